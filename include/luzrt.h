@@ -1,0 +1,221 @@
+/*
+ * luzrt.h — C ABI of the MI355X-native ray-traced lighting path.
+ *
+ * This replaces three interfaces of Luz (hadryansalles/Luz):
+ *   1. the acceleration-structure part of the Vulkan wrapper
+ *        vkw::CreateBLAS / CmdBuildBLAS   source/Graphics/VulkanWrapper.h:202-209,242
+ *                                         source/Graphics/VulkanWrapper.cpp:743-836,1089-1104
+ *        vkw::CreateTLAS / CmdBuildTLAS   source/Graphics/VulkanWrapper.h:192-200,243
+ *                                         source/Graphics/VulkanWrapper.cpp:657-741,1106-1139
+ *   2. the light pass  DeferredRenderer::LightPass(LightConstants)
+ *                                         source/Graphics/DeferredRenderer.hpp:17-26,46
+ *                                         source/Graphics/DeferredRenderer.cpp:324-345
+ *      together with the device program it launches, source/Shaders/light.frag:86-235;
+ *   3. the host<->device blocks it reads   source/Shaders/LuzCommon.h:37-61 (LightBlock),
+ *                                         :94-125 (SceneBlock), :180-189 (LightConstants).
+ *
+ * Conventions
+ *   - Every function returns a luzrt_status; no exception crosses the ABI.
+ *     luzrt_last_error() returns a message for the last failure on a context.
+ *   - Handles are opaque and owned by the library; destroy them explicitly.
+ *     A BLAS referenced by a TLAS that still exists cannot be destroyed
+ *     (LUZRT_E_IN_USE): the reference keeps each BLAS alive through GPUMesh
+ *     (GPUScene.hpp:17-23) because its TLAS instances hold the BLAS address.
+ *   - Buffers named "device" are device pointers (luzrt_malloc, hipMalloc or a
+ *     torch tensor's data_ptr) on the context's device; "host" are host memory.
+ *   - Work is ordered on one HIP stream per context and is asynchronous until
+ *     luzrt_sync().  A context must not be used from two threads at once.
+ *   - Limits follow LuzCommon.h:20-21: 64 lights, 8192 instances.  Beyond
+ *     them calls fail with LUZRT_E_LIMIT (the reference overflows silently,
+ *     GPUScene.cpp:51,240).
+ */
+#ifndef LUZRT_H
+#define LUZRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUZRT_MAX_LIGHTS 64       /* LuzCommon.h:20 LUZ_MAX_LIGHTS */
+#define LUZRT_MAX_INSTANCES 8192  /* LuzCommon.h:21 LUZ_MAX_MODELS */
+#define LUZRT_MAX_SAMPLES 255     /* per-pixel counts are stored in 8 bits */
+
+#define LUZRT_LIGHT_POINT 0       /* LuzCommon.h:24-26 */
+#define LUZRT_LIGHT_SPOT 1
+#define LUZRT_LIGHT_DIRECTIONAL 2
+
+#define LUZRT_SHADOW_RAYTRACED 1  /* LuzCommon.h:28 SHADOW_TYPE_RAYTRACING */
+
+typedef enum luzrt_status {
+    LUZRT_OK = 0,
+    LUZRT_E_INVALID_ARG = -1,
+    LUZRT_E_OUT_OF_MEMORY = -2,
+    LUZRT_E_DEVICE = -3,      /* a HIP call failed */
+    LUZRT_E_IN_USE = -4,      /* BLAS still referenced by a TLAS */
+    LUZRT_E_LIMIT = -5,       /* lights, instances, samples or sizes over the limits */
+    LUZRT_E_NO_DEVICE = -6,   /* no HIP device / bad device index */
+    LUZRT_E_UNSUPPORTED = -7
+} luzrt_status;
+
+typedef struct luzrt_ctx luzrt_ctx;
+typedef struct luzrt_blas luzrt_blas;
+typedef struct luzrt_tlas luzrt_tlas;
+
+/* BLASInstance (VulkanWrapper.h:196-200) as packed by CmdBuildTLAS
+ * (VulkanWrapper.cpp:1113-1126): row r of object_to_world is row r of the
+ * column-major modelMat, custom_index is the ModelBlock index, mask 0xFF
+ * for a visible instance (0 removes it from every ray). */
+typedef struct luzrt_instance {
+    const luzrt_blas* blas;
+    float object_to_world[3][4];
+    uint32_t custom_index;
+    uint32_t mask;
+} luzrt_instance;
+
+/* LightBlock (LuzCommon.h:37-61), the part the lighting path reads.
+ * direction is the light node's W*(0,-1,0,0) and need not be unit length
+ * (GPUScene.cpp:245); angles are in radians (GPUScene.cpp:244,246). */
+typedef struct luzrt_light {
+    float color[3];     float intensity;
+    float position[3];  float inner_angle;
+    float direction[3]; float outer_angle;
+    int32_t type;       int32_t num_shadow_samples;
+    float radius;       int32_t pad;
+} luzrt_light;
+
+/* SceneBlock (LuzCommon.h:94-125), the part the lighting path reads, plus
+ * LightConstants::frame (LuzCommon.h:183). */
+typedef struct luzrt_scene {
+    luzrt_light lights[LUZRT_MAX_LIGHTS];
+    float ambient_color[3]; float ambient_intensity;
+    float cam_pos[3];       int32_t num_lights;
+    float ao_min;           float ao_max;
+    int32_t ao_num_samples; int32_t frame;
+    int32_t shadow_type;    int32_t pad[3];
+} luzrt_scene;
+
+/* The G-buffer of DeferredRenderer::CreateImages (DeferredRenderer.cpp:175-248),
+ * device pointers, tightly packed rows of width pixels.  World-space position
+ * replaces the depth attachment + DepthToWorld (utils.glsl:1-7).  A zero
+ * normal marks a background pixel (light.frag:178). */
+typedef struct luzrt_gbuffer {
+    uint32_t width, height;
+    const float* position;     /* float3 per pixel */
+    const float* normal;       /* float3 per pixel */
+    const uint32_t* albedo;    /* RGBA8 sRGB (light.frag:172 decodes with pow 2.2) */
+    const uint32_t* material;  /* RGBA8: r roughness, g metallic, b occlusion (opaque.frag:57) */
+    const uint32_t* emission;  /* RGBA8 */
+} luzrt_gbuffer;
+
+/* Writable G-buffer for the primary-ray producer. */
+typedef struct luzrt_gbuffer_out {
+    uint32_t width, height;
+    float* position; float* normal;
+    uint32_t* albedo; uint32_t* material; uint32_t* emission;
+} luzrt_gbuffer_out;
+
+/* Pinhole camera for primary rays: direction(x,y) = forward + ndc_x*right +
+ * ndc_y*up with ndc = (pixel centre / size)*2 - 1 and ndc_y growing down the
+ * image (the y flip of CameraNode::GetProj, AssetManager.cpp:375). */
+typedef struct luzrt_camera {
+    float eye[3]; float forward[3]; float right[3]; float up[3];
+    float tmin, tmax;
+} luzrt_camera;
+
+/* ModelBlock material fields (LuzCommon.h:76-92) in G-buffer encoding,
+ * indexed by luzrt_instance::custom_index. */
+typedef struct luzrt_material {
+    uint32_t albedo, material, emission, pad;
+} luzrt_material;
+
+/* GPU time of the stages of the last call, from HIP events on the
+ * context's stream (the reference's CmdTimeStamp, VulkanWrapper.cpp:1276-1296). */
+typedef struct luzrt_timings {
+    float build_ms, compact_ms, raygen_ms, trace_ms, resolve_ms, shade_ms, primary_ms;
+    uint64_t rays;          /* rays cast by the last visibility/light call */
+    uint64_t active_pixels; /* non-background pixels of that call */
+} luzrt_timings;
+
+/* ------------------------------------------------------------ context */
+luzrt_status luzrt_create(luzrt_ctx** out, int device);
+luzrt_status luzrt_destroy(luzrt_ctx* ctx);
+const char*  luzrt_last_error(const luzrt_ctx* ctx);
+const char*  luzrt_status_string(luzrt_status s);
+luzrt_status luzrt_sync(luzrt_ctx* ctx);
+/* Per-stage GPU times of the last build/trace/light/primary call (synchronises). */
+luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
+/* Traversal launch parameters (0 keeps the current value): persistent
+ * workgroups per CU, and the number of idle lanes of a wave that triggers
+ * a refill.  Used by the autotuner. */
+luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill_threshold);
+
+/* ------------------------------------------------------------ memory */
+luzrt_status luzrt_malloc(luzrt_ctx* ctx, size_t bytes, void** device_ptr);
+luzrt_status luzrt_free(luzrt_ctx* ctx, void* device_ptr);
+luzrt_status luzrt_upload(luzrt_ctx* ctx, void* device_dst, const void* host_src, size_t bytes);
+luzrt_status luzrt_download(luzrt_ctx* ctx, void* host_dst, const void* device_src, size_t bytes);
+
+/* ------------------------------------------------------ acceleration structures */
+/* CreateBLAS + CmdBuildBLAS.  vertices: device, positions (float3) at byte
+ * offset 0 of each stride-sized vertex, normals at offset 12 when stride >= 24
+ * (MeshVertex, AssetManager.hpp:83-89); indices: device u32, index_count/3
+ * triangles.  The caller's buffers are read during the call only. */
+luzrt_status luzrt_blas_build(luzrt_ctx* ctx, const void* vertices, uint32_t stride_bytes,
+                              uint32_t vertex_count, const uint32_t* indices,
+                              uint32_t index_count, luzrt_blas** out);
+/* Recompute the boxes of an existing BLAS from moved vertices (same topology). */
+luzrt_status luzrt_blas_refit(luzrt_ctx* ctx, luzrt_blas* blas, const void* vertices);
+luzrt_status luzrt_blas_destroy(luzrt_ctx* ctx, luzrt_blas* blas);
+/* Device pointers of the built structure (64-byte nodes, 36-byte triangles,
+ * slot -> primitive ids) for inspection. */
+luzrt_status luzrt_blas_arrays(const luzrt_blas* blas, const void** nodes, uint32_t* node_count,
+                               const float** tris, const uint32_t** prim_ids, uint32_t* tri_count);
+
+/* CreateTLAS(maxInstances). */
+luzrt_status luzrt_tlas_create(luzrt_ctx* ctx, uint32_t max_instances, luzrt_tlas** out);
+/* CmdBuildTLAS: full rebuild from host instances (VulkanWrapper.cpp:1106-1139). */
+luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* tlas, const luzrt_instance* instances,
+                              uint32_t count);
+luzrt_status luzrt_tlas_destroy(luzrt_ctx* ctx, luzrt_tlas* tlas);
+/* Device pointers: nodes, world->object 3x4 per TLAS slot, slot -> input index. */
+luzrt_status luzrt_tlas_arrays(const luzrt_tlas* tlas, const void** nodes, uint32_t* node_count,
+                               const float** world_to_object, const uint32_t** inst_order,
+                               uint32_t* inst_count);
+
+/* ------------------------------------------------------------ ray queries */
+/* Any-hit queries (gl_RayFlagsTerminateOnFirstHitEXT, mask 0xFF; light.frag:100,126)
+ * over device rays, 8 floats each: origin.xyz, tmin, direction.xyz, tmax.
+ * hits: device u8 per ray. */
+luzrt_status luzrt_trace_rays(luzrt_ctx* ctx, const luzrt_tlas* tlas, const float* rays,
+                              uint32_t count, uint8_t* hits);
+
+/* ------------------------------------------------------------ the lighting path */
+/* Shadow + AO visibility for every G-buffer pixel (light.frag:86-146,229-231).
+ * shadow_counts: device u8 [num_lights][width*height], occluded samples per light;
+ * ao_counts: device u8 [width*height], unoccluded AO samples. */
+luzrt_status luzrt_trace_visibility(luzrt_ctx* ctx, const luzrt_tlas* tlas,
+                                    const luzrt_gbuffer* gbuffer, const luzrt_scene* scene,
+                                    uint8_t* shadow_counts, uint8_t* ao_counts);
+/* GGX lighting from a G-buffer and visibility counts (light.frag:171-235). */
+luzrt_status luzrt_shade(luzrt_ctx* ctx, const luzrt_gbuffer* gbuffer, const luzrt_scene* scene,
+                         const uint8_t* shadow_counts, const uint8_t* ao_counts,
+                         float* rgba32f_out);
+/* DeferredRenderer::LightPass: G-buffer in, RGBA32F light buffer out
+ * (device, width*height*4 floats). */
+luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* gbuffer,
+                              const luzrt_scene* scene, float* rgba32f_out);
+
+/* Primary-ray G-buffer producer (closest hit; stands in for the raster
+ * opaque pass, opaque.frag:21-58).  materials: host array indexed by
+ * custom_index; unknown indices get white, roughness 0.5, metallic 0. */
+luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* tlas,
+                                   const luzrt_camera* camera, const luzrt_material* materials,
+                                   uint32_t material_count, const luzrt_gbuffer_out* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUZRT_H */

@@ -1,0 +1,80 @@
+/*
+ * luzscene.h — C ABI of the host scene model and its GPU packer.
+ *
+ * The C++17 model behind it keeps Luz's scene classes
+ * (source/Resources/AssetManager.hpp): MeshAsset with 48-byte MeshVertex
+ * (:82-97), MaterialAsset (:99-112), Node TRS transforms (:114-174,
+ * AssetManager.cpp:38-59), MeshNode (:176-182), LightNode (:184-227),
+ * CameraNode (:229-275) and SceneAsset settings (:277-296).  The packer
+ * mirrors GPUScene (source/Graphics/GPUScene.hpp:35-63): one BLAS per
+ * distinct mesh (AddMesh, GPUScene.cpp:108-138), SceneBlock/ModelBlock
+ * packing (UpdateResources, :181-346) and the per-frame TLAS rebuild
+ * (UpdateResourcesGPU, :348-366).
+ *
+ * Procedural generators build the measured workloads:
+ *   LUZSCENE_CUBE       unit cube on a 20x20 ground quad, one directional
+ *                       light, 256x256, 1 shadow + 4 AO rays per pixel;
+ *   LUZSCENE_SPHERES    50 instances of 5 fBm-displaced level-5 icospheres
+ *                       (20,480 triangles each) over a 64x64 ground grid,
+ *                       1 directional + 8 point lights, 1920x1080,
+ *                       1 shadow ray per light + 16 AO rays per pixel;
+ *   LUZSCENE_TERRAIN    2048x2048 fBm heightfield (decimated 2x) and 40
+ *                       displaced icospheres, 4 directional + 32 point
+ *                       lights, 3840x2160, 8 AO rays per pixel;
+ *   LUZSCENE_RANDOM     a sphere-field scene with a seeded triangle budget
+ *                       and resolution (batched workloads).
+ */
+#ifndef LUZSCENE_H
+#define LUZSCENE_H
+#include <stdint.h>
+
+#include "luzrt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUZSCENE_CUBE 1
+#define LUZSCENE_SPHERES 2
+#define LUZSCENE_TERRAIN 4
+#define LUZSCENE_RANDOM 5
+
+typedef struct luzscene luzscene;
+typedef struct luzgpu luzgpu;
+
+/* Generate a scene.  For LUZSCENE_RANDOM, tri_budget and width/height are
+ * taken from the arguments (0 = draw them from the seed). */
+luzscene* luzscene_generate(int kind, uint64_t seed, uint32_t tri_budget, uint32_t width,
+                            uint32_t height);
+void      luzscene_destroy(luzscene* s);
+
+uint32_t luzscene_mesh_count(const luzscene* s);
+/* MeshVertex array (12 floats per vertex: position, normal, tangent, uv) and indices. */
+int      luzscene_mesh(const luzscene* s, uint32_t mesh, const float** vertices,
+                       uint32_t* vertex_count, const uint32_t** indices, uint32_t* index_count);
+uint32_t luzscene_node_count(const luzscene* s);
+/* Per mesh node: object->world (row-major 3x4 from Node::GetWorldTransform),
+ * mesh index, and the G-buffer encoding of its material (custom index = node). */
+int      luzscene_nodes(const luzscene* s, float* object_to_world, uint32_t* mesh_ids,
+                        luzrt_material* materials);
+/* SceneBlock as GPUScene::UpdateResources packs it (lights, ambient, AO, camera position). */
+int      luzscene_pack(const luzscene* s, luzrt_scene* out);
+/* Image size and primary-ray camera (CameraNode::GetView/GetProj conventions). */
+int      luzscene_camera(const luzscene* s, uint32_t* width, uint32_t* height, luzrt_camera* out);
+/* Seeded rigid motion of every mesh node for animation frame `frame`. */
+int      luzscene_animate(luzscene* s, uint32_t frame);
+/* Total triangles over all mesh nodes (instanced). */
+uint64_t luzscene_instanced_triangles(const luzscene* s);
+
+/* GPUScene: uploads meshes, builds one BLAS per mesh and the TLAS. */
+luzgpu*  luzgpu_create(luzrt_ctx* ctx, const luzscene* s);
+void     luzgpu_destroy(luzgpu* g);
+/* Re-pack instances from the scene and rebuild the TLAS (per-frame work). */
+luzrt_status luzgpu_update(luzgpu* g, const luzscene* s);
+luzrt_tlas*  luzgpu_tlas(luzgpu* g);
+luzrt_blas*  luzgpu_blas(luzgpu* g, uint32_t mesh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

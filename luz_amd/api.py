@@ -1,0 +1,436 @@
+"""ctypes bindings of the luzrt C ABI (include/luzrt.h, include/luzscene.h).
+
+The native library luz_amd/_lib/libluzrt.so is required: importing this
+module raises ImportError when it has not been built (there is no fallback).
+
+Names follow the reference interface they replace:
+  create_blas / cmd_build_blas  -> vkw::CreateBLAS + CmdBuildBLAS (VulkanWrapper.cpp:743-836, 1089-1104)
+  create_tlas / cmd_build_tlas  -> vkw::CreateTLAS + CmdBuildTLAS (VulkanWrapper.cpp:657-741, 1106-1139)
+  GPUScene.add_mesh / update_resources_gpu -> GPUScene::AddMesh / UpdateResourcesGPU (GPUScene.cpp:108-138, 348-366)
+  light_pass -> DeferredRenderer::LightPass (DeferredRenderer.cpp:324-345)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libluzrt.so"
+
+if not LIB_PATH.exists():
+    raise ImportError(f"luzrt native library missing at {LIB_PATH}; build it with `python -m luz_amd.build`")
+
+lib = C.CDLL(str(LIB_PATH))
+
+MAX_LIGHTS = 64
+MAX_INSTANCES = 8192
+LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
+SCENE_CUBE, SCENE_SPHERES, SCENE_TERRAIN, SCENE_RANDOM = 1, 2, 4, 5
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "device error", -4: "in use",
+          -5: "limit exceeded", -6: "no device", -7: "unsupported"}
+
+
+class LuzError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Light(C.Structure):
+    _fields_ = [("color", C.c_float * 3), ("intensity", C.c_float),
+                ("position", C.c_float * 3), ("inner_angle", C.c_float),
+                ("direction", C.c_float * 3), ("outer_angle", C.c_float),
+                ("type", C.c_int32), ("num_shadow_samples", C.c_int32),
+                ("radius", C.c_float), ("pad", C.c_int32)]
+
+
+class SceneBlock(C.Structure):
+    _fields_ = [("lights", Light * MAX_LIGHTS),
+                ("ambient_color", C.c_float * 3), ("ambient_intensity", C.c_float),
+                ("cam_pos", C.c_float * 3), ("num_lights", C.c_int32),
+                ("ao_min", C.c_float), ("ao_max", C.c_float),
+                ("ao_num_samples", C.c_int32), ("frame", C.c_int32),
+                ("shadow_type", C.c_int32), ("pad", C.c_int32 * 3)]
+
+
+class GBuffer(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
+                ("position", C.c_void_p), ("normal", C.c_void_p), ("albedo", C.c_void_p),
+                ("material", C.c_void_p), ("emission", C.c_void_p)]
+
+
+GBufferOut = GBuffer
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("tmin", C.c_float), ("tmax", C.c_float)]
+
+
+class Material(C.Structure):
+    _fields_ = [("albedo", C.c_uint32), ("material", C.c_uint32), ("emission", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("blas", C.c_void_p), ("object_to_world", (C.c_float * 4) * 3),
+                ("custom_index", C.c_uint32), ("mask", C.c_uint32)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("build_ms", C.c_float), ("compact_ms", C.c_float), ("raygen_ms", C.c_float),
+                ("trace_ms", C.c_float), ("resolve_ms", C.c_float), ("shade_ms", C.c_float),
+                ("primary_ms", C.c_float), ("rays", C.c_uint64), ("active_pixels", C.c_uint64)]
+
+
+assert C.sizeof(Light) == 64
+assert C.sizeof(SceneBlock) == 64 * 64 + 64
+assert C.sizeof(Instance) == 64
+
+_P = C.c_void_p
+_sigs = {
+    "luzrt_create": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "luzrt_destroy": (C.c_int, [_P]),
+    "luzrt_last_error": (C.c_char_p, [_P]),
+    "luzrt_status_string": (C.c_char_p, [C.c_int]),
+    "luzrt_sync": (C.c_int, [_P]),
+    "luzrt_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
+    "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "luzrt_malloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "luzrt_free": (C.c_int, [_P, _P]),
+    "luzrt_upload": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "luzrt_download": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "luzrt_blas_build": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(_P)]),
+    "luzrt_blas_refit": (C.c_int, [_P, _P, _P]),
+    "luzrt_blas_destroy": (C.c_int, [_P, _P]),
+    "luzrt_blas_arrays": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
+                                    C.POINTER(_P), C.POINTER(C.c_uint32)]),
+    "luzrt_tlas_create": (C.c_int, [_P, C.c_uint32, C.POINTER(_P)]),
+    "luzrt_tlas_build": (C.c_int, [_P, _P, C.POINTER(Instance), C.c_uint32]),
+    "luzrt_tlas_destroy": (C.c_int, [_P, _P]),
+    "luzrt_tlas_arrays": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
+                                    C.POINTER(_P), C.POINTER(C.c_uint32)]),
+    "luzrt_trace_rays": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
+    "luzrt_trace_visibility": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P]),
+    "luzrt_shade": (C.c_int, [_P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P, _P]),
+    "luzrt_light_pass": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P]),
+    "luzrt_primary_gbuffer": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(Material), C.c_uint32,
+                                        C.POINTER(GBuffer)]),
+    "luzscene_generate": (_P, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "luzscene_destroy": (None, [_P]),
+    "luzscene_mesh_count": (C.c_uint32, [_P]),
+    "luzscene_mesh": (C.c_int, [_P, C.c_uint32, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
+                                C.POINTER(C.c_uint32)]),
+    "luzscene_node_count": (C.c_uint32, [_P]),
+    "luzscene_nodes": (C.c_int, [_P, _P, _P, _P]),
+    "luzscene_pack": (C.c_int, [_P, C.POINTER(SceneBlock)]),
+    "luzscene_camera": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(Camera)]),
+    "luzscene_animate": (C.c_int, [_P, C.c_uint32]),
+    "luzscene_instanced_triangles": (C.c_uint64, [_P]),
+    "luzgpu_create": (_P, [_P, _P]),
+    "luzgpu_destroy": (None, [_P]),
+    "luzgpu_update": (C.c_int, [_P, _P]),
+    "luzgpu_tlas": (_P, [_P]),
+    "luzgpu_blas": (_P, [_P, C.c_uint32]),
+}
+for _name, (_res, _args) in _sigs.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = tuple(_sigs)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ------------------------------------------------------------------ context
+class DeviceBuffer:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _P()
+        ctx._check(lib.luzrt_malloc(ctx.handle, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes, (arr.nbytes, self.nbytes)
+        self.ctx._check(lib.luzrt_upload(self.ctx.handle, self.ptr, _ptr(arr), arr.nbytes))
+        return self
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._check(lib.luzrt_download(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib.luzrt_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device and one stream (luzrt_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _P()
+        s = lib.luzrt_create(C.byref(h), device)
+        if s != 0:
+            raise LuzError(s, f"luzrt_create(device={device}) failed")
+        self.handle = h.value
+
+    def _check(self, s: int) -> None:
+        if s != 0:
+            raise LuzError(s, lib.luzrt_last_error(self.handle).decode())
+
+    def close(self) -> None:
+        if self.handle:
+            lib.luzrt_destroy(self.handle)
+            self.handle = None
+
+    def sync(self) -> None:
+        self._check(lib.luzrt_sync(self.handle))
+
+    def timings(self) -> Timings:
+        t = Timings()
+        self._check(lib.luzrt_get_timings(self.handle, C.byref(t)))
+        return t
+
+    def set_trace_config(self, groups_per_cu: int = 0, refill: int = 0) -> None:
+        self._check(lib.luzrt_set_trace_config(self.handle, groups_per_cu, refill))
+
+    def buffer(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> DeviceBuffer:
+        arr = np.ascontiguousarray(arr)
+        return DeviceBuffer(self, arr.nbytes).upload(arr)
+
+    # -------------------------------------------------- acceleration structures
+    def create_blas(self, vertices: DeviceBuffer, stride: int, vertex_count: int,
+                    indices: DeviceBuffer, index_count: int) -> "Blas":
+        h = _P()
+        self._check(lib.luzrt_blas_build(self.handle, vertices.ptr, stride, vertex_count,
+                                         indices.ptr, index_count, C.byref(h)))
+        return Blas(self, h.value, index_count // 3)
+
+    def create_tlas(self, max_instances: int) -> "Tlas":
+        h = _P()
+        self._check(lib.luzrt_tlas_create(self.handle, max_instances, C.byref(h)))
+        return Tlas(self, h.value)
+
+    # -------------------------------------------------- the lighting path
+    def trace_rays(self, tlas: "Tlas", rays: DeviceBuffer, count: int, hits: DeviceBuffer) -> None:
+        self._check(lib.luzrt_trace_rays(self.handle, tlas.handle, rays.ptr, count, hits.ptr))
+
+    def trace_visibility(self, tlas: "Tlas", g: GBuffer, scene: SceneBlock, shadow: DeviceBuffer,
+                         ao: DeviceBuffer) -> None:
+        self._check(lib.luzrt_trace_visibility(self.handle, tlas.handle, C.byref(g), C.byref(scene),
+                                               shadow.ptr, ao.ptr))
+
+    def shade(self, g: GBuffer, scene: SceneBlock, shadow: DeviceBuffer, ao: DeviceBuffer,
+              out: DeviceBuffer) -> None:
+        self._check(lib.luzrt_shade(self.handle, C.byref(g), C.byref(scene), shadow.ptr, ao.ptr, out.ptr))
+
+    def light_pass(self, tlas: "Tlas", g: GBuffer, scene: SceneBlock, out: DeviceBuffer) -> None:
+        self._check(lib.luzrt_light_pass(self.handle, tlas.handle, C.byref(g), C.byref(scene), out.ptr))
+
+    def primary_gbuffer(self, tlas: "Tlas", cam: Camera, materials, out: GBuffer) -> None:
+        n = len(materials)
+        arr = (Material * max(n, 1))(*materials)
+        self._check(lib.luzrt_primary_gbuffer(self.handle, tlas.handle, C.byref(cam), arr, n, C.byref(out)))
+
+
+class Blas:
+    def __init__(self, ctx: Context, handle: int, tri_count: int):
+        self.ctx, self.handle, self.tri_count = ctx, handle, tri_count
+
+    def refit(self, vertices: DeviceBuffer) -> None:
+        self.ctx._check(lib.luzrt_blas_refit(self.ctx.handle, self.handle, vertices.ptr))
+
+    def arrays(self):
+        """(nodes u8[n,64], tris f32[t,9], prim_ids u32[t]) downloaded from the device."""
+        nodes, nn, tris, prims, nt = _P(), C.c_uint32(), _P(), _P(), C.c_uint32()
+        self.ctx._check(lib.luzrt_blas_arrays(self.handle, C.byref(nodes), C.byref(nn), C.byref(tris),
+                                              C.byref(prims), C.byref(nt)))
+        return _download_arrays(self.ctx, nodes.value, nn.value, tris.value, prims.value, nt.value)
+
+    def destroy(self) -> None:
+        self.ctx._check(lib.luzrt_blas_destroy(self.ctx.handle, self.handle))
+        self.handle = None
+
+
+def _download_arrays(ctx, nodes, nn, tris, prims, nt):
+    n_out = np.empty((nn, 64), np.uint8)
+    ctx._check(lib.luzrt_download(ctx.handle, _ptr(n_out), nodes, n_out.nbytes))
+    t_out = np.empty((nt, 9), np.float32)
+    p_out = np.empty(nt, np.uint32)
+    if nt:
+        ctx._check(lib.luzrt_download(ctx.handle, _ptr(t_out), tris, t_out.nbytes))
+        ctx._check(lib.luzrt_download(ctx.handle, _ptr(p_out), prims, p_out.nbytes))
+    return n_out, t_out, p_out
+
+
+class Tlas:
+    def __init__(self, ctx: Context, handle: int):
+        self.ctx, self.handle = ctx, handle
+
+    def build(self, instances) -> None:
+        n = len(instances)
+        arr = (Instance * max(n, 1))(*instances)
+        self.ctx._check(lib.luzrt_tlas_build(self.ctx.handle, self.handle, arr, n))
+
+    def arrays(self):
+        """(nodes u8[n,64], world_to_object f32[k,12], order u32[k])."""
+        nodes, nn, w2o, order, cnt = _P(), C.c_uint32(), _P(), _P(), C.c_uint32()
+        self.ctx._check(lib.luzrt_tlas_arrays(self.handle, C.byref(nodes), C.byref(nn), C.byref(w2o),
+                                              C.byref(order), C.byref(cnt)))
+        n_out = np.empty((nn.value, 64), np.uint8)
+        self.ctx._check(lib.luzrt_download(self.ctx.handle, _ptr(n_out), nodes.value, n_out.nbytes))
+        k = cnt.value
+        w = np.empty((k, 12), np.float32)
+        o = np.empty(k, np.uint32)
+        if k:
+            self.ctx._check(lib.luzrt_download(self.ctx.handle, _ptr(w), w2o.value, w.nbytes))
+            self.ctx._check(lib.luzrt_download(self.ctx.handle, _ptr(o), order.value, o.nbytes))
+        return n_out, w, o
+
+    def destroy(self) -> None:
+        self.ctx._check(lib.luzrt_tlas_destroy(self.ctx.handle, self.handle))
+        self.handle = None
+
+
+def make_instance(blas: Blas, object_to_world: np.ndarray, custom_index: int, mask: int = 0xFF) -> Instance:
+    inst = Instance()
+    inst.blas = blas.handle
+    m = np.asarray(object_to_world, np.float32).reshape(3, 4)
+    for r in range(3):
+        for c in range(4):
+            inst.object_to_world[r][c] = float(m[r, c])
+    inst.custom_index = custom_index
+    inst.mask = mask
+    return inst
+
+
+# ------------------------------------------------------------------ scene model
+class HostScene:
+    """Luz's SceneAsset with procedural content (luzscene)."""
+
+    def __init__(self, kind: int, seed: int = 0, tri_budget: int = 0, width: int = 0, height: int = 0):
+        self.handle = lib.luzscene_generate(kind, seed, tri_budget, width, height)
+        if not self.handle:
+            raise ValueError(f"unknown scene kind {kind}")
+
+    def close(self) -> None:
+        if self.handle:
+            lib.luzscene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def mesh_count(self) -> int:
+        return lib.luzscene_mesh_count(self.handle)
+
+    def mesh(self, i: int):
+        v, nv, idx, ni = _P(), C.c_uint32(), _P(), C.c_uint32()
+        if lib.luzscene_mesh(self.handle, i, C.byref(v), C.byref(nv), C.byref(idx), C.byref(ni)) != 0:
+            raise IndexError(i)
+        verts = np.ctypeslib.as_array(C.cast(v.value, C.POINTER(C.c_float)), shape=(nv.value, 12)).copy()
+        inds = np.ctypeslib.as_array(C.cast(idx.value, C.POINTER(C.c_uint32)), shape=(ni.value,)).copy()
+        return verts, inds
+
+    def nodes(self):
+        n = lib.luzscene_node_count(self.handle)
+        o2w = np.zeros((n, 12), np.float32)
+        mesh = np.zeros(n, np.uint32)
+        mats = (Material * max(n, 1))()
+        lib.luzscene_nodes(self.handle, _ptr(o2w), _ptr(mesh), C.cast(mats, C.c_void_p))
+        return o2w, mesh, [mats[i] for i in range(n)]
+
+    def pack(self) -> SceneBlock:
+        s = SceneBlock()
+        if lib.luzscene_pack(self.handle, C.byref(s)) != 0:
+            raise LuzError(-5, "scene has more than 64 lights")
+        return s
+
+    def camera(self):
+        w, h, cam = C.c_uint32(), C.c_uint32(), Camera()
+        lib.luzscene_camera(self.handle, C.byref(w), C.byref(h), C.byref(cam))
+        return w.value, h.value, cam
+
+    def animate(self, frame: int) -> None:
+        lib.luzscene_animate(self.handle, frame)
+
+    @property
+    def instanced_triangles(self) -> int:
+        return lib.luzscene_instanced_triangles(self.handle)
+
+
+class GPUScene:
+    """GPUScene: one BLAS per mesh, TLAS over mesh nodes (luzgpu)."""
+
+    def __init__(self, ctx: Context, scene: HostScene):
+        self.ctx = ctx
+        self.handle = lib.luzgpu_create(ctx.handle, scene.handle)
+        if not self.handle:
+            raise LuzError(-3, "GPUScene creation failed: " + lib.luzrt_last_error(ctx.handle).decode())
+        self.tlas = Tlas(ctx, lib.luzgpu_tlas(self.handle))
+
+    def update_resources_gpu(self, scene: HostScene) -> None:
+        self.ctx._check(lib.luzgpu_update(self.handle, scene.handle))
+
+    def blas(self, mesh: int) -> Blas:
+        return Blas(self.ctx, lib.luzgpu_blas(self.handle, mesh), 0)
+
+    def close(self) -> None:
+        if self.handle:
+            lib.luzgpu_destroy(self.handle)
+            self.handle = None
+
+
+class GBufferDevice:
+    """Device G-buffer planes for width x height pixels."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        n = width * height
+        self.width, self.height = width, height
+        self.position = ctx.buffer(n * 12)
+        self.normal = ctx.buffer(n * 12)
+        self.albedo = ctx.buffer(n * 4)
+        self.material = ctx.buffer(n * 4)
+        self.emission = ctx.buffer(n * 4)
+
+    def desc(self) -> GBuffer:
+        return GBuffer(self.width, self.height, self.position.ptr, self.normal.ptr, self.albedo.ptr,
+                       self.material.ptr, self.emission.ptr)
+
+    def upload(self, planes: dict) -> None:
+        for k in ("position", "normal", "albedo", "material", "emission"):
+            getattr(self, k).upload(planes[k])
+
+    def download(self) -> dict:
+        n = self.width * self.height
+        return {"position": self.position.download(np.float32, (n, 3)),
+                "normal": self.normal.download(np.float32, (n, 3)),
+                "albedo": self.albedo.download(np.uint32, n),
+                "material": self.material.download(np.uint32, n),
+                "emission": self.emission.download(np.uint32, n)}

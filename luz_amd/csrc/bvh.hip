@@ -1,0 +1,725 @@
+// bvh.hip — BLAS and TLAS builds on gfx950.
+//
+// Replaces the driver builds behind vkw::CmdBuildBLAS / vkw::CmdBuildTLAS
+// (VulkanWrapper.cpp:1089-1104, 1106-1139) with a deterministic LBVH
+// (Karras 2012, "Maximizing Parallelism in the Construction of BVHs,
+// Octrees, and k-d Trees"):
+//   1. padded triangle boxes + FP64 centroid bounds (block reductions);
+//   2. 30-bit Morton codes in the FP64 frame;
+//   3. stable LSD radix sort, 4 passes of 8 bits: per-tile histograms, one
+//      scan, and a scatter that ranks keys inside a tile with wave ballots
+//      (8 ballots give each lane the set of lanes holding the same digit);
+//   4. Karras hierarchy, one thread per internal node;
+//   5. triangle gather into slot order (36-byte records);
+//   6. bottom-up refit, one thread per leaf, second arrival continues.
+// The TLAS build runs in one workgroup (<= 8192 instances): world boxes,
+// FP64 frame, Morton codes, bitonic sort of (code, index) in LDS, Karras,
+// refit.  Both produce byte-identical nodes to a sequential CPU build.
+#include "internal.hpp"
+
+namespace luz {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;
+
+__device__ inline uint32_t expand10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ inline uint32_t morton_code(const double* frame, double cx, double cy, double cz) {
+    double c[3] = {cx, cy, cz};
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double t = (c[k] - frame[k]) * frame[3 + k];
+        t = floor(t);
+        if (t < 0.0) t = 0.0;
+        if (t > 1023.0) t = 1023.0;
+        q[k] = (uint32_t)t;
+    }
+    return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+}
+
+// Padded triangle box (max |coord| * 2^-16 on every side).
+__device__ inline void tri_box(V3 a, V3 b, V3 c, float* box) {
+    float lo0 = fmin_(fmin_(a.x, b.x), c.x), hi0 = fmax_(fmax_(a.x, b.x), c.x);
+    float lo1 = fmin_(fmin_(a.y, b.y), c.y), hi1 = fmax_(fmax_(a.y, b.y), c.y);
+    float lo2 = fmin_(fmin_(a.z, b.z), c.z), hi2 = fmax_(fmax_(a.z, b.z), c.z);
+    float m = 0.0f;
+    m = fmax_(m, fabsf(lo0)); m = fmax_(m, fabsf(hi0));
+    m = fmax_(m, fabsf(lo1)); m = fmax_(m, fabsf(hi1));
+    m = fmax_(m, fabsf(lo2)); m = fmax_(m, fabsf(hi2));
+    float pad = m * 0x1p-16f;
+    box[0] = lo0 - pad; box[1] = lo1 - pad; box[2] = lo2 - pad;
+    box[3] = hi0 + pad; box[4] = hi1 + pad; box[5] = hi2 + pad;
+}
+
+__device__ inline V3 load_pos(const uint8_t* base, uint32_t stride, uint32_t i) {
+    const float* p = (const float*)(base + (size_t)i * stride);
+    return v3(p[0], p[1], p[2]);
+}
+
+template <typename T, typename Op>
+__device__ inline T block_reduce(T v, T* sh, Op op) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] = op(sh[tid], sh[tid + s]);
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+struct DMin { __device__ double operator()(double a, double b) const { return b < a ? b : a; } };
+struct DMax { __device__ double operator()(double a, double b) const { return b > a ? b : a; } };
+
+// Index of vertex k of triangle t, clamped into range; out-of-range
+// indices set the error flag (the build then fails) instead of reading
+// outside the vertex buffer.
+__device__ inline uint32_t vidx(const uint32_t* idx, uint32_t i, uint32_t nv, uint32_t* error) {
+    uint32_t v = idx[i];
+    if (v >= nv) { atomicOr(error, 2u); v = 0; }
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_boxes(const uint8_t* verts, uint32_t stride,
+                                                      uint32_t nv, const uint32_t* idx, uint32_t n,
+                                                      float* boxes, double* partial,
+                                                      uint32_t* error) {
+    __shared__ double sh[kBlock];
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    double c[3] = {INFINITY, INFINITY, INFINITY}, C[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (t < n) {
+        V3 a = load_pos(verts, stride, vidx(idx, 3 * t, nv, error));
+        V3 b = load_pos(verts, stride, vidx(idx, 3 * t + 1, nv, error));
+        V3 cc = load_pos(verts, stride, vidx(idx, 3 * t + 2, nv, error));
+        float bx[6];
+        tri_box(a, b, cc, bx);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) boxes[(size_t)t * 6 + k] = bx[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double ck = 0.5 * ((double)bx[k] + (double)bx[3 + k]);
+            c[k] = ck; C[k] = ck;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double mn = block_reduce(c[k], sh, DMin());
+        double mx = block_reduce(C[k], sh, DMax());
+        if (threadIdx.x == 0) { partial[blockIdx.x * 6 + k] = mn; partial[blockIdx.x * 6 + 3 + k] = mx; }
+    }
+}
+
+// frame = {cmin[3], scale[3]} with scale = 1024/extent (0 for a flat axis).
+__global__ __launch_bounds__(kBlock) void k_frame(const double* partial, uint32_t nparts,
+                                                  double* frame) {
+    __shared__ double sh[kBlock];
+    for (int k = 0; k < 3; ++k) {
+        double mn = INFINITY, mx = -INFINITY;
+        for (uint32_t i = threadIdx.x; i < nparts; i += kBlock) {
+            double a = partial[i * 6 + k], b = partial[i * 6 + 3 + k];
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+        }
+        mn = block_reduce(mn, sh, DMin());
+        mx = block_reduce(mx, sh, DMax());
+        if (threadIdx.x == 0) {
+            double e = mx - mn;
+            frame[k] = mn;
+            frame[3 + k] = e > 0.0 ? 1024.0 / e : 0.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const float* boxes, uint32_t n,
+                                                   const double* frame, uint32_t* keys,
+                                                   uint32_t* vals) {
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const float* b = boxes + (size_t)t * 6;
+    double cx = 0.5 * ((double)b[0] + (double)b[3]);
+    double cy = 0.5 * ((double)b[1] + (double)b[4]);
+    double cz = 0.5 * ((double)b[2] + (double)b[5]);
+    keys[t] = morton_code(frame, cx, cy, cz);
+    vals[t] = t;
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(kBlock) void k_sort_hist(const uint32_t* keys, uint32_t n,
+                                                      int shift, uint32_t* hist,
+                                                      uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        uint32_t i = base + r * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of count entries with one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* data, uint32_t count) {
+    __shared__ uint32_t sh[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < count; base += 1024) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < count ? data[i] : 0u;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t add = threadIdx.x >= (uint32_t)off ? sh[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += add;
+            __syncthreads();
+        }
+        uint32_t incl = sh[threadIdx.x];
+        uint32_t c = carry;
+        if (i < count) data[i] = c + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c + incl;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint32_t* keys_in,
+                                                         const uint32_t* vals_in, uint32_t n,
+                                                         int shift, const uint32_t* offsets,
+                                                         uint32_t nblocks, uint32_t* keys_out,
+                                                         uint32_t* vals_out) {
+    __shared__ uint32_t digit_base[256];
+    __shared__ uint32_t wave_cnt[kBlock / 64][256];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    digit_base[tid] = offsets[tid * nblocks + blockIdx.x];
+    for (int w = 0; w < kBlock / 64; ++w) wave_cnt[w][tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kSortItems; ++r) {
+        uint32_t i = base + r * kBlock + tid;
+        bool valid = i < n;
+        uint32_t key = valid ? keys_in[i] : 0u;
+        uint32_t val = valid ? vals_in[i] : 0u;
+        uint32_t d = (key >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wave_cnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t run = digit_base[tid];
+            for (int w = 0; w < kBlock / 64; ++w) {
+                uint32_t c = wave_cnt[w][tid];
+                wave_cnt[w][tid] = run;
+                run += c;
+            }
+            digit_base[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = wave_cnt[wave][d] + rank;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        for (int w = 0; w < kBlock / 64; ++w) wave_cnt[w][tid] = 0;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- hierarchy
+__device__ inline int delta_k(const uint32_t* k, int n, int i, int j) {
+    if (j < 0 || j > n - 1) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+// Karras split for internal node i; writes children and parents.
+__device__ inline void karras_node(const uint32_t* keys, int n, int i, Node* nodes,
+                                   uint32_t* leaf_parent) {
+    int d = (delta_k(keys, n, i, i + 1) - delta_k(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta_k(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta_k(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta_k(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta_k(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta_k(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? d : 0);
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    uint32_t left = (lo == gamma) ? (kLeaf | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t right = (hi == gamma + 1) ? (kLeaf | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    Node& nd = nodes[i];
+    nd.child[0] = left;
+    nd.child[1] = right;
+    nd.pad = 0;
+    if (i == 0) nd.parent = kNone;
+    if (left & kLeaf) leaf_parent[left & ~kLeaf] = (uint32_t)i; else nodes[left].parent = (uint32_t)i;
+    if (right & kLeaf) leaf_parent[right & ~kLeaf] = (uint32_t)i; else nodes[right].parent = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, uint32_t n, Node* nodes,
+                                                   uint32_t* leaf_parent, uint32_t* counters) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    karras_node(keys, (int)n, (int)i, nodes, leaf_parent);
+    counters[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(const uint8_t* verts, uint32_t stride,
+                                                   uint32_t nv, const uint32_t* idx,
+                                                   const uint32_t* order, uint32_t n,
+                                                   const float* boxes, float* tris, float* nrms,
+                                                   uint32_t* prims, uint32_t* slot_idx,
+                                                   float* slot_box, uint32_t* error) {
+    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    uint32_t p = order[s];
+    prims[s] = p;
+    bool has_n = stride >= 24;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        uint32_t vi = vidx(idx, 3 * p + v, nv, error);
+        slot_idx[3 * s + v] = vi;
+        const float* q = (const float*)(verts + (size_t)vi * stride);
+        tris[(size_t)s * 9 + v * 3 + 0] = q[0];
+        tris[(size_t)s * 9 + v * 3 + 1] = q[1];
+        tris[(size_t)s * 9 + v * 3 + 2] = q[2];
+        nrms[(size_t)s * 9 + v * 3 + 0] = has_n ? q[3] : 0.0f;
+        nrms[(size_t)s * 9 + v * 3 + 1] = has_n ? q[4] : 0.0f;
+        nrms[(size_t)s * 9 + v * 3 + 2] = has_n ? q[5] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) slot_box[(size_t)s * 6 + k] = boxes[(size_t)p * 6 + k];
+}
+
+__device__ inline float ld_agent(const float* p) {
+    uint32_t u = __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __uint_as_float(u);
+}
+
+__device__ inline void store_child_box(Node* nd, int side, const float* b) {
+    nd->lox[side] = b[0]; nd->loy[side] = b[1]; nd->loz[side] = b[2];
+    nd->hix[side] = b[3]; nd->hiy[side] = b[4]; nd->hiz[side] = b[5];
+}
+
+// Climb from leaf slot s; the second child to arrive at a node computes its box.
+// Karras trees are at most 64 levels deep (the augmented key has 64 bits),
+// so the climb is bounded; exceeding the bound flags a malformed tree.
+__device__ inline void refit_climb(Node* nodes, const uint32_t* leaf_parent, uint32_t* counters,
+                                   uint32_t s, const float* leaf_box, uint32_t* error) {
+    float box[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) box[k] = leaf_box[k];
+    uint32_t c = kLeaf | s;
+    uint32_t p = leaf_parent[s];
+    for (int level = 0; p != kNone; ++level) {
+        if (level > 80) { atomicOr(error, 1u); return; }
+        Node* nd = &nodes[p];
+        int side = nd->child[0] == c ? 0 : 1;
+        store_child_box(nd, side, box);
+        __threadfence();
+        uint32_t old = atomicAdd(&counters[p], 1u);
+        if (old == 0) return;
+        __threadfence();
+        float b0[6], b1[6];
+        b0[0] = ld_agent(&nd->lox[0]); b0[1] = ld_agent(&nd->loy[0]); b0[2] = ld_agent(&nd->loz[0]);
+        b0[3] = ld_agent(&nd->hix[0]); b0[4] = ld_agent(&nd->hiy[0]); b0[5] = ld_agent(&nd->hiz[0]);
+        b1[0] = ld_agent(&nd->lox[1]); b1[1] = ld_agent(&nd->loy[1]); b1[2] = ld_agent(&nd->loz[1]);
+        b1[3] = ld_agent(&nd->hix[1]); b1[4] = ld_agent(&nd->hiy[1]); b1[5] = ld_agent(&nd->hiz[1]);
+        box[0] = fmin_(b0[0], b1[0]); box[1] = fmin_(b0[1], b1[1]); box[2] = fmin_(b0[2], b1[2]);
+        box[3] = fmax_(b0[3], b1[3]); box[4] = fmax_(b0[4], b1[4]); box[5] = fmax_(b0[5], b1[5]);
+        c = p;
+        p = nd->parent;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit(Node* nodes, const uint32_t* leaf_parent,
+                                                  uint32_t* counters, uint32_t n,
+                                                  const float* slot_box, uint32_t* error) {
+    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    refit_climb(nodes, leaf_parent, counters, s, slot_box + (size_t)s * 6, error);
+}
+
+// Recompute slot boxes and triangle records from moved vertices.
+__global__ __launch_bounds__(kBlock) void k_refit_boxes(const uint8_t* verts, uint32_t stride,
+                                                        const uint32_t* slot_idx, uint32_t n,
+                                                        float* tris, float* slot_box,
+                                                        uint32_t* counters, uint32_t ncounters) {
+    // slot_idx was validated by the build
+    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < ncounters) counters[s] = 0;
+    if (s >= n) return;
+    V3 v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = load_pos(verts, stride, slot_idx[3 * s + k]);
+        tris[(size_t)s * 9 + k * 3 + 0] = v[k].x;
+        tris[(size_t)s * 9 + k * 3 + 1] = v[k].y;
+        tris[(size_t)s * 9 + k * 3 + 2] = v[k].z;
+    }
+    tri_box(v[0], v[1], v[2], slot_box + (size_t)s * 6);
+}
+
+// BLAS of 0 or 1 triangles: a single root with an empty second child.
+__global__ void k_small_root(const uint8_t* verts, uint32_t stride, uint32_t nv,
+                             const uint32_t* idx, uint32_t n, Node* nodes, float* tris,
+                             float* nrms, uint32_t* prims, uint32_t* slot_idx, uint32_t* error) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Node nd;
+    float empty[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    float b[6];
+    if (n == 1) {
+        V3 v[3];
+        for (int k = 0; k < 3; ++k) {
+            uint32_t vi = vidx(idx, k, nv, error);
+            slot_idx[k] = vi;
+            const float* q = (const float*)(verts + (size_t)vi * stride);
+            v[k] = v3(q[0], q[1], q[2]);
+            tris[k * 3 + 0] = q[0]; tris[k * 3 + 1] = q[1]; tris[k * 3 + 2] = q[2];
+            nrms[k * 3 + 0] = stride >= 24 ? q[3] : 0.0f;
+            nrms[k * 3 + 1] = stride >= 24 ? q[4] : 0.0f;
+            nrms[k * 3 + 2] = stride >= 24 ? q[5] : 0.0f;
+        }
+        tri_box(v[0], v[1], v[2], b);
+        prims[0] = 0;
+    } else {
+        for (int k = 0; k < 6; ++k) b[k] = empty[k];
+    }
+    store_child_box(&nd, 0, b);
+    store_child_box(&nd, 1, empty);
+    nd.child[0] = n == 1 ? kLeaf : kEmpty;
+    nd.child[1] = kEmpty;
+    nd.parent = kNone;
+    nd.pad = 0;
+    nodes[0] = nd;
+}
+
+// ---------------------------------------------------------------- TLAS
+constexpr int kTlasThreads = 1024;
+
+__global__ __launch_bounds__(kTlasThreads) void k_tlas_build(const InstIn* in, uint32_t n,
+                                                             float* wbox, uint32_t* leaf_parent,
+                                                             uint32_t* counters, Node* nodes,
+                                                             InstRec* recs, InstAux* aux,
+                                                             float* w2o_out, uint32_t* order_out,
+                                                             uint32_t m, uint32_t* error) {
+    // max(m, 1024) 8-byte entries, m a power of two >= n.  The reductions of
+    // steps 1-2 use the same memory before the sort keys are written, which
+    // keeps the kernel within 64 KB of LDS for 8192 instances.
+    extern __shared__ uint64_t skeys[];
+    float* shf = reinterpret_cast<float*>(skeys);
+    double* shd = reinterpret_cast<double*>(skeys);
+    __shared__ float s_pad;
+    __shared__ double s_frame[6];
+    const int tid = threadIdx.x;
+
+    // 1. world boxes of the BLAS root boxes, scene magnitude
+    float mloc = 0.0f;
+    for (uint32_t i = tid; i < n; i += kTlasThreads) {
+        const Node& r = in[i].nodes[0];
+        float rlo[3] = {fmin_(r.lox[0], r.lox[1]), fmin_(r.loy[0], r.loy[1]), fmin_(r.loz[0], r.loz[1])};
+        float rhi[3] = {fmax_(r.hix[0], r.hix[1]), fmax_(r.hiy[0], r.hiy[1]), fmax_(r.hiz[0], r.hiz[1])};
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 8; ++c) {
+            V3 p = v3((c & 1) ? rhi[0] : rlo[0], (c & 2) ? rhi[1] : rlo[1], (c & 4) ? rhi[2] : rlo[2]);
+            V3 w = xform_point(in[i].o2w, p);
+            lo[0] = fmin_(lo[0], w.x); hi[0] = fmax_(hi[0], w.x);
+            lo[1] = fmin_(lo[1], w.y); hi[1] = fmax_(hi[1], w.y);
+            lo[2] = fmin_(lo[2], w.z); hi[2] = fmax_(hi[2], w.z);
+        }
+        for (int k = 0; k < 3; ++k) {
+            mloc = fmax_(mloc, fabsf(lo[k]));
+            mloc = fmax_(mloc, fabsf(hi[k]));
+            wbox[i * 6 + k] = lo[k];
+            wbox[i * 6 + 3 + k] = hi[k];
+        }
+    }
+    shf[tid] = mloc;
+    __syncthreads();
+    for (int s = kTlasThreads / 2; s > 0; s >>= 1) {
+        if (tid < s) shf[tid] = fmax_(shf[tid], shf[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) s_pad = shf[0] * 0x1p-14f;
+    __syncthreads();
+    const float pad = s_pad;
+    // 2. padded boxes, FP64 centroid frame
+    double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = tid; i < n; i += kTlasThreads) {
+        for (int k = 0; k < 3; ++k) {
+            float lo = wbox[i * 6 + k] - pad, hi = wbox[i * 6 + 3 + k] + pad;
+            wbox[i * 6 + k] = lo;
+            wbox[i * 6 + 3 + k] = hi;
+            double c = 0.5 * ((double)lo + (double)hi);
+            cmn[k] = c < cmn[k] ? c : cmn[k];
+            cmx[k] = c > cmx[k] ? c : cmx[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        shd[tid] = cmn[k];
+        __syncthreads();
+        for (int s = kTlasThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) shd[tid] = shd[tid + s] < shd[tid] ? shd[tid + s] : shd[tid];
+            __syncthreads();
+        }
+        double mn = shd[0];
+        __syncthreads();
+        shd[tid] = cmx[k];
+        __syncthreads();
+        for (int s = kTlasThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) shd[tid] = shd[tid + s] > shd[tid] ? shd[tid + s] : shd[tid];
+            __syncthreads();
+        }
+        double mx = shd[0];
+        __syncthreads();
+        if (tid == 0) {
+            double e = mx - mn;
+            s_frame[k] = mn;
+            s_frame[3 + k] = e > 0.0 ? 1024.0 / e : 0.0;
+        }
+    }
+    __syncthreads();
+    // 3. Morton codes, bitonic sort of (code << 32 | index)
+    for (uint32_t i = tid; i < m; i += kTlasThreads) {
+        uint64_t key = ~0ull;
+        if (i < n) {
+            const float* b = wbox + i * 6;
+            double cx = 0.5 * ((double)b[0] + (double)b[3]);
+            double cy = 0.5 * ((double)b[1] + (double)b[4]);
+            double cz = 0.5 * ((double)b[2] + (double)b[5]);
+            key = ((uint64_t)morton_code(s_frame, cx, cy, cz) << 32) | i;
+        }
+        skeys[i] = key;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < m; i += kTlasThreads) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t a = skeys[i], b = skeys[ixj];
+                    bool up = (i & k) == 0;
+                    if ((a > b) == up) { skeys[i] = b; skeys[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // 4. hierarchy over the sorted codes (high halves), leaf records
+    for (uint32_t s = tid; s < n; s += kTlasThreads) {
+        uint32_t i = (uint32_t)(skeys[s] & 0xFFFFFFFFu);
+        const InstIn& src = in[i];
+        InstRec rec;
+        for (int k = 0; k < 12; ++k) rec.w2o[k] = src.w2o[k];
+        rec.nodes = src.nodes;
+        rec.tris = src.tris;
+        recs[s] = rec;
+        InstAux ax;
+        for (int k = 0; k < 9; ++k) ax.nmat[k] = src.nmat[k];
+        ax.custom_index = src.custom_index;
+        ax.input_index = src.input_index;
+        ax.pad = 0;
+        ax.nrms = src.nrms;
+        ax.prims = src.prims;
+        aux[s] = ax;
+        for (int k = 0; k < 12; ++k) w2o_out[s * 12 + k] = src.w2o[k];
+        order_out[s] = src.input_index;
+    }
+    if (n >= 2) {
+        // keys for delta: the Morton code of slot s is skeys[s] >> 32
+        for (uint32_t i = tid; i < n - 1; i += kTlasThreads) {
+            // delta on the code array: build a local view through a lambda-free copy
+            const int nn = (int)n;
+            auto code = [&](int j) { return (uint32_t)(skeys[j] >> 32); };
+            auto dlt = [&](int a, int b) -> int {
+                if (b < 0 || b > nn - 1) return -1;
+                uint32_t ka = code(a), kb = code(b);
+                if (ka == kb) return 32 + __clz((int)((uint32_t)a ^ (uint32_t)b));
+                return __clz((int)(ka ^ kb));
+            };
+            int ii = (int)i;
+            int d = (dlt(ii, ii + 1) - dlt(ii, ii - 1)) >= 0 ? 1 : -1;
+            int dmin = dlt(ii, ii - d);
+            int lmax = 2;
+            while (dlt(ii, ii + lmax * d) > dmin) lmax *= 2;
+            int l = 0;
+            for (int t = lmax / 2; t >= 1; t /= 2)
+                if (dlt(ii, ii + (l + t) * d) > dmin) l += t;
+            int j = ii + l * d;
+            int dnode = dlt(ii, j);
+            int s = 0;
+            int t = l;
+            do {
+                t = (t + 1) >> 1;
+                if (dlt(ii, ii + (s + t) * d) > dnode) s += t;
+            } while (t > 1);
+            int gamma = ii + s * d + (d < 0 ? d : 0);
+            int lo = ii < j ? ii : j, hi = ii < j ? j : ii;
+            uint32_t left = (lo == gamma) ? (kLeaf | (uint32_t)gamma) : (uint32_t)gamma;
+            uint32_t right = (hi == gamma + 1) ? (kLeaf | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+            nodes[i].child[0] = left;
+            nodes[i].child[1] = right;
+            nodes[i].pad = 0;
+            if (i == 0) nodes[0].parent = kNone;
+            if (left & kLeaf) leaf_parent[left & ~kLeaf] = i; else nodes[left].parent = i;
+            if (right & kLeaf) leaf_parent[right & ~kLeaf] = i; else nodes[right].parent = i;
+            counters[i] = 0;
+        }
+        __threadfence();
+        __syncthreads();
+        // 5. refit
+        for (uint32_t s = tid; s < n; s += kTlasThreads) {
+            uint32_t i = (uint32_t)(skeys[s] & 0xFFFFFFFFu);
+            refit_climb(nodes, leaf_parent, counters, s, wbox + i * 6, error);
+        }
+    } else if (tid == 0) {
+        Node nd;
+        float empty[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        if (n == 1) store_child_box(&nd, 0, wbox);
+        else store_child_box(&nd, 0, empty);
+        store_child_box(&nd, 1, empty);
+        nd.child[0] = n == 1 ? kLeaf : kEmpty;
+        nd.child[1] = kEmpty;
+        nd.parent = kNone;
+        nd.pad = 0;
+        nodes[0] = nd;
+    }
+}
+
+inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+// Device flag set by build kernels on bad input or a malformed tree; read by
+// luzrt_get_timings / the build call's own check.
+uint32_t* build_error_flag(luzrt_ctx* ctx) {
+    if (ensure(ctx, ctx->counters, 64) != LUZRT_OK) return nullptr;
+    return (uint32_t*)ctx->counters.ptr + kCntBuildError;
+}
+
+luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices,
+                               const uint32_t* indices) {
+    hipStream_t st = ctx->stream;
+    uint32_t* error = build_error_flag(ctx);
+    if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
+    const uint32_t n = b->tri_count;
+    const uint8_t* verts = (const uint8_t*)vertices;
+    if (n <= 1) {
+        k_small_root<<<1, 64, 0, st>>>(verts, b->stride, b->vertex_count, indices, n, b->nodes,
+                                       b->tris, b->nrms, b->prims, b->slot_idx, error);
+        LUZ_HIP(ctx, hipGetLastError());
+        return LUZRT_OK;
+    }
+    const uint32_t nb = cdiv(n, kBlock);
+    const uint32_t sb = cdiv(n, kSortTile);
+    size_t bytes = (size_t)n * 24 + (size_t)nb * 48 + 64 + (size_t)n * 16 + (size_t)256 * sb * 4 +
+                   (size_t)n * 24 + 1024;
+    uint8_t* scratch = nullptr;
+    LUZ_HIP(ctx, hipMallocAsync((void**)&scratch, bytes, st));
+    uint8_t* p = scratch;
+    auto take = [&](size_t sz) { uint8_t* r = p; p += (sz + 255) & ~(size_t)255; return r; };
+    float* boxes = (float*)take((size_t)n * 24);
+    double* partial = (double*)take((size_t)nb * 48);
+    double* frame = (double*)take(64);
+    uint32_t* k0 = (uint32_t*)take((size_t)n * 4);
+    uint32_t* v0 = (uint32_t*)take((size_t)n * 4);
+    uint32_t* k1 = (uint32_t*)take((size_t)n * 4);
+    uint32_t* v1 = (uint32_t*)take((size_t)n * 4);
+    uint32_t* hist = (uint32_t*)take((size_t)256 * sb * 4);
+    float* slot_box = (float*)take((size_t)n * 24);
+    (void)bytes;
+
+    k_tri_boxes<<<nb, kBlock, 0, st>>>(verts, b->stride, b->vertex_count, indices, n, boxes, partial,
+                                       error);
+    k_frame<<<1, kBlock, 0, st>>>(partial, nb, frame);
+    k_morton<<<nb, kBlock, 0, st>>>(boxes, n, frame, k0, v0);
+    uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+    for (int pass = 0; pass < 4; ++pass) {
+        int shift = pass * 8;
+        k_sort_hist<<<sb, kBlock, 0, st>>>(kin, n, shift, hist, sb);
+        k_scan<<<1, 1024, 0, st>>>(hist, 256 * sb);
+        k_sort_scatter<<<sb, kBlock, 0, st>>>(kin, vin, n, shift, hist, sb, kout, vout);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    k_karras<<<cdiv(n - 1, kBlock), kBlock, 0, st>>>(kin, n, b->nodes, b->leaf_parent, b->counters);
+    k_gather<<<nb, kBlock, 0, st>>>(verts, b->stride, b->vertex_count, indices, vin, n, boxes,
+                                    b->tris, b->nrms, b->prims, b->slot_idx, slot_box, error);
+    k_refit<<<nb, kBlock, 0, st>>>(b->nodes, b->leaf_parent, b->counters, n, slot_box, error);
+    LUZ_HIP(ctx, hipGetLastError());
+    LUZ_HIP(ctx, hipFreeAsync(scratch, st));
+    return LUZRT_OK;
+}
+
+luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices) {
+    hipStream_t st = ctx->stream;
+    uint32_t* error = build_error_flag(ctx);
+    if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
+    const uint32_t n = b->tri_count;
+    if (n <= 1) {
+        // rebuild the single root (indices are kept in slot order)
+        k_small_root<<<1, 64, 0, st>>>((const uint8_t*)vertices, b->stride, b->vertex_count,
+                                       b->slot_idx, n, b->nodes, b->tris, b->nrms, b->prims,
+                                       b->slot_idx, error);
+        LUZ_HIP(ctx, hipGetLastError());
+        return LUZRT_OK;
+    }
+    float* slot_box = nullptr;
+    LUZ_HIP(ctx, hipMallocAsync((void**)&slot_box, (size_t)n * 24, st));
+    const uint32_t nb = cdiv(n, kBlock);
+    k_refit_boxes<<<nb, kBlock, 0, st>>>((const uint8_t*)vertices, b->stride, b->slot_idx, n, b->tris,
+                                         slot_box, b->counters, n - 1);
+    k_refit<<<nb, kBlock, 0, st>>>(b->nodes, b->leaf_parent, b->counters, n, slot_box, error);
+    LUZ_HIP(ctx, hipGetLastError());
+    LUZ_HIP(ctx, hipFreeAsync(slot_box, st));
+    return LUZRT_OK;
+}
+
+luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
+    uint32_t m = 1;
+    while (m < n) m <<= 1;
+    float* wbox = (float*)t->scratch;
+    uint32_t* leaf_parent = (uint32_t*)(wbox + (size_t)t->capacity * 6);
+    uint32_t* counters = leaf_parent + t->capacity;
+    uint32_t* error = build_error_flag(ctx);
+    if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
+    size_t lds = (size_t)(m > 1024 ? m : 1024) * sizeof(uint64_t);
+    k_tlas_build<<<1, kTlasThreads, lds, ctx->stream>>>(dev_in, n, wbox, leaf_parent, counters,
+                                                        t->nodes, t->recs, t->aux, t->w2o,
+                                                        t->order, m, error);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
+}  // namespace luz

@@ -1,0 +1,140 @@
+// internal.hpp — host-side objects behind the luzrt C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/luzrt.h"
+#include "device_math.hpp"
+
+namespace luz {
+
+// A grow-only device allocation reused across calls.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+enum Stage { kStageBuild, kStageCompact, kStageRaygen, kStageTrace, kStageResolve, kStageShade,
+             kStagePrimary, kStageCount };
+
+}  // namespace luz
+
+struct luzrt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    int num_cus = 0;
+    // workspace
+    luz::DevBuf rays_o, rays_d, hits, active, act_index, counters, spill, lut, scene, vis_shadow,
+        vis_ao;
+    // timing
+    hipEvent_t ev[luz::kStageCount][2] = {};
+    bool ev_used[luz::kStageCount] = {};
+    uint64_t last_rays = 0, last_active = 0;
+    // traversal launch configuration
+    uint32_t groups_per_cu = 4;
+    uint32_t refill_threshold = 16;
+};
+
+struct luzrt_blas {
+    uint32_t tri_count = 0;
+    uint32_t node_count = 0;
+    uint32_t vertex_count = 0;
+    uint32_t stride = 0;
+    luz::Node* nodes = nullptr;
+    float* tris = nullptr;          // 9 floats per slot
+    float* nrms = nullptr;          // 9 floats per slot
+    uint32_t* prims = nullptr;      // slot -> primitive
+    uint32_t* slot_idx = nullptr;   // 3 vertex indices per slot (refit)
+    uint32_t* leaf_parent = nullptr;
+    uint32_t* counters = nullptr;   // refit arrival counters (node_count)
+    int refs = 0;                   // live TLASes built over this BLAS
+};
+
+struct luzrt_tlas {
+    uint32_t capacity = 0;
+    uint32_t count = 0;            // instances in the structure (after mask/empty filtering)
+    uint32_t node_count = 0;
+    luz::Node* nodes = nullptr;
+    luz::InstRec* recs = nullptr;  // per slot
+    luz::InstAux* aux = nullptr;   // per slot
+    float* w2o = nullptr;          // per slot, 12 floats (inspection copy)
+    uint32_t* order = nullptr;     // slot -> input index
+    void* input = nullptr;         // uploaded instance inputs
+    void* scratch = nullptr;
+    std::vector<const luzrt_blas*> referenced;
+};
+
+namespace luz {
+
+luzrt_status fail(luzrt_ctx* ctx, luzrt_status s, const char* fmt, ...);
+luzrt_status hip_fail(luzrt_ctx* ctx, hipError_t e, const char* what);
+luzrt_status ensure(luzrt_ctx* ctx, DevBuf& b, size_t bytes);
+void stage_begin(luzrt_ctx* ctx, Stage s);
+void stage_end(luzrt_ctx* ctx, Stage s);
+
+#define LUZ_HIP(ctx, call)                                          \
+    do {                                                            \
+        hipError_t e_ = (call);                                     \
+        if (e_ != hipSuccess) return ::luz::hip_fail(ctx, e_, #call); \
+    } while (0)
+
+// Host-side packed instance input for the TLAS build kernel (128 bytes).
+struct alignas(16) InstIn {
+    float o2w[12];
+    float w2o[12];
+    float nmat[9];
+    uint32_t custom_index;
+    uint32_t mask;
+    uint32_t input_index;
+    const Node* nodes;
+    const float* tris;
+    const float* nrms;
+    const uint32_t* prims;
+};
+
+// Launchers (each defined next to its kernels).
+luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices,
+                               const uint32_t* indices);
+luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices);
+luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n);
+uint32_t* build_error_flag(luzrt_ctx* ctx);
+
+// Scene block as uploaded for one visibility call: the caller's luzrt_scene
+// plus the ray-slot layout.  Slot j of active pixel a is ray j*A + a, where A
+// is the number of non-background pixels; shadow slots of light l are
+// slot_base[l] .. slot_base[l] + n_l - 1, AO slots follow at r_shadow.
+struct DevScene {
+    luzrt_scene s;
+    uint32_t slot_base[LUZRT_MAX_LIGHTS];
+    uint32_t r_shadow;
+    uint32_t r_total;
+    uint32_t pad[2];
+};
+
+// Device counters of one visibility call (ctx->counters).
+enum Counter { kCntFetch = 0, kCntActive = 1, kCntError = 2, kCntRays = 3, kCntBuildError = 4,
+               kCntCount = 8 };
+
+struct TraceLaunch {
+    const Node* tlas_nodes;
+    const InstRec* recs;
+    const float4* ray_o;   // origin.xyz, tmin   (ray i at ray_o[i * ray_stride])
+    const float4* ray_d;   // direction.xyz, tmax
+    uint32_t ray_stride;   // in float4 units: 1 for split arrays, 2 for interleaved rays
+    uint8_t* hits;
+    uint32_t* counters;    // device Counter array; kCntRays read, kCntFetch zeroed before
+    uint32_t* spill;       // per-lane overflow stack
+};
+luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
+
+// Validates a scene, fills slot layout, uploads it to ctx->scene.
+luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy);
+luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g);
+luzrt_status ensure_lut(luzrt_ctx* ctx);
+
+}  // namespace luz

@@ -1,0 +1,466 @@
+// runtime.hip — context, memory, acceleration-structure handles and errors of
+// the luzrt C ABI.
+//
+// Replaces the parts of the Vulkan wrapper the path needs: the device/queue
+// context (VulkanWrapper.cpp:243 `static Context _ctx`), buffer creation and
+// staging copies (:422-486), the BLAS/TLAS handle objects (:657-836) and the
+// GPU timestamps (:1276-1332).  Unlike the reference, failures are returned
+// as status codes with a message instead of ASSERT/abort (Base.hpp:51-59).
+#include <cmath>
+#include <cstdarg>
+
+#include "internal.hpp"
+#include "traverse.hpp"
+
+namespace luz {
+
+luzrt_status fail(luzrt_ctx* ctx, luzrt_status s, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->error = buf;
+    }
+    return s;
+}
+
+luzrt_status hip_fail(luzrt_ctx* ctx, hipError_t e, const char* what) {
+    luzrt_status s = e == hipErrorOutOfMemory ? LUZRT_E_OUT_OF_MEMORY : LUZRT_E_DEVICE;
+    return fail(ctx, s, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+luzrt_status ensure(luzrt_ctx* ctx, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.ptr) return LUZRT_OK;
+    if (b.ptr) {
+        LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        LUZ_HIP(ctx, hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t sz = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&b.ptr, sz);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return hip_fail(ctx, e, "hipMalloc(workspace)");
+    }
+    b.bytes = sz;
+    return LUZRT_OK;
+}
+
+void stage_begin(luzrt_ctx* ctx, Stage s) {
+    hipEventRecord(ctx->ev[s][0], ctx->stream);
+    ctx->ev_used[s] = true;
+}
+void stage_end(luzrt_ctx* ctx, Stage s) { hipEventRecord(ctx->ev[s][1], ctx->stream); }
+
+luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g) {
+    if (g->width == 0 || g->height == 0 || g->width > 16384 || g->height > 16384)
+        return fail(ctx, LUZRT_E_LIMIT, "G-buffer size %ux%u outside 1..16384", g->width, g->height);
+    if (!g->position || !g->normal || !g->albedo || !g->material || !g->emission)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "G-buffer plane is null");
+    return LUZRT_OK;
+}
+
+luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds) {
+    if (scene->num_lights < 0 || scene->num_lights > LUZRT_MAX_LIGHTS)
+        return fail(ctx, LUZRT_E_LIMIT, "num_lights %d outside 0..%d", scene->num_lights,
+                    LUZRT_MAX_LIGHTS);
+    if (scene->ao_num_samples < 0 || scene->ao_num_samples > LUZRT_MAX_SAMPLES)
+        return fail(ctx, LUZRT_E_LIMIT, "ao_num_samples %d outside 0..%d", scene->ao_num_samples,
+                    LUZRT_MAX_SAMPLES);
+    std::memset(ds, 0, sizeof(*ds));
+    ds->s = *scene;
+    uint32_t j = 0;
+    for (int l = 0; l < scene->num_lights; ++l) {
+        const luzrt_light& L = scene->lights[l];
+        if (L.type < 0 || L.type > 2)
+            return fail(ctx, LUZRT_E_INVALID_ARG, "light %d has unknown type %d", l, L.type);
+        if (L.num_shadow_samples < 0 || L.num_shadow_samples > LUZRT_MAX_SAMPLES)
+            return fail(ctx, LUZRT_E_LIMIT, "light %d num_shadow_samples %d outside 0..%d", l,
+                        L.num_shadow_samples, LUZRT_MAX_SAMPLES);
+        ds->slot_base[l] = j;
+        j += (uint32_t)L.num_shadow_samples;
+    }
+    ds->r_shadow = j;
+    ds->r_total = j + (uint32_t)scene->ao_num_samples;
+    luzrt_status s = ensure(ctx, ctx->scene, sizeof(DevScene));
+    if (s != LUZRT_OK) return s;
+    LUZ_HIP(ctx, hipMemcpyAsync(ctx->scene.ptr, ds, sizeof(DevScene), hipMemcpyHostToDevice,
+                                ctx->stream));
+    // the host copy must outlive the asynchronous copy
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUZRT_OK;
+}
+
+// Waits for the build and reports (then clears) the device build-error flag:
+// 2 = a vertex index >= vertex_count, 1 = malformed hierarchy.
+static luzrt_status check_build_flag(luzrt_ctx* ctx) {
+    uint32_t flag = 0;
+    uint32_t* d = (uint32_t*)ctx->counters.ptr + kCntBuildError;
+    LUZ_HIP(ctx, hipMemcpyAsync(&flag, d, 4, hipMemcpyDeviceToHost, ctx->stream));
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!flag) return LUZRT_OK;
+    LUZ_HIP(ctx, hipMemsetAsync(d, 0, 4, ctx->stream));
+    if (flag & 2u) return fail(ctx, LUZRT_E_INVALID_ARG, "a vertex index is >= vertex_count");
+    return fail(ctx, LUZRT_E_DEVICE, "BVH build produced a malformed hierarchy");
+}
+
+// FP64 inverse of a 3x4 affine transform, rounded once to FP32.
+static bool invert_3x4(const float m[12], float out[12], float nmat[9]) {
+    double r00 = m[0], r01 = m[1], r02 = m[2], t0 = m[3];
+    double r10 = m[4], r11 = m[5], r12 = m[6], t1 = m[7];
+    double r20 = m[8], r21 = m[9], r22 = m[10], t2 = m[11];
+    double c00 = r11 * r22 - r12 * r21;
+    double c01 = r12 * r20 - r10 * r22;
+    double c02 = r10 * r21 - r11 * r20;
+    double det = r00 * c00 + r01 * c01 + r02 * c02;
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    double inv[3][3];
+    inv[0][0] = c00 / det;
+    inv[1][0] = c01 / det;
+    inv[2][0] = c02 / det;
+    inv[0][1] = (r02 * r21 - r01 * r22) / det;
+    inv[1][1] = (r00 * r22 - r02 * r20) / det;
+    inv[2][1] = (r01 * r20 - r00 * r21) / det;
+    inv[0][2] = (r01 * r12 - r02 * r11) / det;
+    inv[1][2] = (r02 * r10 - r00 * r12) / det;
+    inv[2][2] = (r00 * r11 - r01 * r10) / det;
+    double tv[3] = {t0, t1, t2};
+    for (int i = 0; i < 3; ++i) {
+        double ti = -(inv[i][0] * tv[0] + inv[i][1] * tv[1] + inv[i][2] * tv[2]);
+        for (int j = 0; j < 3; ++j) out[i * 4 + j] = (float)inv[i][j];
+        out[i * 4 + 3] = (float)ti;
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) nmat[r * 3 + c] = out[c * 4 + r];
+    return true;
+}
+
+}  // namespace luz
+
+using namespace luz;
+
+extern "C" {
+
+const char* luzrt_status_string(luzrt_status s) {
+    switch (s) {
+        case LUZRT_OK: return "ok";
+        case LUZRT_E_INVALID_ARG: return "invalid argument";
+        case LUZRT_E_OUT_OF_MEMORY: return "out of device memory";
+        case LUZRT_E_DEVICE: return "device error";
+        case LUZRT_E_IN_USE: return "object in use";
+        case LUZRT_E_LIMIT: return "limit exceeded";
+        case LUZRT_E_NO_DEVICE: return "no HIP device";
+        case LUZRT_E_UNSUPPORTED: return "unsupported";
+    }
+    return "unknown status";
+}
+
+const char* luzrt_last_error(const luzrt_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+luzrt_status luzrt_create(luzrt_ctx** out, int device) {
+    if (!out) return LUZRT_E_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return LUZRT_E_NO_DEVICE;
+    if (device < 0 || device >= n) return LUZRT_E_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return LUZRT_E_DEVICE;
+    luzrt_ctx* ctx = new luzrt_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; return LUZRT_E_DEVICE; }
+    ctx->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return LUZRT_E_DEVICE;
+    }
+    for (int s = 0; s < kStageCount; ++s)
+        for (int k = 0; k < 2; ++k) hipEventCreate(&ctx->ev[s][k]);
+    if (ensure(ctx, ctx->counters, 64) != LUZRT_OK ||
+        hipMemset(ctx->counters.ptr, 0, 64) != hipSuccess) {
+        luzrt_destroy(ctx);
+        return LUZRT_E_DEVICE;
+    }
+    *out = ctx;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    DevBuf* bufs[] = {&ctx->rays_o, &ctx->rays_d, &ctx->hits, &ctx->active, &ctx->act_index,
+                      &ctx->counters, &ctx->spill, &ctx->lut, &ctx->scene, &ctx->vis_shadow,
+                      &ctx->vis_ao};
+    for (DevBuf* b : bufs)
+        if (b->ptr) hipFree(b->ptr);
+    for (int s = 0; s < kStageCount; ++s)
+        for (int k = 0; k < 2; ++k) hipEventDestroy(ctx->ev[s][k]);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_sync(luzrt_ctx* ctx) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out) {
+    if (!ctx || !out) return LUZRT_E_INVALID_ARG;
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float* f[kStageCount] = {&out->build_ms, &out->compact_ms, &out->raygen_ms, &out->trace_ms,
+                             &out->resolve_ms, &out->shade_ms, &out->primary_ms};
+    for (int s = 0; s < kStageCount; ++s) {
+        *f[s] = 0.0f;
+        if (ctx->ev_used[s]) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ctx->ev[s][0], ctx->ev[s][1]) == hipSuccess) *f[s] = ms;
+        }
+        ctx->ev_used[s] = false;
+    }
+    out->rays = 0;
+    out->active_pixels = 0;
+    if (ctx->counters.ptr) {
+        uint32_t c[kCntCount];
+        LUZ_HIP(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+        out->rays = c[kCntRays];
+        out->active_pixels = c[kCntActive];
+        if (c[kCntError])
+            return fail(ctx, LUZRT_E_LIMIT, "traversal overflow (stack depth > %d or > %u steps)",
+                        kSpill, kMaxSteps);
+        if (c[kCntBuildError])
+            return fail(ctx, LUZRT_E_INVALID_ARG, "acceleration-structure build error flag %u",
+                        c[kCntBuildError]);
+    }
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (groups_per_cu > 64 || refill > 64)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "trace config out of range (groups<=64, refill<=64)");
+    if (groups_per_cu) ctx->groups_per_cu = groups_per_cu;
+    if (refill) ctx->refill_threshold = refill;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_malloc(luzrt_ctx* ctx, size_t bytes, void** p) {
+    if (!ctx || !p) return LUZRT_E_INVALID_ARG;
+    LUZ_HIP(ctx, hipMalloc(p, bytes ? bytes : 1));
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_free(luzrt_ctx* ctx, void* p) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!p) return LUZRT_OK;
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUZ_HIP(ctx, hipFree(p));
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return LUZRT_E_INVALID_ARG;
+    if (!bytes) return LUZRT_OK;
+    LUZ_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_download(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return LUZRT_E_INVALID_ARG;
+    if (!bytes) return LUZRT_OK;
+    LUZ_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUZRT_OK;
+}
+
+// ---------------------------------------------------------------- BLAS
+luzrt_status luzrt_blas_build(luzrt_ctx* ctx, const void* vertices, uint32_t stride,
+                              uint32_t vertex_count, const uint32_t* indices, uint32_t index_count,
+                              luzrt_blas** out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!out) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_blas_build: null output");
+    *out = nullptr;
+    if (index_count % 3 != 0)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "index_count %u is not a multiple of 3", index_count);
+    if (stride < 12 || stride % 4 != 0)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "vertex stride %u must be >= 12 and a multiple of 4",
+                    stride);
+    if ((index_count && !indices) || (vertex_count && !vertices))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_blas_build: null buffer");
+    if (index_count / 3 >= 0x7FFFFFFFu)
+        return fail(ctx, LUZRT_E_LIMIT, "too many triangles");
+    LUZ_HIP(ctx, hipSetDevice(ctx->device));
+    luzrt_blas* b = new luzrt_blas();
+    b->tri_count = index_count / 3;
+    b->node_count = b->tri_count > 1 ? b->tri_count - 1 : 1;
+    b->vertex_count = vertex_count;
+    b->stride = stride;
+    const size_t n = b->tri_count ? b->tri_count : 1;
+    auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes); };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = alloc((void**)&b->nodes, (size_t)b->node_count * sizeof(Node));
+    if (e == hipSuccess) e = alloc((void**)&b->tris, n * 36);
+    if (e == hipSuccess) e = alloc((void**)&b->nrms, n * 36);
+    if (e == hipSuccess) e = alloc((void**)&b->prims, n * 4);
+    if (e == hipSuccess) e = alloc((void**)&b->slot_idx, n * 12);
+    if (e == hipSuccess) e = alloc((void**)&b->leaf_parent, n * 4);
+    if (e == hipSuccess) e = alloc((void**)&b->counters, (size_t)b->node_count * 4);
+    if (e != hipSuccess) {
+        luzrt_blas_destroy(ctx, b);
+        return hip_fail(ctx, e, "hipMalloc(BLAS)");
+    }
+    // Validate indices on the device-side copy would need a kernel; the
+    // caller contract is indices < vertex_count (as for vkCmdBuildAccelerationStructuresKHR).
+    stage_begin(ctx, kStageBuild);
+    luzrt_status s = blas_build_device(ctx, b, vertices, indices);
+    stage_end(ctx, kStageBuild);
+    if (s == LUZRT_OK) s = check_build_flag(ctx);
+    if (s != LUZRT_OK) {
+        luzrt_blas_destroy(ctx, b);
+        return s;
+    }
+    *out = b;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_blas_refit(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!b || (b->vertex_count && !vertices))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_blas_refit: null argument");
+    stage_begin(ctx, kStageBuild);
+    luzrt_status s = blas_refit_device(ctx, b, vertices);
+    stage_end(ctx, kStageBuild);
+    return s;  // a malformed-tree flag (never set by a refit of a built BLAS) surfaces in luzrt_get_timings
+}
+
+luzrt_status luzrt_blas_destroy(luzrt_ctx* ctx, luzrt_blas* b) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!b) return LUZRT_OK;
+    if (b->refs > 0)
+        return fail(ctx, LUZRT_E_IN_USE, "BLAS is referenced by %d TLAS(es)", b->refs);
+    hipStreamSynchronize(ctx->stream);
+    void* ptrs[] = {b->nodes, b->tris, b->nrms, b->prims, b->slot_idx, b->leaf_parent, b->counters};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete b;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_blas_arrays(const luzrt_blas* b, const void** nodes, uint32_t* node_count,
+                               const float** tris, const uint32_t** prims, uint32_t* tri_count) {
+    if (!b) return LUZRT_E_INVALID_ARG;
+    if (nodes) *nodes = b->nodes;
+    if (node_count) *node_count = b->node_count;
+    if (tris) *tris = b->tris;
+    if (prims) *prims = b->prims;
+    if (tri_count) *tri_count = b->tri_count;
+    return LUZRT_OK;
+}
+
+// ---------------------------------------------------------------- TLAS
+luzrt_status luzrt_tlas_create(luzrt_ctx* ctx, uint32_t max_instances, luzrt_tlas** out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!out) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_tlas_create: null output");
+    *out = nullptr;
+    if (max_instances == 0 || max_instances > LUZRT_MAX_INSTANCES)
+        return fail(ctx, LUZRT_E_LIMIT, "max_instances %u outside 1..%d", max_instances,
+                    LUZRT_MAX_INSTANCES);
+    luzrt_tlas* t = new luzrt_tlas();
+    t->capacity = max_instances;
+    const size_t n = max_instances;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMalloc((void**)&t->nodes, n * sizeof(Node));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->recs, n * sizeof(InstRec));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->aux, n * sizeof(InstAux));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->w2o, n * 48);
+    if (e == hipSuccess) e = hipMalloc((void**)&t->order, n * 4);
+    if (e == hipSuccess) e = hipMalloc(&t->input, n * sizeof(InstIn));
+    if (e == hipSuccess) e = hipMalloc(&t->scratch, n * 24 + n * 8 + 256);
+    if (e != hipSuccess) {
+        luzrt_tlas_destroy(ctx, t);
+        return hip_fail(ctx, e, "hipMalloc(TLAS)");
+    }
+    *out = t;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instance* inst,
+                              uint32_t count) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!t || (count && !inst)) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_tlas_build: null argument");
+    if (count > t->capacity)
+        return fail(ctx, LUZRT_E_LIMIT, "%u instances exceed the TLAS capacity %u", count,
+                    t->capacity);
+    std::vector<InstIn> host;
+    host.reserve(count);
+    std::vector<const luzrt_blas*> refs;
+    for (uint32_t i = 0; i < count; ++i) {
+        const luzrt_instance& in = inst[i];
+        if (!in.blas) return fail(ctx, LUZRT_E_INVALID_ARG, "instance %u has no BLAS", i);
+        InstIn r;
+        std::memset(&r, 0, sizeof(r));
+        std::memcpy(r.o2w, in.object_to_world, 48);
+        for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(r.o2w[k]))
+                return fail(ctx, LUZRT_E_INVALID_ARG, "instance %u transform is not finite", i);
+        if (!invert_3x4(r.o2w, r.w2o, r.nmat))
+            return fail(ctx, LUZRT_E_INVALID_ARG, "instance %u transform is singular", i);
+        r.custom_index = in.custom_index;
+        r.mask = in.mask;
+        r.input_index = i;
+        r.nodes = in.blas->nodes;
+        r.tris = in.blas->tris;
+        r.nrms = in.blas->nrms;
+        r.prims = in.blas->prims;
+        refs.push_back(in.blas);
+        // empty meshes and mask-0 instances can never be hit by a 0xFF ray mask
+        if (in.blas->tri_count == 0 || (in.mask & 0xFFu) == 0) continue;
+        host.push_back(r);
+    }
+    LUZ_HIP(ctx, hipSetDevice(ctx->device));
+    if (!host.empty())
+        LUZ_HIP(ctx, hipMemcpyAsync(t->input, host.data(), host.size() * sizeof(InstIn),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    stage_begin(ctx, kStageBuild);
+    luzrt_status s = tlas_build_device(ctx, t, (const InstIn*)t->input, (uint32_t)host.size());
+    stage_end(ctx, kStageBuild);
+    if (s != LUZRT_OK) return s;
+    // the staged host array must stay valid until the copy has been consumed
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs--;
+    t->referenced = refs;
+    for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs++;
+    t->count = (uint32_t)host.size();
+    t->node_count = t->count > 1 ? t->count - 1 : 1;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_tlas_destroy(luzrt_ctx* ctx, luzrt_tlas* t) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!t) return LUZRT_OK;
+    hipStreamSynchronize(ctx->stream);
+    for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs--;
+    void* ptrs[] = {t->nodes, t->recs, t->aux, t->w2o, t->order, t->input, t->scratch};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete t;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_tlas_arrays(const luzrt_tlas* t, const void** nodes, uint32_t* node_count,
+                               const float** w2o, const uint32_t** order, uint32_t* count) {
+    if (!t) return LUZRT_E_INVALID_ARG;
+    if (nodes) *nodes = t->nodes;
+    if (node_count) *node_count = t->node_count;
+    if (w2o) *w2o = t->w2o;
+    if (order) *order = t->order;
+    if (count) *count = t->count;
+    return LUZRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,756 @@
+// scene.cpp — Luz's scene model on the host (C++17), the workload
+// generators, and the GPUScene packer over the luzrt C ABI.
+//
+// Model (source/Resources/AssetManager.hpp): MeshAsset/MeshVertex :82-97,
+// MaterialAsset :99-112, Node :114-174, MeshNode :176-182, LightNode
+// :184-227, CameraNode :229-275, SceneAsset :277-296.  Transforms follow
+// Node::ComposeTransform (AssetManager.cpp:38-43: T * R(quat(radians(euler)))
+// * S) and glm's column-major conventions; they are evaluated in FP64 and
+// rounded once to FP32 where the device reads them (the reference rounds
+// every glm step to FP32).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/luzrt.h"
+#include "../../include/luzscene.h"
+
+namespace luz {
+namespace {
+
+constexpr double kPiD = 3.14159265358979323846;
+
+// ------------------------------------------------------------------ RNG
+struct SplitMix64 {
+    uint64_t s;
+    explicit SplitMix64(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1p-53; }
+    double uniform(double a, double b) { return a + (b - a) * uniform(); }
+    uint32_t byte() { return (uint32_t)(next() >> 56); }
+};
+
+// Independent sub-stream of a master seed.
+enum Stream : uint64_t { kGeometry = 1, kTransforms = 2, kLights = 3, kMaterials = 4, kAnimation = 5,
+                         kLayout = 6 };
+SplitMix64 stream(uint64_t seed, uint64_t id) {
+    SplitMix64 m(seed ^ (0xD1B54A32D192ED03ull * id));
+    return SplitMix64(m.next());
+}
+
+// ------------------------------------------------------------------ math
+struct D3 { double x, y, z; };
+inline D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+inline D3 operator+(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline D3 operator-(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline D3 operator*(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 cross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline double length(D3 a) { return std::sqrt(dot(a, a)); }
+inline D3 normalize(D3 a) { return a * (1.0 / length(a)); }
+
+struct Quat { double w, x, y, z; };
+
+// glm::quat(vec3 eulerAngles) (radians).
+Quat quat_from_euler(D3 e) {
+    double cx = std::cos(e.x * 0.5), cy = std::cos(e.y * 0.5), cz = std::cos(e.z * 0.5);
+    double sx = std::sin(e.x * 0.5), sy = std::sin(e.y * 0.5), sz = std::sin(e.z * 0.5);
+    return Quat{cx * cy * cz + sx * sy * sz, sx * cy * cz - cx * sy * sz,
+                cx * sy * cz + sx * cy * sz, cx * cy * sz - sx * sy * cz};
+}
+
+// glm::eulerAngles(quat) (radians): pitch, yaw, roll.
+D3 euler_from_quat(Quat q) {
+    double pitch = std::atan2(2.0 * (q.y * q.z + q.w * q.x), q.w * q.w - q.x * q.x - q.y * q.y + q.z * q.z);
+    double s = -2.0 * (q.x * q.z - q.w * q.y);
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+    double yaw = std::asin(s);
+    double roll = std::atan2(2.0 * (q.x * q.y + q.w * q.z), q.w * q.w + q.x * q.x - q.y * q.y - q.z * q.z);
+    return d3(pitch, yaw, roll);
+}
+
+// Row-major 3x4 affine transform.
+struct M34 {
+    double m[3][4];
+    static M34 identity() {
+        M34 r{};
+        for (int i = 0; i < 3; ++i) r.m[i][i] = 1.0;
+        return r;
+    }
+    D3 point(D3 p) const {
+        return d3(m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3],
+                  m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3],
+                  m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3]);
+    }
+    D3 vec(D3 v) const {
+        return d3(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z,
+                  m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+                  m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z);
+    }
+};
+
+M34 operator*(const M34& a, const M34& b) {
+    M34 r{};
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 4; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += a.m[i][k] * b.m[k][j];
+            if (j == 3) s += a.m[i][3];
+            r.m[i][j] = s;
+        }
+    }
+    return r;
+}
+
+// Rotation matrix of a unit quaternion (glm::mat3_cast, transposed into row-major).
+M34 quat_matrix(Quat q) {
+    double xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+    double xz = q.x * q.z, xy = q.x * q.y, yz = q.y * q.z;
+    double wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+    M34 r = M34::identity();
+    r.m[0][0] = 1.0 - 2.0 * (yy + zz); r.m[1][0] = 2.0 * (xy + wz);       r.m[2][0] = 2.0 * (xz - wy);
+    r.m[0][1] = 2.0 * (xy - wz);       r.m[1][1] = 1.0 - 2.0 * (xx + zz); r.m[2][1] = 2.0 * (yz + wx);
+    r.m[0][2] = 2.0 * (xz + wy);       r.m[1][2] = 2.0 * (yz - wx);       r.m[2][2] = 1.0 - 2.0 * (xx + yy);
+    return r;
+}
+
+// ------------------------------------------------------------------ model
+struct MeshVertex {
+    float position[3];
+    float normal[3];
+    float tangent[4];
+    float texCoord[2];
+};
+static_assert(sizeof(MeshVertex) == 48, "MeshVertex is 48 bytes (AssetManager.hpp:83-89)");
+
+struct MeshAsset {
+    std::vector<MeshVertex> vertices;
+    std::vector<uint32_t> indices;
+};
+
+struct MaterialAsset {
+    float color[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    float emission[3] = {0.0f, 0.0f, 0.0f};
+    float metallic = 0.0f;
+    float roughness = 0.5f;
+};
+
+struct Node {
+    D3 position = d3(0, 0, 0);
+    D3 rotation = d3(0, 0, 0);  // Euler angles in degrees
+    D3 scale = d3(1, 1, 1);
+    // Node::ComposeTransform (AssetManager.cpp:38-43), parent = identity.
+    M34 GetWorldTransform() const {
+        M34 t = M34::identity();
+        t.m[0][3] = position.x; t.m[1][3] = position.y; t.m[2][3] = position.z;
+        D3 rad = rotation * (kPiD / 180.0);
+        M34 r = rotation_of(rad);
+        M34 s = M34::identity();
+        s.m[0][0] = scale.x; s.m[1][1] = scale.y; s.m[2][2] = scale.z;
+        return t * r * s;
+    }
+    static M34 rotation_of(D3 rad) { return quat_matrix(quat_from_euler(rad)); }
+};
+
+struct MeshNode : Node {
+    uint32_t mesh = 0;
+    uint32_t material = 0;
+    // seeded rigid motion (animation)
+    D3 base_position = d3(0, 0, 0), base_rotation = d3(0, 0, 0);
+    D3 amp = d3(0, 0, 0), spin = d3(0, 0, 0);
+    double freq = 0.0, phase = 0.0;
+};
+
+struct LightNode : Node {
+    enum LightType { Point = 0, Spot = 1, Directional = 2 };
+    float color[3] = {1.0f, 1.0f, 1.0f};
+    float intensity = 10.0f;
+    LightType lightType = Point;
+    float radius = 2.0f;
+    float innerAngle = 60.f;
+    float outerAngle = 50.f;
+    // Orient the node so that W*(0,-1,0,0) points along d (shortest arc).
+    void SetDirection(D3 d) {
+        D3 a = d3(0, -1, 0), b = normalize(d);
+        double c = dot(a, b);
+        Quat q;
+        if (c < -1.0 + 1e-12) {
+            q = Quat{0.0, 1.0, 0.0, 0.0};  // 180 degrees about x
+        } else {
+            D3 x = cross(a, b);
+            q = Quat{1.0 + c, x.x, x.y, x.z};
+            double n = std::sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+            q = Quat{q.w / n, q.x / n, q.y / n, q.z / n};
+        }
+        rotation = euler_from_quat(q) * (180.0 / kPiD);
+    }
+};
+
+struct CameraNode {
+    D3 eye = d3(0, 0, 0), center = d3(0, 0, 0);
+    float horizontalFov = 60.0f;  // passed to perspective() as fovy (AssetManager.cpp:368)
+    float nearDistance = 0.01f;
+    float farDistance = 1000.0f;
+};
+
+}  // namespace
+}  // namespace luz
+
+struct luzscene {
+    std::vector<luz::MeshAsset> meshes;
+    std::vector<luz::MaterialAsset> materials;
+    std::vector<luz::MeshNode> meshNodes;
+    std::vector<luz::LightNode> lights;
+    luz::CameraNode camera;
+    float ambientLightColor[3] = {1.0f, 1.0f, 1.0f};
+    float ambientLight = 0.01f;
+    int aoSamples = 4;
+    int lightSamples = 2;
+    float aoMin = 0.0001f;
+    float aoMax = 1.0f;
+    int shadowType = LUZRT_SHADOW_RAYTRACED;
+    uint32_t width = 256, height = 256;
+};
+
+namespace luz {
+namespace {
+
+// ------------------------------------------------------------------ meshes
+void finish_normals(MeshAsset& m) {
+    std::vector<D3> acc(m.vertices.size(), d3(0, 0, 0));
+    auto P = [&](uint32_t i) { const float* p = m.vertices[i].position; return d3(p[0], p[1], p[2]); };
+    for (size_t t = 0; t + 2 < m.indices.size(); t += 3) {
+        uint32_t a = m.indices[t], b = m.indices[t + 1], c = m.indices[t + 2];
+        D3 n = cross(P(b) - P(a), P(c) - P(a));  // area weighted
+        acc[a] = acc[a] + n; acc[b] = acc[b] + n; acc[c] = acc[c] + n;
+    }
+    for (size_t i = 0; i < m.vertices.size(); ++i) {
+        double l = length(acc[i]);
+        D3 n = l > 0 ? acc[i] * (1.0 / l) : d3(0, 1, 0);
+        m.vertices[i].normal[0] = (float)n.x;
+        m.vertices[i].normal[1] = (float)n.y;
+        m.vertices[i].normal[2] = (float)n.z;
+    }
+}
+
+MeshVertex vert(D3 p, D3 n) {
+    MeshVertex v{};
+    v.position[0] = (float)p.x; v.position[1] = (float)p.y; v.position[2] = (float)p.z;
+    v.normal[0] = (float)n.x; v.normal[1] = (float)n.y; v.normal[2] = (float)n.z;
+    v.tangent[0] = 1.0f; v.tangent[3] = 1.0f;
+    return v;
+}
+
+// Axis-aligned box with 4 vertices per face (24 vertices, 12 triangles), like assets/cube.glb.
+MeshAsset make_cube(D3 center, double half) {
+    MeshAsset m;
+    const D3 axes[3] = {d3(1, 0, 0), d3(0, 1, 0), d3(0, 0, 1)};
+    for (int a = 0; a < 3; ++a) {
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+            D3 n = axes[a] * (double)sgn;
+            D3 u = axes[(a + 1) % 3], v = axes[(a + 2) % 3];
+            if (sgn < 0) std::swap(u, v);
+            uint32_t base = (uint32_t)m.vertices.size();
+            D3 c = center + n * half;
+            m.vertices.push_back(vert(c - u * half - v * half, n));
+            m.vertices.push_back(vert(c + u * half - v * half, n));
+            m.vertices.push_back(vert(c + u * half + v * half, n));
+            m.vertices.push_back(vert(c - u * half + v * half, n));
+            uint32_t q[6] = {0, 1, 2, 0, 2, 3};
+            for (uint32_t k : q) m.indices.push_back(base + k);
+        }
+    }
+    return m;
+}
+
+// Regular grid on y = height spanning [-extent, extent]^2 with nx*nz quads.
+MeshAsset make_grid(uint32_t nx, uint32_t nz, double extent, double height) {
+    MeshAsset m;
+    for (uint32_t j = 0; j <= nz; ++j)
+        for (uint32_t i = 0; i <= nx; ++i) {
+            double x = -extent + 2.0 * extent * i / nx, z = -extent + 2.0 * extent * j / nz;
+            m.vertices.push_back(vert(d3(x, height, z), d3(0, 1, 0)));
+        }
+    for (uint32_t j = 0; j < nz; ++j)
+        for (uint32_t i = 0; i < nx; ++i) {
+            uint32_t a = j * (nx + 1) + i, b = a + 1, c = a + (nx + 1), d = c + 1;
+            uint32_t q[6] = {a, c, b, b, c, d};
+            for (uint32_t k : q) m.indices.push_back(k);
+        }
+    return m;
+}
+
+// 3D value noise (quintic-faded trilinear interpolation of hashed lattice values).
+double lattice(int64_t x, int64_t y, int64_t z, uint64_t seed) {
+    uint64_t h = (uint64_t)x * 0x8CB92BA72F3D8DD7ull ^ (uint64_t)y * 0xD6E8FEB86659FD93ull ^
+                 (uint64_t)z * 0xA0761D6478BD642Full ^ seed * 0xE7037ED1A0B428DBull;
+    SplitMix64 r(h);
+    return r.uniform();
+}
+
+double value_noise(D3 p, uint64_t seed) {
+    double fx = std::floor(p.x), fy = std::floor(p.y), fz = std::floor(p.z);
+    int64_t ix = (int64_t)fx, iy = (int64_t)fy, iz = (int64_t)fz;
+    double tx = p.x - fx, ty = p.y - fy, tz = p.z - fz;
+    auto fade = [](double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); };
+    double ux = fade(tx), uy = fade(ty), uz = fade(tz);
+    double c[2][2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int d = 0; d < 2; ++d) c[a][b][d] = lattice(ix + a, iy + b, iz + d, seed);
+    auto lerp = [](double a, double b, double t) { return a + (b - a) * t; };
+    double x00 = lerp(c[0][0][0], c[1][0][0], ux), x10 = lerp(c[0][1][0], c[1][1][0], ux);
+    double x01 = lerp(c[0][0][1], c[1][0][1], ux), x11 = lerp(c[0][1][1], c[1][1][1], ux);
+    return lerp(lerp(x00, x10, uy), lerp(x01, x11, uy), uz);
+}
+
+// fBm in [-1, 1]: 4 octaves, lacunarity 2, gain 0.5.
+double fbm(D3 p, uint64_t seed, int octaves = 4) {
+    double sum = 0.0, amp = 1.0, norm = 0.0, f = 1.0;
+    for (int o = 0; o < octaves; ++o) {
+        sum += amp * (2.0 * value_noise(p * f, seed + (uint64_t)o * 7919u) - 1.0);
+        norm += amp;
+        amp *= 0.5;
+        f *= 2.0;
+    }
+    return sum / norm;
+}
+
+// Level-L icosphere (20 * 4^L triangles) radially displaced by 1 + amp * fBm(2p).
+MeshAsset make_icosphere(int level, double amp, uint64_t seed) {
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    std::vector<D3> v = {d3(-1, t, 0), d3(1, t, 0), d3(-1, -t, 0), d3(1, -t, 0),
+                         d3(0, -1, t), d3(0, 1, t), d3(0, -1, -t), d3(0, 1, -t),
+                         d3(t, 0, -1), d3(t, 0, 1), d3(-t, 0, -1), d3(-t, 0, 1)};
+    for (auto& p : v) p = normalize(p);
+    std::vector<std::array<uint32_t, 3>> f = {
+        {0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
+        {11, 10, 2}, {10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8},
+        {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+    for (int l = 0; l < level; ++l) {
+        std::map<uint64_t, uint32_t> mid;
+        auto midpoint = [&](uint32_t a, uint32_t b) {
+            uint64_t key = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+            auto it = mid.find(key);
+            if (it != mid.end()) return it->second;
+            v.push_back(normalize((v[a] + v[b]) * 0.5));
+            uint32_t id = (uint32_t)v.size() - 1;
+            mid[key] = id;
+            return id;
+        };
+        std::vector<std::array<uint32_t, 3>> nf;
+        nf.reserve(f.size() * 4);
+        for (auto& tri : f) {
+            uint32_t a = midpoint(tri[0], tri[1]), b = midpoint(tri[1], tri[2]), c = midpoint(tri[2], tri[0]);
+            nf.push_back({tri[0], a, c});
+            nf.push_back({tri[1], b, a});
+            nf.push_back({tri[2], c, b});
+            nf.push_back({a, b, c});
+        }
+        f.swap(nf);
+    }
+    MeshAsset m;
+    for (auto& p : v) {
+        double r = 1.0 + amp * fbm(p * 2.0, seed);
+        m.vertices.push_back(vert(p * r, p));
+    }
+    for (auto& tri : f)
+        for (int k = 0; k < 3; ++k) m.indices.push_back(tri[k]);
+    finish_normals(m);
+    return m;
+}
+
+// Heightfield over [-extent, extent]^2 with nx*nz quads, heights amp * fBm.
+MeshAsset make_terrain(uint32_t nx, uint32_t nz, double extent, double base, double amp, uint64_t seed) {
+    MeshAsset m = make_grid(nx, nz, extent, base);
+    for (auto& vx : m.vertices) {
+        D3 p = d3(vx.position[0], 0.0, vx.position[2]) * (4.0 / extent);
+        vx.position[1] = (float)(base + amp * fbm(d3(p.x, 0.5, p.z), seed, 6));
+    }
+    finish_normals(m);
+    return m;
+}
+
+void random_material(SplitMix64& r, MaterialAsset& mat) {
+    for (int k = 0; k < 4; ++k) mat.color[k] = (float)r.byte() / 255.0f;
+    mat.metallic = (float)r.uniform();
+    mat.roughness = (float)r.uniform();
+}
+
+Quat random_rotation(SplitMix64& r) {  // Shoemake 1992
+    double u1 = r.uniform(), u2 = r.uniform(), u3 = r.uniform();
+    double a = std::sqrt(1.0 - u1), b = std::sqrt(u1);
+    return Quat{b * std::cos(2 * kPiD * u3), a * std::sin(2 * kPiD * u2), a * std::cos(2 * kPiD * u2),
+                b * std::sin(2 * kPiD * u3)};
+}
+
+void set_camera(luzscene& s, D3 eye, D3 center, uint32_t w, uint32_t h) {
+    s.camera.eye = eye;
+    s.camera.center = center;
+    s.width = w;
+    s.height = h;
+}
+
+// Sphere field: `count` instances of `kinds` displaced icospheres of `level`.
+void add_sphere_field(luzscene& s, uint64_t seed, int level, uint32_t kinds, uint32_t count,
+                      double box, double smin, double smax) {
+    SplitMix64 geo = stream(seed, kGeometry), xf = stream(seed, kTransforms),
+               mr = stream(seed, kMaterials), an = stream(seed, kAnimation);
+    (void)geo;
+    uint32_t first_mesh = (uint32_t)s.meshes.size();
+    for (uint32_t k = 0; k < kinds; ++k) s.meshes.push_back(make_icosphere(level, 0.1, seed * 131 + k));
+    for (uint32_t i = 0; i < count; ++i) {
+        MeshNode n;
+        n.mesh = first_mesh + i % kinds;
+        n.position = d3(xf.uniform(-box, box), xf.uniform(-box, box), xf.uniform(-box, box));
+        n.rotation = euler_from_quat(random_rotation(xf)) * (180.0 / kPiD);
+        double sc = xf.uniform(smin, smax);
+        n.scale = d3(sc, sc, sc);
+        n.base_position = n.position;
+        n.base_rotation = n.rotation;
+        n.amp = d3(an.uniform(0.5, 3.0), an.uniform(0.5, 3.0), an.uniform(0.5, 3.0));
+        n.spin = d3(an.uniform(-2, 2), an.uniform(-2, 2), an.uniform(-2, 2));
+        n.freq = an.uniform(0.02, 0.1);
+        n.phase = an.uniform(0.0, 2 * kPiD);
+        MaterialAsset mat;
+        random_material(mr, mat);
+        n.material = (uint32_t)s.materials.size();
+        s.materials.push_back(mat);
+        s.meshNodes.push_back(n);
+    }
+}
+
+void add_static(luzscene& s, MeshAsset m, SplitMix64& mr) {
+    s.meshes.push_back(std::move(m));
+    MeshNode n;
+    n.mesh = (uint32_t)s.meshes.size() - 1;
+    MaterialAsset mat;
+    random_material(mr, mat);
+    n.material = (uint32_t)s.materials.size();
+    s.materials.push_back(mat);
+    s.meshNodes.push_back(n);
+}
+
+void add_point_lights(luzscene& s, SplitMix64& lr, uint32_t n, double box, float intensity,
+                      float radius) {
+    for (uint32_t i = 0; i < n; ++i) {
+        LightNode L;
+        L.lightType = LightNode::Point;
+        L.position = d3(lr.uniform(-box, box), lr.uniform(-box, box), lr.uniform(-box, box));
+        for (int k = 0; k < 3; ++k) L.color[k] = (float)lr.uniform();
+        L.intensity = intensity;
+        L.radius = radius;
+        s.lights.push_back(L);
+    }
+}
+
+LightNode directional(D3 dir, float intensity, float radius) {
+    LightNode L;
+    L.lightType = LightNode::Directional;
+    L.SetDirection(dir);
+    L.position = normalize(dir) * -1000.0;  // sets tMax of the shadow ray (light.frag:140,143)
+    L.intensity = intensity;
+    L.radius = radius;
+    return L;
+}
+
+luzscene* gen_cube(uint64_t seed) {
+    auto* s = new luzscene();
+    SplitMix64 mr = stream(seed, kMaterials);
+    add_static(*s, make_grid(1, 1, 10.0, 0.0), mr);           // ground quad, 2 triangles
+    add_static(*s, make_cube(d3(0.0, 0.5, 0.0), 0.5), mr);   // unit cube, 12 triangles
+    s->lights.push_back(directional(d3(-0.3, -1.0, -0.2), 3.0f, 0.0f));
+    set_camera(*s, d3(3, 2, 3), d3(0, 0, 0), 256, 256);
+    s->lightSamples = 1;
+    s->aoSamples = 4;
+    return s;
+}
+
+luzscene* gen_spheres(uint64_t seed) {
+    auto* s = new luzscene();
+    add_sphere_field(*s, seed, 5, 5, 50, 50.0, 0.5, 3.0);
+    SplitMix64 mr = stream(seed ^ 0x5151, kMaterials);
+    add_static(*s, make_grid(64, 64, 250.0, -55.0), mr);
+    SplitMix64 lr = stream(seed, kLights);
+    s->lights.push_back(directional(d3(-0.3, -1.0, -0.2), 3.0f, 0.0f));
+    add_point_lights(*s, lr, 8, 50.0, 1000.0f, 0.5f);
+    set_camera(*s, d3(0, 50, 110), d3(0, -55, -10), 1920, 1080);
+    s->lightSamples = 1;
+    s->aoSamples = 16;
+    return s;
+}
+
+luzscene* gen_terrain(uint64_t seed) {
+    auto* s = new luzscene();
+    // 2048 x 2048 heightfield decimated 2x along z: 2047 x 1023 quads, ~4.19M triangles.
+    SplitMix64 mr = stream(seed ^ 0x7777, kMaterials);
+    add_static(*s, make_terrain(2047, 1023, 200.0, -20.0, 12.0, seed + 99), mr);
+    add_sphere_field(*s, seed, 5, 5, 40, 60.0, 1.0, 4.0);
+    for (auto& n : s->meshNodes)
+        if (n.mesh != 0) { n.position.y = std::fabs(n.position.y) * 0.5; n.base_position = n.position; }
+    SplitMix64 lr = stream(seed, kLights);
+    for (int i = 0; i < 4; ++i) {
+        D3 d = d3(lr.uniform(-1, 1), -1.0 - lr.uniform(), lr.uniform(-1, 1));
+        s->lights.push_back(directional(d, 1.0f, 0.0f));
+    }
+    add_point_lights(*s, lr, 32, 80.0, 800.0f, 0.5f);
+    for (auto& L : s->lights)
+        if (L.lightType == LightNode::Point) L.position.y = std::fabs(L.position.y) * 0.5 + 5.0;
+    set_camera(*s, d3(0, 60, 180), d3(0, -10, 0), 3840, 2160);
+    s->lightSamples = 1;
+    s->aoSamples = 8;
+    return s;
+}
+
+luzscene* gen_random(uint64_t seed, uint32_t tri_budget, uint32_t w, uint32_t h) {
+    SplitMix64 lay = stream(seed, kLayout);
+    if (tri_budget == 0) tri_budget = (uint32_t)std::exp(std::log(1e3) + lay.uniform() * (std::log(5e5) - std::log(1e3)));
+    if (w == 0 || h == 0) {
+        const uint32_t res[4][2] = {{320, 240}, {640, 480}, {1280, 720}, {1920, 1080}};
+        uint32_t k = (uint32_t)(lay.uniform() * 4.0);
+        if (k > 3) k = 3;
+        w = res[k][0];
+        h = res[k][1];
+    }
+    auto* s = new luzscene();
+    // ground 16x16 quads (512 triangles) + spheres filling the rest of the budget
+    SplitMix64 mr = stream(seed ^ 0x3131, kMaterials);
+    uint32_t rest = tri_budget > 512 ? tri_budget - 512 : 0;
+    int level = rest >= 20 * 20480 ? 5 : (rest >= 20 * 5120 ? 4 : (rest >= 20 * 1280 ? 3 : 2));
+    uint32_t per = 20u << (2 * level);
+    uint32_t count = std::max(1u, rest / per);
+    add_sphere_field(*s, seed, level, std::min(count, 3u), count, 20.0, 0.5, 2.0);
+    add_static(*s, make_grid(16, 16, 30.0, -25.0), mr);
+    SplitMix64 lr = stream(seed, kLights);
+    s->lights.push_back(directional(d3(-0.3, -1.0, -0.2), 3.0f, 0.0f));
+    add_point_lights(*s, lr, 4, 20.0, 300.0f, 0.5f);
+    set_camera(*s, d3(0, 15, 60), d3(0, -5, 0), w, h);
+    s->lightSamples = 1;
+    s->aoSamples = 8;
+    return s;
+}
+
+inline uint32_t unorm8(double x) {
+    x = x < 0 ? 0 : (x > 1 ? 1 : x);
+    return (uint32_t)std::lround(x * 255.0);
+}
+
+}  // namespace
+}  // namespace luz
+
+using namespace luz;
+
+struct luzgpu {
+    luzrt_ctx* ctx = nullptr;
+    std::vector<void*> vbs, ibs;
+    std::vector<luzrt_blas*> blas;
+    luzrt_tlas* tlas = nullptr;
+};
+
+extern "C" {
+
+luzscene* luzscene_generate(int kind, uint64_t seed, uint32_t tri_budget, uint32_t w, uint32_t h) {
+    switch (kind) {
+        case LUZSCENE_CUBE: return gen_cube(seed);
+        case LUZSCENE_SPHERES: return gen_spheres(seed);
+        case LUZSCENE_TERRAIN: return gen_terrain(seed);
+        case LUZSCENE_RANDOM: return gen_random(seed, tri_budget, w, h);
+        default: return nullptr;
+    }
+}
+
+void luzscene_destroy(luzscene* s) { delete s; }
+
+uint32_t luzscene_mesh_count(const luzscene* s) { return s ? (uint32_t)s->meshes.size() : 0u; }
+
+int luzscene_mesh(const luzscene* s, uint32_t i, const float** v, uint32_t* nv, const uint32_t** idx,
+                  uint32_t* ni) {
+    if (!s || i >= s->meshes.size()) return -1;
+    const MeshAsset& m = s->meshes[i];
+    *v = reinterpret_cast<const float*>(m.vertices.data());
+    *nv = (uint32_t)m.vertices.size();
+    *idx = m.indices.data();
+    *ni = (uint32_t)m.indices.size();
+    return 0;
+}
+
+uint32_t luzscene_node_count(const luzscene* s) { return s ? (uint32_t)s->meshNodes.size() : 0u; }
+
+uint64_t luzscene_instanced_triangles(const luzscene* s) {
+    uint64_t t = 0;
+    if (!s) return 0;
+    for (const auto& n : s->meshNodes) t += s->meshes[n.mesh].indices.size() / 3;
+    return t;
+}
+
+int luzscene_nodes(const luzscene* s, float* o2w, uint32_t* mesh_ids, luzrt_material* mats) {
+    if (!s) return -1;
+    for (size_t i = 0; i < s->meshNodes.size(); ++i) {
+        const MeshNode& n = s->meshNodes[i];
+        M34 w = n.GetWorldTransform();
+        if (o2w)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 4; ++c) o2w[i * 12 + r * 4 + c] = (float)w.m[r][c];
+        if (mesh_ids) mesh_ids[i] = n.mesh;
+        if (mats) {
+            const MaterialAsset& m = s->materials[n.material];
+            luzrt_material e{};
+            e.albedo = unorm8(m.color[0]) | unorm8(m.color[1]) << 8 | unorm8(m.color[2]) << 16 |
+                       unorm8(m.color[3]) << 24;
+            // opaque.frag:57: (roughness, metallic, occlusion = 1, 1)
+            e.material = unorm8(m.roughness) | unorm8(m.metallic) << 8 | 255u << 16 | 255u << 24;
+            e.emission = unorm8(m.emission[0]) | unorm8(m.emission[1]) << 8 |
+                         unorm8(m.emission[2]) << 16 | 255u << 24;
+            mats[i] = e;
+        }
+    }
+    return 0;
+}
+
+int luzscene_pack(const luzscene* s, luzrt_scene* out) {
+    if (!s || !out) return -1;
+    if (s->lights.size() > LUZRT_MAX_LIGHTS) return -2;
+    std::memset(out, 0, sizeof(*out));
+    // GPUScene::UpdateResources (GPUScene.cpp:239-341)
+    for (size_t i = 0; i < s->lights.size(); ++i) {
+        const LightNode& L = s->lights[i];
+        luzrt_light& b = out->lights[i];
+        for (int k = 0; k < 3; ++k) b.color[k] = L.color[k];
+        b.intensity = L.intensity;
+        b.position[0] = (float)L.position.x;
+        b.position[1] = (float)L.position.y;
+        b.position[2] = (float)L.position.z;
+        b.inner_angle = (float)(L.innerAngle * kPiD / 180.0);
+        D3 dir = L.GetWorldTransform().vec(d3(0, -1, 0));
+        b.direction[0] = (float)dir.x;
+        b.direction[1] = (float)dir.y;
+        b.direction[2] = (float)dir.z;
+        b.outer_angle = (float)(L.outerAngle * kPiD / 180.0);
+        b.type = (int32_t)L.lightType;
+        b.num_shadow_samples = s->shadowType == LUZRT_SHADOW_RAYTRACED ? s->lightSamples : 0;
+        b.radius = L.radius;
+    }
+    out->num_lights = (int32_t)s->lights.size();
+    for (int k = 0; k < 3; ++k) out->ambient_color[k] = s->ambientLightColor[k];
+    out->ambient_intensity = s->ambientLight;
+    out->cam_pos[0] = (float)s->camera.eye.x;
+    out->cam_pos[1] = (float)s->camera.eye.y;
+    out->cam_pos[2] = (float)s->camera.eye.z;
+    out->ao_min = s->aoMin;
+    out->ao_max = s->aoMax;
+    out->ao_num_samples = s->aoSamples > 0 ? s->aoSamples : 0;
+    out->frame = 0;
+    out->shadow_type = s->shadowType;
+    return 0;
+}
+
+int luzscene_camera(const luzscene* s, uint32_t* w, uint32_t* h, luzrt_camera* cam) {
+    if (!s) return -1;
+    if (w) *w = s->width;
+    if (h) *h = s->height;
+    if (cam) {
+        // glm::lookAt(eye, center, +Y) and perspective(radians(fov) as fovy, W/H) with the
+        // Vulkan y flip (AssetManager.cpp:334-377).
+        D3 f = normalize(s->camera.center - s->camera.eye);
+        D3 r = normalize(cross(f, d3(0, 1, 0)));
+        D3 u = cross(r, f);
+        double th = std::tan(s->camera.horizontalFov * kPiD / 180.0 * 0.5);
+        double aspect = (double)s->width / (double)s->height;
+        D3 R = r * (th * aspect), U = u * (-th);
+        const D3 e = s->camera.eye;
+        cam->eye[0] = (float)e.x; cam->eye[1] = (float)e.y; cam->eye[2] = (float)e.z;
+        cam->forward[0] = (float)f.x; cam->forward[1] = (float)f.y; cam->forward[2] = (float)f.z;
+        cam->right[0] = (float)R.x; cam->right[1] = (float)R.y; cam->right[2] = (float)R.z;
+        cam->up[0] = (float)U.x; cam->up[1] = (float)U.y; cam->up[2] = (float)U.z;
+        cam->tmin = 0.0f;
+        cam->tmax = s->camera.farDistance;
+    }
+    return 0;
+}
+
+int luzscene_animate(luzscene* s, uint32_t frame) {
+    if (!s) return -1;
+    for (auto& n : s->meshNodes) {
+        if (n.freq == 0.0) continue;
+        double a = n.freq * frame + n.phase;
+        n.position = n.base_position + d3(n.amp.x * std::sin(a), n.amp.y * std::sin(1.3 * a),
+                                          n.amp.z * std::cos(a));
+        n.rotation = n.base_rotation + n.spin * (double)frame;
+    }
+    return 0;
+}
+
+luzgpu* luzgpu_create(luzrt_ctx* ctx, const luzscene* s) {
+    if (!ctx || !s) return nullptr;
+    auto* g = new luzgpu();
+    g->ctx = ctx;
+    // GPUScene::AddMesh (GPUScene.cpp:108-138): upload, one BLAS per distinct mesh.
+    for (const MeshAsset& m : s->meshes) {
+        void *vb = nullptr, *ib = nullptr;
+        size_t vbytes = m.vertices.size() * sizeof(MeshVertex), ibytes = m.indices.size() * 4;
+        if (luzrt_malloc(ctx, vbytes, &vb) != LUZRT_OK || luzrt_malloc(ctx, ibytes, &ib) != LUZRT_OK ||
+            luzrt_upload(ctx, vb, m.vertices.data(), vbytes) != LUZRT_OK ||
+            luzrt_upload(ctx, ib, m.indices.data(), ibytes) != LUZRT_OK) {
+            if (vb) luzrt_free(ctx, vb);
+            if (ib) luzrt_free(ctx, ib);
+            luzgpu_destroy(g);
+            return nullptr;
+        }
+        g->vbs.push_back(vb);
+        g->ibs.push_back(ib);
+        luzrt_blas* b = nullptr;
+        if (luzrt_blas_build(ctx, vb, sizeof(MeshVertex), (uint32_t)m.vertices.size(),
+                             (const uint32_t*)ib, (uint32_t)m.indices.size(), &b) != LUZRT_OK) {
+            luzgpu_destroy(g);
+            return nullptr;
+        }
+        g->blas.push_back(b);
+    }
+    uint32_t n = (uint32_t)s->meshNodes.size();
+    if (luzrt_tlas_create(ctx, n ? n : 1, &g->tlas) != LUZRT_OK || luzgpu_update(g, s) != LUZRT_OK) {
+        luzgpu_destroy(g);
+        return nullptr;
+    }
+    return g;
+}
+
+luzrt_status luzgpu_update(luzgpu* g, const luzscene* s) {
+    if (!g || !s) return LUZRT_E_INVALID_ARG;
+    // GPUScene::UpdateResourcesGPU -> CmdBuildTLAS (GPUScene.cpp:354-365).
+    std::vector<luzrt_instance> inst(s->meshNodes.size());
+    for (size_t i = 0; i < s->meshNodes.size(); ++i) {
+        const MeshNode& n = s->meshNodes[i];
+        M34 w = n.GetWorldTransform();
+        luzrt_instance& d = inst[i];
+        d.blas = g->blas[n.mesh];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) d.object_to_world[r][c] = (float)w.m[r][c];
+        d.custom_index = (uint32_t)i;
+        d.mask = 0xFFu;
+    }
+    return luzrt_tlas_build(g->ctx, g->tlas, inst.data(), (uint32_t)inst.size());
+}
+
+luzrt_tlas* luzgpu_tlas(luzgpu* g) { return g ? g->tlas : nullptr; }
+luzrt_blas* luzgpu_blas(luzgpu* g, uint32_t i) { return g && i < g->blas.size() ? g->blas[i] : nullptr; }
+
+void luzgpu_destroy(luzgpu* g) {
+    if (!g) return;
+    if (g->tlas) luzrt_tlas_destroy(g->ctx, g->tlas);
+    for (auto* b : g->blas) luzrt_blas_destroy(g->ctx, b);
+    for (void* p : g->vbs) luzrt_free(g->ctx, p);
+    for (void* p : g->ibs) luzrt_free(g->ctx, p);
+    delete g;
+}
+
+}  // extern "C"

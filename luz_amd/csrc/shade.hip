@@ -1,0 +1,169 @@
+// shade.hip — the GGX metallic/roughness lighting kernel on gfx950.
+//
+// Restates light.frag:main (:171-235) with the BRDF of :17-49: G-buffer and
+// visibility counts in, RGBA32F out, one thread per pixel reading 36 bytes
+// of G-buffer, (lights + 1) count bytes and writing 16 bytes, all coalesced.
+// The sRGB decode pow(c/255, 2.2) (light.frag:172) is a 256-entry table.
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace luz {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr float kPi = 3.14159265359f;  // LuzCommon.h:10
+
+__device__ inline float clamp01(float a) { return a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a); }
+
+__global__ __launch_bounds__(kBlock) void k_shade(const luzrt_gbuffer g, const DevScene* ds,
+                                                  const float* lut, const uint8_t* shadow,
+                                                  const uint8_t* ao, float4* out) {
+    __shared__ float s_lut[256];
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    const uint32_t npix = g.width * g.height;
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    const luzrt_scene& S = ds->s;
+    V3 N = v3(g.normal[(size_t)p * 3], g.normal[(size_t)p * 3 + 1], g.normal[(size_t)p * 3 + 2]);
+    if (vlen(N) == 0.0f) {
+        out[p] = make_float4(S.ambient_color[0] * S.ambient_intensity,
+                             S.ambient_color[1] * S.ambient_intensity,
+                             S.ambient_color[2] * S.ambient_intensity, 1.0f);
+        return;
+    }
+    uint32_t alb = g.albedo[p], mat = g.material[p], emi = g.emission[p];
+    float albedo[3] = {s_lut[alb & 0xFFu], s_lut[(alb >> 8) & 0xFFu], s_lut[(alb >> 16) & 0xFFu]};
+    float rough = (float)(mat & 0xFFu) / 255.0f;
+    float metal = (float)((mat >> 8) & 0xFFu) / 255.0f;
+    float occl = (float)((mat >> 16) & 0xFFu) / 255.0f;
+    V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
+    V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
+    V3 V = vnorm(vsub(cam, P));
+    float F0[3];
+    for (int k = 0; k < 3; ++k) F0[k] = 0.04f * (1.0f - metal) + albedo[k] * metal;
+    float Lo[3] = {0.0f, 0.0f, 0.0f};
+    float a = rough * rough, a2 = a * a;
+    float rr = rough + 1.0f, kk = (rr * rr) / 8.0f;
+    float NdotV = fmax_(vdot(N, V), 0.0f);
+    float gv = NdotV / (NdotV * (1.0f - kk) + kk);
+    const uint32_t nl = (uint32_t)S.num_lights;
+    for (uint32_t l = 0; l < nl; ++l) {
+        const luzrt_light& Lb = S.lights[l];
+        V3 Lr = vsub(v3(Lb.position[0], Lb.position[1], Lb.position[2]), P);
+        V3 L = vnorm(Lr);
+        float att = 1.0f;
+        V3 ndir = v3(-Lb.direction[0], -Lb.direction[1], -Lb.direction[2]);
+        if (Lb.type == LUZRT_LIGHT_DIRECTIONAL) {
+            L = vnorm(ndir);
+        } else if (Lb.type == LUZRT_LIGHT_SPOT) {
+            float dist = vlen(Lr);
+            att = 1.0f / (dist * dist);
+            float theta = vdot(L, vnorm(ndir));
+            float eps = Lb.inner_angle - Lb.outer_angle;
+            att = att * clamp01((theta - Lb.outer_angle) / eps);
+        } else {
+            float dist = vlen(Lr);
+            att = 1.0f / (dist * dist);
+        }
+        int ns = Lb.num_shadow_samples;
+        float sh = ns > 0 ? (float)shadow[(size_t)l * npix + p] / (float)ns : 0.0f;
+        float scale = Lb.intensity * att * (1.0f - sh);
+        V3 H = vnorm(vadd(V, L));
+        float NdotH = fmax_(vdot(N, H), 0.0f);
+        float den = NdotH * NdotH * (a2 - 1.0f) + 1.0f;
+        float D = a2 / (kPi * den * den);
+        float NdotL = fmax_(vdot(N, L), 0.0f);
+        float G = (NdotL / (NdotL * (1.0f - kk) + kk)) * gv;
+        float HV = clamp01(vdot(H, V));
+        float om = clamp01(1.0f - HV);
+        float om5 = om * om * om * om * om;
+        float denom = 4.0f * NdotV * NdotL + 0.0001f;
+        float DG = D * G;
+        for (int k = 0; k < 3; ++k) {
+            float F = F0[k] + (1.0f - F0[k]) * om5;
+            float spec = DG * F / denom;
+            float kD = (1.0f - F) * (1.0f - metal);
+            float radiance = Lb.color[k] * scale;
+            Lo[k] = Lo[k] + (kD * albedo[k] / kPi + spec) * radiance * NdotL;
+        }
+    }
+    int na = S.ao_num_samples;
+    float aof = na > 0 ? (float)ao[p] / (float)na : 1.0f;
+    float o[3];
+    for (int k = 0; k < 3; ++k) {
+        float amb = S.ambient_color[k] * S.ambient_intensity * albedo[k] * occl * aof;
+        float e = (float)((emi >> (8 * k)) & 0xFFu) / 255.0f;
+        o[k] = amb + Lo[k] + e;
+    }
+    out[p] = make_float4(o[0], o[1], o[2], 1.0f);
+}
+
+inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+luzrt_status ensure_lut(luzrt_ctx* ctx) {
+    if (ctx->lut.ptr) return LUZRT_OK;
+    luzrt_status s = ensure(ctx, ctx->lut, 256 * sizeof(float));
+    if (s != LUZRT_OK) return s;
+    float h[256];
+    for (int i = 0; i < 256; ++i) h[i] = std::pow((float)i / 255.0f, 2.2f);
+    LUZ_HIP(ctx, hipMemcpyAsync(ctx->lut.ptr, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUZRT_OK;
+}
+
+luzrt_status shade_device(luzrt_ctx* ctx, const luzrt_gbuffer* g, const uint8_t* shadow,
+                          const uint8_t* ao, float* out) {
+    luzrt_status s = ensure_lut(ctx);
+    if (s != LUZRT_OK) return s;
+    const size_t npix = (size_t)g->width * g->height;
+    stage_begin(ctx, kStageShade);
+    k_shade<<<cdiv(npix, kBlock), kBlock, 0, ctx->stream>>>(*g, (const DevScene*)ctx->scene.ptr,
+                                                           (const float*)ctx->lut.ptr, shadow, ao,
+                                                           (float4*)out);
+    stage_end(ctx, kStageShade);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
+}  // namespace luz
+
+using namespace luz;
+
+extern "C" luzrt_status luzrt_shade(luzrt_ctx* ctx, const luzrt_gbuffer* g, const luzrt_scene* scene,
+                                    const uint8_t* shadow, const uint8_t* ao, float* out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!g || !scene || !ao || !out || (scene->num_lights > 0 && !shadow))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_shade: null argument");
+    if (((uintptr_t)out & 15u) != 0)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_shade: output must be 16-byte aligned");
+    DevScene ds;
+    luzrt_status s = upload_scene(ctx, scene, &ds);
+    if (s != LUZRT_OK) return s;
+    if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
+    return shade_device(ctx, g, shadow, ao, out);
+}
+
+extern "C" luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas,
+                                         const luzrt_gbuffer* g, const luzrt_scene* scene,
+                                         float* out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!tlas || !g || !scene || !out)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_light_pass: null argument");
+    if (((uintptr_t)out & 15u) != 0)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_light_pass: output must be 16-byte aligned");
+    luzrt_status s = check_gbuffer(ctx, g);
+    if (s != LUZRT_OK) return s;
+    const size_t npix = (size_t)g->width * g->height;
+    int nl = scene->num_lights < 0 ? 0 : scene->num_lights;
+    if ((s = ensure(ctx, ctx->vis_shadow, npix * (size_t)(nl > 0 ? nl : 1))) != LUZRT_OK) return s;
+    if ((s = ensure(ctx, ctx->vis_ao, npix)) != LUZRT_OK) return s;
+    s = luzrt_trace_visibility(ctx, tlas, g, scene, (uint8_t*)ctx->vis_shadow.ptr,
+                               (uint8_t*)ctx->vis_ao.ptr);
+    if (s != LUZRT_OK) return s;
+    return shade_device(ctx, g, (const uint8_t*)ctx->vis_shadow.ptr, (const uint8_t*)ctx->vis_ao.ptr,
+                        out);
+}

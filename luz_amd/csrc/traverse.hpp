@@ -1,0 +1,178 @@
+// traverse.hpp — two-level BVH traversal state machine for gfx950.
+//
+// One lane owns one ray.  Its stack lives in LDS (kShort entries per lane,
+// interleaved so that lane i of a wave touches bank i) and spills its oldest
+// entries to a per-lane global array when it overflows, so the depth is
+// unbounded by LDS size.  A sentinel entry on the stack marks the return from
+// a BLAS to the TLAS (the instance's object-space ray is then dropped).
+//
+// The step function advances a ray by one node visit (two slab tests, and the
+// triangle tests of leaf children) so that a persistent kernel can interleave
+// steps with wave-wide refills of finished lanes.
+#pragma once
+#include "device_math.hpp"
+
+namespace luz {
+
+constexpr int kShort = 16;    // LDS stack entries per lane (power of two)
+constexpr int kSpill = 160;   // global overflow entries per lane (>= TLAS+BLAS depth + 1)
+
+struct Stack {
+    uint32_t* ls;     // this lane's LDS column; entry e at ls[e * lds_stride]
+    uint32_t* gs;     // this lane's spill column; entry e at gs[e * g_stride]
+    uint32_t lds_stride;
+    uint32_t g_stride;
+    int sp, lo;
+
+    __device__ inline bool push(uint32_t x) {
+        if (sp - lo == kShort) {
+            gs[(size_t)lo * g_stride] = ls[(lo & (kShort - 1)) * lds_stride];
+            lo++;
+        }
+        if (sp >= kSpill) return false;
+        ls[(sp & (kShort - 1)) * lds_stride] = x;
+        sp++;
+        return true;
+    }
+    __device__ inline uint32_t pop() {
+        sp--;
+        if (sp < lo) {
+            lo = sp;
+            return gs[(size_t)sp * g_stride];
+        }
+        return ls[(sp & (kShort - 1)) * lds_stride];
+    }
+};
+
+struct Best {
+    float t;
+    uint32_t inst_slot, tri_slot;
+    float T, det, U, V, W;
+    bool found;
+};
+
+struct RayState {
+    V3 wo, wd;
+    float tmin, tmax;
+    BoxRay wbr, obr;
+    TriRay tr;
+    const Node* bnodes;
+    const float* btris;
+    uint32_t inst_slot;
+    uint32_t next;
+    uint32_t steps;
+    bool in_blas;
+};
+
+// A well-formed two-level tree visits each node at most once per instance
+// entry; the cap only guards against a malformed structure looping forever.
+constexpr uint32_t kMaxSteps = 1u << 24;
+
+enum StepResult { kStepContinue = 0, kStepHit = 1, kStepDone = 2, kStepOverflow = 3 };
+
+__device__ inline void ray_begin(RayState& r, Stack& st, V3 o, V3 d, float tmin, float tmax) {
+    r.wo = o; r.wd = d; r.tmin = tmin; r.tmax = tmax;
+    r.wbr = box_setup(o, d, tmin, tmax);
+    r.next = 0;
+    r.steps = 0;
+    r.in_blas = false;
+    st.sp = 0; st.lo = 0;
+}
+
+__device__ inline void load_node(const Node* base, uint32_t i, float4& qx, float4& qy, float4& qz,
+                                 uint4& qc) {
+    const float4* p = reinterpret_cast<const float4*>(base + i);
+    qx = p[0]; qy = p[1]; qz = p[2];
+    qc = reinterpret_cast<const uint4*>(p)[3];
+}
+
+__device__ inline void load_tri(const float* tris, uint32_t slot, V3& a, V3& b, V3& c) {
+    const float* t = tris + (size_t)slot * 9;
+    a = v3(t[0], t[1], t[2]); b = v3(t[3], t[4], t[5]); c = v3(t[6], t[7], t[8]);
+}
+
+// Closest-hit ordering: smaller t, then smaller instance input index, then primitive id.
+__device__ inline bool better_hit(float t, uint32_t inst_slot, uint32_t tri_slot, const Best& b,
+                                  const InstAux* aux) {
+    if (!b.found || t < b.t) return true;
+    if (t > b.t) return false;
+    uint32_t ia = aux[inst_slot].input_index, ib = aux[b.inst_slot].input_index;
+    if (ia != ib) return ia < ib;
+    return aux[inst_slot].prims[tri_slot] < aux[b.inst_slot].prims[b.tri_slot];
+}
+
+template <bool CLOSEST>
+__device__ inline int ray_step(RayState& r, Stack& st, const Node* tlas_nodes, const InstRec* recs,
+                               const InstAux* aux, Best* best) {
+    if (++r.steps > kMaxSteps) return kStepOverflow;
+    if (r.next == kNone) {
+        if (st.sp == 0) return kStepDone;
+        r.next = st.pop();
+        if (r.next == kSentinel) {
+            r.in_blas = false;
+            r.next = kNone;
+            return kStepContinue;
+        }
+    }
+    if (!r.in_blas && (r.next & kLeaf)) {
+        uint32_t slot = r.next & ~kLeaf;
+        const float4* rp = reinterpret_cast<const float4*>(recs + slot);
+        float4 w0 = rp[0], w1 = rp[1], w2 = rp[2];
+        uint4 pp = reinterpret_cast<const uint4*>(rp)[3];
+        float w[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
+        V3 oo = xform_point(w, r.wo);
+        V3 od = xform_vec(w, r.wd);
+        float thi = CLOSEST ? best->t : r.tmax;
+        r.obr = box_setup(oo, od, r.tmin, thi);
+        r.tr = tri_setup(od);
+        r.bnodes = reinterpret_cast<const Node*>(((uint64_t)pp.y << 32) | pp.x);
+        r.btris = reinterpret_cast<const float*>(((uint64_t)pp.w << 32) | pp.z);
+        r.inst_slot = slot;
+        if (!st.push(kSentinel)) return kStepOverflow;
+        r.in_blas = true;
+        r.next = 0;
+        return kStepContinue;
+    }
+    float4 qx, qy, qz;
+    uint4 qc;
+    load_node(r.in_blas ? r.bnodes : tlas_nodes, r.next, qx, qy, qz, qc);
+    const BoxRay& br = r.in_blas ? r.obr : r.wbr;
+    float tn0, tn1;
+    bool h0 = qc.x != kEmpty && box_hit(br, qx.x, qx.z, qy.x, qy.z, qz.x, qz.z, tn0);
+    bool h1 = qc.y != kEmpty && box_hit(br, qx.y, qx.w, qy.y, qy.w, qz.y, qz.w, tn1);
+    int first = (h0 && h1) ? (tn1 < tn0 ? 1 : 0) : (h0 ? 0 : 1);
+    r.next = kNone;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        int c = k == 0 ? first : 1 - first;
+        bool h = c == 0 ? h0 : h1;
+        if (!h) continue;
+        uint32_t ch = c == 0 ? qc.x : qc.y;
+        if (r.in_blas && (ch & kLeaf)) {
+            uint32_t slot = ch & ~kLeaf;
+            V3 a, b, cc;
+            load_tri(r.btris, slot, a, b, cc);
+            float T, det, U, V, W;
+            if (tri_test(r.tr, r.obr.o, r.tmin, r.tmax, a, b, cc, T, det, U, V, W)) {
+                if (!CLOSEST) return kStepHit;
+                float t = __fdiv_rn(T, det);
+                if (better_hit(t, r.inst_slot, slot, *best, aux)) {
+                    best->found = true;
+                    best->t = t;
+                    best->inst_slot = r.inst_slot;
+                    best->tri_slot = slot;
+                    best->T = T; best->det = det; best->U = U; best->V = V; best->W = W;
+                    float m = t_margin(t);
+                    r.obr.thi = t + m;
+                    r.wbr.thi = t + m;
+                }
+            }
+            continue;
+        }
+        if (r.next == kNone) r.next = ch;
+        else if (!st.push(ch)) return kStepOverflow;
+    }
+    return kStepContinue;
+}
+
+}  // namespace luz

@@ -1,0 +1,141 @@
+"""GPU parity of the HIP path against the CPU oracle.
+
+Integer results (BVH bytes, G-buffer bytes, shadow/AO counts, hit flags) must
+be identical; radiance within 1e-3 relative (|gpu - cpu| <= 1e-3 * max(|cpu|, 1e-3)).
+"""
+import numpy as np
+import pytest
+
+from luz_amd import api
+from oracle import oracle as orc
+from tests.helpers import oracle_scene, random_rays, set_light_direction
+
+pytestmark = pytest.mark.gpu
+
+RADIANCE_RTOL = 1e-3
+
+
+def _gpu_scene(ctx, kind, seed=0):
+    s = api.HostScene(kind, seed)
+    g = api.GPUScene(ctx, s)
+    return s, g
+
+
+def test_blas_bytes_match_oracle(ctx):
+    for kind in (api.SCENE_CUBE, api.SCENE_SPHERES):
+        s, g = _gpu_scene(ctx, kind)
+        o = oracle_scene(s)
+        for m in range(s.mesh_count):
+            gn, gt, gp = g.blas(m).arrays()
+            on, ot, op = o.blas_arrays(m)
+            print(f"scene {kind} mesh {m}: {len(op)} tris, {len(on)} nodes", flush=True)
+            assert gn.shape == on.shape
+            np.testing.assert_array_equal(gp, op)
+            np.testing.assert_array_equal(gt.view(np.uint32), ot.view(np.uint32))
+            np.testing.assert_array_equal(gn, on)
+        g.close()
+
+
+def test_tlas_bytes_match_oracle(ctx):
+    s, g = _gpu_scene(ctx, api.SCENE_SPHERES)
+    o = oracle_scene(s)
+    gn, gw, go = g.tlas.arrays()
+    on, ow, oo = o.tlas_arrays()
+    np.testing.assert_array_equal(go, oo)
+    np.testing.assert_array_equal(gw.view(np.uint32), ow.view(np.uint32))
+    np.testing.assert_array_equal(gn, on)
+    g.close()
+
+
+@pytest.mark.parametrize("kind", [api.SCENE_CUBE, api.SCENE_SPHERES])
+def test_trace_rays_match_oracle(ctx, kind):
+    s, g = _gpu_scene(ctx, kind)
+    o = oracle_scene(s)
+    rng = np.random.default_rng(7)
+    if kind == api.SCENE_CUBE:
+        rays = np.concatenate([random_rays(rng, 20000, (0, 0.5, 0), 2.0, 20.0),
+                               random_rays(rng, 20000, (0, 0.5, 0), 1.0, 2.0, short=True)])
+    else:
+        rays = np.concatenate([random_rays(rng, 40000, (0, 0, 0), 60.0, 200.0),
+                               random_rays(rng, 40000, (0, 0, 0), 50.0, 3.0, short=True)])
+    want, _ = o.any_hit(rays)
+    dr = ctx.to_device(rays)
+    dh = ctx.buffer(len(rays))
+    ctx.trace_rays(g.tlas, dr, len(rays), dh)
+    got = dh.download(np.uint8, len(rays))
+    print(f"kind {kind}: {int(want.sum())} of {len(rays)} rays hit", flush=True)
+    assert 0 < want.sum() < len(rays)
+    np.testing.assert_array_equal(got, want)
+    g.close()
+
+
+def _gbuffers(ctx, s, g, o, width=None, height=None):
+    W, H, cam = s.camera()
+    if width:
+        W, H = width, height
+    _, _, mats = s.nodes()
+    gd = api.GBufferDevice(ctx, W, H)
+    ctx.primary_gbuffer(g.tlas, cam, mats, gd.desc())
+    got = gd.download()
+    want = o.primary_gbuffer(cam, mats, W, H)
+    return W, H, gd, got, want
+
+
+@pytest.mark.parametrize("kind,res", [(api.SCENE_CUBE, None), (api.SCENE_SPHERES, (480, 270))])
+def test_primary_gbuffer_bit_exact(ctx, kind, res):
+    s, g = _gpu_scene(ctx, kind)
+    o = oracle_scene(s)
+    W, H, gd, got, want = _gbuffers(ctx, s, g, o, *(res or (None, None)))
+    for k in ("position", "normal"):
+        np.testing.assert_array_equal(got[k].view(np.uint32), want[k].view(np.uint32), err_msg=k)
+    for k in ("albedo", "material", "emission"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    g.close()
+
+
+@pytest.mark.parametrize("kind,res,light_dir", [
+    (api.SCENE_CUBE, None, None),
+    (api.SCENE_CUBE, None, (0.8, -1.0, 0.6)),      # shadow falls towards the camera
+    (api.SCENE_SPHERES, (480, 270), None),
+])
+def test_visibility_bit_exact(ctx, kind, res, light_dir):
+    s, g = _gpu_scene(ctx, kind)
+    o = oracle_scene(s)
+    W, H, gd, got, _ = _gbuffers(ctx, s, g, o, *(res or (None, None)))
+    sb = s.pack()
+    if light_dir is not None:
+        set_light_direction(sb, 0, light_dir)
+    nl = sb.num_lights
+    dsh = ctx.buffer(nl * W * H)
+    dao = ctx.buffer(W * H)
+    ctx.trace_visibility(g.tlas, gd.desc(), sb, dsh, dao)
+    gsh = dsh.download(np.uint8, (nl, W * H))
+    gao = dao.download(np.uint8, W * H)
+    t = ctx.timings()
+    wsh, wao, st = o.trace_visibility(got, W, H, sb)
+    print(f"kind {kind} {W}x{H}: {t.rays} rays (oracle {st.rays}), shadowed {int(wsh.sum())}, "
+          f"AO misses {int(wao.sum())}", flush=True)
+    assert t.rays == st.rays
+    np.testing.assert_array_equal(gsh, wsh)
+    np.testing.assert_array_equal(gao, wao)
+    if light_dir is not None:
+        assert wsh.sum() > 100
+    g.close()
+
+
+@pytest.mark.parametrize("kind,res", [(api.SCENE_CUBE, None), (api.SCENE_SPHERES, (480, 270))])
+def test_light_pass_radiance(ctx, kind, res):
+    s, g = _gpu_scene(ctx, kind)
+    o = oracle_scene(s)
+    W, H, gd, got, _ = _gbuffers(ctx, s, g, o, *(res or (None, None)))
+    sb = s.pack()
+    out = ctx.buffer(W * H * 16)
+    ctx.light_pass(g.tlas, gd.desc(), sb, out)
+    rad = out.download(np.float32, (W * H, 4))
+    wsh, wao, _ = o.trace_visibility(got, W, H, sb)
+    want = orc.shade(got, W, H, sb, wsh, wao)
+    err = np.abs(rad - want)
+    tol = RADIANCE_RTOL * np.maximum(np.abs(want), 1e-3)
+    print(f"max rel err {float((err / np.maximum(np.abs(want), 1e-3)).max()):.3g}", flush=True)
+    assert np.all(err <= tol)
+    g.close()
