@@ -3,6 +3,9 @@
 // Every FP32 expression here is evaluated exactly as written: the library is
 // compiled with -ffp-contract=off (no fused multiply-add), IEEE division and
 // square root (HIP's default correctly rounded forms) and denormals kept.
+// Square roots use sqrtf: on gfx950 it lowers to v_sqrt_f32 plus a
+// correction step (correctly rounded), whereas __fsqrt_rn lowers to the bare
+// v_sqrt_f32 approximation.
 // The shadow mask and AO counts are therefore reproducible bit for bit by a
 // CPU implementation of the same specification.
 //
@@ -71,7 +74,7 @@ __host__ __device__ inline float vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y
 __host__ __device__ inline V3 vcross(V3 a, V3 b) {
     return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ inline float vlen(V3 a) { return __fsqrt_rn(vdot(a, a)); }
+__device__ inline float vlen(V3 a) { return sqrtf(vdot(a, a)); }
 __device__ inline V3 vnorm(V3 a) {
     float l = vlen(a);
     return v3(__fdiv_rn(a.x, l), __fdiv_rn(a.y, l), __fdiv_rn(a.z, l));

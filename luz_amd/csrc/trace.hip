@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const float* position, const 
         for (int s = 0; s < n; ++s) {
             float u, v;
             sample2(x, y, frame, (uint32_t)(l * 256 + s), u, v);
-            float pr = L.radius * __fsqrt_rn(u);
+            float pr = L.radius * sqrtf(u);
             float sn, cs;
             sincos_turns(v, sn, cs);
             float dx = pr * cs, dy = pr * sn;
@@ -137,12 +137,12 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const float* position, const 
         for (int s = 0; s < na; ++s) {
             float u, v;
             sample2(x, y, frame, (uint32_t)(16384 + s), u, v);
-            float r = __fsqrt_rn(u);
+            float r = sqrtf(u);
             float sn, cs;
             sincos_turns(v * kHemiTurns, sn, cs);
             float hx = r * cs, hy = r * sn;
             float om = 1.0f - u;
-            float hz = __fsqrt_rn(om > 0.0f ? om : 0.0f);
+            float hz = sqrtf(om > 0.0f ? om : 0.0f);
             V3 d = v3(T.x * hx + B.x * hy + N.x * hz,
                       T.y * hx + B.y * hy + N.y * hz,
                       T.z * hx + B.z * hy + N.z * hz);
