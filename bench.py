@@ -39,7 +39,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
     ap.add_argument("--groups-per-cu", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (PMC counter calibration)")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "k_trace_any_traffic.json"),
+                    help="HBM bytes per traversal launch measured with rocprofv3 --pmc (JSON)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "the lighting path does not shard across GPUs here", file=sys.stderr)
@@ -113,9 +119,17 @@ def main() -> int:
         },
     }
 
+    traffic = None
+    tpath = Path(args.traffic)
+    if tpath.exists():
+        try:
+            traffic = json.loads(tpath.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+
     if not args.no_cpu:
         from oracle import oracle as orc
-        # the same scene on the oracle, its FP32 visibility on a row sample
+        # the same scene on the oracle: FP32 visibility of the whole frame
         host = gbuf.download()
         gpu_sh = ctx.buffer(max(nl, 1) * W * H)
         gpu_ao = ctx.buffer(W * H)
@@ -128,13 +142,19 @@ def main() -> int:
             o.add_mesh(v, idx)
         o2w, mesh_ids, _ = scene.nodes()
         o.set_instances(o2w, mesh_ids)
-        threads = orc.threads_default()
+        threads = min(orc.threads_default(), 16)
+        o.trace_visibility(host, W, H, sb, threads=threads)      # warm-up frame
+        times = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            wsh, wao, st = o.trace_visibility(host, W, H, sb, threads=threads)
+            times.append(time.perf_counter() - c0)
+        cpu_s = float(np.median(times))
+        # single core on a row sample
         c0 = time.perf_counter()
-        wsh, wao, st = o.trace_visibility(host, W, H, sb, threads=threads, row_step=args.cpu_row_step)
-        cpu_s = time.perf_counter() - c0
-        rows = np.zeros((H, W), bool)
-        rows[:: args.cpu_row_step] = True
-        m = rows.reshape(-1)
+        _, _, st1 = o.trace_visibility(host, W, H, sb, threads=1, row_step=args.cpu_row_step * 2)
+        cpu1_s = time.perf_counter() - c0
+        m = np.ones(W * H, bool)
         # FP64 oracle on a sparser sample for the AO RMSE
         f64_step = args.cpu_row_step * 4
         rows64 = np.zeros((H, W), bool)
@@ -149,7 +169,7 @@ def main() -> int:
         achieved = trace_bytes / (trace_avg * 1e-3) / 1e9
         record["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_trace_any",
             "algorithmic_bytes_per_ray": round(per_ray + 33.0, 2),
             "per_ray_counts": {"node": round(st.node_visits / max(st.rays, 1), 3),
@@ -159,19 +179,38 @@ def main() -> int:
         record["cpu_baseline"] = {
             "value": round(st.rays / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle FP32 visibility on every {args.cpu_row_step}th row of the same frame "
-                      f"({st.rays} rays, {cpu_s:.2f} s)",
+            "sample": f"oracle FP32 visibility of the whole frame ({st.rays} rays), median of 3 "
+                      f"after a warm-up frame: {cpu_s * 1e3:.0f} ms/frame on {threads} threads; "
+                      f"single thread on every {args.cpu_row_step * 2}th row: "
+                      f"{st1.rays / cpu1_s / 1e6:.2f} Mrays/s",
+            "ms_per_frame": round(cpu_s * 1e3, 1),
+            "single_thread_mrays_per_s": round(st1.rays / cpu1_s / 1e6, 3),
         }
         record["parity"] = {
-            "sample_rows_every": args.cpu_row_step,
+            "compared_pixels": int(m.sum()),
             "shadow_mismatch": int((gsh[:, m] != wsh[:, m]).sum()),
             "ao_mismatch": int((gao[m] != wao[m]).sum()),
             "ao_rmse_vs_fp64": round(ao_rmse, 6),
-            "hit_miss_mismatch_vs_fp64": int((gsh[:, m64] != dsh[:, m64]).sum() + (gao[m64] != dao[m64]).sum()),
+            "count_mismatch_vs_fp64": int((gsh[:, m64] != dsh[:, m64]).sum() + (gao[m64] != dao[m64]).sum()),
+            "fp64_sample_rows_every": f64_step,
         }
     print(json.dumps(record), flush=True)
     gs.close()
     ctx.close()
+    return 0
+
+
+def calibrate_copy(mib: int) -> int:
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    readings scale the traversal kernel's counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "launches": 3}), flush=True)
     return 0
 
 
