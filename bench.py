@@ -37,6 +37,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-row-step", type=int, default=8, help="oracle traces every k-th row")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
+    ap.add_argument("--autotune", action="store_true",
+                    help="search the traversal launch shape on this workload before timing")
     ap.add_argument("--groups-per-cu", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -67,6 +69,10 @@ def main() -> int:
     sb = scene.pack()
     out = ctx.buffer(W * H * 16)
     nl = sb.num_lights
+    tuned = None
+    if args.autotune:
+        from luz_amd import autotune
+        tuned = autotune.tune_visibility(ctx, gs.tlas, gbuf.desc(), sb)
 
     def frame():
         gs.update_resources_gpu(scene)            # TLAS rebuild (GPUScene.cpp:354-365)
@@ -118,6 +124,9 @@ def main() -> int:
             "visibility_mrays_per_s": round(rays / (float(np.mean(vis_ms)) * 1e-3) / 1e6, 2),
         },
     }
+    if tuned is not None:
+        record["config"]["autotune"] = {"groups_per_cu": tuned["groups_per_cu"], "refill": tuned["refill"],
+                                        "table": tuned["table"]}
 
     traffic = None
     tpath = Path(args.traffic)

@@ -36,7 +36,7 @@ struct luzrt_ctx {
     bool ev_used[luz::kStageCount] = {};
     uint64_t last_rays = 0, last_active = 0;
     // traversal launch configuration
-    uint32_t groups_per_cu = 4;
+    uint32_t groups_per_cu = 8;
     uint32_t refill_threshold = 16;
 };
 

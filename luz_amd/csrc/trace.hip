@@ -157,6 +157,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const float* position, const 
     }
 }
 
+// Rays are claimed in chunks of kChunk per wave (one device-scope atomic per
+// chunk): a single counter hit by every refill serialises across the 8 XCDs.
+constexpr uint32_t kChunk = 2048;
+
 __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t refill) {
     __shared__ uint32_t lds[kShort * kBlock];
     const uint32_t tid = threadIdx.x;
@@ -174,25 +178,33 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
     RayState r;
     uint32_t ray = kNone;
     bool open = true;
+    uint32_t chunk_next = 0, chunk_end = 0;  // wave-uniform
     for (;;) {
         if (open) {
             uint64_t idle = __ballot(ray == kNone);
             uint32_t nidle = (uint32_t)__popcll(idle);
             if (nidle >= refill || nidle == 64u) {
-                int leader = __ffsll((unsigned long long)idle) - 1;
-                uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(&L.counters[kCntFetch], nidle);
-                base = __shfl(base, leader);
-                if (base + nidle >= nrays) open = false;
+                if (chunk_next >= chunk_end) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&L.counters[kCntFetch], kChunk);
+                    base = __shfl(base, 0);
+                    chunk_next = base;
+                    chunk_end = base + kChunk < nrays ? base + kChunk : nrays;
+                    if (base >= nrays) open = false;
+                }
+                uint32_t avail = chunk_end > chunk_next ? chunk_end - chunk_next : 0u;
+                uint32_t take = nidle < avail ? nidle : avail;
                 if (ray == kNone) {
-                    uint32_t rid = base + (uint32_t)__popcll(idle & lt);
-                    if (rid < nrays) {
+                    uint32_t rank = (uint32_t)__popcll(idle & lt);
+                    if (rank < take) {
+                        uint32_t rid = chunk_next + rank;
                         ray = rid;
                         float4 o = L.ray_o[(size_t)rid * L.ray_stride];
                         float4 d = L.ray_d[(size_t)rid * L.ray_stride];
                         ray_begin(r, st, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, d.w);
                     }
                 }
+                chunk_next += take;
             }
         }
         if (__ballot(ray != kNone) == 0ull) {
