@@ -4,8 +4,8 @@
 // TraceAORays :111-135, EvaluateShadow :137-146, AO origin :229-231) with a
 // wavefront pipeline over coalesced structure-of-arrays buffers:
 //   k_vis_init  zero the call's device counters (graph-safe, no memset);
-//   k_compact   non-background pixels in 8x8-tile order -> active list
-//               (wave-aggregated atomics keep tiles contiguous);
+//   k_compact_* non-background pixels in 8x8-tile order -> active list
+//               (count, scan, write: deterministic, no atomics);
 //   k_raygen    one thread per active pixel writes its shadow and AO rays,
 //               slot-major (ray j*A + a), so a wave holds 64 neighbouring
 //               pixels of the same light sample or AO sample;
@@ -33,35 +33,93 @@ __device__ inline void tile_pixel(uint32_t g, uint32_t W, uint32_t& x, uint32_t&
     y = (tile / tiles_x) * 8 + (t >> 3);
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact(const float* normal, uint32_t W, uint32_t H,
-                                                    uint32_t ntiles_px, uint32_t* act_list,
-                                                    uint32_t* act_index, uint32_t* counters) {
-    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+// Compaction of non-background pixels without atomics, in three passes over
+// 8x8-tile order: per-block counts, one exclusive scan, per-block writes.  The
+// active list is therefore deterministic (tile order) and contiguous per tile.
+__device__ inline bool pixel_active(const float* normal, uint32_t g, uint32_t W, uint32_t H,
+                                    uint32_t ntiles_px, uint32_t& p) {
     uint32_t x = 0, y = 0;
-    bool inside = false, active = false;
+    p = 0;
+    if (g >= ntiles_px) return false;
+    tile_pixel(g, W, x, y);
+    if (x >= W || y >= H) return false;
+    p = y * W + x;
+    const float* n = normal + (size_t)p * 3;
+    return vlen(v3(n[0], n[1], n[2])) != 0.0f;
+}
+
+__device__ inline uint32_t block_prefix(bool active, uint32_t* wave_tot, uint32_t& block_total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t m = __ballot(active);
+    uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    if (lane == 0) wave_tot[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t w = 0; w < kBlock / 64; ++w) {
+        uint32_t c = wave_tot[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    block_total = tot;
+    return off + (uint32_t)__popcll(m & lt);
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_count(const float* normal, uint32_t W, uint32_t H,
+                                                          uint32_t ntiles_px, uint32_t* block_counts) {
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    uint32_t p;
+    bool active = pixel_active(normal, blockIdx.x * kBlock + threadIdx.x, W, H, ntiles_px, p);
+    uint32_t total;
+    block_prefix(active, wave_tot, total);
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block counts (one workgroup); writes the total.
+__global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, uint32_t nblocks,
+                                                       uint32_t* counters) {
+    __shared__ uint32_t sh[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nblocks; base += 1024) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < nblocks ? block_counts[i] : 0u;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024; off <<= 1) {
+            uint32_t add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += add;
+            __syncthreads();
+        }
+        uint32_t incl = sh[threadIdx.x];
+        uint32_t c = carry;
+        if (i < nblocks) block_counts[i] = c + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counters[kCntActive] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_write(const float* normal, uint32_t W, uint32_t H,
+                                                          uint32_t ntiles_px,
+                                                          const uint32_t* block_offsets,
+                                                          uint32_t* act_list, uint32_t* act_index) {
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t p;
+    bool active = pixel_active(normal, g, W, H, ntiles_px, p);
+    uint32_t total;
+    uint32_t a = block_offsets[blockIdx.x] + block_prefix(active, wave_tot, total);
+    bool inside = false;
     if (g < ntiles_px) {
+        uint32_t x, y;
         tile_pixel(g, W, x, y);
         inside = x < W && y < H;
     }
-    uint32_t p = y * W + x;
-    if (inside) {
-        const float* n = normal + (size_t)p * 3;
-        active = vlen(v3(n[0], n[1], n[2])) != 0.0f;
-    }
-    uint64_t m = __ballot(active);
-    uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    if (m) {
-        int leader = __ffsll((unsigned long long)m) - 1;
-        if ((int)lane == leader) base = atomicAdd(&counters[kCntActive], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-    }
-    if (inside) {
-        uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-        uint32_t a = base + (uint32_t)__popcll(m & lt);
-        if (active) act_list[a] = p;
-        act_index[p] = active ? a : kNone;
-    }
+    if (active) act_list[a] = p;
+    if (inside) act_index[p] = active ? a : kNone;
 }
 
 struct ShadowSetup { V3 Lvec, T, B; float tmax; };
@@ -283,7 +341,10 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
         return fail(ctx, LUZRT_E_LIMIT, "%zu rays exceed the 32-bit ray index", max_rays);
     if ((s = ensure(ctx, ctx->counters, 64)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->active, npix * 4)) != LUZRT_OK) return s;
-    if ((s = ensure(ctx, ctx->act_index, npix * 4)) != LUZRT_OK) return s;
+    // act_index holds the per-pixel active index followed by the per-block counts
+    const size_t ntiles_px_sz = (size_t)((W + 7) / 8) * ((H + 7) / 8) * 64;
+    if ((s = ensure(ctx, ctx->act_index, npix * 4 + (ntiles_px_sz / kBlock + 2) * 4)) != LUZRT_OK)
+        return s;
     if ((s = ensure(ctx, ctx->rays_o, max_rays * 16 + 16)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->rays_d, max_rays * 16 + 16)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->hits, max_rays + 16)) != LUZRT_OK) return s;
@@ -294,9 +355,13 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
     stage_begin(ctx, kStageCompact);
     k_vis_init<<<1, 64, 0, st>>>(counters, 0u);
     uint32_t ntiles_px = ((W + 7) / 8) * ((H + 7) / 8) * 64;
-    k_compact<<<cdiv(ntiles_px, kBlock), kBlock, 0, st>>>(g->normal, W, H, ntiles_px,
-                                                         (uint32_t*)ctx->active.ptr,
-                                                         (uint32_t*)ctx->act_index.ptr, counters);
+    uint32_t nblocks = cdiv(ntiles_px, kBlock);
+    uint32_t* block_counts = (uint32_t*)ctx->act_index.ptr + npix;
+    k_compact_count<<<nblocks, kBlock, 0, st>>>(g->normal, W, H, ntiles_px, block_counts);
+    k_compact_scan<<<1, 1024, 0, st>>>(block_counts, nblocks, counters);
+    k_compact_write<<<nblocks, kBlock, 0, st>>>(g->normal, W, H, ntiles_px, block_counts,
+                                               (uint32_t*)ctx->active.ptr,
+                                               (uint32_t*)ctx->act_index.ptr);
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
