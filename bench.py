@@ -37,6 +37,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-row-step", type=int, default=8, help="oracle traces every k-th row")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
+    ap.add_argument("--workload", default="spheres1080p", choices=["spheres1080p", "terrain4k"],
+                    help="spheres1080p: the headline 1080p sphere field (plain FP32 path); "
+                         "terrain4k: 3840x2160 terrain + spheres, ~5.2M tris, precision form")
     ap.add_argument("--autotune", action="store_true",
                     help="search the traversal launch shape on this workload before timing")
     ap.add_argument("--groups-per-cu", type=int, default=0)
@@ -60,7 +63,10 @@ def main() -> int:
     ctx = api.Context(0)
     if args.groups_per_cu or args.refill:
         ctx.set_trace_config(args.groups_per_cu, args.refill)
-    scene = api.HostScene(api.SCENE_SPHERES, 0)
+    compact = args.workload == "terrain4k"
+    if compact:
+        ctx.set_precision(api.PRECISION_COMPACT)
+    scene = api.HostScene(api.SCENE_TERRAIN if compact else api.SCENE_SPHERES, 0)
     gs = api.GPUScene(ctx, scene)
     W, H, cam = scene.camera()
     _, _, mats = scene.nodes()
@@ -109,12 +115,15 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32+fp16" if compact else "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "1920x1080 sphere field: 50 instances of 5 fBm icospheres (20,480 tris) + "
-                        "64x64 ground grid, 1,032,192 instanced tris, 1 dir + 8 point lights, "
-                        "1 shadow ray/light + 16 AO rays/px",
+            "workload": ("3840x2160 fBm terrain (2047x1023 quads) + 40 fBm icospheres, 8-wide BVH with "
+                         "8-bit child boxes, packed-FP16 GGX, 4 dir + 32 point lights, 1 shadow ray/light "
+                         "+ 8 AO rays/px") if compact else
+                        ("1920x1080 sphere field: 50 instances of 5 fBm icospheres (20,480 tris) + "
+                         "64x64 ground grid, 1,032,192 instanced tris, 1 dir + 8 point lights, "
+                         "1 shadow ray/light + 16 AO rays/px"),
             "width": W, "height": H, "instanced_tris": int(scene.instanced_triangles),
             "rays_per_frame": rays, "active_pixels": active,
             "frame_ms_breakdown": {"tlas_build": round(float(np.mean(build_ms)), 4),
@@ -136,7 +145,9 @@ def main() -> int:
         except (ValueError, OSError):
             traffic = None
 
-    if not args.no_cpu:
+    if not args.no_cpu and compact:
+        record.update(compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays))
+    elif not args.no_cpu:
         from oracle import oracle as orc
         # the same scene on the oracle: FP32 visibility of the whole frame
         host = gbuf.download()
@@ -207,6 +218,63 @@ def main() -> int:
     gs.close()
     ctx.close()
     return 0
+
+
+def compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays) -> dict:
+    """Precision form: hit/miss mismatch and radiance RMSE against the FP64 oracle,
+    exactness against the FP32 oracle, on row samples of the 4K frame."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    nl = sb.num_lights
+    host = gbuf.download()
+    dsh, dao = ctx.buffer(nl * W * H), ctx.buffer(W * H)
+    ctx.trace_visibility(gs.tlas, gbuf.desc(), sb, dsh, dao)
+    ctx.shade(gbuf.desc(), sb, dsh, dao, out)
+    gsh = dsh.download(np.uint8, (nl, W * H))
+    gao = dao.download(np.uint8, W * H)
+    rad = out.download(np.float32, (W * H, 4))[:, :3].astype(np.float64)
+    o = orc.OracleScene()
+    for i in range(scene.mesh_count):
+        v, idx = scene.mesh(i)
+        o.add_mesh(v, idx)
+    o2w, mesh_ids, _ = scene.nodes()
+    o.set_instances(o2w, mesh_ids)
+    o.build_wide()
+    threads = min(orc.threads_default(), 16)
+    step32, step64 = 16, 64
+    c0 = time.perf_counter()
+    wsh, wao, st = o.trace_visibility(host, W, H, sb, threads=threads, row_step=step32, wide=True)
+    cpu_s = time.perf_counter() - c0
+    dsh64, dao64, _ = o.trace_visibility(host, W, H, sb, threads=threads, row_step=step64, f64=True)
+    act = np.abs(host["normal"]).sum(1) != 0
+    rows32 = np.zeros((H, W), bool); rows32[::step32] = True; m32 = rows32.reshape(-1)
+    rows64 = np.zeros((H, W), bool); rows64[::step64] = True; m64 = rows64.reshape(-1) & act
+    ref64 = orc.shade(host, W, H, sb, dsh64, dao64, f64=True)[:, :3]
+    rmse = float(np.sqrt(np.mean((rad[m64] - ref64[m64]) ** 2)))
+    scale = float(np.sqrt(np.mean(ref64[m64] ** 2)))
+    na = max(sb.ao_num_samples, 1)
+    per_ray = (96.0 * st.wide_node_visits + 64.0 * st.node_visits + 36.0 * st.tri_tests +
+               64.0 * st.inst_visits) / max(st.rays, 1)
+    achieved = rays * (per_ray + 33.0) / (trace_avg * 1e-3) / 1e9
+    return {
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace_any<wide>",
+                     "algorithmic_bytes_per_ray": round(per_ray + 33.0, 2)},
+        "cpu_baseline": {"value": round(st.rays / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads,
+                         "kind": "port",
+                         "sample": f"oracle FP32 8-wide visibility on every {step32}th row ({st.rays} rays)"},
+        "parity": {
+            "fp32_oracle_rows_every": step32,
+            "shadow_mismatch_vs_fp32": int((gsh[:, m32] != wsh[:, m32]).sum()),
+            "ao_mismatch_vs_fp32": int((gao[m32] != wao[m32]).sum()),
+            "fp64_oracle_rows_every": step64,
+            "count_mismatch_vs_fp64": int((gsh[:, m64] != dsh64[:, m64]).sum() + (gao[m64] != dao64[m64]).sum()),
+            "ao_rmse_vs_fp64": round(float(np.sqrt(np.mean((gao[m64] / na - dao64[m64] / na) ** 2))), 6),
+            "radiance_rmse_vs_fp64": round(rmse, 6),
+            "radiance_rmse_relative": round(rmse / max(scale, 1e-30), 6),
+        },
+    }
 
 
 def calibrate_copy(mib: int) -> int:

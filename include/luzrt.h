@@ -152,6 +152,14 @@ luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
  * a refill.  Used by the autotuner. */
 luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill_threshold);
 
+/* Precision modes.  PLAIN: binary BVH with FP32 boxes, FP32 shading.
+ * COMPACT: BLASes traversed as 8-wide nodes with 8-bit child boxes rounded
+ * outward (96-byte nodes), FP32 watertight triangle tests, GGX shading in
+ * packed FP16 with FP32 accumulation.  Takes effect at the next TLAS build. */
+#define LUZRT_PRECISION_PLAIN 0
+#define LUZRT_PRECISION_COMPACT 1
+luzrt_status luzrt_set_precision(luzrt_ctx* ctx, int mode);
+
 /* ------------------------------------------------------------ memory */
 luzrt_status luzrt_malloc(luzrt_ctx* ctx, size_t bytes, void** device_ptr);
 luzrt_status luzrt_free(luzrt_ctx* ctx, void* device_ptr);
@@ -173,6 +181,12 @@ luzrt_status luzrt_blas_destroy(luzrt_ctx* ctx, luzrt_blas* blas);
  * slot -> primitive ids) for inspection. */
 luzrt_status luzrt_blas_arrays(const luzrt_blas* blas, const void** nodes, uint32_t* node_count,
                                const float** tris, const uint32_t** prim_ids, uint32_t* tri_count);
+
+/* Device pointers of the 8-wide form (96-byte nodes, 36-byte triangles in
+ * wide-leaf order, primitive ids); built on first use in COMPACT mode. */
+luzrt_status luzrt_blas_wide_arrays(luzrt_ctx* ctx, luzrt_blas* blas, const void** nodes,
+                                    uint32_t* node_count, const float** tris,
+                                    const uint32_t** prim_ids, uint32_t* tri_count);
 
 /* CreateTLAS(maxInstances). */
 luzrt_status luzrt_tlas_create(luzrt_ctx* ctx, uint32_t max_instances, luzrt_tlas** out);

@@ -27,6 +27,7 @@ MAX_LIGHTS = 64
 MAX_INSTANCES = 8192
 LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
 SCENE_CUBE, SCENE_SPHERES, SCENE_TERRAIN, SCENE_RANDOM = 1, 2, 4, 5
+PRECISION_PLAIN, PRECISION_COMPACT = 0, 1
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "device error", -4: "in use",
           -5: "limit exceeded", -6: "no device", -7: "unsupported"}
@@ -98,6 +99,9 @@ _sigs = {
     "luzrt_sync": (C.c_int, [_P]),
     "luzrt_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "luzrt_set_precision": (C.c_int, [_P, C.c_int]),
+    "luzrt_blas_wide_arrays": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
+                                         C.POINTER(_P), C.POINTER(C.c_uint32)]),
     "luzrt_malloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "luzrt_free": (C.c_int, [_P, _P]),
     "luzrt_upload": (C.c_int, [_P, _P, _P, C.c_size_t]),
@@ -213,6 +217,10 @@ class Context:
     def set_trace_config(self, groups_per_cu: int = 0, refill: int = 0) -> None:
         self._check(lib.luzrt_set_trace_config(self.handle, groups_per_cu, refill))
 
+    def set_precision(self, mode: int) -> None:
+        """PRECISION_PLAIN or PRECISION_COMPACT (8-wide 8-bit BVH, packed-FP16 shading)."""
+        self._check(lib.luzrt_set_precision(self.handle, mode))
+
     def buffer(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
 
@@ -268,6 +276,21 @@ class Blas:
         self.ctx._check(lib.luzrt_blas_arrays(self.handle, C.byref(nodes), C.byref(nn), C.byref(tris),
                                               C.byref(prims), C.byref(nt)))
         return _download_arrays(self.ctx, nodes.value, nn.value, tris.value, prims.value, nt.value)
+
+    def wide_arrays(self):
+        """(nodes u8[n,96], tris f32[t,9], prim_ids u32[t]) of the 8-wide form."""
+        nodes, nn, tris, prims, nt = _P(), C.c_uint32(), _P(), _P(), C.c_uint32()
+        self.ctx._check(lib.luzrt_blas_wide_arrays(self.ctx.handle, self.handle, C.byref(nodes), C.byref(nn),
+                                                   C.byref(tris), C.byref(prims), C.byref(nt)))
+        n_out = np.empty((nn.value, 96), np.uint8)
+        if nn.value:
+            self.ctx._check(lib.luzrt_download(self.ctx.handle, _ptr(n_out), nodes.value, n_out.nbytes))
+        t_out = np.empty((nt.value, 9), np.float32)
+        p_out = np.empty(nt.value, np.uint32)
+        if nt.value:
+            self.ctx._check(lib.luzrt_download(self.ctx.handle, _ptr(t_out), tris.value, t_out.nbytes))
+            self.ctx._check(lib.luzrt_download(self.ctx.handle, _ptr(p_out), prims.value, p_out.nbytes))
+        return n_out, t_out, p_out
 
     def destroy(self) -> None:
         self.ctx._check(lib.luzrt_blas_destroy(self.ctx.handle, self.handle))

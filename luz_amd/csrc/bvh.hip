@@ -425,15 +425,157 @@ __global__ void k_small_root(const uint8_t* verts, uint32_t stride, uint32_t nv,
     nodes[0] = nd;
 }
 
+// ---------------------------------------------------------------- 8-wide form
+// Collapse of the binary LBVH into 8-wide nodes, one breadth-first level per
+// pass: k_wide_collapse lists each wide node's slots (repeatedly replacing the
+// internal slot of largest surface area, first on ties, by its two children),
+// two scans number the next level's nodes and this level's leaves, and
+// k_wide_emit quantises the child boxes and writes nodes and leaf triangles.
+struct WRef {
+    uint32_t c;
+    float lo[3], hi[3];
+    uint32_t pad;
+};
+
+__device__ inline float surface_area(const float* lo, const float* hi) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ inline WRef child_ref(const Node& n, int c) {
+    WRef r;
+    r.c = n.child[c];
+    r.lo[0] = n.lox[c]; r.lo[1] = n.loy[c]; r.lo[2] = n.loz[c];
+    r.hi[0] = n.hix[c]; r.hi[1] = n.hiy[c]; r.hi[2] = n.hiz[c];
+    r.pad = 0;
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_wide_collapse(const Node* bn, const uint32_t* level,
+                                                          uint32_t count, WRef* slots_out,
+                                                          uint32_t* nslots, uint32_t* n_int,
+                                                          uint32_t* n_leaf) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    WRef s[8];
+    int n = 0;
+    const Node& root = bn[level[i]];
+    for (int c = 0; c < 2; ++c)
+        if (root.child[c] != kEmpty) s[n++] = child_ref(root, c);
+    while (n < 8) {
+        int best = -1;
+        float best_sa = 0.0f;
+        for (int k = 0; k < n; ++k) {
+            if (s[k].c & kLeaf) continue;
+            float sa = surface_area(s[k].lo, s[k].hi);
+            if (best < 0 || sa > best_sa) { best = k; best_sa = sa; }
+        }
+        if (best < 0) break;
+        const Node& in = bn[s[best].c];
+        for (int k = n; k > best + 1; --k) s[k] = s[k - 1];
+        s[best] = child_ref(in, 0);
+        s[best + 1] = child_ref(in, 1);
+        ++n;
+    }
+    uint32_t ni = 0, nl = 0;
+    for (int k = 0; k < n; ++k) {
+        slots_out[(size_t)i * 8 + k] = s[k];
+        if (s[k].c & kLeaf) ++nl; else ++ni;
+    }
+    nslots[i] = (uint32_t)n;
+    n_int[i] = ni;
+    n_leaf[i] = nl;
+}
+
+__device__ inline void quant_frame(float lo, float hi, float& origin, int& e) {
+    float t = (hi - lo) / 255.0f;
+    int e0 = -126, em = -126;
+    if (t > 0.0f) frexpf(t, &e0);
+    float mag = fabsf(lo) > fabsf(hi) ? fabsf(lo) : fabsf(hi);
+    if (mag > 0.0f) frexpf(mag, &em);
+    e = e0 > em - 23 ? e0 : em - 23;
+    if (e < -126) e = -126;
+    for (;;) {
+        float sc = ldexpf(1.0f, e);
+        origin = floorf(lo / sc) * sc;
+        if (origin > lo) origin = origin - sc;
+        if (origin + 255.0f * sc >= hi) break;
+        ++e;
+    }
+}
+
+__device__ inline uint8_t quant_lo(float v, float origin, float sc) {
+    float q = floorf((v - origin) / sc);
+    q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
+    while (q > 0.0f && origin + q * sc > v) q = q - 1.0f;
+    return (uint8_t)q;
+}
+
+__device__ inline uint8_t quant_hi(float v, float origin, float sc) {
+    float q = ceilf((v - origin) / sc);
+    q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
+    while (q < 255.0f && origin + q * sc < v) q = q + 1.0f;
+    return (uint8_t)q;
+}
+
+__global__ __launch_bounds__(kBlock) void k_wide_emit(const WRef* slots, const uint32_t* nslots,
+                                                      uint32_t count, uint32_t level_start,
+                                                      uint32_t next_start, uint32_t tri_start,
+                                                      const uint32_t* int_off,
+                                                      const uint32_t* leaf_off, Node8* out,
+                                                      uint32_t* next_level, const float* tris,
+                                                      const uint32_t* prims, float* tris8,
+                                                      uint32_t* prims8) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t n = nslots[i];
+    const WRef* s = slots + (size_t)i * 8;
+    Node8 w;
+    uint32_t* wz = reinterpret_cast<uint32_t*>(&w);
+    for (int k = 0; k < 24; ++k) wz[k] = 0;
+    w.nchild = (uint8_t)n;
+    w.child_base = next_start + int_off[i];
+    w.tri_base = tri_start + leaf_off[i];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) { lo[a] = fmin_(lo[a], s[k].lo[a]); hi[a] = fmax_(hi[a], s[k].hi[a]); }
+    float origin[3] = {0.0f, 0.0f, 0.0f};
+    int e[3] = {0, 0, 0};
+    if (n > 0)
+        for (int a = 0; a < 3; ++a) quant_frame(lo[a], hi[a], origin[a], e[a]);
+    w.ox = origin[0]; w.oy = origin[1]; w.oz = origin[2];
+    w.ex = (int8_t)e[0]; w.ey = (int8_t)e[1]; w.ez = (int8_t)e[2];
+    uint32_t ni = 0, nl = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            float sc = ldexpf(1.0f, e[a]);
+            w.qlo[a][k] = quant_lo(s[k].lo[a], origin[a], sc);
+            w.qhi[a][k] = quant_hi(s[k].hi[a], origin[a], sc);
+        }
+        if (s[k].c & kLeaf) {
+            uint32_t slot = s[k].c & ~kLeaf;
+            uint32_t dst = w.tri_base + nl;
+            for (int q = 0; q < 9; ++q) tris8[(size_t)dst * 9 + q] = tris[(size_t)slot * 9 + q];
+            prims8[dst] = prims[slot];
+            w.meta[k] = (uint8_t)nl++;
+        } else {
+            next_level[int_off[i] + ni] = s[k].c;
+            w.meta[k] = (uint8_t)(0x80u | ni++);
+        }
+    }
+    out[level_start + i] = w;
+}
+
 // ---------------------------------------------------------------- TLAS
 constexpr int kTlasThreads = 1024;
 
 __global__ __launch_bounds__(kTlasThreads) void k_tlas_build(const InstIn* in, uint32_t n,
                                                              float* wbox, uint32_t* leaf_parent,
                                                              uint32_t* counters, Node* nodes,
-                                                             InstRec* recs, InstAux* aux,
-                                                             float* w2o_out, uint32_t* order_out,
-                                                             uint32_t m, uint32_t* error) {
+                                                             InstRec* recs, InstRec* recs_w,
+                                                             InstAux* aux, float* w2o_out,
+                                                             uint32_t* order_out, uint32_t m,
+                                                             uint32_t* error) {
     // max(m, 1024) 8-byte entries, m a power of two >= n.  The reductions of
     // steps 1-2 use the same memory before the sort keys are written, which
     // keeps the kernel within 64 KB of LDS for 8192 instances.
@@ -545,6 +687,11 @@ __global__ __launch_bounds__(kTlasThreads) void k_tlas_build(const InstIn* in, u
         rec.nodes = src.nodes;
         rec.tris = src.tris;
         recs[s] = rec;
+        if (recs_w) {
+            rec.nodes = reinterpret_cast<const Node*>(src.wnodes);
+            rec.tris = src.wtris;
+            recs_w[s] = rec;
+        }
         InstAux ax;
         for (int k = 0; k < 9; ++k) ax.nmat[k] = src.nmat[k];
         ax.custom_index = src.custom_index;
@@ -706,6 +853,66 @@ luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
     return LUZRT_OK;
 }
 
+luzrt_status blas_wide_device(luzrt_ctx* ctx, luzrt_blas* b) {
+    hipStream_t st = ctx->stream;
+    const uint32_t n = b->tri_count;
+    const size_t cap = n > 1 ? n - 1 : 1;   // wide nodes <= binary internal nodes
+    if (!b->wide) {
+        LUZ_HIP(ctx, hipMalloc((void**)&b->wide, cap * sizeof(Node8)));
+        LUZ_HIP(ctx, hipMalloc((void**)&b->tris8, (size_t)(n ? n : 1) * 36));
+        LUZ_HIP(ctx, hipMalloc((void**)&b->prims8, (size_t)(n ? n : 1) * 4));
+    }
+    // scratch: two level lists, slots, counts and offsets
+    size_t bytes = cap * 4 * 2 + cap * 8 * sizeof(WRef) + cap * 4 * 5 + 1024;
+    uint8_t* scratch = nullptr;
+    LUZ_HIP(ctx, hipMallocAsync((void**)&scratch, bytes, st));
+    uint8_t* p = scratch;
+    auto take = [&](size_t sz) { uint8_t* r = p; p += (sz + 255) & ~(size_t)255; return r; };
+    uint32_t* lvl[2] = {(uint32_t*)take(cap * 4), (uint32_t*)take(cap * 4)};
+    WRef* slots = (WRef*)take(cap * 8 * sizeof(WRef));
+    uint32_t* nslots = (uint32_t*)take(cap * 4);
+    uint32_t* n_int = (uint32_t*)take(cap * 4);
+    uint32_t* n_leaf = (uint32_t*)take(cap * 4);
+    uint32_t* int_off = (uint32_t*)take(cap * 4);
+    uint32_t* leaf_off = (uint32_t*)take(cap * 4);
+    (void)bytes;
+    LUZ_HIP(ctx, hipMemsetAsync(lvl[0], 0, 4, st));   // level 0 = binary root
+    uint32_t count = 1, level_start = 0, tri_start = 0;
+    int cur = 0;
+    while (count > 0) {
+        const uint32_t blocks = cdiv(count, kBlock);
+        k_wide_collapse<<<blocks, kBlock, 0, st>>>(b->nodes, lvl[cur], count, slots, nslots, n_int, n_leaf);
+        LUZ_HIP(ctx, hipMemcpyAsync(int_off, n_int, (size_t)count * 4, hipMemcpyDeviceToDevice, st));
+        LUZ_HIP(ctx, hipMemcpyAsync(leaf_off, n_leaf, (size_t)count * 4, hipMemcpyDeviceToDevice, st));
+        k_scan<<<1, 1024, 0, st>>>(int_off, count);
+        k_scan<<<1, 1024, 0, st>>>(leaf_off, count);
+        uint32_t tail[4];
+        LUZ_HIP(ctx, hipMemcpyAsync(&tail[0], int_off + count - 1, 4, hipMemcpyDeviceToHost, st));
+        LUZ_HIP(ctx, hipMemcpyAsync(&tail[1], n_int + count - 1, 4, hipMemcpyDeviceToHost, st));
+        LUZ_HIP(ctx, hipMemcpyAsync(&tail[2], leaf_off + count - 1, 4, hipMemcpyDeviceToHost, st));
+        LUZ_HIP(ctx, hipMemcpyAsync(&tail[3], n_leaf + count - 1, 4, hipMemcpyDeviceToHost, st));
+        LUZ_HIP(ctx, hipStreamSynchronize(st));
+        const uint32_t next_count = tail[0] + tail[1], leaves = tail[2] + tail[3];
+        const uint32_t next_start = level_start + count;
+        if ((size_t)next_start + next_count > cap || (size_t)tri_start + leaves > (n ? n : 1)) {
+            hipFreeAsync(scratch, st);
+            return fail(ctx, LUZRT_E_DEVICE, "8-wide collapse overflowed its arrays");
+        }
+        k_wide_emit<<<blocks, kBlock, 0, st>>>(slots, nslots, count, level_start, next_start, tri_start,
+                                               int_off, leaf_off, b->wide, lvl[cur ^ 1], b->tris,
+                                               b->prims, b->tris8, b->prims8);
+        LUZ_HIP(ctx, hipGetLastError());
+        level_start = next_start;
+        tri_start += leaves;
+        count = next_count;
+        cur ^= 1;
+    }
+    LUZ_HIP(ctx, hipFreeAsync(scratch, st));
+    b->wide_count = level_start;
+    b->wide_valid = true;
+    return LUZRT_OK;
+}
+
 luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
     uint32_t m = 1;
     while (m < n) m <<= 1;
@@ -716,8 +923,8 @@ luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
     size_t lds = (size_t)(m > 1024 ? m : 1024) * sizeof(uint64_t);
     k_tlas_build<<<1, kTlasThreads, lds, ctx->stream>>>(dev_in, n, wbox, leaf_parent, counters,
-                                                        t->nodes, t->recs, t->aux, t->w2o,
-                                                        t->order, m, error);
+                                                        t->nodes, t->recs, t->wide ? t->recs_w : nullptr,
+                                                        t->aux, t->w2o, t->order, m, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
 }

@@ -54,6 +54,24 @@ struct alignas(16) InstRec {
 };
 static_assert(sizeof(InstRec) == 64, "instance record is 64 bytes");
 
+// 8-wide node of the precision form, 96 bytes: a per-axis power-of-two
+// frame (origin, exponent) and 8-bit child bounds rounded outward, so the
+// decoded child box origin + q * 2^e is exact in FP32 and contains the child.
+// meta[k]: bit 7 set = internal child child_base + (meta & 0x7f), clear =
+// triangle tri_base + meta.  Slots >= nchild are unused.
+struct alignas(16) Node8 {
+    float ox, oy, oz;
+    int8_t ex, ey, ez;
+    uint8_t nchild;
+    uint32_t child_base;
+    uint32_t tri_base;
+    uint8_t meta[8];
+    uint8_t qlo[3][8];
+    uint8_t qhi[3][8];
+    uint32_t pad[4];
+};
+static_assert(sizeof(Node8) == 96, "wide node is 96 bytes");
+
 // Per-instance data used only by the primary-ray producer.
 struct InstAux {
     float nmat[9];            // inverse-transpose of the 3x3 part, row-major

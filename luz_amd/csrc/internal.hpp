@@ -38,6 +38,8 @@ struct luzrt_ctx {
     // traversal launch configuration
     uint32_t groups_per_cu = 8;
     uint32_t refill_threshold = 16;
+    // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
+    int precision = 0;
 };
 
 struct luzrt_blas {
@@ -53,6 +55,12 @@ struct luzrt_blas {
     uint32_t* leaf_parent = nullptr;
     uint32_t* counters = nullptr;   // refit arrival counters (node_count)
     int refs = 0;                   // live TLASes built over this BLAS
+    // precision form: 8-wide quantised nodes, built on demand from the binary tree
+    luz::Node8* wide = nullptr;
+    uint32_t wide_count = 0;
+    float* tris8 = nullptr;          // 9 floats per wide leaf
+    uint32_t* prims8 = nullptr;
+    bool wide_valid = false;
 };
 
 struct luzrt_tlas {
@@ -60,13 +68,15 @@ struct luzrt_tlas {
     uint32_t count = 0;            // instances in the structure (after mask/empty filtering)
     uint32_t node_count = 0;
     luz::Node* nodes = nullptr;
-    luz::InstRec* recs = nullptr;  // per slot
+    luz::InstRec* recs = nullptr;  // per slot, binary BLASes
+    luz::InstRec* recs_w = nullptr;  // per slot, 8-wide BLASes (compact form)
     luz::InstAux* aux = nullptr;   // per slot
     float* w2o = nullptr;          // per slot, 12 floats (inspection copy)
     uint32_t* order = nullptr;     // slot -> input index
     void* input = nullptr;         // uploaded instance inputs
     void* scratch = nullptr;
     std::vector<const luzrt_blas*> referenced;
+    bool wide = false;             // records point to 8-wide BLASes (compact form)
 };
 
 namespace luz {
@@ -91,16 +101,19 @@ struct alignas(16) InstIn {
     uint32_t custom_index;
     uint32_t mask;
     uint32_t input_index;
-    const Node* nodes;
+    const Node* nodes;      // binary BLAS
     const float* tris;
     const float* nrms;
     const uint32_t* prims;
+    const Node8* wnodes;    // 8-wide BLAS (compact form) or null
+    const float* wtris;
 };
 
 // Launchers (each defined next to its kernels).
 luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices,
                                const uint32_t* indices);
 luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices);
+luzrt_status blas_wide_device(luzrt_ctx* ctx, luzrt_blas* b);
 luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n);
 uint32_t* build_error_flag(luzrt_ctx* ctx);
 
@@ -129,6 +142,7 @@ struct TraceLaunch {
     uint8_t* hits;
     uint32_t* counters;    // device Counter array; kCntRays read, kCntFetch zeroed before
     uint32_t* spill;       // per-lane overflow stack
+    bool wide;             // instance records point to 8-wide BLASes
 };
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
 

@@ -248,6 +248,31 @@ luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint
     return LUZRT_OK;
 }
 
+luzrt_status luzrt_set_precision(luzrt_ctx* ctx, int mode) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (mode != LUZRT_PRECISION_PLAIN && mode != LUZRT_PRECISION_COMPACT)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "unknown precision mode %d", mode);
+    ctx->precision = mode;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_blas_wide_arrays(luzrt_ctx* ctx, luzrt_blas* b, const void** nodes,
+                                    uint32_t* node_count, const float** tris,
+                                    const uint32_t** prims, uint32_t* tri_count) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!b) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_blas_wide_arrays: null BLAS");
+    if (b->tri_count > 0 && !b->wide_valid) {
+        luzrt_status s = blas_wide_device(ctx, b);
+        if (s != LUZRT_OK) return s;
+    }
+    if (nodes) *nodes = b->wide;
+    if (node_count) *node_count = b->tri_count > 0 ? b->wide_count : 0;
+    if (tris) *tris = b->tris8;
+    if (prims) *prims = b->prims8;
+    if (tri_count) *tri_count = b->tri_count;
+    return LUZRT_OK;
+}
+
 luzrt_status luzrt_malloc(luzrt_ctx* ctx, size_t bytes, void** p) {
     if (!ctx || !p) return LUZRT_E_INVALID_ARG;
     LUZ_HIP(ctx, hipMalloc(p, bytes ? bytes : 1));
@@ -335,6 +360,7 @@ luzrt_status luzrt_blas_refit(luzrt_ctx* ctx, luzrt_blas* b, const void* vertice
     stage_begin(ctx, kStageBuild);
     luzrt_status s = blas_refit_device(ctx, b, vertices);
     stage_end(ctx, kStageBuild);
+    b->wide_valid = false;  // the 8-wide form is rebuilt from the refitted tree on next use
     return s;  // a malformed-tree flag (never set by a refit of a built BLAS) surfaces in luzrt_get_timings
 }
 
@@ -344,7 +370,8 @@ luzrt_status luzrt_blas_destroy(luzrt_ctx* ctx, luzrt_blas* b) {
     if (b->refs > 0)
         return fail(ctx, LUZRT_E_IN_USE, "BLAS is referenced by %d TLAS(es)", b->refs);
     hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {b->nodes, b->tris, b->nrms, b->prims, b->slot_idx, b->leaf_parent, b->counters};
+    void* ptrs[] = {b->nodes, b->tris, b->nrms, b->prims, b->slot_idx, b->leaf_parent, b->counters,
+                    b->wide, b->tris8, b->prims8};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete b;
@@ -417,12 +444,24 @@ luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instanc
         r.tris = in.blas->tris;
         r.nrms = in.blas->nrms;
         r.prims = in.blas->prims;
+        if (ctx->precision == LUZRT_PRECISION_COMPACT && in.blas->tri_count > 0) {
+            luzrt_blas* b = const_cast<luzrt_blas*>(in.blas);
+            if (!b->wide_valid) {
+                luzrt_status ws = blas_wide_device(ctx, b);
+                if (ws != LUZRT_OK) return ws;
+            }
+            r.wnodes = b->wide;
+            r.wtris = b->tris8;
+        }
         refs.push_back(in.blas);
         // empty meshes and mask-0 instances can never be hit by a 0xFF ray mask
         if (in.blas->tri_count == 0 || (in.mask & 0xFFu) == 0) continue;
         host.push_back(r);
     }
     LUZ_HIP(ctx, hipSetDevice(ctx->device));
+    t->wide = ctx->precision == LUZRT_PRECISION_COMPACT;
+    if (t->wide && !t->recs_w)
+        LUZ_HIP(ctx, hipMalloc((void**)&t->recs_w, (size_t)t->capacity * sizeof(InstRec)));
     if (!host.empty())
         LUZ_HIP(ctx, hipMemcpyAsync(t->input, host.data(), host.size() * sizeof(InstIn),
                                     hipMemcpyHostToDevice, ctx->stream));
@@ -445,7 +484,7 @@ luzrt_status luzrt_tlas_destroy(luzrt_ctx* ctx, luzrt_tlas* t) {
     if (!t) return LUZRT_OK;
     hipStreamSynchronize(ctx->stream);
     for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs--;
-    void* ptrs[] = {t->nodes, t->recs, t->aux, t->w2o, t->order, t->input, t->scratch};
+    void* ptrs[] = {t->nodes, t->recs, t->recs_w, t->aux, t->w2o, t->order, t->input, t->scratch};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete t;

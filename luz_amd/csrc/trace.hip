@@ -219,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const float* position, const 
 // chunk): a single counter hit by every refill serialises across the 8 XCDs.
 constexpr uint32_t kChunk = 2048;
 
+template <bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t refill) {
     __shared__ uint32_t lds[kShort * kBlock];
     const uint32_t tid = threadIdx.x;
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
         }
         if (ray != kNone) {
             for (;;) {
-                int res = ray_step<false>(r, st, L.tlas_nodes, L.recs, nullptr, nullptr);
+                int res = ray_step<false, WIDE>(r, st, L.tlas_nodes, L.recs, nullptr, nullptr);
                 if (res != kStepContinue) {
                     L.hits[ray] = res == kStepHit ? 1 : 0;
                     if (res == kStepOverflow) atomicOr(&L.counters[kCntError], 1u);
@@ -322,7 +323,10 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
     TraceLaunch l = L;
     l.spill = (uint32_t*)ctx->spill.ptr;
     uint32_t refill = ctx->refill_threshold ? ctx->refill_threshold : 64u;
-    k_trace_any<<<groups, kBlock, 0, ctx->stream>>>(l, refill);
+    if (L.wide)
+        k_trace_any<true><<<groups, kBlock, 0, ctx->stream>>>(l, refill);
+    else
+        k_trace_any<false><<<groups, kBlock, 0, ctx->stream>>>(l, refill);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
 }
@@ -378,12 +382,13 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
     if (ds.r_total > 0 && tlas->count > 0) {
         TraceLaunch L{};
         L.tlas_nodes = tlas->nodes;
-        L.recs = tlas->recs;
+        L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
         L.ray_o = (const float4*)ctx->rays_o.ptr;
         L.ray_d = (const float4*)ctx->rays_d.ptr;
         L.ray_stride = 1;
         L.hits = (uint8_t*)ctx->hits.ptr;
         L.counters = counters;
+        L.wide = tlas->wide;
         if ((s = launch_trace(ctx, L)) != LUZRT_OK) return s;
     } else if (ds.r_total > 0) {
         // empty TLAS: nothing can be hit
@@ -422,12 +427,13 @@ extern "C" luzrt_status luzrt_trace_rays(luzrt_ctx* ctx, const luzrt_tlas* tlas,
     } else {
         TraceLaunch L{};
         L.tlas_nodes = tlas->nodes;
-        L.recs = tlas->recs;
+        L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
         L.ray_o = (const float4*)rays;
         L.ray_d = (const float4*)(rays + 4);
         L.ray_stride = 2;
         L.hits = hits;
         L.counters = counters;
+        L.wide = tlas->wide;
         if ((s = launch_trace(ctx, L)) != LUZRT_OK) return s;
     }
     stage_end(ctx, kStageTrace);

@@ -101,7 +101,54 @@ __device__ inline bool better_hit(float t, uint32_t inst_slot, uint32_t tri_slot
     return aux[inst_slot].prims[tri_slot] < aux[b.inst_slot].prims[b.tri_slot];
 }
 
-template <bool CLOSEST>
+__device__ inline uint32_t byte_of(uint32_t lo, uint32_t hi, int k) {
+    return ((k < 4 ? lo : hi) >> (8 * (k & 3))) & 0xFFu;
+}
+
+// Visit of an 8-wide quantised BLAS node (any-hit): 8 slab tests on the
+// decoded child boxes, then the hit slots in slot order.
+__device__ inline int wide_visit(RayState& r, Stack& st) {
+    const uint4* p = reinterpret_cast<const uint4*>(reinterpret_cast<const Node8*>(r.bnodes) + r.next);
+    uint4 h0 = p[0], h1 = p[1], qa = p[2], qb = p[3], qc = p[4];
+    const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
+    const int ex = (int)(int8_t)(h0.w & 0xFFu), ey = (int)(int8_t)((h0.w >> 8) & 0xFFu),
+              ez = (int)(int8_t)((h0.w >> 16) & 0xFFu);
+    const int nchild = (int)(h0.w >> 24);
+    const float sx = __int_as_float((ex + 127) << 23), sy = __int_as_float((ey + 127) << 23),
+                sz = __int_as_float((ez + 127) << 23);
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float lx = ox + (float)byte_of(qa.x, qa.y, k) * sx;
+        float ly = oy + (float)byte_of(qa.z, qa.w, k) * sy;
+        float lz = oz + (float)byte_of(qb.x, qb.y, k) * sz;
+        float hx = ox + (float)byte_of(qb.z, qb.w, k) * sx;
+        float hy = oy + (float)byte_of(qc.x, qc.y, k) * sy;
+        float hz = oz + (float)byte_of(qc.z, qc.w, k) * sz;
+        float tn;
+        if (k < nchild && box_hit(r.obr, lx, hx, ly, hy, lz, hz, tn)) mask |= 1u << k;
+    }
+    const uint64_t meta = ((uint64_t)h1.w << 32) | h1.z;
+    r.next = kNone;
+    while (mask) {
+        int k = __ffs(mask) - 1;
+        mask &= mask - 1;
+        uint32_t m = (uint32_t)(meta >> (8 * k)) & 0xFFu;
+        if (m & 0x80u) {
+            uint32_t child = h1.x + (m & 0x7Fu);
+            if (r.next == kNone) r.next = child;
+            else if (!st.push(child)) return kStepOverflow;
+        } else {
+            V3 a, b, c;
+            load_tri(r.btris, h1.y + m, a, b, c);
+            float T, det, U, V, W;
+            if (tri_test(r.tr, r.obr.o, r.tmin, r.tmax, a, b, c, T, det, U, V, W)) return kStepHit;
+        }
+    }
+    return kStepContinue;
+}
+
+template <bool CLOSEST, bool WIDE = false>
 __device__ inline int ray_step(RayState& r, Stack& st, const Node* tlas_nodes, const InstRec* recs,
                                const InstAux* aux, Best* best) {
     if (++r.steps > kMaxSteps) return kStepOverflow;
@@ -133,6 +180,7 @@ __device__ inline int ray_step(RayState& r, Stack& st, const Node* tlas_nodes, c
         r.next = 0;
         return kStepContinue;
     }
+    if (WIDE && r.in_blas) return wide_visit(r, st);
     float4 qx, qy, qz;
     uint4 qc;
     load_node(r.in_blas ? r.bnodes : tlas_nodes, r.next, qx, qy, qz, qc);

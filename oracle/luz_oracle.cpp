@@ -477,6 +477,152 @@ void build_lbvh(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::ve
 
 inline Box root_box(const Node& r) { return box_union(get_child(r, 0), get_child(r, 1)); }
 
+// ---------------------------------------------------------------- S7
+// 8-wide BVH with 8-bit child boxes (the precision form).  Collapse of the
+// binary LBVH: a wide node starts from a binary node's two children and
+// repeatedly replaces its internal child of largest surface area (first on
+// ties) by that child's two children, up to 8 slots.  Wide nodes are numbered
+// breadth first; internal slots point to child_base + rank, leaf slots to
+// triangle tri_base + rank (leaves hold one triangle).  Child boxes are
+// quantised per axis in a power-of-two frame: scale = 2^e, origin a multiple
+// of 2^e, lo rounded down and hi rounded up, so that decode (origin + q*2^e,
+// exact in FP32) always contains the child box.
+struct Node8 {              // 96 bytes; identical layout on the device
+    float ox, oy, oz;
+    int8_t ex, ey, ez;
+    uint8_t nchild;
+    uint32_t child_base;
+    uint32_t tri_base;
+    uint8_t meta[8];        // bit 7: internal child; bits 0-6: rank among internal or leaf slots
+    uint8_t qlo[3][8];
+    uint8_t qhi[3][8];
+    uint32_t pad[4];
+};
+static_assert(sizeof(Node8) == 96, "wide node layout");
+
+struct Ref { uint32_t c; Box b; };
+
+inline float surface_area(const Box& b) {
+    float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+void wide_collapse(const std::vector<Node>& bn, uint32_t node, std::vector<Ref>& slots) {
+    slots.clear();
+    for (int c = 0; c < 2; ++c)
+        if (bn[node].child[c] != kEmpty) slots.push_back(Ref{bn[node].child[c], get_child(bn[node], c)});
+    while (slots.size() < 8) {
+        int best = -1;
+        float best_sa = 0.0f;
+        for (size_t k = 0; k < slots.size(); ++k) {
+            if (slots[k].c & kLeaf) continue;
+            float sa = surface_area(slots[k].b);
+            if (best < 0 || sa > best_sa) { best = (int)k; best_sa = sa; }
+        }
+        if (best < 0) break;
+        uint32_t in = slots[best].c;
+        Ref a{bn[in].child[0], get_child(bn[in], 0)}, b{bn[in].child[1], get_child(bn[in], 1)};
+        slots[best] = a;
+        slots.insert(slots.begin() + best + 1, b);
+    }
+}
+
+void quant_frame(float lo, float hi, float& origin, int& e) {
+    float t = (hi - lo) / 255.0f;
+    int e0 = -126, em = -126;
+    if (t > 0.0f) std::frexp(t, &e0);
+    float mag = std::fabs(lo) > std::fabs(hi) ? std::fabs(lo) : std::fabs(hi);
+    if (mag > 0.0f) std::frexp(mag, &em);
+    e = e0 > em - 23 ? e0 : em - 23;
+    if (e < -126) e = -126;
+    for (;;) {
+        float sc = std::ldexp(1.0f, e);
+        origin = std::floor(lo / sc) * sc;
+        if (origin > lo) origin = origin - sc;
+        if (origin + 255.0f * sc >= hi) break;
+        ++e;
+    }
+}
+
+inline uint8_t quant_lo(float v, float origin, float sc) {
+    float q = std::floor((v - origin) / sc);
+    q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
+    while (q > 0.0f && origin + q * sc > v) q = q - 1.0f;
+    return (uint8_t)q;
+}
+
+inline uint8_t quant_hi(float v, float origin, float sc) {
+    float q = std::ceil((v - origin) / sc);
+    q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
+    while (q < 255.0f && origin + q * sc < v) q = q + 1.0f;
+    return (uint8_t)q;
+}
+
+inline float wide_scale(int8_t e) { return std::ldexp(1.0f, (int)e); }
+
+// Decoded box of slot k (exact FP32).
+inline void wide_box(const Node8& n, int k, float lo[3], float hi[3]) {
+    const float o[3] = {n.ox, n.oy, n.oz};
+    const int8_t ex[3] = {n.ex, n.ey, n.ez};
+    for (int a = 0; a < 3; ++a) {
+        float sc = wide_scale(ex[a]);
+        lo[a] = o[a] + (float)n.qlo[a][k] * sc;
+        hi[a] = o[a] + (float)n.qhi[a][k] * sc;
+    }
+}
+
+void build_wide(const std::vector<Node>& bn, const std::vector<float>& tris, const std::vector<uint32_t>& prims,
+                std::vector<Node8>& out, std::vector<float>& tris8, std::vector<uint32_t>& prims8) {
+    out.clear();
+    tris8.clear();
+    prims8.clear();
+    std::vector<uint32_t> level = {0u};
+    uint32_t level_start = 0;
+    std::vector<Ref> slots;
+    while (!level.empty()) {
+        const uint32_t next_start = level_start + (uint32_t)level.size();
+        std::vector<uint32_t> next;
+        for (uint32_t node : level) {
+            wide_collapse(bn, node, slots);
+            Node8 w;
+            std::memset(&w, 0, sizeof(w));
+            w.nchild = (uint8_t)slots.size();
+            w.child_base = next_start + (uint32_t)next.size();
+            w.tri_base = (uint32_t)prims8.size();
+            Box nb = empty_box();
+            for (auto& r : slots) nb = box_union(nb, r.b);
+            float origin[3];
+            int e[3] = {0, 0, 0};
+            for (int a = 0; a < 3; ++a) {
+                if (slots.empty()) { origin[a] = 0.0f; e[a] = 0; continue; }
+                quant_frame(nb.lo[a], nb.hi[a], origin[a], e[a]);
+            }
+            w.ox = origin[0]; w.oy = origin[1]; w.oz = origin[2];
+            w.ex = (int8_t)e[0]; w.ey = (int8_t)e[1]; w.ez = (int8_t)e[2];
+            uint32_t ni = 0, nl = 0;
+            for (size_t k = 0; k < slots.size(); ++k) {
+                for (int a = 0; a < 3; ++a) {
+                    float sc = std::ldexp(1.0f, e[a]);
+                    w.qlo[a][k] = quant_lo(slots[k].b.lo[a], origin[a], sc);
+                    w.qhi[a][k] = quant_hi(slots[k].b.hi[a], origin[a], sc);
+                }
+                if (slots[k].c & kLeaf) {
+                    uint32_t slot = slots[k].c & ~kLeaf;
+                    w.meta[k] = (uint8_t)nl++;
+                    tris8.insert(tris8.end(), tris.begin() + (size_t)slot * 9, tris.begin() + (size_t)slot * 9 + 9);
+                    prims8.push_back(prims[slot]);
+                } else {
+                    w.meta[k] = (uint8_t)(0x80u | ni++);
+                    next.push_back(slots[k].c);
+                }
+            }
+            out.push_back(w);
+        }
+        level_start = next_start;
+        level.swap(next);
+    }
+}
+
 // ---------------------------------------------------------------- S6
 void invert_3x4(const float m[12], float out[12], double* dout = nullptr) {
     double r00 = m[0], r01 = m[1], r02 = m[2], t0 = m[3];
@@ -528,6 +674,10 @@ struct Blas {
     uint32_t vertex_count = 0;
     uint32_t stride = 48;
     bool has_normals = false;
+    // precision form
+    std::vector<Node8> wide;
+    std::vector<float> tris8;
+    std::vector<uint32_t> prims8;
 };
 
 struct Inst {
@@ -795,6 +945,80 @@ bool brute(const orc_scene& s, f3 wo, f3 wd, float tmin, float tmax, bool closes
         }
     }
     return found;
+}
+
+// Any-hit traversal with the 8-wide quantised BLASes (TLAS stays binary).
+// Hit slots are visited in slot order: leaves are tested at once, the first
+// internal child is visited next and the others are pushed.
+bool traverse_wide(const orc_scene& s, f3 wo, f3 wd, float tmin, float tmax, orc_stats* st) {
+    if (s.tlas_inst.empty()) return false;
+    LevelRay world = level_ray(wo, wd, tmin, tmax);
+    LevelRay obj;
+    const Blas* blas = nullptr;
+    bool in_blas = false;
+    uint32_t stack[512];
+    int sp = 0;
+    uint32_t next = 0;
+    for (;;) {
+        if (next == kNone) {
+            if (sp == 0) break;
+            next = stack[--sp];
+            if (next == kSentinel) { in_blas = false; next = kNone; continue; }
+        }
+        if (!in_blas && (next & kLeaf)) {
+            uint32_t slot = next & ~kLeaf;
+            const Inst& in = s.tlas_inst[s.tlas_order[slot]];
+            if (st) st->inst_visits++;
+            const float* w = &s.tlas_w2o[(size_t)slot * 12];
+            obj = level_ray(xform_point(w, wo), xform_vec(w, wd), tmin, tmax);
+            blas = &s.blas[in.mesh];
+            in_blas = true;
+            stack[sp++] = kSentinel;
+            next = 0;
+            continue;
+        }
+        if (!in_blas) {
+            const Node& nd = s.tlas_nodes[next];
+            if (st) st->node_visits++;
+            float tn[2] = {0.0f, 0.0f};
+            bool h[2];
+            for (int c = 0; c < 2; ++c)
+                h[c] = nd.child[c] != kEmpty &&
+                       box_hit(world.br, nd.lox[c], nd.hix[c], nd.loy[c], nd.hiy[c], nd.loz[c], nd.hiz[c], tn[c]);
+            int first = (h[0] && h[1]) ? (tn[1] < tn[0] ? 1 : 0) : (h[0] ? 0 : 1);
+            next = kNone;
+            for (int k = 0; k < 2; ++k) {
+                int c = k == 0 ? first : 1 - first;
+                if (!h[c]) continue;
+                if (next == kNone) next = nd.child[c];
+                else stack[sp++] = nd.child[c];
+            }
+            continue;
+        }
+        const Node8& n = blas->wide[next];
+        if (st) st->wide_node_visits++;
+        next = kNone;
+        for (int k = 0; k < n.nchild; ++k) {
+            float lo[3], hi[3], tn;
+            wide_box(n, k, lo, hi);
+            if (!box_hit(obj.br, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], tn)) continue;
+            if (n.meta[k] & 0x80u) {
+                uint32_t child = n.child_base + (n.meta[k] & 0x7Fu);
+                if (next == kNone) next = child;
+                else if (sp < 511) stack[sp++] = child;
+                else return true;  // unreachable: depth * 7 < 511
+            } else {
+                const float* v = &blas->tris8[(size_t)(n.tri_base + n.meta[k]) * 9];
+                if (st) st->tri_tests++;
+                float T, det, U, V, W;
+                if (tri_test(obj.tr, obj.o, tmin, tmax, load3(v), load3(v + 3), load3(v + 6), T, det, U, V, W)) {
+                    if (st) st->hits++;
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
 }
 
 // FP64 any-hit over the same BVH (boxes tested in FP64 with a margin).
@@ -1246,9 +1470,9 @@ int orc_primary_gbuffer(const orc_scene* s, const orc_camera* cam, const orc_mat
     return 0;
 }
 
-int orc_trace_visibility(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
-                         uint8_t* shadow, uint8_t* ao, orc_stats* stats, int threads,
-                         uint32_t row_step) {
+static int trace_visibility_impl(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
+                                 uint8_t* shadow, uint8_t* ao, orc_stats* stats, int threads,
+                                 uint32_t row_step, bool wide) {
     if (!s || !g || !p || p->num_lights < 0 || p->num_lights > 64) return -1;
     const uint32_t W = g->width, H = g->height;
     const size_t npix = (size_t)W * H;
@@ -1273,7 +1497,9 @@ int orc_trace_visibility(const orc_scene* s, const orc_gbuffer* g, const orc_par
                     sample2(x, y, (uint32_t)p->frame, (uint32_t)(l * 256 + smp), u, v);
                     f3 d = shadow_dir(ss, u, v);
                     st.rays++;
-                    if (traverse(*s, pr.so, d, 0.001f, ss.tmax, false, nullptr, &st)) cnt++;
+                    bool h = wide ? traverse_wide(*s, pr.so, d, 0.001f, ss.tmax, &st)
+                                  : traverse(*s, pr.so, d, 0.001f, ss.tmax, false, nullptr, &st);
+                    if (h) cnt++;
                 }
                 shadow[(size_t)l * npix + idx] = (uint8_t)cnt;
             }
@@ -1284,7 +1510,9 @@ int orc_trace_visibility(const orc_scene* s, const orc_gbuffer* g, const orc_par
                 sample2(x, y, (uint32_t)p->frame, (uint32_t)(16384 + smp), u, v);
                 f3 d = ao_dir(pr.aoT, pr.aoB, pr.N, u, v);
                 st.rays++;
-                if (!traverse(*s, pr.ao_o, d, p->ao_min, p->ao_max, false, nullptr, &st)) miss++;
+                bool h = wide ? traverse_wide(*s, pr.ao_o, d, p->ao_min, p->ao_max, &st)
+                              : traverse(*s, pr.ao_o, d, p->ao_min, p->ao_max, false, nullptr, &st);
+                if (!h) miss++;
             }
             ao[idx] = (uint8_t)miss;
         }
@@ -1293,10 +1521,54 @@ int orc_trace_visibility(const orc_scene* s, const orc_gbuffer* g, const orc_par
         orc_stats t{};
         for (auto& q : per) {
             t.rays += q.rays; t.node_visits += q.node_visits; t.tri_tests += q.tri_tests;
-            t.inst_visits += q.inst_visits; t.hits += q.hits;
+            t.inst_visits += q.inst_visits; t.hits += q.hits; t.wide_node_visits += q.wide_node_visits;
         }
         *stats = t;
     }
+    return 0;
+}
+
+int orc_trace_visibility(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
+                         uint8_t* shadow, uint8_t* ao, orc_stats* stats, int threads,
+                         uint32_t row_step) {
+    return trace_visibility_impl(s, g, p, shadow, ao, stats, threads, row_step, false);
+}
+
+int orc_trace_visibility_wide(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
+                              uint8_t* shadow, uint8_t* ao, orc_stats* stats, int threads,
+                              uint32_t row_step) {
+    for (const Blas& b : s->blas)
+        if (b.wide.empty()) return -2;
+    return trace_visibility_impl(s, g, p, shadow, ao, stats, threads, row_step, true);
+}
+
+int orc_scene_build_wide(orc_scene* s) {
+    if (!s) return -1;
+    for (Blas& b : s->blas) build_wide(b.nodes, b.tris, b.prims, b.wide, b.tris8, b.prims8);
+    return 0;
+}
+
+int orc_blas_wide_arrays(const orc_scene* s, int mesh, const void** nodes, uint32_t* node_count,
+                         const float** tris, const uint32_t** prim_ids, uint32_t* tri_count) {
+    if (!s || mesh < 0 || mesh >= (int)s->blas.size()) return -1;
+    const Blas& b = s->blas[mesh];
+    *nodes = b.wide.data();
+    *node_count = (uint32_t)b.wide.size();
+    *tris = b.tris8.data();
+    *prim_ids = b.prims8.data();
+    *tri_count = (uint32_t)b.prims8.size();
+    return 0;
+}
+
+int orc_any_hit_wide(const orc_scene* s, const float* rays, uint32_t n, uint8_t* hits, orc_stats* stats) {
+    if (!s) return -1;
+    orc_stats st{};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        st.rays++;
+        hits[i] = traverse_wide(*s, mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]), r[3], r[7], &st) ? 1 : 0;
+    }
+    if (stats) *stats = st;
     return 0;
 }
 

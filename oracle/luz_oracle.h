@@ -69,6 +69,7 @@ typedef struct orc_material {  /* per custom index, RGBA8 each */
 
 typedef struct orc_stats {
     uint64_t rays, node_visits, tri_tests, inst_visits, hits;
+    uint64_t wide_node_visits;   /* 96-byte nodes of the 8-wide BLASes */
 } orc_stats;
 
 /* --- primitives exposed for known-answer tests ------------------------- */
@@ -124,6 +125,16 @@ int  orc_trace_visibility(const orc_scene* s, const orc_gbuffer* g, const orc_pa
 /* Same rays in FP64 (ray set-up, sampling angles and intersection). */
 int  orc_trace_visibility_f64(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
                               uint8_t* shadow, uint8_t* ao, int threads, uint32_t row_step);
+/* Precision form: 8-wide BLASes with 8-bit quantised child boxes, built from
+ * each mesh's binary LBVH; visibility traced through them (TLAS binary). */
+int  orc_scene_build_wide(orc_scene* s);
+int  orc_blas_wide_arrays(const orc_scene* s, int mesh, const void** nodes, uint32_t* node_count,
+                          const float** tris, const uint32_t** prim_ids, uint32_t* tri_count);
+int  orc_any_hit_wide(const orc_scene* s, const float* rays, uint32_t n, uint8_t* hits,
+                      orc_stats* stats);
+int  orc_trace_visibility_wide(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
+                               uint8_t* shadow, uint8_t* ao, orc_stats* stats, int threads,
+                               uint32_t row_step);
 /* Emit the ray list the visibility pass casts (for brute-force checks). */
 int64_t orc_visibility_rays(const orc_gbuffer* g, const orc_params* p, float* rays,
                             int64_t capacity);

@@ -56,7 +56,7 @@ class OrcMaterial(C.Structure):
 
 class OrcStats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
-                ("inst_visits", C.c_uint64), ("hits", C.c_uint64)]
+                ("inst_visits", C.c_uint64), ("hits", C.c_uint64), ("wide_node_visits", C.c_uint64)]
 
 
 _P = C.c_void_p
@@ -87,6 +87,12 @@ _sigs = {
                                        C.POINTER(OrcStats), C.c_int, C.c_uint32]),
     "orc_trace_visibility_f64": (C.c_int, [_P, C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P,
                                            C.c_int, C.c_uint32]),
+    "orc_scene_build_wide": (C.c_int, [_P]),
+    "orc_blas_wide_arrays": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
+                                       C.POINTER(_P), C.POINTER(C.c_uint32)]),
+    "orc_any_hit_wide": (C.c_int, [_P, _P, C.c_uint32, _P, C.POINTER(OrcStats)]),
+    "orc_trace_visibility_wide": (C.c_int, [_P, C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P,
+                                            C.POINTER(OrcStats), C.c_int, C.c_uint32]),
     "orc_visibility_rays": (C.c_int64, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, C.c_int64]),
     "orc_shade": (C.c_int, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P, _P, C.c_int]),
     "orc_shade_f64": (C.c_int, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P, _P, C.c_int]),
@@ -230,6 +236,30 @@ class OracleScene:
             w = np.ctypeslib.as_array(C.cast(w2o.value, C.POINTER(C.c_float)), shape=(k, 12)).copy()
         return n, w, o
 
+    def build_wide(self) -> None:
+        if lib.orc_scene_build_wide(self.h) != 0:
+            raise ValueError("orc_scene_build_wide failed")
+
+    def blas_wide_arrays(self, mesh: int):
+        """(nodes u8[n,96], tris f32[t,9], prim ids u32[t]) of the 8-wide BLAS."""
+        nodes, nn, tris, prims, nt = C.c_void_p(), C.c_uint32(), C.c_void_p(), C.c_void_p(), C.c_uint32()
+        lib.orc_blas_wide_arrays(self.h, mesh, C.byref(nodes), C.byref(nn), C.byref(tris), C.byref(prims),
+                                 C.byref(nt))
+        n = np.ctypeslib.as_array(C.cast(nodes.value, C.POINTER(C.c_uint8)), shape=(nn.value, 96)).copy()
+        t = np.zeros((nt.value, 9), np.float32)
+        p = np.zeros(nt.value, np.uint32)
+        if nt.value:
+            t = np.ctypeslib.as_array(C.cast(tris.value, C.POINTER(C.c_float)), shape=(nt.value, 9)).copy()
+            p = np.ctypeslib.as_array(C.cast(prims.value, C.POINTER(C.c_uint32)), shape=(nt.value,)).copy()
+        return n, t, p
+
+    def any_hit_wide(self, rays: np.ndarray):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros(len(r), np.uint8)
+        st = OrcStats()
+        lib.orc_any_hit_wide(self.h, _p(r), len(r), _p(hits), C.byref(st))
+        return hits, st
+
     def any_hit(self, rays: np.ndarray, brute: bool = False):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         hits = np.zeros(len(r), np.uint8)
@@ -261,7 +291,7 @@ class OracleScene:
         return out
 
     def trace_visibility(self, g: dict, width: int, height: int, params, threads: int = 0, f64: bool = False,
-                         row_step: int = 1):
+                         row_step: int = 1, wide: bool = False):
         """Shadow counts [lights][W*H], AO counts [W*H], stats.  With row_step > 1 only
         rows 0, row_step, ... are traced; the other rows stay zero."""
         gb, keep = gbuffer_struct(g, width, height)
@@ -273,6 +303,9 @@ class OracleScene:
         if f64:
             r = lib.orc_trace_visibility_f64(self.h, C.byref(gb), C.byref(p), _p(shadow), _p(ao),
                                              threads or threads_default(), row_step)
+        elif wide:
+            r = lib.orc_trace_visibility_wide(self.h, C.byref(gb), C.byref(p), _p(shadow), _p(ao), C.byref(st),
+                                              threads or threads_default(), row_step)
         else:
             r = lib.orc_trace_visibility(self.h, C.byref(gb), C.byref(p), _p(shadow), _p(ao), C.byref(st),
                                          threads or threads_default(), row_step)
