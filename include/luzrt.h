@@ -154,8 +154,10 @@ luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint
 
 /* Precision modes.  PLAIN: binary BVH with FP32 boxes, FP32 shading.
  * COMPACT: BLASes traversed as 8-wide nodes with 8-bit child boxes rounded
- * outward (96-byte nodes), FP32 watertight triangle tests, GGX shading in
- * packed FP16 with FP32 accumulation.  Takes effect at the next TLAS build. */
+ * outward (96-byte nodes), FP32 watertight triangle tests, GGX shading with
+ * the Fresnel and diffuse terms in packed FP16 (two lights per half2), the
+ * specular scalar D*G/denominator in FP32, FP32 accumulation.  Takes effect
+ * at the next TLAS build. */
 #define LUZRT_PRECISION_PLAIN 0
 #define LUZRT_PRECISION_COMPACT 1
 luzrt_status luzrt_set_precision(luzrt_ctx* ctx, int mode);

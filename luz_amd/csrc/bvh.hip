@@ -790,8 +790,9 @@ luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
     }
     const uint32_t nb = cdiv(n, kBlock);
     const uint32_t sb = cdiv(n, kSortTile);
+    // 10 sub-allocations, each rounded up to 256 bytes
     size_t bytes = (size_t)n * 24 + (size_t)nb * 48 + 64 + (size_t)n * 16 + (size_t)256 * sb * 4 +
-                   (size_t)n * 24 + 1024;
+                   (size_t)n * 24 + 10 * 256;
     uint8_t* scratch = nullptr;
     LUZ_HIP(ctx, hipMallocAsync((void**)&scratch, bytes, st));
     uint8_t* p = scratch;
@@ -863,7 +864,8 @@ luzrt_status blas_wide_device(luzrt_ctx* ctx, luzrt_blas* b) {
         LUZ_HIP(ctx, hipMalloc((void**)&b->prims8, (size_t)(n ? n : 1) * 4));
     }
     // scratch: two level lists, slots, counts and offsets
-    size_t bytes = cap * 4 * 2 + cap * 8 * sizeof(WRef) + cap * 4 * 5 + 1024;
+    // 8 sub-allocations, each rounded up to 256 bytes
+    size_t bytes = cap * 4 * 2 + cap * 8 * sizeof(WRef) + cap * 4 * 5 + 8 * 256;
     uint8_t* scratch = nullptr;
     LUZ_HIP(ctx, hipMallocAsync((void**)&scratch, bytes, st));
     uint8_t* p = scratch;

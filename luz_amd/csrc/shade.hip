@@ -102,18 +102,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(const luzrt_gbuffer g, const D
     out[p] = make_float4(o[0], o[1], o[2], 1.0f);
 }
 
-// Compact form of the lighting kernel: per-light geometry and the GGX
-// distribution term stay FP32 (its range reaches 1/(pi a^2)); the masking,
-// Fresnel, specular and diffuse terms of two lights at a time run in packed
-// FP16 (one half2 lane per light) and each light's contribution is
-// accumulated in FP32 after the FP32 radiance scale.  Roughness is floored at
-// 0.089 (the FP16 minimum used by Filament) so every FP16 term stays in range.
+// Compact form of the lighting kernel: per-light geometry and the specular
+// scalar D*G/(4 NdotV NdotL + 1e-4) stay FP32 (the GGX distribution spans
+// 1/(pi a^2), beyond FP16 range for smooth surfaces); the Fresnel and diffuse
+// colour terms of two lights at a time run in packed FP16 (one half2 lane
+// per light), and every light's contribution is accumulated in FP32.
 struct LightGeo {
-    float D, NdotL, HV, scale;
+    float spec, NdotL, HV, scale;
 };
 
-__device__ inline LightGeo light_geo(const luzrt_light& Lb, V3 P, V3 N, V3 V, float a2,
-                                     float sh) {
+__device__ inline LightGeo light_geo(const luzrt_light& Lb, V3 P, V3 N, V3 V, float a2, float kk,
+                                     float NdotV, float gv, float sh) {
     LightGeo g;
     V3 Lr = vsub(v3(Lb.position[0], Lb.position[1], Lb.position[2]), P);
     V3 L = vnorm(Lr);
@@ -134,8 +133,10 @@ __device__ inline LightGeo light_geo(const luzrt_light& Lb, V3 P, V3 N, V3 V, fl
     V3 H = vnorm(vadd(V, L));
     float NdotH = fmax_(vdot(N, H), 0.0f);
     float den = NdotH * NdotH * (a2 - 1.0f) + 1.0f;
-    g.D = a2 / (kPi * den * den);
+    float D = a2 / (kPi * den * den);
     g.NdotL = fmax_(vdot(N, L), 0.0f);
+    float G = (g.NdotL / (g.NdotL * (1.0f - kk) + kk)) * gv;
+    g.spec = D * G / (4.0f * NdotV * g.NdotL + 0.0001f);
     g.HV = clamp01(vdot(H, V));
     g.scale = Lb.intensity * att * (1.0f - sh);
     return g;
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_compact(const luzrt_gbuffer g,
     }
     uint32_t alb = g.albedo[p], mat = g.material[p], emi = g.emission[p];
     float albedo[3] = {s_lut[alb & 0xFFu], s_lut[(alb >> 8) & 0xFFu], s_lut[(alb >> 16) & 0xFFu]};
-    float rough = fmax_((float)(mat & 0xFFu) / 255.0f, 0.089f);
+    float rough = (float)(mat & 0xFFu) / 255.0f;
     float metal = (float)((mat >> 8) & 0xFFu) / 255.0f;
     float occl = (float)((mat >> 16) & 0xFFu) / 255.0f;
     V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
@@ -168,16 +169,13 @@ __global__ __launch_bounds__(kBlock) void k_shade_compact(const luzrt_gbuffer g,
     float a = rough * rough, a2 = a * a;
     float rr = rough + 1.0f, kk = (rr * rr) / 8.0f;
     float NdotV = fmax_(vdot(N, V), 0.0f);
+    float gv = NdotV / (NdotV * (1.0f - kk) + kk);
     const __half2 one = __float2half2_rn(1.0f);
-    const __half2 hk = __float2half2_rn(kk), h1k = __float2half2_rn(1.0f - kk);
     const __half2 hmet = __float2half2_rn(1.0f - metal);
-    const __half2 hinvpi = __float2half2_rn(1.0f / kPi);
-    const __half2 hNdotV = __float2half2_rn(NdotV);
-    const __half2 gv = __h2div(hNdotV, __hadd2(__hmul2(hNdotV, h1k), hk));
     __half2 F0[3], alb_pi[3];
     for (int c = 0; c < 3; ++c) {
         F0[c] = __float2half2_rn(0.04f * (1.0f - metal) + albedo[c] * metal);
-        alb_pi[c] = __hmul2(__float2half2_rn(albedo[c]), hinvpi);
+        alb_pi[c] = __float2half2_rn(albedo[c] / kPi);
     }
     float Lo[3] = {0.0f, 0.0f, 0.0f};
     const int nl = S.num_lights;
@@ -185,32 +183,23 @@ __global__ __launch_bounds__(kBlock) void k_shade_compact(const luzrt_gbuffer g,
         const bool pair = l + 1 < nl;
         int n0 = S.lights[l].num_shadow_samples;
         float sh0 = n0 > 0 ? (float)shadow[(size_t)l * npix + p] / (float)n0 : 0.0f;
-        LightGeo g0 = light_geo(S.lights[l], P, N, V, a2, sh0);
+        LightGeo g0 = light_geo(S.lights[l], P, N, V, a2, kk, NdotV, gv, sh0);
         LightGeo g1 = g0;
-        g1.scale = 0.0f;
         if (pair) {
             int n1 = S.lights[l + 1].num_shadow_samples;
             float sh1 = n1 > 0 ? (float)shadow[(size_t)(l + 1) * npix + p] / (float)n1 : 0.0f;
-            g1 = light_geo(S.lights[l + 1], P, N, V, a2, sh1);
+            g1 = light_geo(S.lights[l + 1], P, N, V, a2, kk, NdotV, gv, sh1);
         }
-        // packed FP16: lane 0 = light l, lane 1 = light l+1
-        __half2 NdotL = __floats2half2_rn(g0.NdotL, g1.NdotL);
-        __half2 D = __floats2half2_rn(g0.D, g1.D);
-        __half2 G = __hmul2(__h2div(NdotL, __hadd2(__hmul2(NdotL, h1k), hk)), gv);
+        // packed FP16, lane x = light l, lane y = light l+1
         __half2 om = __hsub2(one, __floats2half2_rn(g0.HV, g1.HV));
         __half2 om2 = __hmul2(om, om);
         __half2 om5 = __hmul2(__hmul2(om2, om2), om);
-        __half2 denom = __hadd2(__hmul2(__hmul2(__float2half2_rn(4.0f), hNdotV), NdotL),
-                                __float2half2_rn(0.0001f));
-        __half2 DG = __hmul2(D, G);
         for (int c = 0; c < 3; ++c) {
             __half2 F = __hadd2(F0[c], __hmul2(__hsub2(one, F0[c]), om5));
-            __half2 spec = __h2div(__hmul2(DG, F), denom);
-            __half2 kD = __hmul2(__hsub2(one, F), hmet);
-            __half2 brdf = __hadd2(__hmul2(kD, alb_pi[c]), spec);
-            float2 b = __half22float2(brdf);
-            Lo[c] += b.x * (S.lights[l].color[c] * g0.scale) * g0.NdotL;
-            if (pair) Lo[c] += b.y * (S.lights[l + 1].color[c] * g1.scale) * g1.NdotL;
+            __half2 kd = __hmul2(__hmul2(__hsub2(one, F), hmet), alb_pi[c]);
+            float2 f = __half22float2(F), d = __half22float2(kd);
+            Lo[c] += (d.x + g0.spec * f.x) * (S.lights[l].color[c] * g0.scale) * g0.NdotL;
+            if (pair) Lo[c] += (d.y + g1.spec * f.y) * (S.lights[l + 1].color[c] * g1.scale) * g1.NdotL;
         }
     }
     int na = S.ao_num_samples;
