@@ -37,9 +37,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-row-step", type=int, default=8, help="oracle traces every k-th row")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
-    ap.add_argument("--workload", default="spheres1080p", choices=["spheres1080p", "terrain4k"],
+    ap.add_argument("--workload", default="spheres1080p", choices=["spheres1080p", "terrain4k", "batch64"],
                     help="spheres1080p: the headline 1080p sphere field (plain FP32 path); "
-                         "terrain4k: 3840x2160 terrain + spheres, ~5.2M tris, precision form")
+                         "terrain4k: 3840x2160 terrain + spheres, ~5.2M tris, precision form; "
+                         "batch64: 64 animated scenes of uneven size and resolution per frame")
+    ap.add_argument("--scenes", type=int, default=64, help="batch64: scenes per frame")
     ap.add_argument("--autotune", action="store_true",
                     help="search the traversal launch shape on this workload before timing")
     ap.add_argument("--groups-per-cu", type=int, default=0)
@@ -63,6 +65,8 @@ def main() -> int:
     ctx = api.Context(0)
     if args.groups_per_cu or args.refill:
         ctx.set_trace_config(args.groups_per_cu, args.refill)
+    if args.workload == "batch64":
+        return bench_batch(ctx, args)
     compact = args.workload == "terrain4k"
     if compact:
         ctx.set_precision(api.PRECISION_COMPACT)
@@ -134,8 +138,8 @@ def main() -> int:
         },
     }
     if tuned is not None:
-        record["config"]["autotune"] = {"groups_per_cu": tuned["groups_per_cu"], "refill": tuned["refill"],
-                                        "table": tuned["table"]}
+        record["config"]["autotune"] = {k: tuned[k] for k in
+                                        ("groups_per_cu", "refill", "stack_entries", "tile_width", "table")}
 
     traffic = None
     tpath = Path(args.traffic)
@@ -216,6 +220,85 @@ def main() -> int:
         }
     print(json.dumps(record), flush=True)
     gs.close()
+    ctx.close()
+    return 0
+
+
+def bench_batch(ctx, args) -> int:
+    """Batched serving frames: per-frame p50/p99 latency and aggregate Mrays/s."""
+    import numpy as np
+
+    from luz_amd import autotune
+    from luz_amd.batching import BatchWorkload
+
+    t0 = time.perf_counter()
+    wl = BatchWorkload(ctx, list(range(args.scenes)))
+    setup_s = time.perf_counter() - t0
+    print(f"batch: {args.scenes} scenes, {wl.pixels} px, {wl.triangles} instanced tris, "
+          f"set-up {setup_s:.1f} s", file=sys.stderr, flush=True)
+    tuned = None
+    if args.autotune:
+        tuned = autotune.tune_batch(ctx, wl)
+        print(f"batch autotune: {({k: tuned[k] for k in autotune.KEYS})} {tuned['ms']:.2f} ms",
+              file=sys.stderr, flush=True)
+    res = wl.run(args.steps, warmup=args.warmup, first=1000)
+    # one replayed frame checked scene by scene against the oracle on the smallest scenes
+    parity = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        frame = 1000 + args.warmup + args.steps
+        wl.frame(frame, True)
+        order = sorted(range(args.scenes), key=lambda i: wl.gbufs[i].width * wl.gbufs[i].height)[:4]
+        mism = 0
+        checked = 0
+        for i in order:
+            h = wl.hosts[i]
+            gbuf, sh, ao, _ = wl.outputs(i)
+            sb = h.pack()
+            sb.frame = frame
+            o = orc.OracleScene()
+            for m in range(h.mesh_count):
+                v, idx = h.mesh(m)
+                o.add_mesh(v, idx)
+            o2w, mesh_ids, _ = h.nodes()
+            o.set_instances(o2w, mesh_ids)
+            W, H, _ = h.camera()
+            wsh, wao, _ = o.trace_visibility(gbuf, W, H, sb, threads=min(orc.threads_default(), 16))
+            mism += int((sh != wsh).sum() + (ao != wao).sum())
+            checked += W * H
+        parity = {"scenes_checked": len(order), "pixels_checked": checked, "count_mismatch": mism}
+    res_set = sorted({(g.width, g.height) for g in wl.gbufs})
+    record = {
+        "metric": "batched frames: per-frame p50/p99 latency and aggregate shadow+AO Mrays/s",
+        "value": round(res["mrays_per_s"], 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["mean_ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.scenes} seeded sphere-field scenes, tris log-uniform in [1e3, 5e5], "
+                         "resolutions from {320x240, 640x480, 1280x720, 1920x1080}; per frame: rigid "
+                         "animation, BLAS refit, TLAS rebuild, primary G-buffer, 5 shadow + 8 AO rays/px, "
+                         "GGX shading; one hipGraph replay per frame"),
+            "scenes": args.scenes, "pixels": wl.pixels, "instanced_tris": wl.triangles,
+            "resolutions": [f"{w}x{h}" for w, h in res_set],
+            "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+            "rays_per_frame": int(res["rays_per_frame"]),
+        },
+    }
+    if tuned is not None:
+        record["config"]["autotune"] = {k: tuned[k] for k in autotune.KEYS}
+        record["config"]["autotune"]["table"] = tuned["table"]
+    if parity is not None:
+        record["parity"] = parity
+    print(json.dumps(record), flush=True)
+    wl.close()
     ctx.close()
     return 0
 

@@ -42,6 +42,7 @@ extern "C" {
 #define LUZRT_MAX_LIGHTS 64       /* LuzCommon.h:20 LUZ_MAX_LIGHTS */
 #define LUZRT_MAX_INSTANCES 8192  /* LuzCommon.h:21 LUZ_MAX_MODELS */
 #define LUZRT_MAX_SAMPLES 255     /* per-pixel counts are stored in 8 bits */
+#define LUZRT_MAX_BATCH 256       /* scenes per batched frame */
 
 #define LUZRT_LIGHT_POINT 0       /* LuzCommon.h:24-26 */
 #define LUZRT_LIGHT_SPOT 1
@@ -148,9 +149,12 @@ luzrt_status luzrt_sync(luzrt_ctx* ctx);
 /* Per-stage GPU times of the last build/trace/light/primary call (synchronises). */
 luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
 /* Traversal launch parameters (0 keeps the current value): persistent
- * workgroups per CU, and the number of idle lanes of a wave that triggers
- * a refill.  Used by the autotuner. */
-luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill_threshold);
+ * workgroups per CU (wave occupancy), idle lanes of a wave that trigger a
+ * refill, LDS stack entries per lane (8, 16 or 32; deeper entries spill to
+ * global memory) and the width of the 64-pixel ray tiles (8, 16 or 32).
+ * Chosen by the autotuner (luz_amd/autotune.py). */
+luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill_threshold,
+                                    uint32_t stack_entries, uint32_t tile_width);
 
 /* Precision modes.  PLAIN: binary BVH with FP32 boxes, FP32 shading.
  * COMPACT: BLASes traversed as 8-wide nodes with 8-bit child boxes rounded
@@ -230,6 +234,49 @@ luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzr
 luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* tlas,
                                    const luzrt_camera* camera, const luzrt_material* materials,
                                    uint32_t material_count, const luzrt_gbuffer_out* out);
+
+/* ------------------------------------------------------------ batched frames */
+/* Many independent scenes of uneven size and resolution rendered as one
+ * frame (no counterpart in the reference, which renders one scene per
+ * frame; each scene's result equals luzrt_primary_gbuffer + luzrt_light_pass
+ * of that scene bit for bit).  Per frame: the BLASes registered with
+ * luzrt_batch_add_refit are refit, every scene's TLAS is rebuilt from the
+ * instances staged by luzrt_batch_set_instances (one launch for all TLASes),
+ * G-buffers of scenes with a camera are rendered by primary rays, the shadow
+ * and AO rays of ALL scenes are traced by one persistent traversal launch,
+ * and every scene is shaded.  With use_graph != 0 the frame is captured once
+ * into a hipGraph and replayed; it is recaptured when staged instance counts
+ * or the trace configuration change.  Plain precision only.  The TLASes,
+ * BLASes and buffers named here belong to the batch's frames until
+ * luzrt_batch_destroy; the batch does not take ownership of them. */
+typedef struct luzrt_batch luzrt_batch;
+
+typedef struct luzrt_batch_scene {
+    luzrt_tlas* tlas;                  /* rebuilt every frame from the staged instances */
+    luzrt_gbuffer gbuffer;             /* device planes (written each frame when camera != NULL) */
+    luzrt_scene scene;                 /* scene.frame + the frame index seeds the samples */
+    const luzrt_camera* camera;        /* NULL: the caller supplies the G-buffer */
+    const luzrt_material* materials;   /* host table indexed by instance custom_index */
+    uint32_t material_count;
+    uint8_t* shadow_counts;            /* device, [num_lights][width*height] */
+    uint8_t* ao_counts;                /* device, [width*height] */
+    float* rgba32f_out;                /* device, width*height*4 */
+} luzrt_batch_scene;
+
+luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes, uint32_t count,
+                                luzrt_batch** out);
+/* Refit `blas` from `vertices` (device, the build's vertex stride) in every frame. */
+luzrt_status luzrt_batch_add_refit(luzrt_ctx* ctx, luzrt_batch* batch, luzrt_blas* blas,
+                                   const void* vertices);
+/* Stage the instances of scene `scene` for the next frame (host array; waits
+ * for the previous frame, whose staging copy reads the same memory). */
+luzrt_status luzrt_batch_set_instances(luzrt_ctx* ctx, luzrt_batch* batch, uint32_t scene,
+                                       const luzrt_instance* instances, uint32_t count);
+/* Render one frame; asynchronous like the other calls. */
+luzrt_status luzrt_batch_frame(luzrt_ctx* ctx, luzrt_batch* batch, uint32_t frame, int use_graph);
+/* Waits for the last frame; its total ray count and any device error. */
+luzrt_status luzrt_batch_rays(luzrt_ctx* ctx, luzrt_batch* batch, uint64_t* total_rays);
+luzrt_status luzrt_batch_destroy(luzrt_ctx* ctx, luzrt_batch* batch);
 
 #ifdef __cplusplus
 }

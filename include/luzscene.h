@@ -73,6 +73,10 @@ void     luzgpu_destroy(luzgpu* g);
 luzrt_status luzgpu_update(luzgpu* g, const luzscene* s);
 luzrt_tlas*  luzgpu_tlas(luzgpu* g);
 luzrt_blas*  luzgpu_blas(luzgpu* g, uint32_t mesh);
+/* Device vertex buffer (MeshVertex array) of a mesh, as its BLAS was built from. */
+const void*  luzgpu_vertices(luzgpu* g, uint32_t mesh);
+/* The TLAS instances of the scene's mesh nodes (out may be NULL); returns the count. */
+uint32_t     luzgpu_instances(luzgpu* g, const luzscene* s, luzrt_instance* out);
 
 #ifdef __cplusplus
 }

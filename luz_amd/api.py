@@ -86,6 +86,13 @@ class Timings(C.Structure):
                 ("primary_ms", C.c_float), ("rays", C.c_uint64), ("active_pixels", C.c_uint64)]
 
 
+class BatchScene(C.Structure):
+    _fields_ = [("tlas", C.c_void_p), ("gbuffer", GBuffer), ("scene", SceneBlock),
+                ("camera", C.POINTER(Camera)), ("materials", C.POINTER(Material)),
+                ("material_count", C.c_uint32), ("shadow_counts", C.c_void_p),
+                ("ao_counts", C.c_void_p), ("rgba32f_out", C.c_void_p)]
+
+
 assert C.sizeof(Light) == 64
 assert C.sizeof(SceneBlock) == 64 * 64 + 64
 assert C.sizeof(Instance) == 64
@@ -98,7 +105,7 @@ _sigs = {
     "luzrt_status_string": (C.c_char_p, [C.c_int]),
     "luzrt_sync": (C.c_int, [_P]),
     "luzrt_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
-    "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "luzrt_set_precision": (C.c_int, [_P, C.c_int]),
     "luzrt_blas_wide_arrays": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
                                          C.POINTER(_P), C.POINTER(C.c_uint32)]),
@@ -138,6 +145,14 @@ _sigs = {
     "luzgpu_update": (C.c_int, [_P, _P]),
     "luzgpu_tlas": (_P, [_P]),
     "luzgpu_blas": (_P, [_P, C.c_uint32]),
+    "luzgpu_vertices": (_P, [_P, C.c_uint32]),
+    "luzgpu_instances": (C.c_uint32, [_P, _P, _P]),
+    "luzrt_batch_create": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(_P)]),
+    "luzrt_batch_add_refit": (C.c_int, [_P, _P, _P, _P]),
+    "luzrt_batch_set_instances": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(Instance), C.c_uint32]),
+    "luzrt_batch_frame": (C.c_int, [_P, _P, C.c_uint32, C.c_int]),
+    "luzrt_batch_rays": (C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
+    "luzrt_batch_destroy": (C.c_int, [_P, _P]),
 }
 for _name, (_res, _args) in _sigs.items():
     _f = getattr(lib, _name)
@@ -214,8 +229,9 @@ class Context:
         self._check(lib.luzrt_get_timings(self.handle, C.byref(t)))
         return t
 
-    def set_trace_config(self, groups_per_cu: int = 0, refill: int = 0) -> None:
-        self._check(lib.luzrt_set_trace_config(self.handle, groups_per_cu, refill))
+    def set_trace_config(self, groups_per_cu: int = 0, refill: int = 0, stack_entries: int = 0,
+                         tile_width: int = 0) -> None:
+        self._check(lib.luzrt_set_trace_config(self.handle, groups_per_cu, refill, stack_entries, tile_width))
 
     def set_precision(self, mode: int) -> None:
         """PRECISION_PLAIN or PRECISION_COMPACT (8-wide 8-bit BVH, packed-FP16 shading)."""
@@ -424,6 +440,17 @@ class GPUScene:
     def blas(self, mesh: int) -> Blas:
         return Blas(self.ctx, lib.luzgpu_blas(self.handle, mesh), 0)
 
+    def vertices(self, mesh: int) -> int:
+        """Device address of the mesh's vertex buffer (MeshVertex array)."""
+        return lib.luzgpu_vertices(self.handle, mesh)
+
+    def instances(self, scene: HostScene):
+        """The scene's TLAS instances (ctypes array) and their count."""
+        n = lib.luzgpu_instances(self.handle, scene.handle, None)
+        arr = (Instance * max(n, 1))()
+        lib.luzgpu_instances(self.handle, scene.handle, C.cast(arr, C.c_void_p))
+        return arr, n
+
     def close(self) -> None:
         if self.handle:
             lib.luzgpu_destroy(self.handle)
@@ -457,3 +484,60 @@ class GBufferDevice:
                 "albedo": self.albedo.download(np.uint32, n),
                 "material": self.material.download(np.uint32, n),
                 "emission": self.emission.download(np.uint32, n)}
+
+
+# ------------------------------------------------------------------ batched frames
+class Batch:
+    """Many scenes rendered per frame (luzrt_batch): batched BLAS refit and
+    TLAS rebuild, primary G-buffers, one persistent traversal launch over all
+    scenes' shadow and AO rays, shading; captured into a hipGraph."""
+
+    def __init__(self, ctx: Context, scenes):
+        """scenes: list of dicts with keys tlas (Tlas), gbuffer (GBuffer),
+        scene (SceneBlock), camera (Camera or None), materials (list),
+        shadow, ao, out (DeviceBuffer)."""
+        self.ctx = ctx
+        n = len(scenes)
+        self._keep = []
+        arr = (BatchScene * n)()
+        for i, d in enumerate(scenes):
+            e = arr[i]
+            e.tlas = d["tlas"].handle
+            e.gbuffer = d["gbuffer"]
+            e.scene = d["scene"]
+            if d.get("camera") is not None:
+                cam = Camera()
+                C.memmove(C.byref(cam), C.byref(d["camera"]), C.sizeof(Camera))
+                self._keep.append(cam)
+                e.camera = C.pointer(cam)
+            mats = d.get("materials") or []
+            if mats:
+                m = (Material * len(mats))(*mats)
+                self._keep.append(m)
+                e.materials = C.cast(m, C.POINTER(Material))
+            e.material_count = len(mats)
+            e.shadow_counts = d["shadow"].ptr
+            e.ao_counts = d["ao"].ptr
+            e.rgba32f_out = d["out"].ptr
+        h = _P()
+        ctx._check(lib.luzrt_batch_create(ctx.handle, C.cast(arr, C.c_void_p), n, C.byref(h)))
+        self.handle = h.value
+
+    def add_refit(self, blas: Blas, vertices: int) -> None:
+        self.ctx._check(lib.luzrt_batch_add_refit(self.ctx.handle, self.handle, blas.handle, vertices))
+
+    def set_instances(self, scene: int, instances, count: int) -> None:
+        self.ctx._check(lib.luzrt_batch_set_instances(self.ctx.handle, self.handle, scene, instances, count))
+
+    def frame(self, frame: int, use_graph: bool = True) -> None:
+        self.ctx._check(lib.luzrt_batch_frame(self.ctx.handle, self.handle, frame, 1 if use_graph else 0))
+
+    def rays(self) -> int:
+        r = C.c_uint64()
+        self.ctx._check(lib.luzrt_batch_rays(self.ctx.handle, self.handle, C.byref(r)))
+        return r.value
+
+    def close(self) -> None:
+        if self.handle:
+            lib.luzrt_batch_destroy(self.ctx.handle, self.handle)
+            self.handle = None

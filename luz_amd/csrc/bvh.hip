@@ -569,13 +569,18 @@ __global__ __launch_bounds__(kBlock) void k_wide_emit(const WRef* slots, const u
 // ---------------------------------------------------------------- TLAS
 constexpr int kTlasThreads = 1024;
 
-__global__ __launch_bounds__(kTlasThreads) void k_tlas_build(const InstIn* in, uint32_t n,
-                                                             float* wbox, uint32_t* leaf_parent,
-                                                             uint32_t* counters, Node* nodes,
-                                                             InstRec* recs, InstRec* recs_w,
-                                                             InstAux* aux, float* w2o_out,
-                                                             uint32_t* order_out, uint32_t m,
-                                                             uint32_t* error) {
+__device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
+    const InstIn* in = J.in;
+    const uint32_t n = J.n, m = J.m;
+    float* wbox = J.wbox;
+    uint32_t* leaf_parent = J.leaf_parent;
+    uint32_t* counters = J.counters;
+    Node* nodes = J.nodes;
+    InstRec* recs = J.recs;
+    InstRec* recs_w = J.recs_w;
+    InstAux* aux = J.aux;
+    float* w2o_out = J.w2o_out;
+    uint32_t* order_out = J.order_out;
     // max(m, 1024) 8-byte entries, m a power of two >= n.  The reductions of
     // steps 1-2 use the same memory before the sort keys are written, which
     // keeps the kernel within 64 KB of LDS for 8192 instances.
@@ -764,6 +769,53 @@ __global__ __launch_bounds__(kTlasThreads) void k_tlas_build(const InstIn* in, u
     }
 }
 
+__global__ __launch_bounds__(kTlasThreads) void k_tlas_build(TlasJob J, uint32_t* error) {
+    tlas_build_body(J, error);
+}
+
+// Many TLASes in one launch: workgroup i builds jobs[i].
+__global__ __launch_bounds__(kTlasThreads) void k_tlas_build_batched(const TlasJob* jobs,
+                                                                     uint32_t* error) {
+    tlas_build_body(jobs[blockIdx.x], error);
+}
+
+// Batched refit: global triangle t belongs to the job whose tri_offset range holds it.
+__device__ inline uint32_t refit_job_of(const RefitJob* jobs, uint32_t njobs, uint32_t t) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].tri_offset <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_boxes_batched(const RefitJob* jobs, uint32_t njobs,
+                                                                uint32_t total, float* slot_box) {
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const RefitJob& J = jobs[refit_job_of(jobs, njobs, t)];
+    uint32_t s = t - J.tri_offset;
+    if (s < J.n - 1) J.counters[s] = 0;
+    V3 v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = load_pos(J.verts, J.stride, J.slot_idx[3 * s + k]);
+        J.tris[(size_t)s * 9 + k * 3 + 0] = v[k].x;
+        J.tris[(size_t)s * 9 + k * 3 + 1] = v[k].y;
+        J.tris[(size_t)s * 9 + k * 3 + 2] = v[k].z;
+    }
+    tri_box(v[0], v[1], v[2], slot_box + (size_t)t * 6);
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_batched(const RefitJob* jobs, uint32_t njobs,
+                                                          uint32_t total, const float* slot_box,
+                                                          uint32_t* error) {
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const RefitJob& J = jobs[refit_job_of(jobs, njobs, t)];
+    refit_climb(J.nodes, J.leaf_parent, J.counters, t - J.tri_offset, slot_box + (size_t)t * 6, error);
+}
+
 inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
@@ -915,18 +967,54 @@ luzrt_status blas_wide_device(luzrt_ctx* ctx, luzrt_blas* b) {
     return LUZRT_OK;
 }
 
-luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
-    uint32_t m = 1;
-    while (m < n) m <<= 1;
-    float* wbox = (float*)t->scratch;
-    uint32_t* leaf_parent = (uint32_t*)(wbox + (size_t)t->capacity * 6);
-    uint32_t* counters = leaf_parent + t->capacity;
+luzrt_status launch_tlas_batched(luzrt_ctx* ctx, const TlasJob* dev_jobs, uint32_t njobs,
+                                 uint32_t max_m) {
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
-    size_t lds = (size_t)(m > 1024 ? m : 1024) * sizeof(uint64_t);
-    k_tlas_build<<<1, kTlasThreads, lds, ctx->stream>>>(dev_in, n, wbox, leaf_parent, counters,
-                                                        t->nodes, t->recs, t->wide ? t->recs_w : nullptr,
-                                                        t->aux, t->w2o, t->order, m, error);
+    size_t lds = (size_t)(max_m > 1024 ? max_m : 1024) * sizeof(uint64_t);
+    k_tlas_build_batched<<<njobs, kTlasThreads, lds, ctx->stream>>>(dev_jobs, error);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
+luzrt_status launch_refit_batched(luzrt_ctx* ctx, const RefitJob* dev_jobs, uint32_t njobs,
+                                  uint32_t total_tris, float* slot_box) {
+    uint32_t* error = build_error_flag(ctx);
+    if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
+    if (njobs == 0 || total_tris == 0) return LUZRT_OK;
+    k_refit_boxes_batched<<<cdiv(total_tris, kBlock), kBlock, 0, ctx->stream>>>(dev_jobs, njobs,
+                                                                              total_tris, slot_box);
+    k_refit_batched<<<cdiv(total_tris, kBlock), kBlock, 0, ctx->stream>>>(dev_jobs, njobs, total_tris,
+                                                                        slot_box, error);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
+TlasJob tlas_job(luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
+    uint32_t m = 1;
+    while (m < n) m <<= 1;
+    TlasJob J;
+    J.in = dev_in;
+    J.n = n;
+    J.m = m;
+    J.wbox = (float*)t->scratch;
+    J.leaf_parent = (uint32_t*)(J.wbox + (size_t)t->capacity * 6);
+    J.counters = J.leaf_parent + t->capacity;
+    J.nodes = t->nodes;
+    J.recs = t->recs;
+    J.recs_w = t->wide ? t->recs_w : nullptr;
+    J.aux = t->aux;
+    J.w2o_out = t->w2o;
+    J.order_out = t->order;
+    return J;
+}
+
+luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
+    uint32_t* error = build_error_flag(ctx);
+    if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
+    TlasJob J = tlas_job(t, dev_in, n);
+    size_t lds = (size_t)(J.m > 1024 ? J.m : 1024) * sizeof(uint64_t);
+    k_tlas_build<<<1, kTlasThreads, lds, ctx->stream>>>(J, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
 }

@@ -36,8 +36,10 @@ struct luzrt_ctx {
     bool ev_used[luz::kStageCount] = {};
     uint64_t last_rays = 0, last_active = 0;
     // traversal launch configuration
-    uint32_t groups_per_cu = 8;
-    uint32_t refill_threshold = 16;
+    uint32_t groups_per_cu = 10;
+    uint32_t refill_threshold = 32;
+    uint32_t stack_entries = 16;   // LDS stack entries per lane: 8, 16 or 32
+    uint32_t tile_width = 8;       // pixel tiles of 64: 8x8, 16x4 or 32x2
     // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
     int precision = 0;
 };
@@ -117,6 +119,45 @@ luzrt_status blas_wide_device(luzrt_ctx* ctx, luzrt_blas* b);
 luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n);
 uint32_t* build_error_flag(luzrt_ctx* ctx);
 
+// Host packing of TLAS instances (FP64 inverses, mask/empty filtering); updates
+// the TLAS's bookkeeping (count, referenced BLASes) for the build that follows.
+luzrt_status pack_instances(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instance* inst,
+                            uint32_t count, std::vector<InstIn>& out);
+
+// One TLAS build job, as consumed by the single and the batched build kernels.
+struct TlasJob {
+    const InstIn* in;
+    uint32_t n, m;          // instances, power of two >= n
+    float* wbox;
+    uint32_t* leaf_parent;
+    uint32_t* counters;
+    Node* nodes;
+    InstRec* recs;
+    InstRec* recs_w;        // null unless the compact form is built
+    InstAux* aux;
+    float* w2o_out;
+    uint32_t* order_out;
+};
+TlasJob tlas_job(luzrt_tlas* t, const InstIn* dev_in, uint32_t n);
+luzrt_status launch_tlas_batched(luzrt_ctx* ctx, const TlasJob* dev_jobs, uint32_t njobs,
+                                 uint32_t max_m);
+
+// One BLAS refit job of a batched refit (binary form only).
+struct RefitJob {
+    Node* nodes;
+    uint32_t* leaf_parent;
+    uint32_t* counters;
+    const uint32_t* slot_idx;
+    float* tris;
+    const uint8_t* verts;
+    uint32_t stride;
+    uint32_t n;             // triangles (>= 2)
+    uint32_t tri_offset;    // first global triangle of this job
+    uint32_t pad;
+};
+luzrt_status launch_refit_batched(luzrt_ctx* ctx, const RefitJob* dev_jobs, uint32_t njobs,
+                                  uint32_t total_tris, float* slot_box);
+
 // Scene block as uploaded for one visibility call: the caller's luzrt_scene
 // plus the ray-slot layout.  Slot j of active pixel a is ray j*A + a, where A
 // is the number of non-background pixels; shadow slots of light l are
@@ -143,10 +184,25 @@ struct TraceLaunch {
     uint32_t* counters;    // device Counter array; kCntRays read, kCntFetch zeroed before
     uint32_t* spill;       // per-lane overflow stack
     bool wide;             // instance records point to 8-wide BLASes
+    // Multi-scene launch (nscenes > 0): ray i belongs to the scene s with
+    // scene_ray_base[s] <= i < scene_ray_base[s + 1] and is traced against
+    // scene_tlas[s]; tlas_nodes and recs are then unused.
+    uint32_t nscenes;
+    const uint32_t* scene_ray_base;   // device, nscenes + 1 entries
+    const struct BatchTlas* scene_tlas;
 };
+
+// TLAS of one scene of a multi-scene launch (nodes null: empty TLAS).
+struct BatchTlas {
+    const Node* nodes;
+    const InstRec* recs;
+};
+constexpr uint32_t kMaxBatch = LUZRT_MAX_BATCH;
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
 
-// Validates a scene, fills slot layout, uploads it to ctx->scene.
+// Validates a scene and fills its slot layout.
+luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds);
+// fill_scene, then uploads the block to ctx->scene.
 luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy);
 luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g);
 luzrt_status ensure_lut(luzrt_ctx* ctx);

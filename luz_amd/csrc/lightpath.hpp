@@ -1,0 +1,309 @@
+// lightpath.hpp — per-pixel stages of the lighting path, shared by the
+// single-scene kernels (trace.hip, shade.hip) and the batched multi-scene
+// kernels (batch.hip), so both evaluate identical arithmetic.
+//
+//   tile_pixel / pixel_active / block_prefix  compaction of non-background pixels
+//   shadow_setup / raygen_pixel               light.frag:86-146, 111-135, 229-231
+//   resolve_pixel                             occluded / unoccluded sample counts
+//   shade_pixel                               light.frag:17-49, 171-235 (FP32)
+//   primary_pixel                             G-buffer texel by a closest-hit primary ray
+#pragma once
+#include "internal.hpp"
+#include "traverse.hpp"
+
+namespace luz {
+
+constexpr float kPiF = 3.14159265359f;  // LuzCommon.h:10
+
+__device__ inline float clamp01(float a) { return a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a); }
+
+// Pixel g of the 64-pixel-tile order with tiles tw wide and 64/tw tall.
+__device__ inline void tile_pixel(uint32_t g, uint32_t W, uint32_t tw, uint32_t& x, uint32_t& y) {
+    uint32_t tile = g >> 6, t = g & 63u;
+    uint32_t th = 64u / tw;
+    uint32_t tiles_x = (W + tw - 1) / tw;
+    x = (tile % tiles_x) * tw + (t % tw);
+    y = (tile / tiles_x) * th + t / tw;
+}
+
+// Tile-order pixel g of a W x H G-buffer: its index p and whether it is
+// inside the image and not background (non-zero normal, light.frag:178).
+__device__ inline bool pixel_active(const float* normal, uint32_t g, uint32_t W, uint32_t H,
+                                    uint32_t tw, uint32_t ntiles_px, uint32_t& p, bool& inside) {
+    uint32_t x = 0, y = 0;
+    p = 0;
+    inside = false;
+    if (g >= ntiles_px) return false;
+    tile_pixel(g, W, tw, x, y);
+    if (x >= W || y >= H) return false;
+    inside = true;
+    p = y * W + x;
+    const float* n = normal + (size_t)p * 3;
+    return vlen(v3(n[0], n[1], n[2])) != 0.0f;
+}
+
+// Exclusive rank of this thread's flag within its 256-thread block; the
+// block total in block_total.  All threads of the block must call it.
+__device__ inline uint32_t block_prefix(bool active, uint32_t* wave_tot, uint32_t& block_total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t m = __ballot(active);
+    uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    if (lane == 0) wave_tot[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+        uint32_t c = wave_tot[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    block_total = tot;
+    return off + (uint32_t)__popcll(m & lt);
+}
+
+struct ShadowSetup { V3 Lvec, T, B; float tmax; };
+
+// light.frag:137-146 (origin, ray vector) and :90-97 (tangent frame, tMax).
+__device__ inline ShadowSetup shadow_setup(const luzrt_light& L, V3 P) {
+    ShadowSetup r;
+    V3 lpos = v3(L.position[0], L.position[1], L.position[2]);
+    V3 Lraw = vsub(lpos, P);
+    float dist = vlen(Lraw);
+    if (L.type == LUZRT_LIGHT_DIRECTIONAL) {
+        V3 dir = v3(L.direction[0], L.direction[1], L.direction[2]);
+        V3 Ld = vnorm(v3(-dir.x, -dir.y, -dir.z));
+        r.Lvec = vmul(vmul(dir, vdot(dir, Ld)), dist);
+    } else {
+        r.Lvec = vmul(vnorm(Lraw), dist);
+    }
+    V3 c = vcross(r.Lvec, v3(0.0f, 1.0f, 0.0f));
+    if (vlen(c) == 0.0f) c = vcross(r.Lvec, v3(1.0f, 0.0f, 0.0f));
+    r.T = vnorm(c);
+    r.B = vnorm(vcross(r.T, r.Lvec));
+    r.tmax = vlen(r.Lvec);
+    return r;
+}
+
+// Writes the shadow and AO rays of active pixel a (pixel p of a W-wide
+// image) as ray (slot * A + a), slot-major, into the structure-of-arrays
+// buffers ray_o (origin, tmin) and ray_d (direction, tmax).
+__device__ inline void raygen_pixel(const float* position, const float* normal, uint32_t W,
+                                    const DevScene* ds, uint32_t p, uint32_t a, uint32_t A,
+                                    float4* ray_o, float4* ray_d) {
+    const luzrt_scene& S = ds->s;
+    uint32_t x = p % W, y = p / W;
+    V3 P = v3(position[(size_t)p * 3], position[(size_t)p * 3 + 1], position[(size_t)p * 3 + 2]);
+    V3 N = v3(normal[(size_t)p * 3], normal[(size_t)p * 3 + 1], normal[(size_t)p * 3 + 2]);
+    V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
+    float lc = vlen(vsub(P, cam));
+    float sb = fmax_(lc * 0.01f, 0.05f);
+    V3 so = vadd(P, vmul(N, sb));
+    const uint32_t frame = (uint32_t)S.frame;
+    for (int l = 0; l < S.num_lights; ++l) {
+        const luzrt_light& L = S.lights[l];
+        int n = L.num_shadow_samples;
+        if (n <= 0) continue;
+        ShadowSetup ss = shadow_setup(L, P);
+        uint32_t j0 = ds->slot_base[l];
+        for (int s = 0; s < n; ++s) {
+            float u, v;
+            sample2(x, y, frame, (uint32_t)(l * 256 + s), u, v);
+            float pr = L.radius * sqrtf(u);
+            float sn, cs;
+            sincos_turns(v, sn, cs);
+            float dx = pr * cs, dy = pr * sn;
+            V3 d = v3(ss.Lvec.x + dx * ss.T.x + dy * ss.B.x,
+                      ss.Lvec.y + dx * ss.T.y + dy * ss.B.y,
+                      ss.Lvec.z + dx * ss.T.z + dy * ss.B.z);
+            d = vnorm(d);
+            size_t idx = (size_t)(j0 + s) * A + a;
+            ray_o[idx] = make_float4(so.x, so.y, so.z, 0.001f);
+            ray_d[idx] = make_float4(d.x, d.y, d.z, ss.tmax);
+        }
+    }
+    int na = S.ao_num_samples;
+    if (na > 0) {
+        float ab = lc * 0.01f;
+        V3 ao = vadd(P, vmul(N, ab));
+        V3 T = fabsf(N.z) > 0.5f ? v3(0.0f, -N.z, N.y) : v3(-N.y, N.x, 0.0f);
+        V3 B = vcross(N, T);
+        uint32_t j0 = ds->r_shadow;
+        for (int s = 0; s < na; ++s) {
+            float u, v;
+            sample2(x, y, frame, (uint32_t)(16384 + s), u, v);
+            float r = sqrtf(u);
+            float sn, cs;
+            sincos_turns(v * kHemiTurns, sn, cs);
+            float hx = r * cs, hy = r * sn;
+            float om = 1.0f - u;
+            float hz = sqrtf(om > 0.0f ? om : 0.0f);
+            V3 d = v3(T.x * hx + B.x * hy + N.x * hz,
+                      T.y * hx + B.y * hy + N.y * hz,
+                      T.z * hx + B.z * hy + N.z * hz);
+            size_t idx = (size_t)(j0 + s) * A + a;
+            ray_o[idx] = make_float4(ao.x, ao.y, ao.z, S.ao_min);
+            ray_d[idx] = make_float4(d.x, d.y, d.z, S.ao_max);
+        }
+    }
+}
+
+// Per-pixel counts from hit bytes: occluded shadow samples per light
+// (plane l of `shadow`), unoccluded AO samples; a = kNone for background.
+__device__ inline void resolve_pixel(const DevScene* ds, uint32_t a, uint32_t A, const uint8_t* hits,
+                                     uint32_t npix, uint32_t p, uint8_t* shadow, uint8_t* ao) {
+    const luzrt_scene& S = ds->s;
+    for (int l = 0; l < S.num_lights; ++l) {
+        uint32_t cnt = 0;
+        int n = S.lights[l].num_shadow_samples;
+        if (a != kNone && n > 0) {
+            uint32_t j0 = ds->slot_base[l];
+            for (int s = 0; s < n; ++s) cnt += hits[(size_t)(j0 + s) * A + a];
+        }
+        shadow[(size_t)l * npix + p] = (uint8_t)cnt;
+    }
+    uint32_t miss = 0;
+    if (a != kNone) {
+        uint32_t j0 = ds->r_shadow;
+        for (int s = 0; s < S.ao_num_samples; ++s) miss += 1u - hits[(size_t)(j0 + s) * A + a];
+    }
+    ao[p] = (uint8_t)miss;
+}
+
+// light.frag:main in FP32; lut = pow(i/255, 2.2).
+__device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds, const float* lut,
+                                     const uint8_t* shadow, const uint8_t* ao, uint32_t p) {
+    const uint32_t npix = g.width * g.height;
+    const luzrt_scene& S = ds->s;
+    V3 N = v3(g.normal[(size_t)p * 3], g.normal[(size_t)p * 3 + 1], g.normal[(size_t)p * 3 + 2]);
+    if (vlen(N) == 0.0f)
+        return make_float4(S.ambient_color[0] * S.ambient_intensity,
+                           S.ambient_color[1] * S.ambient_intensity,
+                           S.ambient_color[2] * S.ambient_intensity, 1.0f);
+    uint32_t alb = g.albedo[p], mat = g.material[p], emi = g.emission[p];
+    float albedo[3] = {lut[alb & 0xFFu], lut[(alb >> 8) & 0xFFu], lut[(alb >> 16) & 0xFFu]};
+    float rough = (float)(mat & 0xFFu) / 255.0f;
+    float metal = (float)((mat >> 8) & 0xFFu) / 255.0f;
+    float occl = (float)((mat >> 16) & 0xFFu) / 255.0f;
+    V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
+    V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
+    V3 V = vnorm(vsub(cam, P));
+    float F0[3];
+    for (int k = 0; k < 3; ++k) F0[k] = 0.04f * (1.0f - metal) + albedo[k] * metal;
+    float Lo[3] = {0.0f, 0.0f, 0.0f};
+    float a = rough * rough, a2 = a * a;
+    float rr = rough + 1.0f, kk = (rr * rr) / 8.0f;
+    float NdotV = fmax_(vdot(N, V), 0.0f);
+    float gv = NdotV / (NdotV * (1.0f - kk) + kk);
+    const uint32_t nl = (uint32_t)S.num_lights;
+    for (uint32_t l = 0; l < nl; ++l) {
+        const luzrt_light& Lb = S.lights[l];
+        V3 Lr = vsub(v3(Lb.position[0], Lb.position[1], Lb.position[2]), P);
+        V3 L = vnorm(Lr);
+        float att = 1.0f;
+        V3 ndir = v3(-Lb.direction[0], -Lb.direction[1], -Lb.direction[2]);
+        if (Lb.type == LUZRT_LIGHT_DIRECTIONAL) {
+            L = vnorm(ndir);
+        } else if (Lb.type == LUZRT_LIGHT_SPOT) {
+            float dist = vlen(Lr);
+            att = 1.0f / (dist * dist);
+            float theta = vdot(L, vnorm(ndir));
+            float eps = Lb.inner_angle - Lb.outer_angle;
+            att = att * clamp01((theta - Lb.outer_angle) / eps);
+        } else {
+            float dist = vlen(Lr);
+            att = 1.0f / (dist * dist);
+        }
+        int ns = Lb.num_shadow_samples;
+        float sh = ns > 0 ? (float)shadow[(size_t)l * npix + p] / (float)ns : 0.0f;
+        float scale = Lb.intensity * att * (1.0f - sh);
+        V3 H = vnorm(vadd(V, L));
+        float NdotH = fmax_(vdot(N, H), 0.0f);
+        float den = NdotH * NdotH * (a2 - 1.0f) + 1.0f;
+        float D = a2 / (kPiF * den * den);
+        float NdotL = fmax_(vdot(N, L), 0.0f);
+        float G = (NdotL / (NdotL * (1.0f - kk) + kk)) * gv;
+        float HV = clamp01(vdot(H, V));
+        float om = clamp01(1.0f - HV);
+        float om5 = om * om * om * om * om;
+        float denom = 4.0f * NdotV * NdotL + 0.0001f;
+        float DG = D * G;
+        for (int k = 0; k < 3; ++k) {
+            float F = F0[k] + (1.0f - F0[k]) * om5;
+            float spec = DG * F / denom;
+            float kD = (1.0f - F) * (1.0f - metal);
+            float radiance = Lb.color[k] * scale;
+            Lo[k] = Lo[k] + (kD * albedo[k] / kPiF + spec) * radiance * NdotL;
+        }
+    }
+    int na = S.ao_num_samples;
+    float aof = na > 0 ? (float)ao[p] / (float)na : 1.0f;
+    float o[3];
+    for (int k = 0; k < 3; ++k) {
+        float amb = S.ambient_color[k] * S.ambient_intensity * albedo[k] * occl * aof;
+        float e = (float)((emi >> (8 * k)) & 0xFFu) / 255.0f;
+        o[k] = amb + Lo[k] + e;
+    }
+    return make_float4(o[0], o[1], o[2], 1.0f);
+}
+
+// G-buffer texel (x, y) from the closest hit of its primary ray: world
+// position, normalised interpolated vertex normal through the instance's
+// inverse-transpose, and the material of the hit instance (background: all
+// zero).  The stack must be empty; it is empty again on return.
+template <typename STACK>
+__device__ inline void primary_pixel(const Node* tlas_nodes, const InstRec* recs, const InstAux* aux,
+                                     const luzrt_camera& cam, const luzrt_material* mats,
+                                     uint32_t nmats, const luzrt_gbuffer_out& out, uint32_t x,
+                                     uint32_t y, STACK& st, uint32_t* error_flag, bool empty) {
+    const uint32_t W = out.width, H = out.height;
+    const V3 eye = v3(cam.eye[0], cam.eye[1], cam.eye[2]);
+    Best best;
+    best.found = false;
+    best.t = cam.tmax;
+    float nx = ((float)x + 0.5f) / (float)W * 2.0f - 1.0f;
+    float ny = ((float)y + 0.5f) / (float)H * 2.0f - 1.0f;
+    V3 d = v3(cam.forward[0] + nx * cam.right[0] + ny * cam.up[0],
+              cam.forward[1] + nx * cam.right[1] + ny * cam.up[1],
+              cam.forward[2] + nx * cam.right[2] + ny * cam.up[2]);
+    if (!empty) {
+        RayState r;
+        ray_begin(r, st, eye, d, cam.tmin, cam.tmax);
+        for (;;) {
+            int res = ray_step<true>(r, st, tlas_nodes, recs, aux, &best);
+            if (res == kStepDone) break;
+            if (res == kStepOverflow) { atomicOr(error_flag, 1u); st.sp = st.lo = 0; break; }
+        }
+    }
+    size_t p = (size_t)y * W + x;
+    if (!best.found) {
+        for (int k = 0; k < 3; ++k) { out.position[p * 3 + k] = 0.0f; out.normal[p * 3 + k] = 0.0f; }
+        out.albedo[p] = 0; out.material[p] = 0; out.emission[p] = 0;
+        return;
+    }
+    const InstAux& ax = aux[best.inst_slot];
+    V3 P = vadd(eye, vmul(d, best.t));
+    float bA = __fdiv_rn(best.U, best.det), bB = __fdiv_rn(best.V, best.det),
+          bC = __fdiv_rn(best.W, best.det);
+    const float* nn = ax.nrms + (size_t)best.tri_slot * 9;
+    V3 no = v3(nn[0] * bA + nn[3] * bB + nn[6] * bC,
+               nn[1] * bA + nn[4] * bB + nn[7] * bC,
+               nn[2] * bA + nn[5] * bB + nn[8] * bC);
+    const float* m = ax.nmat;
+    V3 nw = v3(m[0] * no.x + m[1] * no.y + m[2] * no.z,
+               m[3] * no.x + m[4] * no.y + m[5] * no.z,
+               m[6] * no.x + m[7] * no.y + m[8] * no.z);
+    float ln = vlen(nw);
+    V3 N = ln == 0.0f ? v3(0.0f, 1.0f, 0.0f)
+                      : v3(__fdiv_rn(nw.x, ln), __fdiv_rn(nw.y, ln), __fdiv_rn(nw.z, ln));
+    out.position[p * 3 + 0] = P.x; out.position[p * 3 + 1] = P.y; out.position[p * 3 + 2] = P.z;
+    out.normal[p * 3 + 0] = N.x; out.normal[p * 3 + 1] = N.y; out.normal[p * 3 + 2] = N.z;
+    if (ax.custom_index < nmats) {
+        luzrt_material mt = mats[ax.custom_index];
+        out.albedo[p] = mt.albedo; out.material[p] = mt.material; out.emission[p] = mt.emission;
+    } else {
+        out.albedo[p] = 0xFFFFFFFFu;
+        out.material[p] = 0xFFFF0080u;  // roughness 0.5, metallic 0, occlusion 1
+        out.emission[p] = 0xFF000000u;
+    }
+}
+
+}  // namespace luz

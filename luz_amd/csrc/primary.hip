@@ -8,7 +8,7 @@
 // Ties in t are broken by (instance input index, primitive id), so the result
 // does not depend on traversal order.
 #include "internal.hpp"
-#include "traverse.hpp"
+#include "lightpath.hpp"
 
 namespace luz {
 namespace {
@@ -28,64 +28,16 @@ __global__ __launch_bounds__(kBlock) void k_primary(const Node* tlas_nodes, cons
     st.lds_stride = kBlock;
     st.gs = spill + (size_t)blockIdx.x * kBlock + tid;
     st.g_stride = total;
+    st.sp = 0;
+    st.lo = 0;
     const uint32_t W = out.width, H = out.height;
     const uint32_t tiles_x = (W + 7) / 8;
     const uint32_t ntiles_px = tiles_x * ((H + 7) / 8) * 64;
-    const V3 eye = v3(cam.eye[0], cam.eye[1], cam.eye[2]);
     for (uint32_t g = blockIdx.x * kBlock + tid; g < ntiles_px; g += total) {
         uint32_t tile = g >> 6, t = g & 63u;
         uint32_t x = (tile % tiles_x) * 8 + (t & 7u), y = (tile / tiles_x) * 8 + (t >> 3);
-        bool inside = x < W && y < H;
-        Best best;
-        best.found = false;
-        best.t = cam.tmax;
-        V3 d = v3(0.0f, 0.0f, 0.0f);
-        if (inside && !empty) {
-            float nx = ((float)x + 0.5f) / (float)W * 2.0f - 1.0f;
-            float ny = ((float)y + 0.5f) / (float)H * 2.0f - 1.0f;
-            d = v3(cam.forward[0] + nx * cam.right[0] + ny * cam.up[0],
-                   cam.forward[1] + nx * cam.right[1] + ny * cam.up[1],
-                   cam.forward[2] + nx * cam.right[2] + ny * cam.up[2]);
-            RayState r;
-            ray_begin(r, st, eye, d, cam.tmin, cam.tmax);
-            for (;;) {
-                int res = ray_step<true>(r, st, tlas_nodes, recs, aux, &best);
-                if (res == kStepDone) break;
-                if (res == kStepOverflow) { atomicOr(error_flag, 1u); break; }
-            }
-        }
-        if (!inside) continue;
-        size_t p = (size_t)y * W + x;
-        if (!best.found) {
-            for (int k = 0; k < 3; ++k) { out.position[p * 3 + k] = 0.0f; out.normal[p * 3 + k] = 0.0f; }
-            out.albedo[p] = 0; out.material[p] = 0; out.emission[p] = 0;
-            continue;
-        }
-        const InstAux& ax = aux[best.inst_slot];
-        V3 P = vadd(eye, vmul(d, best.t));
-        float bA = __fdiv_rn(best.U, best.det), bB = __fdiv_rn(best.V, best.det),
-              bC = __fdiv_rn(best.W, best.det);
-        const float* nn = ax.nrms + (size_t)best.tri_slot * 9;
-        V3 no = v3(nn[0] * bA + nn[3] * bB + nn[6] * bC,
-                   nn[1] * bA + nn[4] * bB + nn[7] * bC,
-                   nn[2] * bA + nn[5] * bB + nn[8] * bC);
-        const float* m = ax.nmat;
-        V3 nw = v3(m[0] * no.x + m[1] * no.y + m[2] * no.z,
-                   m[3] * no.x + m[4] * no.y + m[5] * no.z,
-                   m[6] * no.x + m[7] * no.y + m[8] * no.z);
-        float ln = vlen(nw);
-        V3 N = ln == 0.0f ? v3(0.0f, 1.0f, 0.0f)
-                          : v3(__fdiv_rn(nw.x, ln), __fdiv_rn(nw.y, ln), __fdiv_rn(nw.z, ln));
-        out.position[p * 3 + 0] = P.x; out.position[p * 3 + 1] = P.y; out.position[p * 3 + 2] = P.z;
-        out.normal[p * 3 + 0] = N.x; out.normal[p * 3 + 1] = N.y; out.normal[p * 3 + 2] = N.z;
-        if (ax.custom_index < nmats) {
-            luzrt_material mt = mats[ax.custom_index];
-            out.albedo[p] = mt.albedo; out.material[p] = mt.material; out.emission[p] = mt.emission;
-        } else {
-            out.albedo[p] = 0xFFFFFFFFu;
-            out.material[p] = 0xFFFF0080u;  // roughness 0.5, metallic 0, occlusion 1
-            out.emission[p] = 0xFF000000u;
-        }
+        if (x < W && y < H)
+            primary_pixel(tlas_nodes, recs, aux, cam, mats, nmats, out, x, y, st, error_flag, empty);
     }
 }
 

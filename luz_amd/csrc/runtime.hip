@@ -63,7 +63,7 @@ luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g) {
     return LUZRT_OK;
 }
 
-luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds) {
+luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds) {
     if (scene->num_lights < 0 || scene->num_lights > LUZRT_MAX_LIGHTS)
         return fail(ctx, LUZRT_E_LIMIT, "num_lights %d outside 0..%d", scene->num_lights,
                     LUZRT_MAX_LIGHTS);
@@ -85,7 +85,13 @@ luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds
     }
     ds->r_shadow = j;
     ds->r_total = j + (uint32_t)scene->ao_num_samples;
-    luzrt_status s = ensure(ctx, ctx->scene, sizeof(DevScene));
+    return LUZRT_OK;
+}
+
+luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds) {
+    luzrt_status s = fill_scene(ctx, scene, ds);
+    if (s != LUZRT_OK) return s;
+    s = ensure(ctx, ctx->scene, sizeof(DevScene));
     if (s != LUZRT_OK) return s;
     LUZ_HIP(ctx, hipMemcpyAsync(ctx->scene.ptr, ds, sizeof(DevScene), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -239,12 +245,19 @@ luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out) {
     return LUZRT_OK;
 }
 
-luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill) {
+luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill,
+                                    uint32_t stack_entries, uint32_t tile_width) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     if (groups_per_cu > 64 || refill > 64)
         return fail(ctx, LUZRT_E_INVALID_ARG, "trace config out of range (groups<=64, refill<=64)");
+    if (stack_entries && stack_entries != 8 && stack_entries != 16 && stack_entries != 32)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "stack_entries must be 8, 16 or 32");
+    if (tile_width && tile_width != 8 && tile_width != 16 && tile_width != 32)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "tile_width must be 8, 16 or 32");
     if (groups_per_cu) ctx->groups_per_cu = groups_per_cu;
     if (refill) ctx->refill_threshold = refill;
+    if (stack_entries) ctx->stack_entries = stack_entries;
+    if (tile_width) ctx->tile_width = tile_width;
     return LUZRT_OK;
 }
 
@@ -416,14 +429,17 @@ luzrt_status luzrt_tlas_create(luzrt_ctx* ctx, uint32_t max_instances, luzrt_tla
     return LUZRT_OK;
 }
 
-luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instance* inst,
-                              uint32_t count) {
-    if (!ctx) return LUZRT_E_INVALID_ARG;
-    if (!t || (count && !inst)) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_tlas_build: null argument");
+}  // extern "C"
+
+namespace luz {
+
+luzrt_status pack_instances(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instance* inst,
+                            uint32_t count, std::vector<InstIn>& host) {
+    if (!t || (count && !inst)) return fail(ctx, LUZRT_E_INVALID_ARG, "instances: null argument");
     if (count > t->capacity)
         return fail(ctx, LUZRT_E_LIMIT, "%u instances exceed the TLAS capacity %u", count,
                     t->capacity);
-    std::vector<InstIn> host;
+    host.clear();
     host.reserve(count);
     std::vector<const luzrt_blas*> refs;
     for (uint32_t i = 0; i < count; ++i) {
@@ -458,6 +474,24 @@ luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instanc
         if (in.blas->tri_count == 0 || (in.mask & 0xFFu) == 0) continue;
         host.push_back(r);
     }
+    for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs--;
+    t->referenced = refs;
+    for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs++;
+    t->count = (uint32_t)host.size();
+    t->node_count = t->count > 1 ? t->count - 1 : 1;
+    return LUZRT_OK;
+}
+
+}  // namespace luz
+
+extern "C" {
+
+luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instance* inst,
+                              uint32_t count) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    std::vector<InstIn> host;
+    luzrt_status s = pack_instances(ctx, t, inst, count, host);
+    if (s != LUZRT_OK) return s;
     LUZ_HIP(ctx, hipSetDevice(ctx->device));
     t->wide = ctx->precision == LUZRT_PRECISION_COMPACT;
     if (t->wide && !t->recs_w)
@@ -466,16 +500,11 @@ luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instanc
         LUZ_HIP(ctx, hipMemcpyAsync(t->input, host.data(), host.size() * sizeof(InstIn),
                                     hipMemcpyHostToDevice, ctx->stream));
     stage_begin(ctx, kStageBuild);
-    luzrt_status s = tlas_build_device(ctx, t, (const InstIn*)t->input, (uint32_t)host.size());
+    s = tlas_build_device(ctx, t, (const InstIn*)t->input, (uint32_t)host.size());
     stage_end(ctx, kStageBuild);
     if (s != LUZRT_OK) return s;
     // the staged host array must stay valid until the copy has been consumed
     LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs--;
-    t->referenced = refs;
-    for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs++;
-    t->count = (uint32_t)host.size();
-    t->node_count = t->count > 1 ? t->count - 1 : 1;
     return LUZRT_OK;
 }
 

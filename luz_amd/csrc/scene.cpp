@@ -724,25 +724,32 @@ luzgpu* luzgpu_create(luzrt_ctx* ctx, const luzscene* s) {
     return g;
 }
 
-luzrt_status luzgpu_update(luzgpu* g, const luzscene* s) {
-    if (!g || !s) return LUZRT_E_INVALID_ARG;
+uint32_t luzgpu_instances(luzgpu* g, const luzscene* s, luzrt_instance* out) {
+    if (!g || !s) return 0;
     // GPUScene::UpdateResourcesGPU -> CmdBuildTLAS (GPUScene.cpp:354-365).
-    std::vector<luzrt_instance> inst(s->meshNodes.size());
-    for (size_t i = 0; i < s->meshNodes.size(); ++i) {
+    for (size_t i = 0; i < s->meshNodes.size() && out; ++i) {
         const MeshNode& n = s->meshNodes[i];
         M34 w = n.GetWorldTransform();
-        luzrt_instance& d = inst[i];
+        luzrt_instance& d = out[i];
         d.blas = g->blas[n.mesh];
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 4; ++c) d.object_to_world[r][c] = (float)w.m[r][c];
         d.custom_index = (uint32_t)i;
         d.mask = 0xFFu;
     }
+    return (uint32_t)s->meshNodes.size();
+}
+
+luzrt_status luzgpu_update(luzgpu* g, const luzscene* s) {
+    if (!g || !s) return LUZRT_E_INVALID_ARG;
+    std::vector<luzrt_instance> inst(s->meshNodes.size());
+    luzgpu_instances(g, s, inst.data());
     return luzrt_tlas_build(g->ctx, g->tlas, inst.data(), (uint32_t)inst.size());
 }
 
 luzrt_tlas* luzgpu_tlas(luzgpu* g) { return g ? g->tlas : nullptr; }
 luzrt_blas* luzgpu_blas(luzgpu* g, uint32_t i) { return g && i < g->blas.size() ? g->blas[i] : nullptr; }
+const void* luzgpu_vertices(luzgpu* g, uint32_t i) { return g && i < g->vbs.size() ? g->vbs[i] : nullptr; }
 
 void luzgpu_destroy(luzgpu* g) {
     if (!g) return;

@@ -9,14 +9,13 @@
 #include <hip/hip_fp16.h>
 
 #include "internal.hpp"
+#include "lightpath.hpp"
 
 namespace luz {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr float kPi = 3.14159265359f;  // LuzCommon.h:10
-
-__device__ inline float clamp01(float a) { return a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a); }
+constexpr float kPi = kPiF;
 
 __global__ __launch_bounds__(kBlock) void k_shade(const luzrt_gbuffer g, const DevScene* ds,
                                                   const float* lut, const uint8_t* shadow,
@@ -24,82 +23,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(const luzrt_gbuffer g, const D
     __shared__ float s_lut[256];
     s_lut[threadIdx.x] = lut[threadIdx.x];
     __syncthreads();
-    const uint32_t npix = g.width * g.height;
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= npix) return;
-    const luzrt_scene& S = ds->s;
-    V3 N = v3(g.normal[(size_t)p * 3], g.normal[(size_t)p * 3 + 1], g.normal[(size_t)p * 3 + 2]);
-    if (vlen(N) == 0.0f) {
-        out[p] = make_float4(S.ambient_color[0] * S.ambient_intensity,
-                             S.ambient_color[1] * S.ambient_intensity,
-                             S.ambient_color[2] * S.ambient_intensity, 1.0f);
-        return;
-    }
-    uint32_t alb = g.albedo[p], mat = g.material[p], emi = g.emission[p];
-    float albedo[3] = {s_lut[alb & 0xFFu], s_lut[(alb >> 8) & 0xFFu], s_lut[(alb >> 16) & 0xFFu]};
-    float rough = (float)(mat & 0xFFu) / 255.0f;
-    float metal = (float)((mat >> 8) & 0xFFu) / 255.0f;
-    float occl = (float)((mat >> 16) & 0xFFu) / 255.0f;
-    V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
-    V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
-    V3 V = vnorm(vsub(cam, P));
-    float F0[3];
-    for (int k = 0; k < 3; ++k) F0[k] = 0.04f * (1.0f - metal) + albedo[k] * metal;
-    float Lo[3] = {0.0f, 0.0f, 0.0f};
-    float a = rough * rough, a2 = a * a;
-    float rr = rough + 1.0f, kk = (rr * rr) / 8.0f;
-    float NdotV = fmax_(vdot(N, V), 0.0f);
-    float gv = NdotV / (NdotV * (1.0f - kk) + kk);
-    const uint32_t nl = (uint32_t)S.num_lights;
-    for (uint32_t l = 0; l < nl; ++l) {
-        const luzrt_light& Lb = S.lights[l];
-        V3 Lr = vsub(v3(Lb.position[0], Lb.position[1], Lb.position[2]), P);
-        V3 L = vnorm(Lr);
-        float att = 1.0f;
-        V3 ndir = v3(-Lb.direction[0], -Lb.direction[1], -Lb.direction[2]);
-        if (Lb.type == LUZRT_LIGHT_DIRECTIONAL) {
-            L = vnorm(ndir);
-        } else if (Lb.type == LUZRT_LIGHT_SPOT) {
-            float dist = vlen(Lr);
-            att = 1.0f / (dist * dist);
-            float theta = vdot(L, vnorm(ndir));
-            float eps = Lb.inner_angle - Lb.outer_angle;
-            att = att * clamp01((theta - Lb.outer_angle) / eps);
-        } else {
-            float dist = vlen(Lr);
-            att = 1.0f / (dist * dist);
-        }
-        int ns = Lb.num_shadow_samples;
-        float sh = ns > 0 ? (float)shadow[(size_t)l * npix + p] / (float)ns : 0.0f;
-        float scale = Lb.intensity * att * (1.0f - sh);
-        V3 H = vnorm(vadd(V, L));
-        float NdotH = fmax_(vdot(N, H), 0.0f);
-        float den = NdotH * NdotH * (a2 - 1.0f) + 1.0f;
-        float D = a2 / (kPi * den * den);
-        float NdotL = fmax_(vdot(N, L), 0.0f);
-        float G = (NdotL / (NdotL * (1.0f - kk) + kk)) * gv;
-        float HV = clamp01(vdot(H, V));
-        float om = clamp01(1.0f - HV);
-        float om5 = om * om * om * om * om;
-        float denom = 4.0f * NdotV * NdotL + 0.0001f;
-        float DG = D * G;
-        for (int k = 0; k < 3; ++k) {
-            float F = F0[k] + (1.0f - F0[k]) * om5;
-            float spec = DG * F / denom;
-            float kD = (1.0f - F) * (1.0f - metal);
-            float radiance = Lb.color[k] * scale;
-            Lo[k] = Lo[k] + (kD * albedo[k] / kPi + spec) * radiance * NdotL;
-        }
-    }
-    int na = S.ao_num_samples;
-    float aof = na > 0 ? (float)ao[p] / (float)na : 1.0f;
-    float o[3];
-    for (int k = 0; k < 3; ++k) {
-        float amb = S.ambient_color[k] * S.ambient_intensity * albedo[k] * occl * aof;
-        float e = (float)((emi >> (8 * k)) & 0xFFu) / 255.0f;
-        o[k] = amb + Lo[k] + e;
-    }
-    out[p] = make_float4(o[0], o[1], o[2], 1.0f);
+    if (p >= g.width * g.height) return;
+    out[p] = shade_pixel(g, ds, s_lut, shadow, ao, p);
 }
 
 // Compact form of the lighting kernel: per-light geometry and the specular

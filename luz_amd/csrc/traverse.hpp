@@ -14,10 +14,12 @@
 
 namespace luz {
 
-constexpr int kShort = 16;    // LDS stack entries per lane (power of two)
+constexpr int kShort = 16;    // default LDS stack entries per lane (power of two)
 constexpr int kSpill = 160;   // global overflow entries per lane (>= TLAS+BLAS depth + 1)
 
-struct Stack {
+// SHORT: LDS entries per lane (a power of two, chosen by the autotuner).
+template <int SHORT>
+struct StackT {
     uint32_t* ls;     // this lane's LDS column; entry e at ls[e * lds_stride]
     uint32_t* gs;     // this lane's spill column; entry e at gs[e * g_stride]
     uint32_t lds_stride;
@@ -25,12 +27,12 @@ struct Stack {
     int sp, lo;
 
     __device__ inline bool push(uint32_t x) {
-        if (sp - lo == kShort) {
-            gs[(size_t)lo * g_stride] = ls[(lo & (kShort - 1)) * lds_stride];
+        if (sp - lo == SHORT) {
+            gs[(size_t)lo * g_stride] = ls[(lo & (SHORT - 1)) * lds_stride];
             lo++;
         }
         if (sp >= kSpill) return false;
-        ls[(sp & (kShort - 1)) * lds_stride] = x;
+        ls[(sp & (SHORT - 1)) * lds_stride] = x;
         sp++;
         return true;
     }
@@ -40,9 +42,10 @@ struct Stack {
             lo = sp;
             return gs[(size_t)sp * g_stride];
         }
-        return ls[(sp & (kShort - 1)) * lds_stride];
+        return ls[(sp & (SHORT - 1)) * lds_stride];
     }
 };
+using Stack = StackT<kShort>;
 
 struct Best {
     float t;
@@ -70,7 +73,8 @@ constexpr uint32_t kMaxSteps = 1u << 24;
 
 enum StepResult { kStepContinue = 0, kStepHit = 1, kStepDone = 2, kStepOverflow = 3 };
 
-__device__ inline void ray_begin(RayState& r, Stack& st, V3 o, V3 d, float tmin, float tmax) {
+template <typename STACK>
+__device__ inline void ray_begin(RayState& r, STACK& st, V3 o, V3 d, float tmin, float tmax) {
     r.wo = o; r.wd = d; r.tmin = tmin; r.tmax = tmax;
     r.wbr = box_setup(o, d, tmin, tmax);
     r.next = 0;
@@ -107,7 +111,8 @@ __device__ inline uint32_t byte_of(uint32_t lo, uint32_t hi, int k) {
 
 // Visit of an 8-wide quantised BLAS node (any-hit): 8 slab tests on the
 // decoded child boxes, then the hit slots in slot order.
-__device__ inline int wide_visit(RayState& r, Stack& st) {
+template <typename STACK>
+__device__ inline int wide_visit(RayState& r, STACK& st) {
     const uint4* p = reinterpret_cast<const uint4*>(reinterpret_cast<const Node8*>(r.bnodes) + r.next);
     uint4 h0 = p[0], h1 = p[1], qa = p[2], qb = p[3], qc = p[4];
     const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
@@ -148,8 +153,8 @@ __device__ inline int wide_visit(RayState& r, Stack& st) {
     return kStepContinue;
 }
 
-template <bool CLOSEST, bool WIDE = false>
-__device__ inline int ray_step(RayState& r, Stack& st, const Node* tlas_nodes, const InstRec* recs,
+template <bool CLOSEST, bool WIDE = false, typename STACK = Stack>
+__device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, const InstRec* recs,
                                const InstAux* aux, Best* best) {
     if (++r.steps > kMaxSteps) return kStepOverflow;
     if (r.next == kNone) {

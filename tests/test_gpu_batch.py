@@ -1,0 +1,111 @@
+"""GPU parity of batched frames: every scene of a batch (refit, TLAS rebuild,
+primary G-buffer, one multi-scene traversal launch, shading; eager and
+hipGraph replays) reproduces the single-scene calls bit for bit, and its
+visibility counts equal the oracle's on the same G-buffer."""
+import numpy as np
+import pytest
+
+from luz_amd import api
+from luz_amd.batching import BatchWorkload
+from tests.helpers import oracle_scene
+
+pytestmark = pytest.mark.gpu
+
+# uneven scenes: (seed, triangle budget, width, height)
+SCENES = [(3, 1500, 96, 64), (11, 30000, 160, 120), (5, 6000, 64, 64), (8, 90000, 200, 150),
+          (21, 1000, 40, 30)]
+
+
+def single_scene(ctx, wl, i, frame):
+    """The single-scene path on scene i at animation `frame`."""
+    h, g = wl.hosts[i], wl.gpus[i]
+    h.animate(frame)
+    g.update_resources_gpu(h)
+    W, H, cam = h.camera()
+    _, _, mats = h.nodes()
+    gd = api.GBufferDevice(ctx, W, H)
+    ctx.primary_gbuffer(g.tlas, cam, mats, gd.desc())
+    sb = h.pack()
+    sb.frame = frame
+    n = W * H
+    dsh, dao = ctx.buffer(max(sb.num_lights, 1) * n), ctx.buffer(n)
+    out = ctx.buffer(n * 16)
+    ctx.trace_visibility(g.tlas, gd.desc(), sb, dsh, dao)
+    ctx.shade(gd.desc(), sb, dsh, dao, out)
+    return (gd.download(), dsh.download(np.uint8, (sb.num_lights, n)), dao.download(np.uint8, n),
+            out.download(np.float32, (n, 4)))
+
+
+def assert_same(got, want, tag):
+    gg, gs, ga, go = got
+    wg, ws, wa, wo = want
+    for k in gg:
+        np.testing.assert_array_equal(gg[k].view(np.uint32), wg[k].view(np.uint32), err_msg=f"{tag} {k}")
+    np.testing.assert_array_equal(gs, ws, err_msg=f"{tag} shadow")
+    np.testing.assert_array_equal(ga, wa, err_msg=f"{tag} ao")
+    np.testing.assert_array_equal(go.view(np.uint32), wo.view(np.uint32), err_msg=f"{tag} radiance")
+
+
+@pytest.fixture()
+def workload(ctx):
+    wl = BatchWorkload(ctx, SCENES)
+    yield wl
+    wl.close()
+    ctx.set_trace_config(10, 32, 16, 8)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_single_scene_calls(ctx, workload, use_graph):
+    # several frames (graph replays) with eager single-scene work in between
+    for frame in (0, 3, 7):
+        rays = workload.frame(frame, use_graph)
+        print(f"graph={use_graph} frame {frame}: {rays} rays", flush=True)
+        got = [workload.outputs(i) for i in range(len(SCENES))]
+        total = 0
+        for i in range(len(SCENES)):
+            want = single_scene(ctx, workload, i, frame)
+            assert_same(got[i], want, f"scene {i} frame {frame}")
+            active = int((np.abs(want[0]["normal"]).sum(1) != 0).sum())
+            sb = workload.blocks[i]
+            per_px = sum(sb.lights[l].num_shadow_samples for l in range(sb.num_lights)) + sb.ao_num_samples
+            total += active * per_px
+        assert rays == total
+
+
+def test_batch_visibility_matches_oracle(ctx, workload):
+    frame = 5
+    workload.frame(frame, True)
+    for i in (0, 1, 3):
+        h = workload.hosts[i]
+        h.animate(frame)
+        gbuf, sh, ao, _ = workload.outputs(i)
+        sb = h.pack()
+        sb.frame = frame
+        o = oracle_scene(h)
+        W, H, _ = h.camera()
+        wsh, wao, _ = o.trace_visibility(gbuf, W, H, sb)
+        print(f"scene {i}: {W}x{H}, oracle shadow sum {int(wsh.sum())}, ao sum {int(wao.sum())}", flush=True)
+        np.testing.assert_array_equal(sh, wsh)
+        np.testing.assert_array_equal(ao, wao)
+
+
+def test_batch_recaptures_on_config_change(ctx, workload):
+    want = None
+    for cfg in [(10, 32, 16, 8), (6, 16, 8, 16), (8, 48, 32, 32)]:
+        ctx.set_trace_config(*cfg)
+        workload.frame(2, True)
+        got = [workload.outputs(i) for i in range(len(SCENES))]
+        if want is None:
+            want = got
+            continue
+        for i in range(len(SCENES)):
+            assert_same(got[i], want[i], f"config {cfg} scene {i}")
+
+
+def test_batch_rejects_compact_precision(ctx, workload):
+    ctx.set_precision(api.PRECISION_COMPACT)
+    try:
+        with pytest.raises(api.LuzError):
+            workload.batch.frame(0, True)
+    finally:
+        ctx.set_precision(api.PRECISION_PLAIN)
