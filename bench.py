@@ -37,10 +37,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-row-step", type=int, default=8, help="oracle traces every k-th row")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
-    ap.add_argument("--workload", default="spheres1080p", choices=["spheres1080p", "terrain4k", "batch64"],
+    ap.add_argument("--workload", default="spheres1080p", choices=["spheres1080p", "terrain4k", "batch64", "denoise1080p"],
                     help="spheres1080p: the headline 1080p sphere field (plain FP32 path); "
                          "terrain4k: 3840x2160 terrain + spheres, ~5.2M tris, precision form; "
-                         "batch64: 64 animated scenes of uneven size and resolution per frame")
+                         "batch64: 64 animated scenes of uneven size and resolution per frame; "
+                         "denoise1080p: 1-spp + learned denoiser against the plain multi-spp tracer")
     ap.add_argument("--scenes", type=int, default=64, help="batch64: scenes per frame")
     ap.add_argument("--autotune", action="store_true",
                     help="search the traversal launch shape on this workload before timing")
@@ -67,6 +68,8 @@ def main() -> int:
         ctx.set_trace_config(args.groups_per_cu, args.refill)
     if args.workload == "batch64":
         return bench_batch(ctx, args)
+    if args.workload == "denoise1080p":
+        return bench_denoise(ctx, args)
     compact = args.workload == "terrain4k"
     if compact:
         ctx.set_precision(api.PRECISION_COMPACT)
@@ -299,6 +302,162 @@ def bench_batch(ctx, args) -> int:
         record["parity"] = parity
     print(json.dumps(record), flush=True)
     wl.close()
+    ctx.close()
+    return 0
+
+
+def bench_denoise(ctx, args) -> int:
+    """1-spp visibility + learned denoiser + lighting against the plain k-spp
+    tracer on a held-out 1080p sphere-field scene: AO / visibility RMSE
+    against 256-spp references (GPU FP32 on every pixel; FP64 oracle on a row
+    sample) and GPU frame time of each; MFMA fraction of the denoiser."""
+    import ctypes as C
+
+    import numpy as np
+
+    from luz_amd import api, denoiser as dnm
+
+    if not dnm.WEIGHTS.exists():
+        print(f"bench.py: no trained denoiser weights at {dnm.WEIGHTS}", file=sys.stderr)
+        return 2
+    seed = 900100
+    scene = api.HostScene(api.SCENE_SPHERES, seed)
+    gs = api.GPUScene(ctx, scene)
+    W, H, cam = scene.camera()
+    _, _, mats = scene.nodes()
+    gd = api.GBufferDevice(ctx, W, H)
+    ctx.primary_gbuffer(gs.tlas, cam, mats, gd.desc())
+    sb = scene.pack()
+    L, n = sb.num_lights, W * H
+
+    def block(spp, frame):
+        b = api.SceneBlock()
+        C.memmove(C.byref(b), C.byref(sb), C.sizeof(api.SceneBlock))
+        for l in range(b.num_lights):
+            b.lights[l].num_shadow_samples = spp
+        b.ao_num_samples = spp
+        b.frame = frame
+        return b
+
+    sh = ctx.buffer(max(L, 1) * n)
+    ao = ctx.buffer(n)
+    out = ctx.buffer(n * 16)
+    vis_o, ao_o = ctx.buffer(max(L, 1) * n * 4), ctx.buffer(n * 4)
+    dn = dnm.Denoiser(ctx, 0)
+    dn.load(dnm.WEIGHTS)
+    fr = dnm.Frame(ctx, W, H, gd.position, gd.normal, tuple(sb.cam_pos), L, sh, 1, ao, 1)
+    reps = max(args.steps // 5, 3)
+
+    def vis_ms(t):
+        return t.compact_ms + t.raygen_ms + t.trace_ms + t.resolve_ms
+
+    # 256-spp reference on the GPU (two independent 128-spp passes)
+    ref_sh = np.zeros((max(L, 1), n))
+    ref_ao = np.zeros(n)
+    for frame in (1, 2):
+        ctx.trace_visibility(gs.tlas, gd.desc(), block(128, frame), sh, ao)
+        ref_sh += sh.download(np.uint8, (max(L, 1), n))
+        ref_ao += ao.download(np.uint8, n)
+    ref_ao /= 256.0
+    ref_vis = 1.0 - ref_sh[:L] / 256.0
+    host = gd.download()
+    fg = np.abs(host["normal"]).sum(1) != 0
+
+    def rmse(a, b, m):
+        return float(np.sqrt(np.mean((a[..., m] - b[..., m]) ** 2)))
+
+    # the denoised frame: 1-spp visibility + denoiser + lighting
+    b1 = block(1, 0)
+    times = []
+    for r in range(reps + 1):
+        ctx.trace_visibility(gs.tlas, gd.desc(), b1, sh, ao)
+        t_v = ctx.timings()
+        dn.denoise(fr, ao_o, vis_o)
+        t_d = ctx.timings()
+        ctx.shade_fractions(gd.desc(), sb, vis_o, ao_o, out)
+        t_s = ctx.timings()
+        if r:
+            times.append((vis_ms(t_v), t_d.denoise_ms, t_s.shade_ms))
+    tv, td, ts = (float(np.median([t[i] for t in times])) for i in range(3))
+    dn_ao = ao_o.download(np.float32, n).astype(np.float64)
+    dn_vis = vis_o.download(np.float32, (max(L, 1), n)).astype(np.float64)[:L]
+    raw_ao = ao.download(np.uint8, n).astype(np.float64)
+    raw_vis = 1.0 - sh.download(np.uint8, (max(L, 1), n))[:L].astype(np.float64)
+    rows = []
+    for k in (1, 2, 4, 8, 16, 32, 64):
+        bk = block(k, 0)
+        tt = []
+        for r in range(reps + 1):
+            ctx.trace_visibility(gs.tlas, gd.desc(), bk, sh, ao)
+            t_v = ctx.timings()
+            ctx.shade(gd.desc(), bk, sh, ao, out)
+            t_s = ctx.timings()
+            if r:
+                tt.append(vis_ms(t_v) + t_s.shade_ms)
+        a_k = ao.download(np.uint8, n).astype(np.float64) / k
+        v_k = 1.0 - sh.download(np.uint8, (max(L, 1), n))[:L].astype(np.float64) / k
+        rows.append({"spp": k, "frame_ms": round(float(np.median(tt)), 4),
+                     "ao_rmse": round(rmse(a_k, ref_ao, fg), 6), "vis_rmse": round(rmse(v_k, ref_vis, fg), 6),
+                     "_ao": a_k})
+    den = {"frame_ms": round(tv + td + ts, 4), "visibility_ms": round(tv, 4), "denoise_ms": round(td, 4),
+           "shade_ms": round(ts, 4), "ao_rmse": round(rmse(dn_ao, ref_ao, fg), 6),
+           "vis_rmse": round(rmse(dn_vis, ref_vis, fg), 6),
+           "raw_1spp_ao_rmse": round(rmse(raw_ao, ref_ao, fg), 6),
+           "raw_1spp_vis_rmse": round(rmse(raw_vis, ref_vis, fg), 6)}
+    match = next((r for r in rows if r["ao_rmse"] <= den["ao_rmse"]), None)
+    spp64 = next(r for r in rows if r["spp"] == 64)
+    flops = float(fg.sum()) * max(L, 1) * 44032.0
+    achieved = flops / (td * 1e-3) / 1e12
+    record = {
+        "metric": "AO RMSE vs FP64 256-spp oracle and ms/frame: 1-spp + learned denoiser vs plain multi-spp",
+        "value": den["frame_ms"], "unit": "ms/frame", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": den["frame_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (MFMA, FP32 accumulate)", "data": "synthetic",
+        "config": {"workload": "1920x1080 sphere field (held-out scene seed 900100), 1 shadow ray per "
+                               "light (9 lights) + 1 AO ray per pixel, per-pixel MLP 150-64x4-2 on 9 (pixel, "
+                               "light) samples per pixel",
+                   "denoised": den,
+                   "plain": [{k: v for k, v in r.items() if k != "_ao"} for r in rows],
+                   "equal_accuracy_spp": match["spp"] if match else None,
+                   "speedup_vs_equal_accuracy": round(match["frame_ms"] / den["frame_ms"], 2) if match else None,
+                   "speedup_vs_64spp": round(spp64["frame_ms"] / den["frame_ms"], 2)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(achieved / 2500.0, 4), "traffic": None, "kernel": "k_dn_forward",
+                     "flops_per_sample": 44032},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        o = orc.OracleScene()
+        for i in range(scene.mesh_count):
+            v, idx = scene.mesh(i)
+            o.add_mesh(v, idx)
+        o2w, mesh_ids, _ = scene.nodes()
+        o.set_instances(o2w, mesh_ids)
+        step = 16
+        threads = min(orc.threads_default(), 16)
+        f_sh = np.zeros((max(L, 1), n))
+        f_ao = np.zeros(n)
+        c0 = time.perf_counter()
+        for frame in (1, 2):
+            dsh, dao, _ = o.trace_visibility(host, W, H, block(128, frame), threads=threads, row_step=step, f64=True)
+            f_sh += dsh
+            f_ao += dao
+        cpu_s = time.perf_counter() - c0
+        rows_m = np.zeros((H, W), bool)
+        rows_m[::step] = True
+        m = rows_m.reshape(-1) & fg
+        f_ao /= 256.0
+        f_vis = 1.0 - f_sh[:L] / 256.0
+        record["config"]["fp64_oracle"] = {
+            "rows_every": step, "pixels": int(m.sum()), "cpu_s": round(cpu_s, 1), "threads": threads,
+            "denoised_ao_rmse": round(rmse(dn_ao, f_ao, m), 6),
+            "denoised_vis_rmse": round(rmse(dn_vis, f_vis, m), 6),
+            "gpu_ref256_vs_fp64_ao_rmse": round(rmse(ref_ao, f_ao, m), 6),
+            "plain_ao_rmse": {r["spp"]: round(rmse(r["_ao"], f_ao, m), 6) for r in rows},
+        }
+    print(json.dumps(record), flush=True)
+    dn.close()
+    gs.close()
     ctx.close()
     return 0
 

@@ -135,7 +135,7 @@ typedef struct luzrt_material {
 /* GPU time of the stages of the last call, from HIP events on the
  * context's stream (the reference's CmdTimeStamp, VulkanWrapper.cpp:1276-1296). */
 typedef struct luzrt_timings {
-    float build_ms, compact_ms, raygen_ms, trace_ms, resolve_ms, shade_ms, primary_ms;
+    float build_ms, compact_ms, raygen_ms, trace_ms, resolve_ms, shade_ms, primary_ms, denoise_ms;
     uint64_t rays;          /* rays cast by the last visibility/light call */
     uint64_t active_pixels; /* non-background pixels of that call */
 } luzrt_timings;
@@ -225,6 +225,12 @@ luzrt_status luzrt_shade(luzrt_ctx* ctx, const luzrt_gbuffer* gbuffer, const luz
                          float* rgba32f_out);
 /* DeferredRenderer::LightPass: G-buffer in, RGBA32F light buffer out
  * (device, width*height*4 floats). */
+/* The lighting pass from visibility fractions instead of counts: visibility
+ * [num_lights][npix] (1 = unshadowed) and AO [npix] fractions in [0, 1], such
+ * as the learned denoiser's outputs.  Same arithmetic as luzrt_shade with
+ * shadow = 1 - visibility and AO as given. */
+luzrt_status luzrt_shade_fractions(luzrt_ctx* ctx, const luzrt_gbuffer* g, const luzrt_scene* scene,
+                                   const float* visibility, const float* ao, float* rgba32f_out);
 luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* gbuffer,
                               const luzrt_scene* scene, float* rgba32f_out);
 
@@ -277,6 +283,69 @@ luzrt_status luzrt_batch_frame(luzrt_ctx* ctx, luzrt_batch* batch, uint32_t fram
 /* Waits for the last frame; its total ray count and any device error. */
 luzrt_status luzrt_batch_rays(luzrt_ctx* ctx, luzrt_batch* batch, uint64_t* total_rays);
 luzrt_status luzrt_batch_destroy(luzrt_ctx* ctx, luzrt_batch* batch);
+
+/* ------------------------------------------------------------ learned denoiser */
+/* A per-pixel MLP that denoises low-sample AO and per-light soft-shadow
+ * visibility (no counterpart in the reference, whose light.frag uses the
+ * noisy ray-query estimates directly).  One (pixel, light) sample has 150
+ * features: a 5x5 neighbourhood (clamped to the image) x {AO fraction,
+ * visibility of the light, relative depth clamp(16 (d_q - d_p) / d_p, -1, 1),
+ * normal.xyz}, background neighbours (zero normal) giving {0, 0, 1, 0, 0, 0};
+ * then 4 hidden layers of 64 (ReLU) and 2 outputs {AO, visibility}.  Weights
+ * BF16 (MFMA), FP32 accumulation and biases; FP32 master copy for training.
+ * Parameter order (FP32): W1[64][150] b1[64] W2[64][64] b2 W3 b3 W4 b4
+ * W5[2][64] b5[2]. */
+#define LUZRT_DN_FEATURES 150
+#define LUZRT_DN_HIDDEN 64
+#define LUZRT_DN_PARAMS 22274
+
+typedef struct luzrt_denoiser luzrt_denoiser;
+
+typedef struct luzrt_dn_frame {
+    uint32_t width, height;
+    const float* position;          /* device G-buffer planes (float3 per pixel) */
+    const float* normal;
+    float cam_pos[3];
+    uint32_t num_lights;
+    const uint8_t* shadow_counts;   /* device [num_lights][npix]: occluded samples */
+    uint32_t shadow_samples;        /* samples per light behind shadow_counts */
+    const uint8_t* ao_counts;       /* device [npix]: unoccluded AO samples */
+    uint32_t ao_samples;
+} luzrt_dn_frame;
+
+/* batch_capacity: training samples per gradient (a multiple of 64, may be 0
+ * for inference only).  Initial weights are He-uniform from `seed`. */
+luzrt_status luzrt_denoiser_create(luzrt_ctx* ctx, uint32_t batch_capacity, uint64_t seed,
+                                   luzrt_denoiser** out);
+luzrt_status luzrt_denoiser_destroy(luzrt_ctx* ctx, luzrt_denoiser* dn);
+luzrt_status luzrt_denoiser_get_params(luzrt_ctx* ctx, luzrt_denoiser* dn, float* host, uint32_t count);
+/* Replaces the weights (LUZRT_DN_PARAMS floats) and resets the Adam state. */
+luzrt_status luzrt_denoiser_set_params(luzrt_ctx* ctx, luzrt_denoiser* dn, const float* host,
+                                       uint32_t count);
+/* Denoised AO fraction [npix] and per-light visibility [num_lights][npix]
+ * (device outputs in [0, 1]; 0 on background pixels). */
+luzrt_status luzrt_denoise(luzrt_ctx* ctx, luzrt_denoiser* dn, const luzrt_dn_frame* frame,
+                           float* ao_out, float* visibility_out);
+/* The BF16 input features (raw bits, [count][150]) of explicit samples
+ * (device array, pixel | light << 24), for inspection and external trainers. */
+luzrt_status luzrt_denoiser_features(luzrt_ctx* ctx, const luzrt_dn_frame* frame, const uint32_t* samples,
+                                     uint32_t count, uint16_t* features_out);
+/* Training: forward and backward of `count` samples (device array, pixel |
+ * light << 24; count a multiple of 64) of this frame against the targets
+ * (device AO fraction [npix], visibility [num_lights][npix]).  The gradient
+ * of the loss grad_scale/2 * sum (y - target)^2 is accumulated until
+ * luzrt_denoiser_gradient. */
+luzrt_status luzrt_denoiser_accumulate(luzrt_ctx* ctx, luzrt_denoiser* dn, const luzrt_dn_frame* frame,
+                                       const float* target_ao, const float* target_visibility,
+                                       const uint32_t* samples, uint32_t count, float grad_scale);
+/* Weight gradient of the samples accumulated so far (deterministic order);
+ * optional host copies of the gradient (LUZRT_DN_PARAMS floats) and of the
+ * sum of squared errors (synchronise when requested).  Clears the batch. */
+luzrt_status luzrt_denoiser_gradient(luzrt_ctx* ctx, luzrt_denoiser* dn, float* host_grad,
+                                     float* loss_sum);
+/* One Adam step with the last gradient; re-packs the BF16 weights. */
+luzrt_status luzrt_denoiser_adam(luzrt_ctx* ctx, luzrt_denoiser* dn, float lr, float beta1, float beta2,
+                                 float eps);
 
 #ifdef __cplusplus
 }

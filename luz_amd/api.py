@@ -83,7 +83,8 @@ class Instance(C.Structure):
 class Timings(C.Structure):
     _fields_ = [("build_ms", C.c_float), ("compact_ms", C.c_float), ("raygen_ms", C.c_float),
                 ("trace_ms", C.c_float), ("resolve_ms", C.c_float), ("shade_ms", C.c_float),
-                ("primary_ms", C.c_float), ("rays", C.c_uint64), ("active_pixels", C.c_uint64)]
+                ("primary_ms", C.c_float), ("denoise_ms", C.c_float), ("rays", C.c_uint64),
+                ("active_pixels", C.c_uint64)]
 
 
 class BatchScene(C.Structure):
@@ -127,6 +128,7 @@ _sigs = {
     "luzrt_trace_visibility": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P]),
     "luzrt_shade": (C.c_int, [_P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P, _P]),
     "luzrt_light_pass": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P]),
+    "luzrt_shade_fractions": (C.c_int, [_P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P, _P]),
     "luzrt_primary_gbuffer": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(Material), C.c_uint32,
                                         C.POINTER(GBuffer)]),
     "luzscene_generate": (_P, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
@@ -269,6 +271,12 @@ class Context:
     def shade(self, g: GBuffer, scene: SceneBlock, shadow: DeviceBuffer, ao: DeviceBuffer,
               out: DeviceBuffer) -> None:
         self._check(lib.luzrt_shade(self.handle, C.byref(g), C.byref(scene), shadow.ptr, ao.ptr, out.ptr))
+
+    def shade_fractions(self, g: GBuffer, scene: SceneBlock, visibility: DeviceBuffer, ao: DeviceBuffer,
+                        out: DeviceBuffer) -> None:
+        """Lighting from visibility/AO fractions (e.g. the denoiser's outputs)."""
+        self._check(lib.luzrt_shade_fractions(self.handle, C.byref(g), C.byref(scene),
+                                              visibility.ptr if scene.num_lights else None, ao.ptr, out.ptr))
 
     def light_pass(self, tlas: "Tlas", g: GBuffer, scene: SceneBlock, out: DeviceBuffer) -> None:
         self._check(lib.luzrt_light_pass(self.handle, tlas.handle, C.byref(g), C.byref(scene), out.ptr))

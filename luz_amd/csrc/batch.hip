@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_bshade(BatchTables T, const float* l
     const BatchSceneDev& S = T.sc[s];
     const uint32_t p = (blockIdx.x - T.pblk[s]) * kBlock + threadIdx.x;
     if (p >= S.npix) return;
-    S.out[p] = shade_pixel(S.g, S.ds, lut, S.shadow, S.ao, p);
+    S.out[p] = shade_pixel(S.g, S.ds, lut, CountVisibility{S.shadow, S.ao, S.npix}, p);
 }
 
 }  // namespace

@@ -1,0 +1,797 @@
+// denoise.hip — learned per-pixel denoiser of 1-spp ambient occlusion and
+// soft shadows on gfx950, inference and training with BF16 MFMA.
+//
+// Network (per (pixel, light) sample): 150 features = a 5x5 neighbourhood
+// (clamped to the image) x 6 channels {AO, visibility of the light, relative
+// depth, normal.xyz}; 4 hidden layers of 64 with ReLU; 2 outputs {AO,
+// visibility}.  A pixel's AO is the mean of the AO outputs of its L samples.
+// Weights are BF16, accumulation FP32, biases FP32.
+//
+// MFMA mapping (v_mfma_f32_16x16x32_bf16, 64 lanes): weights are the A
+// operand (16 output features x 32 inputs), samples the B operand (32 inputs
+// x 16 samples).  The 16x16 FP32 result leaves lane (j, g) = (lane % 16,
+// lane / 16) holding features 4g..4g+3 of sample j, which is exactly the B
+// operand lane layout of the next layer when its input index k of element e
+// of K-step s is  k = 16 * (2s + e/4) + 4g + e%4.  Weight fragments are
+// packed in that order, so activations stay in registers through all five
+// layers, forward and backward (the transposed weights W^T are packed the
+// same way for the backward pass).  The features of layer 1 use k = 32s +
+// 8g + e.  Any K order works as long as A and B agree, because the hardware
+// pairs element e of lane group g of A with the same element of B.
+//
+// Training: k_dn_forward<true> runs forward, loss and the backward pass to
+// every layer's pre-activation gradient dz, and stores x, h and dz (BF16)
+// sample-minor ([feature][sample]); k_dn_wgrad forms dW = dz^T h per chunk of
+// 1024 samples with MFMA (K = samples) and bias sums; k_dn_reduce adds the
+// chunk partials in a fixed order (deterministic gradients); k_dn_adam and
+// k_dn_pack update the FP32 master weights and re-pack the BF16 fragments.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace luz {
+
+struct DnFrameDev {
+    uint32_t width, height, num_lights, pad;
+    const float* position;
+    const float* normal;
+    const uint8_t* shadow;
+    const uint8_t* ao;
+    float cam[3];
+    float shadow_n, ao_n;
+    float pad2;
+};
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = LUZRT_DN_FEATURES, kHid = LUZRT_DN_HIDDEN, kOut = 2;
+constexpr int kBlock = 256;
+constexpr uint32_t kChunkRows = 1024;
+
+constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn;
+constexpr int oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
+constexpr int oW3 = oB2 + kHid, oB3 = oW3 + kHid * kHid;
+constexpr int oW4 = oB3 + kHid, oB4 = oW4 + kHid * kHid;
+constexpr int oW5 = oB4 + kHid, oB5 = oW5 + kOut * kHid;
+constexpr int kParams = oB5 + kOut;
+static_assert(kParams == LUZRT_DN_PARAMS, "parameter count");
+constexpr int kWOff[5] = {oW1, oW2, oW3, oW4, oW5};
+constexpr int kBOff[5] = {oB1, oB2, oB3, oB4, oB5};
+
+// Fragment table (one fragment = 64 lanes x 8 BF16 = 1 KB):
+//   F1(t,s) t<4 s<5 | F2..F4(t,s) t<4 s<2 | F5(s) s<2 | G2..G4(t,s) | G5(t) t<4
+constexpr int kF1 = 0, kF2 = 20, kF3 = 28, kF4 = 36, kF5 = 44, kG2 = 46, kG3 = 54, kG4 = 62,
+              kG5 = 70, kFragFwd = 46, kFragAll = 74;
+
+__host__ __device__ inline int khid(int s, int g, int e) { return 16 * (2 * s + (e >> 2)) + 4 * g + (e & 3); }
+
+__device__ inline uint32_t bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
+    return u >> 16;
+}
+__device__ inline float bf16_val(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ inline float clamp01f(float a) { return a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a); }
+__device__ inline bf16x8 as_bf16x8(uint4 q) { return __builtin_bit_cast(bf16x8, q); }
+__device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+struct DnArgs {
+    DnFrameDev F;
+    const uint4* frags;
+    const float* params;
+    uint32_t nsamples;
+    uint32_t lights;           // samples per pixel in inference (max(L, 1))
+    float* ao_part;            // inference: [lights][npix] AO output per sample
+    float* vis_out;            // inference: [L][npix]
+    const uint32_t* samples;   // training: pixel | light << 24
+    const float* target_ao;    // training: [npix]
+    const float* target_vis;   // training: [L][npix]
+    uint32_t row_base, ldn;    // training rows of this call, leading dimension
+    uint16_t* XT;              // [160][ldn]
+    uint16_t* HT;              // [4][64][ldn]
+    uint16_t* ZT;              // [4][64][ldn]
+    uint16_t* YT;              // [16][ldn]
+    float grad_scale;          // dLoss/dy = grad_scale * (y - target)
+    float* loss_sum;
+};
+
+// The 6 channels of neighbour (qx, qy) of a pixel at distance dc from the camera.
+struct Nb { float c0, c1, c2, c3, c4, c5; };
+
+__device__ inline Nb dn_neighbour(const DnFrameDev& F, uint32_t qx, uint32_t qy, uint32_t light, float dc) {
+    const size_t q = (size_t)qy * F.width + qx;
+    const float* n = F.normal + q * 3;
+    Nb r;
+    float nx = n[0], ny = n[1], nz = n[2];
+    if (vlen(v3(nx, ny, nz)) == 0.0f) {   // background
+        r.c0 = 0.0f; r.c1 = 0.0f; r.c2 = 1.0f; r.c3 = 0.0f; r.c4 = 0.0f; r.c5 = 0.0f;
+        return r;
+    }
+    const float* p = F.position + q * 3;
+    float d = vlen(v3(p[0] - F.cam[0], p[1] - F.cam[1], p[2] - F.cam[2]));
+    r.c0 = (float)F.ao[q] / F.ao_n;
+    r.c1 = F.num_lights ? 1.0f - (float)F.shadow[(size_t)light * F.width * F.height + q] / F.shadow_n
+                        : 0.0f;
+    float rel = dc > 0.0f ? (d - dc) / dc * 16.0f : 0.0f;
+    r.c2 = rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel);
+    r.c3 = nx; r.c4 = ny; r.c5 = nz;
+    return r;
+}
+
+__device__ inline float nb_channel(const Nb& b, int c) {
+    return c == 0 ? b.c0 : c == 1 ? b.c1 : c == 2 ? b.c2 : c == 3 ? b.c3 : c == 4 ? b.c4 : b.c5;
+}
+
+// Features f0 .. f0+7 (BF16 bits, element e in half e of the uint4) of the sample
+// at pixel (x, y), light `light`; zero beyond 150 and for invalid samples.
+__device__ inline uint4 dn_features(const DnFrameDev& F, bool valid, uint32_t x, uint32_t y,
+                                   uint32_t light, float dc, int f0) {
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    if (valid) {
+        int cur = -1;
+        Nb nb{};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int f = f0 + e;
+            if (f < kIn) {
+                int n = f / 6, c = f - n * 6;
+                if (n != cur) {
+                    int dx = n % 5 - 2, dy = n / 5 - 2;
+                    int qx = (int)x + dx, qy = (int)y + dy;
+                    qx = qx < 0 ? 0 : (qx >= (int)F.width ? (int)F.width - 1 : qx);
+                    qy = qy < 0 ? 0 : (qy >= (int)F.height ? (int)F.height - 1 : qy);
+                    nb = dn_neighbour(F, (uint32_t)qx, (uint32_t)qy, light, dc);
+                    cur = n;
+                }
+                w[e >> 1] |= bf16_bits(nb_channel(nb, c)) << (16 * (e & 1));
+            }
+        }
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Bias + ReLU of a hidden layer's 4x4 tiles of one sample column; the next
+// layer's B fragments, the ReLU mask (bit t*4+i) and, when training, h stored.
+template <bool TRAIN>
+__device__ inline void dn_hidden_epilogue(const f32x4 (&acc)[4], const float* bias, int g, uint4 (&hb)[2],
+                                          uint32_t& mask, uint16_t* HTl, uint32_t ldn, uint32_t row,
+                                          bool store) {
+    uint32_t w[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
+    mask = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int o = 16 * t + 4 * g + i;
+            float z = acc[t][i] + bias[o];
+            bool on = z > 0.0f;
+            float h = on ? z : 0.0f;
+            mask |= (on ? 1u : 0u) << (t * 4 + i);
+            uint32_t hbits = bf16_bits(h);
+            // element e = (t & 1) * 4 + i of K-step s = t / 2
+            int s = t >> 1, e = (t & 1) * 4 + i;
+            w[s][e >> 1] |= hbits << (16 * (e & 1));
+            if (TRAIN && store) HTl[(size_t)o * ldn + row] = (uint16_t)hbits;
+        }
+    }
+    hb[0] = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
+    hb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
+}
+
+// Layer (k >= 2) with 2 K-steps: acc[q][t] = sum_s frag(t, s) * hb[q][s].
+__device__ inline void dn_layer64(const uint4* sfrag, int fbase, int lane, const uint4 (&hb)[4][2],
+                                  f32x4 (&acc)[4][4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bf16x8 a = as_bf16x8(sfrag[(fbase + t * 2 + s) * 64 + lane]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q][t] = mfma(a, as_bf16x8(hb[q][s]), acc[q][t]);
+        }
+}
+
+// Backward through a hidden layer's ReLU: dz = dh * mask; stores dz (BF16)
+// when training rows are valid, and builds the B fragments of dz.
+__device__ inline void dn_relu_back(const f32x4 (&dh)[4], uint32_t mask, int g, uint4 (&zb)[2],
+                                    uint16_t* ZTl, uint32_t ldn, uint32_t row, bool store) {
+    uint32_t w[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int o = 16 * t + 4 * g + i;
+            float dz = (mask >> (t * 4 + i)) & 1u ? dh[t][i] : 0.0f;
+            uint32_t bits = bf16_bits(dz);
+            int s = t >> 1, e = (t & 1) * 4 + i;
+            w[s][e >> 1] |= bits << (16 * (e & 1));
+            if (store) ZTl[(size_t)o * ldn + row] = (uint16_t)bits;
+        }
+    zb[0] = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
+    zb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
+}
+
+template <bool TRAIN>
+__global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
+    constexpr int NFRAG = TRAIN ? kFragAll : kFragFwd;
+    __shared__ uint4 sfrag[NFRAG * 64];
+    __shared__ float sbias[4 * kHid + kOut];
+    for (int i = threadIdx.x; i < NFRAG * 64; i += kBlock) sfrag[i] = A.frags[i];
+    for (int i = threadIdx.x; i < 4 * kHid + kOut; i += kBlock)
+        sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nwaves = gridDim.x * (kBlock / 64);
+    const DnFrameDev& F = A.F;
+    const uint32_t npix = F.width * F.height;
+    for (uint32_t base = (blockIdx.x * (kBlock / 64) + wave) * 64u; base < A.nsamples;
+         base += nwaves * 64u) {
+        uint32_t px[4], py[4], pl[4], pp[4], row[4];
+        bool valid[4], inb[4];
+        float dc[4];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t i = base + 16u * q + (uint32_t)j;
+            uint32_t p = 0, l = 0;
+            bool in = i < A.nsamples, v = false;
+            if (in) {
+                if (TRAIN) {
+                    uint32_t sm = A.samples[i];
+                    p = sm & 0xFFFFFFu;
+                    l = sm >> 24;
+                } else {
+                    p = i / A.lights;
+                    l = i - p * A.lights;
+                }
+                // caller-supplied samples outside the frame contribute nothing
+                if (p >= npix || l >= A.lights) { p = 0; l = 0; in = TRAIN ? in : false; }
+                else {
+                    const float* n = F.normal + (size_t)p * 3;
+                    v = vlen(v3(n[0], n[1], n[2])) != 0.0f;
+                }
+            }
+            pp[q] = p; pl[q] = l;
+            px[q] = p % F.width; py[q] = p / F.width;
+            row[q] = A.row_base + i;
+            dc[q] = 0.0f;
+            if (v) {
+                const float* P = F.position + (size_t)p * 3;
+                dc[q] = vlen(v3(P[0] - F.cam[0], P[1] - F.cam[1], P[2] - F.cam[2]));
+            }
+            valid[q] = v;
+            inb[q] = in;
+            any |= v;
+        }
+        if (!TRAIN && __ballot(any) == 0ull) {
+            // background only: AO and visibility 0
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (g == 0 && inb[q]) {
+                    A.ao_part[(size_t)pl[q] * npix + pp[q]] = 0.0f;
+                    if (F.num_lights) A.vis_out[(size_t)pl[q] * npix + pp[q]] = 0.0f;
+                }
+            continue;
+        }
+        // ---- layer 1: 150 (160) -> 64
+        f32x4 acc[4][4];   // [q][t]
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            bf16x8 b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint4 fb = dn_features(F, valid[q], px[q], py[q], pl[q], dc[q], 32 * s + 8 * g);
+                b[q] = as_bf16x8(fb);
+                if (TRAIN && inb[q]) {
+                    const uint32_t ws[4] = {fb.x, fb.y, fb.z, fb.w};
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        A.XT[(size_t)(32 * s + 8 * g + e) * A.ldn + row[q]] =
+                            (uint16_t)(ws[e >> 1] >> (16 * (e & 1)));
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                bf16x8 a = as_bf16x8(sfrag[(kF1 + t * 5 + s) * 64 + lane]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q][t] = mfma(a, b[q], acc[q][t]);
+            }
+        }
+        uint4 hb[4][2];
+        uint32_t mask[4][4];   // [layer][q]
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            dn_hidden_epilogue<TRAIN>(acc[q], sbias, g, hb[q], mask[0][q], A.HT, A.ldn, row[q], inb[q]);
+        // ---- layers 2..4: 64 -> 64
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            dn_layer64(sfrag, k == 1 ? kF2 : (k == 2 ? kF3 : kF4), lane, hb, acc);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                dn_hidden_epilogue<TRAIN>(acc[q], sbias + k * kHid, g, hb[q], mask[k][q],
+                                          A.HT + (size_t)k * kHid * A.ldn, A.ldn, row[q], inb[q]);
+        }
+        // ---- layer 5: 64 -> 2 (one 16-row tile, rows 0 and 1 used)
+        f32x4 out[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            bf16x8 a = as_bf16x8(sfrag[(kF5 + s) * 64 + lane]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) out[q] = mfma(a, as_bf16x8(hb[q][s]), out[q]);
+        }
+        const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
+        if (!TRAIN) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (g == 0 && inb[q]) {
+                    float yao = valid[q] ? out[q][0] + b50 : 0.0f;
+                    float yvis = valid[q] ? clamp01f(out[q][1] + b51) : 0.0f;
+                    A.ao_part[(size_t)pl[q] * npix + pp[q]] = yao;
+                    if (F.num_lights) A.vis_out[(size_t)pl[q] * npix + pp[q]] = yvis;
+                }
+            continue;
+        }
+        // ---- loss and backward
+        float loss = 0.0f;
+        uint4 db[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float d0 = 0.0f, d1 = 0.0f;
+            if (g == 0 && valid[q]) {
+                float e0 = out[q][0] + b50 - A.target_ao[pp[q]];
+                float e1 = out[q][1] + b51 - (F.num_lights ? A.target_vis[(size_t)pl[q] * npix + pp[q]] : 0.0f);
+                loss += e0 * e0 + e1 * e1;
+                d0 = A.grad_scale * e0;
+                d1 = A.grad_scale * e1;
+            }
+            uint32_t b0 = bf16_bits(d0), b1 = bf16_bits(d1);
+            if (g == 0 && inb[q]) {
+                A.YT[row[q]] = (uint16_t)b0;
+                A.YT[(size_t)A.ldn + row[q]] = (uint16_t)b1;
+            }
+            db[q] = make_uint4(g == 0 ? (b0 | b1 << 16) : 0u, 0u, 0u, 0u);
+        }
+        // dh4 = W5^T dy
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[q][t] = mfma(as_bf16x8(sfrag[(kG5 + t) * 64 + lane]), as_bf16x8(db[q]),
+                                 f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+        for (int k = 3; k >= 0; --k) {
+            uint4 zb[4][2];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                dn_relu_back(acc[q], mask[k][q], g, zb[q], A.ZT + (size_t)k * kHid * A.ldn, A.ldn, row[q],
+                             inb[q]);
+            if (k == 0) break;
+            // dh_{k} = W_{k+1}^T dz_{k+1}  (fragments G2..G4 hold W2^T..W4^T)
+            dn_layer64(sfrag, k == 3 ? kG4 : (k == 2 ? kG3 : kG2), lane, zb, acc);
+        }
+        // per-wave loss sum (lanes g == 0 hold the outputs)
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+        if (lane == 0) atomicAdd(A.loss_sum, loss);
+    }
+}
+
+// dW[o][i] = sum_n dz[o][n] h[i][n] over one chunk of samples, for layer
+// blockIdx.y (0..4); bias gradients = sum_n dz[o][n].  One partial per chunk.
+struct WgArgs {
+    const uint16_t* XT;
+    const uint16_t* HT;
+    const uint16_t* ZT;
+    const uint16_t* YT;
+    uint32_t ldn, rows;
+    float* partial;   // [chunks][kParams]
+};
+
+__global__ __launch_bounds__(kBlock) void k_dn_wgrad(WgArgs A) {
+    const int layer = blockIdx.y;
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const int wave = threadIdx.x >> 6;
+    const uint16_t* Z = layer < 4 ? A.ZT + (size_t)layer * kHid * A.ldn : A.YT;
+    const uint16_t* H = layer == 0 ? A.XT : A.HT + (size_t)(layer - 1) * kHid * A.ldn;
+    const int Mt = layer < 4 ? 4 : 1, Kt = layer == 0 ? 10 : 4;
+    const int nin = layer == 0 ? kIn : kHid, nout = layer < 4 ? kHid : kOut;
+    const uint32_t n_begin = blockIdx.x * kChunkRows;
+    const uint32_t n_end = n_begin + kChunkRows < A.rows ? n_begin + kChunkRows : A.rows;
+    float* part = A.partial + (size_t)blockIdx.x * kParams;
+    for (int tile = wave; tile < Mt * Kt; tile += kBlock / 64) {
+        const int mt = tile / Kt, kt = tile % Kt;
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const uint16_t* za = Z + (size_t)(16 * mt + j) * A.ldn + 8 * g;
+        const uint16_t* hb = H + (size_t)(16 * kt + j) * A.ldn + 8 * g;
+        for (uint32_t n0 = n_begin; n0 < n_end; n0 += 32) {
+            uint4 a = *reinterpret_cast<const uint4*>(za + n0);
+            uint4 b = *reinterpret_cast<const uint4*>(hb + n0);
+            acc = mfma(as_bf16x8(a), as_bf16x8(b), acc);
+        }
+        const int in = 16 * kt + j;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int o = 16 * mt + 4 * g + i;
+            if (o < nout && in < nin) part[kWOff[layer] + o * nin + in] = acc[i];
+        }
+    }
+    // bias: four threads per output row, each a quarter of the chunk
+    for (int r = threadIdx.x; r < nout * 4; r += kBlock) {
+        const int o = r >> 2, part4 = r & 3;
+        const uint16_t* zr = Z + (size_t)o * A.ldn;
+        float s = 0.0f;
+        for (uint32_t n = n_begin + part4 * 8; n < n_end; n += 32) {
+            uint4 v = *reinterpret_cast<const uint4*>(zr + n);
+            const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s += bf16_val((ws[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
+        }
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (part4 == 0) part[kBOff[layer] + o] = s;
+    }
+}
+
+__global__ void k_dn_reduce(const float* partial, uint32_t chunks, float* grad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)kParams) return;
+    float s = 0.0f;
+    for (uint32_t c = 0; c < chunks; ++c) s += partial[(size_t)c * kParams + i];
+    grad[i] = s;
+}
+
+__global__ void k_dn_adam(float* params, float* m, float* v, const float* grad, float lr_t, float b1,
+                          float b2, float eps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)kParams) return;
+    float gr = grad[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gr;
+    float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+    m[i] = mi;
+    v[i] = vi;
+    params[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+
+// BF16 fragments of every weight matrix from the FP32 master copy.
+__global__ void k_dn_pack(const float* P, uint16_t* frags) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (uint32_t)kFragAll * 64 * 8) return;
+    const int f = id / 512, l = (id / 8) % 64, e = id % 8;
+    const int j = l & 15, g = l >> 4;
+    float val = 0.0f;
+    if (f < kF2) {
+        const int t = f / 5, s = f % 5, o = 16 * t + j, k = 32 * s + 8 * g + e;
+        if (k < kIn) val = P[oW1 + o * kIn + k];
+    } else if (f < kF5) {
+        const int layer = 1 + (f - kF2) / 8, r = (f - kF2) % 8, t = r / 2, s = r % 2;
+        val = P[kWOff[layer] + (16 * t + j) * kHid + khid(s, g, e)];
+    } else if (f < kG2) {
+        const int s = f - kF5;
+        if (j < kOut) val = P[oW5 + j * kHid + khid(s, g, e)];
+    } else if (f < kG5) {
+        const int layer = 1 + (f - kG2) / 8, r = (f - kG2) % 8, t = r / 2, s = r % 2;
+        val = P[kWOff[layer] + khid(s, g, e) * kHid + (16 * t + j)];
+    } else {
+        const int t = f - kG5, o = khid(0, g, e);
+        if (o < kOut) val = P[oW5 + o * kHid + (16 * t + j)];
+    }
+    frags[id] = (uint16_t)bf16_bits(val);
+}
+
+// AO of a pixel: mean over its samples' AO outputs, clamped to [0, 1].
+__global__ void k_dn_ao_mean(const float* ao_part, uint32_t lights, uint32_t npix, float* ao_out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float s = 0.0f;
+    for (uint32_t l = 0; l < lights; ++l) s += ao_part[(size_t)l * npix + p];
+    ao_out[p] = clamp01f(s / (float)lights);
+}
+
+// Input features of explicit samples, [count][150] BF16 bits (inspection).
+__global__ __launch_bounds__(kBlock) void k_dn_features(DnFrameDev F, const uint32_t* samples,
+                                                        uint32_t count, uint16_t* out) {
+    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = id / 19, grp = id % 19;   // 19 groups of 8 cover 150 features
+    if (i >= count) return;
+    uint32_t sm = samples[i], p = sm & 0xFFFFFFu, l = sm >> 24;
+    const uint32_t lights = F.num_lights ? F.num_lights : 1u;
+    bool valid = p < F.width * F.height && l < lights;
+    if (!valid) { p = 0; l = 0; }
+    const float* n = F.normal + (size_t)p * 3;
+    valid = valid && vlen(v3(n[0], n[1], n[2])) != 0.0f;
+    float dc = 0.0f;
+    if (valid) {
+        const float* P = F.position + (size_t)p * 3;
+        dc = vlen(v3(P[0] - F.cam[0], P[1] - F.cam[1], P[2] - F.cam[2]));
+    }
+    uint4 fb = dn_features(F, valid, p % F.width, p / F.width, l, dc, 8 * (int)grp);
+    const uint32_t ws[4] = {fb.x, fb.y, fb.z, fb.w};
+    for (int e = 0; e < 8; ++e) {
+        int f = 8 * (int)grp + e;
+        if (f < kIn) out[(size_t)i * kIn + f] = (uint16_t)(ws[e >> 1] >> (16 * (e & 1)));
+    }
+}
+
+inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+}  // namespace luz
+
+using namespace luz;
+
+struct luzrt_denoiser {
+    uint32_t capacity = 0;     // training samples per gradient
+    uint32_t rows = 0;         // samples accumulated since the last gradient
+    uint32_t step = 0;         // Adam steps taken
+    float* params = nullptr;   // FP32 master
+    float* grad = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* partial = nullptr;
+    float* loss = nullptr;     // device loss sum
+    uint16_t* frags = nullptr;
+    uint16_t* XT = nullptr;
+    uint16_t* HT = nullptr;
+    uint16_t* ZT = nullptr;
+    uint16_t* YT = nullptr;
+    DevBuf ao_part;
+};
+
+namespace {
+
+void dn_free(luzrt_denoiser* d) {
+    for (void* p : {(void*)d->params, (void*)d->grad, (void*)d->adam_m, (void*)d->adam_v,
+                    (void*)d->partial, (void*)d->loss, (void*)d->frags, (void*)d->XT, (void*)d->HT,
+                    (void*)d->ZT, (void*)d->YT, d->ao_part.ptr})
+        if (p) (void)hipFree(p);
+    delete d;
+}
+
+luzrt_status frame_dev(luzrt_ctx* ctx, const luzrt_dn_frame* f, DnFrameDev& F) {
+    if (!f || !f->position || !f->normal || !f->ao_counts || (f->num_lights && !f->shadow_counts))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "denoiser frame: null plane");
+    if (f->width == 0 || f->height == 0 || (uint64_t)f->width * f->height > 0xFFFFFFull)
+        return fail(ctx, LUZRT_E_LIMIT, "denoiser frame: %ux%u outside 1..2^24 pixels", f->width, f->height);
+    if (f->num_lights > LUZRT_MAX_LIGHTS || f->ao_samples == 0 || (f->num_lights && f->shadow_samples == 0))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "denoiser frame: bad light or sample counts");
+    std::memset(&F, 0, sizeof(F));
+    F.width = f->width;
+    F.height = f->height;
+    F.num_lights = f->num_lights;
+    F.position = f->position;
+    F.normal = f->normal;
+    F.shadow = f->shadow_counts;
+    F.ao = f->ao_counts;
+    for (int k = 0; k < 3; ++k) F.cam[k] = f->cam_pos[k];
+    F.shadow_n = (float)(f->shadow_samples ? f->shadow_samples : 1u);
+    F.ao_n = (float)f->ao_samples;
+    return LUZRT_OK;
+}
+
+luzrt_status repack(luzrt_ctx* ctx, luzrt_denoiser* d) {
+    const uint32_t n = kFragAll * 64 * 8;
+    k_dn_pack<<<cdiv(n, 256), 256, 0, ctx->stream>>>(d->params, d->frags);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+luzrt_status luzrt_denoiser_create(luzrt_ctx* ctx, uint32_t batch_capacity, uint64_t seed,
+                                   luzrt_denoiser** out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!out) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_denoiser_create: null argument");
+    *out = nullptr;
+    if (batch_capacity % 64 != 0 || batch_capacity > (1u << 22))
+        return fail(ctx, LUZRT_E_LIMIT, "batch capacity %u: a multiple of 64 up to 2^22", batch_capacity);
+    LUZ_HIP(ctx, hipSetDevice(ctx->device));
+    auto* d = new luzrt_denoiser();
+    d->capacity = batch_capacity;
+    const uint32_t cap = batch_capacity ? batch_capacity : 64;
+    const uint32_t chunks = cdiv(cap, kChunkRows);
+    const size_t pb = (size_t)kParams * 4;
+    struct A { void** p; size_t bytes; } allocs[] = {
+        {(void**)&d->params, pb}, {(void**)&d->grad, pb}, {(void**)&d->adam_m, pb},
+        {(void**)&d->adam_v, pb}, {(void**)&d->partial, pb * chunks}, {(void**)&d->loss, 256},
+        {(void**)&d->frags, (size_t)kFragAll * 1024}, {(void**)&d->XT, (size_t)160 * cap * 2},
+        {(void**)&d->HT, (size_t)4 * kHid * cap * 2}, {(void**)&d->ZT, (size_t)4 * kHid * cap * 2},
+        {(void**)&d->YT, (size_t)16 * cap * 2}};
+    for (auto& a : allocs) {
+        hipError_t e = hipMalloc(a.p, a.bytes);
+        if (e != hipSuccess) { dn_free(d); return hip_fail(ctx, e, "hipMalloc(denoiser)"); }
+    }
+    // rows 2..15 of dy are never written: zero them once
+    LUZ_HIP(ctx, hipMemset(d->YT, 0, (size_t)16 * cap * 2));
+    LUZ_HIP(ctx, hipMemset(d->adam_m, 0, pb));
+    LUZ_HIP(ctx, hipMemset(d->adam_v, 0, pb));
+    LUZ_HIP(ctx, hipMemset(d->grad, 0, pb));
+    LUZ_HIP(ctx, hipMemset(d->loss, 0, 256));
+    // He-uniform initial weights (SplitMix64), zero biases
+    std::vector<float> h(kParams, 0.0f);
+    uint64_t st = seed;
+    auto next = [&st]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int fan_in[5] = {kIn, kHid, kHid, kHid, kHid}, rows_out[5] = {kHid, kHid, kHid, kHid, kOut};
+    for (int L = 0; L < 5; ++L) {
+        double lim = std::sqrt(6.0 / fan_in[L]);
+        for (int i = 0; i < rows_out[L] * fan_in[L]; ++i) h[kWOff[L] + i] = (float)((2.0 * next() - 1.0) * lim);
+    }
+    LUZ_HIP(ctx, hipMemcpy(d->params, h.data(), pb, hipMemcpyHostToDevice));
+    luzrt_status s = repack(ctx, d);
+    if (s != LUZRT_OK) { dn_free(d); return s; }
+    *out = d;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoiser_destroy(luzrt_ctx* ctx, luzrt_denoiser* d) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!d) return LUZRT_OK;
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dn_free(d);
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoiser_get_params(luzrt_ctx* ctx, luzrt_denoiser* d, float* host, uint32_t count) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!d || !host || count != (uint32_t)kParams)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "denoiser parameters: need %d floats", kParams);
+    LUZ_HIP(ctx, hipMemcpyAsync(host, d->params, (size_t)kParams * 4, hipMemcpyDeviceToHost, ctx->stream));
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoiser_set_params(luzrt_ctx* ctx, luzrt_denoiser* d, const float* host, uint32_t count) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!d || !host || count != (uint32_t)kParams)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "denoiser parameters: need %d floats", kParams);
+    for (uint32_t i = 0; i < count; ++i)
+        if (!std::isfinite(host[i])) return fail(ctx, LUZRT_E_INVALID_ARG, "parameter %u is not finite", i);
+    LUZ_HIP(ctx, hipMemcpyAsync(d->params, host, (size_t)kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    LUZ_HIP(ctx, hipMemsetAsync(d->adam_m, 0, (size_t)kParams * 4, ctx->stream));
+    LUZ_HIP(ctx, hipMemsetAsync(d->adam_v, 0, (size_t)kParams * 4, ctx->stream));
+    d->step = 0;
+    luzrt_status s = repack(ctx, d);
+    if (s != LUZRT_OK) return s;
+    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoise(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_frame* f, float* ao_out,
+                           float* vis_out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!d || !ao_out || (f && f->num_lights && !vis_out))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_denoise: null argument");
+    DnArgs A{};
+    luzrt_status s = frame_dev(ctx, f, A.F);
+    if (s != LUZRT_OK) return s;
+    const uint32_t npix = f->width * f->height;
+    const uint32_t lights = f->num_lights ? f->num_lights : 1u;
+    if ((s = ensure(ctx, d->ao_part, (size_t)lights * npix * 4)) != LUZRT_OK) return s;
+    A.frags = reinterpret_cast<const uint4*>(d->frags);
+    A.params = d->params;
+    A.nsamples = npix * lights;
+    A.lights = lights;
+    A.ao_part = (float*)d->ao_part.ptr;
+    A.vis_out = vis_out;
+    uint32_t groups = cdiv(A.nsamples, kBlock);
+    const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * 8;
+    if (groups > maxg) groups = maxg;
+    stage_begin(ctx, kStageDenoise);
+    k_dn_forward<false><<<groups, kBlock, 0, ctx->stream>>>(A);
+    k_dn_ao_mean<<<cdiv(npix, 256), 256, 0, ctx->stream>>>((const float*)d->ao_part.ptr, lights, npix, ao_out);
+    stage_end(ctx, kStageDenoise);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoiser_features(luzrt_ctx* ctx, const luzrt_dn_frame* f, const uint32_t* samples,
+                                     uint32_t count, uint16_t* out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!samples || !out) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_denoiser_features: null argument");
+    DnFrameDev F;
+    luzrt_status s = frame_dev(ctx, f, F);
+    if (s != LUZRT_OK) return s;
+    if (count == 0) return LUZRT_OK;
+    k_dn_features<<<cdiv((uint64_t)count * 19, kBlock), kBlock, 0, ctx->stream>>>(F, samples, count, out);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoiser_accumulate(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_frame* f,
+                                       const float* target_ao, const float* target_vis,
+                                       const uint32_t* samples, uint32_t count, float grad_scale) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!d || !target_ao || !samples || (f && f->num_lights && !target_vis))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_denoiser_accumulate: null argument");
+    if (count % 64 != 0 || d->rows + count > d->capacity)
+        return fail(ctx, LUZRT_E_LIMIT, "%u samples (a multiple of 64) over the batch capacity %u (%u used)",
+                    count, d->capacity, d->rows);
+    DnArgs A{};
+    luzrt_status s = frame_dev(ctx, f, A.F);
+    if (s != LUZRT_OK) return s;
+    if (count == 0) return LUZRT_OK;
+    A.frags = reinterpret_cast<const uint4*>(d->frags);
+    A.params = d->params;
+    A.nsamples = count;
+    A.lights = f->num_lights ? f->num_lights : 1u;
+    A.samples = samples;
+    A.target_ao = target_ao;
+    A.target_vis = target_vis;
+    A.row_base = d->rows;
+    A.ldn = d->capacity;
+    A.XT = d->XT;
+    A.HT = d->HT;
+    A.ZT = d->ZT;
+    A.YT = d->YT;
+    A.grad_scale = grad_scale;
+    A.loss_sum = d->loss;
+    uint32_t groups = cdiv(count, kBlock);
+    const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * 4;
+    if (groups > maxg) groups = maxg;
+    k_dn_forward<true><<<groups, kBlock, 0, ctx->stream>>>(A);
+    LUZ_HIP(ctx, hipGetLastError());
+    d->rows += count;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoiser_gradient(luzrt_ctx* ctx, luzrt_denoiser* d, float* host_grad, float* loss_sum) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!d) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_denoiser_gradient: null denoiser");
+    hipStream_t st = ctx->stream;
+    if (d->rows > 0) {
+        WgArgs W{d->XT, d->HT, d->ZT, d->YT, d->capacity, d->rows, d->partial};
+        const uint32_t chunks = cdiv(d->rows, kChunkRows);
+        k_dn_wgrad<<<dim3(chunks, 5), kBlock, 0, st>>>(W);
+        k_dn_reduce<<<cdiv(kParams, 256), 256, 0, st>>>(d->partial, chunks, d->grad);
+        LUZ_HIP(ctx, hipGetLastError());
+    } else {
+        LUZ_HIP(ctx, hipMemsetAsync(d->grad, 0, (size_t)kParams * 4, st));
+    }
+    d->rows = 0;
+    if (host_grad) LUZ_HIP(ctx, hipMemcpyAsync(host_grad, d->grad, (size_t)kParams * 4, hipMemcpyDeviceToHost, st));
+    if (loss_sum) LUZ_HIP(ctx, hipMemcpyAsync(loss_sum, d->loss, 4, hipMemcpyDeviceToHost, st));
+    LUZ_HIP(ctx, hipMemsetAsync(d->loss, 0, 4, st));
+    if (host_grad || loss_sum) LUZ_HIP(ctx, hipStreamSynchronize(st));
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_denoiser_adam(luzrt_ctx* ctx, luzrt_denoiser* d, float lr, float beta1, float beta2,
+                                 float eps) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!d || !(lr > 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f) || !(eps > 0.0f))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_denoiser_adam: bad arguments");
+    d->step++;
+    // bias-corrected step size
+    double c1 = 1.0 - std::pow((double)beta1, (double)d->step);
+    double c2 = 1.0 - std::pow((double)beta2, (double)d->step);
+    float lr_t = (float)(lr * std::sqrt(c2) / c1);
+    k_dn_adam<<<cdiv(kParams, 256), 256, 0, ctx->stream>>>(d->params, d->adam_m, d->adam_v, d->grad, lr_t,
+                                                          beta1, beta2, eps);
+    LUZ_HIP(ctx, hipGetLastError());
+    return repack(ctx, d);
+}
+
+}  // extern "C"

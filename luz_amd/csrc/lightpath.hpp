@@ -168,10 +168,42 @@ __device__ inline void resolve_pixel(const DevScene* ds, uint32_t a, uint32_t A,
     ao[p] = (uint8_t)miss;
 }
 
+// Visibility from per-pixel sample counts: occluded shadow samples of light
+// l (plane l) out of its sample count, unoccluded AO samples out of n_AO
+// (light.frag:104-108, 130-134).
+struct CountVisibility {
+    const uint8_t* shadow;
+    const uint8_t* ao;
+    uint32_t npix;
+    __device__ float shadow_frac(const luzrt_scene& S, uint32_t l, uint32_t p) const {
+        int ns = S.lights[l].num_shadow_samples;
+        return ns > 0 ? (float)shadow[(size_t)l * npix + p] / (float)ns : 0.0f;
+    }
+    __device__ float ao_frac(const luzrt_scene& S, uint32_t p) const {
+        int na = S.ao_num_samples;
+        return na > 0 ? (float)ao[p] / (float)na : 1.0f;
+    }
+};
+
+// Visibility from denoised fractions: visibility of light l (plane l) and
+// the AO fraction; lights without shadow samples are unshadowed, and AO is
+// 1 when the scene casts no AO rays, as with counts.
+struct FractionVisibility {
+    const float* vis;
+    const float* ao;
+    uint32_t npix;
+    __device__ float shadow_frac(const luzrt_scene& S, uint32_t l, uint32_t p) const {
+        return S.lights[l].num_shadow_samples > 0 ? 1.0f - vis[(size_t)l * npix + p] : 0.0f;
+    }
+    __device__ float ao_frac(const luzrt_scene& S, uint32_t p) const {
+        return S.ao_num_samples > 0 ? ao[p] : 1.0f;
+    }
+};
+
 // light.frag:main in FP32; lut = pow(i/255, 2.2).
+template <typename VIS>
 __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds, const float* lut,
-                                     const uint8_t* shadow, const uint8_t* ao, uint32_t p) {
-    const uint32_t npix = g.width * g.height;
+                                     const VIS& vis, uint32_t p) {
     const luzrt_scene& S = ds->s;
     V3 N = v3(g.normal[(size_t)p * 3], g.normal[(size_t)p * 3 + 1], g.normal[(size_t)p * 3 + 2]);
     if (vlen(N) == 0.0f)
@@ -212,8 +244,7 @@ __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds,
             float dist = vlen(Lr);
             att = 1.0f / (dist * dist);
         }
-        int ns = Lb.num_shadow_samples;
-        float sh = ns > 0 ? (float)shadow[(size_t)l * npix + p] / (float)ns : 0.0f;
+        float sh = vis.shadow_frac(S, l, p);
         float scale = Lb.intensity * att * (1.0f - sh);
         V3 H = vnorm(vadd(V, L));
         float NdotH = fmax_(vdot(N, H), 0.0f);
@@ -234,8 +265,7 @@ __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds,
             Lo[k] = Lo[k] + (kD * albedo[k] / kPiF + spec) * radiance * NdotL;
         }
     }
-    int na = S.ao_num_samples;
-    float aof = na > 0 ? (float)ao[p] / (float)na : 1.0f;
+    float aof = vis.ao_frac(S, p);
     float o[3];
     for (int k = 0; k < 3; ++k) {
         float amb = S.ambient_color[k] * S.ambient_intensity * albedo[k] * occl * aof;

@@ -25,7 +25,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(const luzrt_gbuffer g, const D
     __syncthreads();
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= g.width * g.height) return;
-    out[p] = shade_pixel(g, ds, s_lut, shadow, ao, p);
+    out[p] = shade_pixel(g, ds, s_lut, CountVisibility{shadow, ao, g.width * g.height}, p);
+}
+
+// The same lighting from denoised visibility and AO fractions.
+__global__ __launch_bounds__(kBlock) void k_shade_fractions(const luzrt_gbuffer g, const DevScene* ds,
+                                                            const float* lut, const float* vis,
+                                                            const float* ao, float4* out) {
+    __shared__ float s_lut[256];
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= g.width * g.height) return;
+    out[p] = shade_pixel(g, ds, s_lut, FractionVisibility{vis, ao, g.width * g.height}, p);
 }
 
 // Compact form of the lighting kernel: per-light geometry and the specular
@@ -187,6 +199,28 @@ extern "C" luzrt_status luzrt_shade(luzrt_ctx* ctx, const luzrt_gbuffer* g, cons
     if (s != LUZRT_OK) return s;
     if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
     return shade_device(ctx, g, shadow, ao, out);
+}
+
+extern "C" luzrt_status luzrt_shade_fractions(luzrt_ctx* ctx, const luzrt_gbuffer* g,
+                                              const luzrt_scene* scene, const float* visibility,
+                                              const float* ao, float* out) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!g || !scene || !ao || !out || (scene->num_lights > 0 && !visibility))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_shade_fractions: null argument");
+    if (((uintptr_t)out & 15u) != 0)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_shade_fractions: output must be 16-byte aligned");
+    DevScene ds;
+    luzrt_status s = upload_scene(ctx, scene, &ds);
+    if (s != LUZRT_OK) return s;
+    if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
+    if ((s = ensure_lut(ctx)) != LUZRT_OK) return s;
+    const uint32_t npix = g->width * g->height;
+    stage_begin(ctx, kStageShade);
+    k_shade_fractions<<<(npix + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(
+        *g, (const DevScene*)ctx->scene.ptr, (const float*)ctx->lut.ptr, visibility, ao, (float4*)out);
+    stage_end(ctx, kStageShade);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
 }
 
 extern "C" luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas,

@@ -41,3 +41,29 @@ def set_light_direction(sb: api.SceneBlock, index: int, d) -> None:
     for k in range(3):
         sb.lights[index].direction[k] = float(d[k])
         sb.lights[index].position[k] = float(-d[k] * 1000.0)
+
+
+def denoise_rmse(ctx, dn, fr) -> dict:
+    """RMSE over foreground pixels against the frame's multi-spp targets of the
+    raw 1-spp estimate, the 5x5 box filter (non-learned) and the network."""
+    from oracle import denoise as odn
+    n = fr.width * fr.height
+    L = fr.num_lights
+    ao_o, vis_o = ctx.buffer(n * 4), ctx.buffer(max(L, 1) * n * 4)
+    dn.denoise(fr, ao_o, vis_o)
+    ao_d = ao_o.download(np.float32, n).astype(np.float64)
+    vis_d = vis_o.download(np.float32, (max(L, 1), n)).astype(np.float64)[:L]
+    host = fr.host()
+    fg = fr.foreground
+    ao_raw = host["ao"].astype(np.float64) / host["ao_n"]
+    vis_raw = 1.0 - host["shadow"].astype(np.float64) / max(host["shadow_n"], 1)
+    box = [odn.box_filter(host, l) for l in range(max(L, 1))]
+    box_ao = np.mean([b[0] for b in box], 0)
+    box_vis = np.array([b[1] for b in box[:L]])
+
+    def r(a, b):
+        return float(np.sqrt(np.mean((a[..., fg] - b[..., fg]) ** 2)))
+
+    ta, tv = fr.target_ao_host.astype(np.float64), fr.target_vis_host.astype(np.float64)
+    return {"ao": {"raw": r(ao_raw, ta), "box": r(box_ao, ta), "mlp": r(ao_d, ta)},
+            "vis": {"raw": r(vis_raw, tv), "box": r(box_vis, tv), "mlp": r(vis_d, tv)}}

@@ -2,11 +2,13 @@
 and keeps the documented struct layouts and error behaviour (no GPU needed)."""
 import ctypes as C
 import re
+import shutil
+import subprocess
 from pathlib import Path
 
 import pytest
 
-from luz_amd import api
+from luz_amd import api, denoiser
 
 ROOT = Path(__file__).resolve().parents[1]
 DECL = re.compile(r"^\s*(?:const\s+)?(?:\w+\s*\*?\s+)+\**\s*(luz(?:rt|scene|gpu)_\w+)\s*\(", re.M)
@@ -25,7 +27,7 @@ def test_every_declared_function_is_exported():
     missing = [n for n in sorted(names) if not hasattr(api.lib, n)]
     assert not missing, missing
     # and the Python binding covers them all
-    assert names <= set(api.EXPORTED) | {"luzrt_status_string"}
+    assert names <= set(api.EXPORTED) | set(denoiser.EXPORTED) | {"luzrt_status_string"}
 
 
 def test_struct_layouts():
@@ -35,6 +37,31 @@ def test_struct_layouts():
     assert C.sizeof(api.Material) == 16
     assert C.sizeof(api.Camera) == 56
     assert api.SceneBlock.num_lights.offset == 64 * 64 + 28
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs a C compiler")
+def test_struct_layouts_match_the_c_header(tmp_path):
+    """sizeof/offsetof of the structs passed by pointer, from the C compiler."""
+    src = tmp_path / "layout.c"
+    src.write_text("""
+#include <stddef.h>
+#include <stdio.h>
+#include "luzrt.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\\n", sizeof(luzrt_batch_scene), offsetof(luzrt_batch_scene, camera),
+         offsetof(luzrt_batch_scene, rgba32f_out), sizeof(luzrt_dn_frame));
+  printf("%zu %zu %zu %zu\\n", offsetof(luzrt_dn_frame, ao_samples), sizeof(luzrt_timings),
+         offsetof(luzrt_timings, denoise_ms), offsetof(luzrt_timings, rays));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(api.BatchScene), api.BatchScene.camera.offset, api.BatchScene.rgba32f_out.offset,
+            C.sizeof(denoiser.DnFrame), denoiser.DnFrame.ao_samples.offset, C.sizeof(api.Timings),
+            api.Timings.denoise_ms.offset, api.Timings.rays.offset]
+    assert got == want
 
 
 def test_status_strings_and_null_context():

@@ -1,0 +1,156 @@
+"""GPU parity of the learned denoiser: BF16 input features bit-exact with the
+oracle, MFMA forward and weight gradients against the FP64 oracle, gradients
+deterministic, training on tracer-generated pairs reduces the error, and the
+trained network beats the raw 1-spp estimate and a non-learned box filter."""
+import numpy as np
+import pytest
+
+from luz_amd import denoiser as dnm
+from oracle import denoise as odn
+from tests.helpers import denoise_rmse
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def frame(ctx):
+    return dnm.make_frame(ctx, 3, crop=(96, 64), target_spp=64, focus="instance")
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    mats = []
+    for shape in odn.SHAPES:
+        fan = shape[1] if len(shape) == 2 else 64
+        if len(shape) == 2:
+            mats.append(rng.normal(0.0, 1.0 / np.sqrt(fan), shape))
+        else:
+            mats.append(rng.uniform(-0.1, 0.2, shape))
+    return odn.pack(mats).astype(np.float32)
+
+
+def _all_samples(fr):
+    L = max(fr.num_lights, 1)
+    px = np.repeat(fr.foreground.astype(np.int64), L)
+    li = np.tile(np.arange(L), len(fr.foreground))
+    return px, li
+
+
+def test_features_bit_exact(ctx, frame):
+    px, li = _all_samples(frame)
+    print(f"{len(px)} samples of a {frame.width}x{frame.height} frame", flush=True)
+    got = dnm.features(ctx, frame, (px | (li << 24)).astype(np.uint32))
+    want = odn.features(frame.host(), px, li)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    bg = np.setdiff1d(np.arange(frame.width * frame.height), frame.foreground)[:5]
+    if len(bg):
+        assert not dnm.features(ctx, frame, bg.astype(np.uint32)).any()
+
+
+def test_forward_matches_fp64_oracle(ctx, frame):
+    dn = dnm.Denoiser(ctx, 0, seed=1)
+    dn.params = _params(7)
+    n, L = frame.width * frame.height, frame.num_lights
+    ao_o, vis_o = ctx.buffer(n * 4), ctx.buffer(L * n * 4)
+    dn.denoise(frame, ao_o, vis_o)
+    ao = ao_o.download(np.float32, n).astype(np.float64)
+    vis = vis_o.download(np.float32, (L, n)).astype(np.float64)
+    px, li = _all_samples(frame)
+    y, _, _ = odn.forward(odn.device_weights(dn.params), odn.features(frame.host(), px, li))
+    want_vis = np.clip(y[:, 1], 0, 1)
+    want_ao = np.clip(y[:, 0].reshape(-1, L).mean(1), 0, 1)
+    fg = frame.foreground
+    err_v = vis[li, px] - want_vis
+    err_a = ao[fg] - want_ao
+    print(f"outputs: y0 in [{y[:, 0].min():.3f}, {y[:, 0].max():.3f}], y1 in [{y[:, 1].min():.3f}, "
+          f"{y[:, 1].max():.3f}]; vis err max {np.abs(err_v).max():.2e} rms {np.sqrt(np.mean(err_v ** 2)):.2e}; "
+          f"ao err max {np.abs(err_a).max():.2e}", flush=True)
+    assert np.abs(err_v).max() < 2e-2 and np.sqrt(np.mean(err_v ** 2)) < 3e-3
+    assert np.abs(err_a).max() < 2e-2 and np.sqrt(np.mean(err_a ** 2)) < 3e-3
+    bgm = np.ones(n, bool)
+    bgm[fg] = False
+    assert not ao[bgm].any() and not vis[:, bgm].any()
+    dn.close()
+
+
+def test_gradient_matches_fp64_oracle_and_is_deterministic(ctx, frame):
+    N = 4096
+    dn = dnm.Denoiser(ctx, N, seed=1)
+    dn.params = _params(9)
+    rng = np.random.default_rng(4)
+    px = frame.foreground[rng.integers(0, len(frame.foreground), N)].astype(np.int64)
+    li = rng.integers(0, frame.num_lights, N)
+    buf = ctx.to_device((px | (li << 24)).astype(np.uint32))
+    grads = []
+    for _ in range(2):
+        dn.accumulate(frame, buf, N, 1.0 / N)
+        g, sse = dn.gradient(host=True, loss=True)
+        grads.append(g)
+    np.testing.assert_array_equal(grads[0].view(np.uint32), grads[1].view(np.uint32))
+    w = odn.device_weights(dn.params)
+    X = odn.features(frame.host(), px, li)
+    T = np.stack([frame.target_ao_host[px], frame.target_vis_host[li, px]], 1).astype(np.float64)
+    want = odn.backward(w, X, T, scale=1.0 / N)
+    y, _, _ = odn.forward(w, X)
+    want_sse = float(np.sum((y - T) ** 2))
+    print(f"sse gpu {sse:.6f} oracle {want_sse:.6f}", flush=True)
+    assert abs(sse - want_sse) <= 1e-2 * want_sse
+    o = 0
+    for k, shape in enumerate(odn.SHAPES):
+        m = int(np.prod(shape))
+        a, b = grads[0][o:o + m].astype(np.float64), want[o:o + m]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        print(f"param block {k} {shape}: rel err {rel:.2e} cos {cos:.6f}", flush=True)
+        assert rel < 2e-2 and cos > 0.9995
+        o += m
+    dn.close()
+
+
+def test_training_on_tracer_pairs_reduces_error(ctx):
+    frames = [dnm.make_frame(ctx, 200 + i, crop=(64, 64), target_spp=64, focus="instance") for i in range(16)]
+    held = dnm.make_frame(ctx, 300, crop=(128, 128), target_spp=128, focus="instance")
+    print(f"held-out frame: {len(held.hard)} noisy samples", flush=True)
+    dn = dnm.Denoiser(ctx, 8 * 2048, seed=0)
+    before = denoise_rmse(ctx, dn, held)
+    hist = dnm.train(ctx, dn, frames, steps=300, frames_per_step=8, samples_per_frame=2048, lr=3e-3,
+                     log_every=50, log=lambda m: print(m, flush=True))
+    after = denoise_rmse(ctx, dn, held)
+    print(f"held-out before {before}\nafter {after}", flush=True)
+    assert hist[-1][1] < 0.5 * hist[0][1]
+    assert after["ao"]["mlp"] < after["ao"]["raw"] and after["vis"]["mlp"] < after["vis"]["raw"]
+    dn.close()
+
+
+@pytest.mark.skipif(not dnm.WEIGHTS.exists(), reason="no trained weights committed")
+def test_trained_network_beats_unlearned_estimates(ctx):
+    dn = dnm.Denoiser(ctx, 0)
+    dn.load(dnm.WEIGHTS)
+    for seed in (900001, 900002):
+        fr = dnm.make_frame(ctx, seed, crop=(256, 256), target_spp=256, focus="instance")
+        r = denoise_rmse(ctx, dn, fr)
+        print(f"scene {seed}: {r}", flush=True)
+        for ch in ("ao", "vis"):
+            assert r[ch]["mlp"] < r[ch]["box"] and r[ch]["mlp"] < r[ch]["raw"]
+    dn.close()
+
+
+def test_shade_fractions_equals_shade_on_exact_fractions(ctx, frame):
+    """With 1-spp counts the fractions are exactly 0 or 1, so lighting from
+    fractions must reproduce lighting from counts bit for bit."""
+    n, L = frame.width * frame.height, frame.num_lights
+    g = frame.gbuffer.desc()
+    sb = frame.scene_block
+    for l in range(L):
+        sb.lights[l].num_shadow_samples = 1
+    sb.ao_num_samples = 1
+    # the G-buffer's material planes were released: give it neutral ones
+    planes = [ctx.to_device(np.full(n, v, np.uint32)) for v in (0xFF8040C0, 0xFFFF8060, 0xFF000000)]
+    g.albedo, g.material, g.emission = (p.ptr for p in planes)
+    out_c, out_f = ctx.buffer(n * 16), ctx.buffer(n * 16)
+    ctx.shade(g, sb, frame.shadow, frame.ao, out_c)
+    sh = frame.shadow.download(np.uint8, (L, n)).astype(np.float32)
+    vis = ctx.to_device(1.0 - sh)
+    aof = ctx.to_device(frame.ao.download(np.uint8, n).astype(np.float32))
+    ctx.shade_fractions(g, sb, vis, aof, out_f)
+    np.testing.assert_array_equal(out_f.download(np.uint32, (n, 4)), out_c.download(np.uint32, (n, 4)))
