@@ -135,7 +135,8 @@ typedef struct luzrt_material {
 /* GPU time of the stages of the last call, from HIP events on the
  * context's stream (the reference's CmdTimeStamp, VulkanWrapper.cpp:1276-1296). */
 typedef struct luzrt_timings {
-    float build_ms, compact_ms, raygen_ms, trace_ms, resolve_ms, shade_ms, primary_ms, denoise_ms;
+    float build_ms, compact_ms, raygen_ms, trace_ms, resolve_ms, shade_ms, primary_ms, denoise_ms,
+          post_ms;
     uint64_t rays;          /* rays cast by the last visibility/light call */
     uint64_t active_pixels; /* non-background pixels of that call */
 } luzrt_timings;
@@ -240,6 +241,38 @@ luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzr
 luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* tlas,
                                    const luzrt_camera* camera, const luzrt_material* materials,
                                    uint32_t material_count, const luzrt_gbuffer_out* out);
+
+/* ------------------------------------------------------------ after the light pass */
+/* Temporal accumulation of the light buffer (taa.comp:main, :271-307) and the
+ * compose/tonemap of the result (present.frag:87-90, TonemapACES :27-35).
+ * Matrices are column-major 4x4 (glm): view_proj is the current (jittered)
+ * projection x view, prev_view_proj the previous frame's, inv_view_proj the
+ * inverse of view_proj (used for background texels, whose depth is the far
+ * plane).  Depth is the NDC depth of the position plane; images use the
+ * reference's REPEAT addressing.  Set reconstruct to use the 3x3 Mitchell
+ * reconstruction of the source (PostProcessingConstants.reconstruct). */
+typedef struct luzrt_taa_desc {
+    uint32_t width, height;
+    const float* light;         /* device RGBA32F, this frame's light buffer */
+    const float* history;       /* device RGBA32F, the previous TAA output */
+    float* out;                 /* device RGBA32F (must not alias light or history) */
+    const float* position;      /* device G-buffer position plane (float3) */
+    const float* normal;        /* device G-buffer normal plane (zero = background) */
+    float view_proj[16];
+    float prev_view_proj[16];
+    float inv_view_proj[16];
+    float jitter[2];            /* NDC jitter of view_proj (CameraNode::GetJitter) */
+    float prev_jitter[2];
+    int32_t reconstruct;
+    int32_t pad;
+} luzrt_taa_desc;
+
+luzrt_status luzrt_taa(luzrt_ctx* ctx, const luzrt_taa_desc* desc);
+/* ACES tonemap + 1/2.2 gamma of the light buffer: RGBA8 UNORM (round half
+ * up, alpha 255) and/or the FP32 tonemapped colour (alpha 1); either output
+ * may be NULL. */
+luzrt_status luzrt_compose(luzrt_ctx* ctx, uint32_t width, uint32_t height, const float* light_rgba32f,
+                           uint32_t* rgba8_out, float* rgba32f_out);
 
 /* ------------------------------------------------------------ batched frames */
 /* Many independent scenes of uneven size and resolution rendered as one

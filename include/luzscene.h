@@ -61,6 +61,12 @@ int      luzscene_nodes(const luzscene* s, float* object_to_world, uint32_t* mes
 int      luzscene_pack(const luzscene* s, luzrt_scene* out);
 /* Image size and primary-ray camera (CameraNode::GetView/GetProj conventions). */
 int      luzscene_camera(const luzscene* s, uint32_t* width, uint32_t* height, luzrt_camera* out);
+/* Column-major view-projection (CameraNode::GetProjJittered * GetView) with
+ * the given NDC jitter (NULL = none), and its inverse (may be NULL). */
+int      luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj[16],
+                            float inv_view_proj[16]);
+/* The camera jitter of frame `index` (CameraNode::NextJitter): Halton(2,3). */
+void     luzscene_jitter(uint32_t index, uint32_t width, uint32_t height, float jitter[2]);
 /* Seeded rigid motion of every mesh node for animation frame `frame`. */
 int      luzscene_animate(luzscene* s, uint32_t frame);
 /* Total triangles over all mesh nodes (instanced). */

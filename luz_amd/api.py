@@ -80,10 +80,19 @@ class Instance(C.Structure):
                 ("custom_index", C.c_uint32), ("mask", C.c_uint32)]
 
 
+class TaaDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("light", C.c_void_p),
+                ("history", C.c_void_p), ("out", C.c_void_p), ("position", C.c_void_p),
+                ("normal", C.c_void_p), ("view_proj", C.c_float * 16), ("prev_view_proj", C.c_float * 16),
+                ("inv_view_proj", C.c_float * 16), ("jitter", C.c_float * 2), ("prev_jitter", C.c_float * 2),
+                ("reconstruct", C.c_int32), ("pad", C.c_int32)]
+
+
 class Timings(C.Structure):
     _fields_ = [("build_ms", C.c_float), ("compact_ms", C.c_float), ("raygen_ms", C.c_float),
                 ("trace_ms", C.c_float), ("resolve_ms", C.c_float), ("shade_ms", C.c_float),
-                ("primary_ms", C.c_float), ("denoise_ms", C.c_float), ("rays", C.c_uint64),
+                ("primary_ms", C.c_float), ("denoise_ms", C.c_float), ("post_ms", C.c_float),
+                ("rays", C.c_uint64),
                 ("active_pixels", C.c_uint64)]
 
 
@@ -129,6 +138,8 @@ _sigs = {
     "luzrt_shade": (C.c_int, [_P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P, _P]),
     "luzrt_light_pass": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P]),
     "luzrt_shade_fractions": (C.c_int, [_P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P, _P]),
+    "luzrt_taa": (C.c_int, [_P, C.POINTER(TaaDesc)]),
+    "luzrt_compose": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, _P]),
     "luzrt_primary_gbuffer": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(Material), C.c_uint32,
                                         C.POINTER(GBuffer)]),
     "luzscene_generate": (_P, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
@@ -141,6 +152,8 @@ _sigs = {
     "luzscene_pack": (C.c_int, [_P, C.POINTER(SceneBlock)]),
     "luzscene_camera": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(Camera)]),
     "luzscene_animate": (C.c_int, [_P, C.c_uint32]),
+    "luzscene_view_proj": (C.c_int, [_P, _P, _P, _P]),
+    "luzscene_jitter": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "luzscene_instanced_triangles": (C.c_uint64, [_P]),
     "luzgpu_create": (_P, [_P, _P]),
     "luzgpu_destroy": (None, [_P]),
@@ -277,6 +290,16 @@ class Context:
         """Lighting from visibility/AO fractions (e.g. the denoiser's outputs)."""
         self._check(lib.luzrt_shade_fractions(self.handle, C.byref(g), C.byref(scene),
                                               visibility.ptr if scene.num_lights else None, ao.ptr, out.ptr))
+
+    def taa(self, desc: TaaDesc) -> None:
+        """Temporal accumulation of desc.light with desc.history into desc.out."""
+        self._check(lib.luzrt_taa(self.handle, C.byref(desc)))
+
+    def compose(self, width: int, height: int, light: DeviceBuffer, rgba8: DeviceBuffer = None,
+                rgba32f: DeviceBuffer = None) -> None:
+        """ACES tonemap + gamma of a light buffer to RGBA8 and/or RGBA32F."""
+        self._check(lib.luzrt_compose(self.handle, width, height, light.ptr, rgba8.ptr if rgba8 else None,
+                                      rgba32f.ptr if rgba32f else None))
 
     def light_pass(self, tlas: "Tlas", g: GBuffer, scene: SceneBlock, out: DeviceBuffer) -> None:
         self._check(lib.luzrt_light_pass(self.handle, tlas.handle, C.byref(g), C.byref(scene), out.ptr))
@@ -424,6 +447,15 @@ class HostScene:
         lib.luzscene_camera(self.handle, C.byref(w), C.byref(h), C.byref(cam))
         return w.value, h.value, cam
 
+    def view_proj(self, jitter=None):
+        """(view_proj, inverse) as column-major float32[16] arrays."""
+        vp = np.zeros(16, np.float32)
+        ivp = np.zeros(16, np.float32)
+        j = None if jitter is None else np.asarray(jitter, np.float32)
+        if lib.luzscene_view_proj(self.handle, _ptr(j) if j is not None else None, _ptr(vp), _ptr(ivp)) != 0:
+            raise LuzError(-1, "singular view-projection")
+        return vp, ivp
+
     def animate(self, frame: int) -> None:
         lib.luzscene_animate(self.handle, frame)
 
@@ -495,6 +527,13 @@ class GBufferDevice:
 
 
 # ------------------------------------------------------------------ batched frames
+def jitter(index: int, width: int, height: int) -> np.ndarray:
+    """CameraNode::NextJitter's Halton(2,3) NDC jitter of frame `index`."""
+    j = np.zeros(2, np.float32)
+    lib.luzscene_jitter(index, width, height, _ptr(j))
+    return j
+
+
 class Batch:
     """Many scenes rendered per frame (luzrt_batch): batched BLAS refit and
     TLAS rebuild, primary G-buffers, one persistent traversal launch over all

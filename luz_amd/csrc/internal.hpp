@@ -19,7 +19,7 @@ struct DevBuf {
 };
 
 enum Stage { kStageBuild, kStageCompact, kStageRaygen, kStageTrace, kStageResolve, kStageShade,
-             kStagePrimary, kStageDenoise, kStageCount };
+             kStagePrimary, kStageDenoise, kStagePost, kStageCount };
 
 }  // namespace luz
 

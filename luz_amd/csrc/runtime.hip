@@ -219,7 +219,8 @@ luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out) {
     if (!ctx || !out) return LUZRT_E_INVALID_ARG;
     LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float* f[kStageCount] = {&out->build_ms, &out->compact_ms, &out->raygen_ms, &out->trace_ms,
-                             &out->resolve_ms, &out->shade_ms, &out->primary_ms, &out->denoise_ms};
+                             &out->resolve_ms, &out->shade_ms, &out->primary_ms, &out->denoise_ms,
+                             &out->post_ms};
     for (int s = 0; s < kStageCount; ++s) {
         *f[s] = 0.0f;
         if (ctx->ev_used[s]) {

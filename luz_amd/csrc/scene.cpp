@@ -678,6 +678,86 @@ int luzscene_camera(const luzscene* s, uint32_t* w, uint32_t* h, luzrt_camera* c
     return 0;
 }
 
+int luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj[16], float inv_view_proj[16]) {
+    if (!s || !view_proj) return -1;
+    // CameraNode::GetView (glm::lookAt, right-handed) and GetProjJittered =
+    // translate(jitter) * perspective(radians(horizontalFov) as fovy, W/H,
+    // near, far) in [0, 1] depth with proj[1][1] negated (AssetManager.cpp:334-377).
+    const CameraNode& c = s->camera;
+    D3 f = normalize(c.center - c.eye);
+    D3 r = normalize(cross(f, d3(0, 1, 0)));
+    D3 u = cross(r, f);
+    double V[16] = {r.x, u.x, -f.x, 0, r.y, u.y, -f.y, 0, r.z, u.z, -f.z, 0,
+                    -dot(r, c.eye), -dot(u, c.eye), dot(f, c.eye), 1};
+    const double th = std::tan(c.horizontalFov * kPiD / 180.0 * 0.5);
+    const double aspect = (double)s->width / (double)s->height;
+    const double n = c.nearDistance, fa = c.farDistance;
+    double P[16] = {0};
+    P[0] = 1.0 / (aspect * th);
+    P[5] = -1.0 / th;
+    P[10] = fa / (n - fa);
+    P[11] = -1.0;
+    P[14] = -(fa * n) / (fa - n);
+    if (jitter) {   // translate(jx, jy, 0) * P adds jitter * row 3 to rows 0 and 1
+        for (int col = 0; col < 4; ++col) {
+            P[col * 4 + 0] += jitter[0] * P[col * 4 + 3];
+            P[col * 4 + 1] += jitter[1] * P[col * 4 + 3];
+        }
+    }
+    double M[16];
+    for (int col = 0; col < 4; ++col)
+        for (int row = 0; row < 4; ++row) {
+            double acc = 0;
+            for (int k = 0; k < 4; ++k) acc += P[k * 4 + row] * V[col * 4 + k];
+            M[col * 4 + row] = acc;
+        }
+    for (int i = 0; i < 16; ++i) view_proj[i] = (float)M[i];
+    if (inv_view_proj) {
+        // Gauss-Jordan with partial pivoting on the FP64 matrix (row-major copy)
+        double a[4][8];
+        for (int row = 0; row < 4; ++row)
+            for (int col = 0; col < 4; ++col) {
+                a[row][col] = M[col * 4 + row];
+                a[row][4 + col] = row == col ? 1.0 : 0.0;
+            }
+        for (int col = 0; col < 4; ++col) {
+            int piv = col;
+            for (int row = col + 1; row < 4; ++row)
+                if (std::fabs(a[row][col]) > std::fabs(a[piv][col])) piv = row;
+            if (a[piv][col] == 0.0) return -2;
+            if (piv != col)
+                for (int k = 0; k < 8; ++k) std::swap(a[piv][k], a[col][k]);
+            double d = a[col][col];
+            for (int k = 0; k < 8; ++k) a[col][k] /= d;
+            for (int row = 0; row < 4; ++row) {
+                if (row == col) continue;
+                double m = a[row][col];
+                for (int k = 0; k < 8; ++k) a[row][k] -= m * a[col][k];
+            }
+        }
+        for (int row = 0; row < 4; ++row)
+            for (int col = 0; col < 4; ++col) inv_view_proj[col * 4 + row] = (float)a[row][4 + col];
+    }
+    return 0;
+}
+
+void luzscene_jitter(uint32_t index, uint32_t width, uint32_t height, float jitter[2]) {
+    // CameraNode::NextJitter (AssetManager.cpp:383-392): Halton(2, 3) of index % 16 + 1,
+    // mapped to [-1, 1] and divided by the extent.
+    auto halton = [](uint32_t i, uint32_t b) {
+        double f = 1.0, r = 0.0;
+        while (i > 0) {
+            f /= b;
+            r += f * (i % b);
+            i /= b;
+        }
+        return r;
+    };
+    uint32_t k = index % 16 + 1;
+    jitter[0] = (float)((2.0 * halton(k, 2) - 1.0) / width);
+    jitter[1] = (float)((2.0 * halton(k, 3) - 1.0) / height);
+}
+
 int luzscene_animate(luzscene* s, uint32_t frame) {
     if (!s) return -1;
     for (auto& n : s->meshNodes) {

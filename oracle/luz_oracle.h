@@ -143,6 +143,21 @@ int  orc_shade(const orc_gbuffer* g, const orc_params* p, const uint8_t* shadow,
 int  orc_shade_f64(const orc_gbuffer* g, const orc_params* p, const uint8_t* shadow,
                    const uint8_t* ao, double* out_rgba, int threads);
 
+/* ---- passes after the light pass (post_oracle.cpp) ---- */
+typedef struct orc_taa_desc {
+    uint32_t width, height;
+    const float* light;      /* RGBA32F */
+    const float* history;    /* RGBA32F */
+    const float* position;   /* float3 */
+    const float* normal;     /* float3, zero = background */
+    float view_proj[16], prev_view_proj[16], inv_view_proj[16];   /* column-major */
+    float jitter[2], prev_jitter[2];
+    int32_t reconstruct;
+} orc_taa_desc;
+
+int orc_taa(const orc_taa_desc* d, float* out_rgba32f);
+int orc_compose(const float* light_rgba32f, uint32_t npix, uint32_t* rgba8, float* rgba32f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,14 @@ class OrcStats(C.Structure):
                 ("inst_visits", C.c_uint64), ("hits", C.c_uint64), ("wide_node_visits", C.c_uint64)]
 
 
+class OrcTaaDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("light", C.c_void_p),
+                ("history", C.c_void_p), ("position", C.c_void_p), ("normal", C.c_void_p),
+                ("view_proj", C.c_float * 16), ("prev_view_proj", C.c_float * 16),
+                ("inv_view_proj", C.c_float * 16), ("jitter", C.c_float * 2), ("prev_jitter", C.c_float * 2),
+                ("reconstruct", C.c_int32)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _sigs = {
@@ -96,6 +104,8 @@ _sigs = {
     "orc_visibility_rays": (C.c_int64, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, C.c_int64]),
     "orc_shade": (C.c_int, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P, _P, C.c_int]),
     "orc_shade_f64": (C.c_int, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P, _P, C.c_int]),
+    "orc_taa": (C.c_int, [C.POINTER(OrcTaaDesc), _P]),
+    "orc_compose": (C.c_int, [_P, C.c_uint32, _P, _P]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(lib, _n)
@@ -339,3 +349,31 @@ def shade(g: dict, width: int, height: int, params, shadow, ao, f64: bool = Fals
     fn = lib.orc_shade_f64 if f64 else lib.orc_shade
     fn(C.byref(gb), C.byref(p), _p(sh), _p(a), _p(out), threads or threads_default())
     return out
+
+
+def taa(light, history, position, normal, width, height, view_proj, prev_view_proj, inv_view_proj,
+        jitter=(0.0, 0.0), prev_jitter=(0.0, 0.0), reconstruct=0) -> np.ndarray:
+    """Temporal accumulation (taa.comp:main) of (H*W, 4) FP32 images; returns (H*W, 4)."""
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (light, history, position, normal)]
+    d = OrcTaaDesc()
+    d.width, d.height = width, height
+    d.light, d.history, d.position, d.normal = (_p(a) for a in arrs)
+    for name, m in (("view_proj", view_proj), ("prev_view_proj", prev_view_proj), ("inv_view_proj", inv_view_proj)):
+        getattr(d, name)[:] = [float(v) for v in np.asarray(m, np.float32).reshape(16)]
+    d.jitter[:] = [float(v) for v in np.asarray(jitter, np.float32)]
+    d.prev_jitter[:] = [float(v) for v in np.asarray(prev_jitter, np.float32)]
+    d.reconstruct = reconstruct
+    out = np.empty((width * height, 4), np.float32)
+    if lib.orc_taa(C.byref(d), _p(out)) != 0:
+        raise ValueError("orc_taa failed")
+    return out
+
+
+def compose(light):
+    """ACES + gamma of (N, 4) FP32: (RGBA8 u32 (N,), RGBA32F (N, 4))."""
+    lt = np.ascontiguousarray(light, np.float32).reshape(-1, 4)
+    n = len(lt)
+    rgba8 = np.empty(n, np.uint32)
+    rgb = np.empty((n, 4), np.float32)
+    lib.orc_compose(_p(lt), n, _p(rgba8), _p(rgb))
+    return rgba8, rgb
