@@ -72,6 +72,26 @@ int      luzscene_animate(luzscene* s, uint32_t frame);
 /* Total triangles over all mesh nodes (instanced). */
 uint64_t luzscene_instanced_triangles(const luzscene* s);
 
+/* Scene ingest (AssetIO.cpp:117-528, AssetManager.cpp:182-257).  The format
+ * follows the extension: ".luz" (project JSON; the mesh blob is bin_path, or
+ * the .luz path + "bin" when NULL; meshes whose data is absent are skipped
+ * and counted in *missing_meshes), ".gltf"/".glb" (glTF 2.0) or ".obj" (with
+ * its MTL).  Imported glTF/OBJ scenes get a camera framing their bounds and
+ * no lights.  On failure returns NULL and writes a message into error. */
+luzscene* luzscene_load(const char* path, const char* bin_path, uint32_t* missing_meshes, char* error,
+                        uint32_t error_len);
+/* Writes the scene as a .luz project and its .luzbin blob. */
+int       luzscene_save(const luzscene* s, const char* path, const char* bin_path, char* error,
+                        uint32_t error_len);
+/* Camera (eye, center, fov passed as fovy like CameraNode) and image size. */
+int       luzscene_set_camera(luzscene* s, const float eye[3], const float center[3], float fov_degrees,
+                              uint32_t width, uint32_t height);
+/* Adds a light (LUZRT_LIGHT_*; direction may be NULL for point lights). */
+int       luzscene_add_light(luzscene* s, int type, const float position[3], const float direction[3],
+                             const float color[3], float intensity, float radius);
+/* SceneAsset::lightSamples / aoSamples / aoMin / aoMax. */
+int       luzscene_set_samples(luzscene* s, int light_samples, int ao_samples, float ao_min, float ao_max);
+
 /* GPUScene: uploads meshes, builds one BLAS per mesh and the TLAS. */
 luzgpu*  luzgpu_create(luzrt_ctx* ctx, const luzscene* s);
 void     luzgpu_destroy(luzgpu* g);

@@ -154,6 +154,11 @@ _sigs = {
     "luzscene_animate": (C.c_int, [_P, C.c_uint32]),
     "luzscene_view_proj": (C.c_int, [_P, _P, _P, _P]),
     "luzscene_jitter": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "luzscene_load": (_P, [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint32), C.c_char_p, C.c_uint32]),
+    "luzscene_save": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32]),
+    "luzscene_set_camera": (C.c_int, [_P, _P, _P, C.c_float, C.c_uint32, C.c_uint32]),
+    "luzscene_add_light": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_float, C.c_float]),
+    "luzscene_set_samples": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_float]),
     "luzscene_instanced_triangles": (C.c_uint64, [_P]),
     "luzgpu_create": (_P, [_P, _P]),
     "luzgpu_destroy": (None, [_P]),
@@ -401,9 +406,46 @@ class HostScene:
     """Luz's SceneAsset with procedural content (luzscene)."""
 
     def __init__(self, kind: int, seed: int = 0, tri_budget: int = 0, width: int = 0, height: int = 0):
-        self.handle = lib.luzscene_generate(kind, seed, tri_budget, width, height)
-        if not self.handle:
+        self.missing_meshes = 0
+        self.handle = lib.luzscene_generate(kind, seed, tri_budget, width, height) if kind else None
+        if kind and not self.handle:
             raise ValueError(f"unknown scene kind {kind}")
+
+    @classmethod
+    def load(cls, path, bin_path=None) -> "HostScene":
+        """Scene ingest: .luz (+ .luzbin), .gltf/.glb or .obj."""
+        s = cls(0)
+        err = C.create_string_buffer(512)
+        missing = C.c_uint32()
+        s.handle = lib.luzscene_load(str(path).encode(), str(bin_path).encode() if bin_path else None,
+                                     C.byref(missing), err, 512)
+        if not s.handle:
+            raise LuzError(-1, err.value.decode())
+        s.missing_meshes = missing.value
+        return s
+
+    def save(self, path, bin_path=None) -> None:
+        err = C.create_string_buffer(512)
+        if lib.luzscene_save(self.handle, str(path).encode(), str(bin_path).encode() if bin_path else None, err, 512):
+            raise LuzError(-1, err.value.decode())
+
+    def set_camera(self, eye, center, fov_degrees: float, width: int, height: int) -> None:
+        e, c = np.asarray(eye, np.float32), np.asarray(center, np.float32)
+        if lib.luzscene_set_camera(self.handle, _ptr(e), _ptr(c), fov_degrees, width, height):
+            raise LuzError(-1, "bad camera")
+
+    def add_light(self, kind: int, position, direction=None, color=(1, 1, 1), intensity: float = 10.0,
+                  radius: float = 0.0) -> None:
+        p = np.asarray(position, np.float32)
+        d = None if direction is None else np.asarray(direction, np.float32)
+        c = np.asarray(color, np.float32)
+        if lib.luzscene_add_light(self.handle, kind, _ptr(p), _ptr(d) if d is not None else None, _ptr(c),
+                                  intensity, radius):
+            raise LuzError(-5, "bad light or more than 64 lights")
+
+    def set_samples(self, light_samples: int, ao_samples: int, ao_min: float = 1e-4, ao_max: float = 1.0) -> None:
+        if lib.luzscene_set_samples(self.handle, light_samples, ao_samples, ao_min, ao_max):
+            raise LuzError(-5, "bad sample counts")
 
     def close(self) -> None:
         if self.handle:
