@@ -49,6 +49,7 @@ extern "C" {
 #define LUZRT_LIGHT_DIRECTIONAL 2
 
 #define LUZRT_SHADOW_RAYTRACED 1  /* LuzCommon.h:28 SHADOW_TYPE_RAYTRACING */
+#define LUZRT_SHADOW_MAP 2        /* LuzCommon.h:29 SHADOW_TYPE_MAP */
 
 typedef enum luzrt_status {
     LUZRT_OK = 0,
@@ -241,6 +242,40 @@ luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzr
 luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* tlas,
                                    const luzrt_camera* camera, const luzrt_material* materials,
                                    uint32_t material_count, const luzrt_gbuffer_out* out);
+
+/* ------------------------------------------------------------ shadow maps */
+/* The alternative visibility source of EvaluateShadow (light.frag:147-165,
+ * shadowType == LUZRT_SHADOW_MAP).  A directional or spot light's map is
+ * the depth (NDC z of an orthographic view-projection fitted to the camera
+ * frustum, GPUScene.cpp:278-311) of the nearest surface through each texel
+ * centre; a point light's is a cube map (faces +X -X +Y -Y +Z -Z, Vulkan
+ * cube-face conventions) of distance / z_far (shadowMap.frag:11-19).  Maps
+ * are rendered here by closest-hit rays through texel centres (the
+ * reference rasterises them), cleared to 1.  Lookups sample the map
+ * bilinearly: REPEAT addressing for 2D maps (VulkanWrapper.cpp:2433-2437),
+ * clamped within the face for cube maps. */
+typedef struct luzrt_shadow_map {
+    int32_t light_type;         /* LUZRT_LIGHT_* of the light */
+    uint32_t resolution;        /* texels per side */
+    float view_proj[16];        /* directional/spot: column-major light view-projection */
+    float inv_view_proj[16];    /* its inverse */
+    float light_pos[3];         /* point lights */
+    float z_far;                /* point lights: LightNode::shadowMapFar */
+    float* depth;               /* device: res*res floats, or 6*res*res for point lights */
+} luzrt_shadow_map;
+
+/* Renders map->depth (allocated by the caller) over the TLAS. */
+luzrt_status luzrt_shadow_map_build(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_shadow_map* map);
+/* Shadow decisions (1 = shadowed) per light and pixel from maps[l] for light
+ * l (count = scene->num_lights); a light whose map has no depth is fully
+ * shadowed, as the reference's shadowMap == -1 case. */
+luzrt_status luzrt_shadow_map_visibility(luzrt_ctx* ctx, const luzrt_gbuffer* g, const luzrt_scene* scene,
+                                         const luzrt_shadow_map* maps, uint32_t count, uint8_t* shadow_out);
+/* The light pass with shadow-map shadows and ray-traced AO (light.frag:main
+ * with shadowType == Map). */
+luzrt_status luzrt_light_pass_shadow_maps(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* g,
+                                          const luzrt_scene* scene, const luzrt_shadow_map* maps,
+                                          uint32_t count, float* rgba32f_out);
 
 /* ------------------------------------------------------------ after the light pass */
 /* Temporal accumulation of the light buffer (taa.comp:main, :271-307) and the

@@ -65,6 +65,13 @@ int      luzscene_camera(const luzscene* s, uint32_t* width, uint32_t* height, l
  * the given NDC jitter (NULL = none), and its inverse (may be NULL). */
 int      luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj[16],
                             float inv_view_proj[16]);
+/* Shadow-map descriptor of light `light` (GPUScene.cpp:266-312): the
+ * orthographic view-projection fitted to the camera frustum for directional
+ * and spot lights, the position and shadowMapFar for point lights; depth is
+ * left NULL for the caller to allocate (res^2 or 6 res^2 floats). */
+int      luzscene_shadow_map(const luzscene* s, uint32_t light, uint32_t resolution, luzrt_shadow_map* out);
+/* SceneAsset::shadowType: LUZRT_SHADOW_RAYTRACED or LUZRT_SHADOW_MAP. */
+int      luzscene_set_shadow_type(luzscene* s, int type);
 /* The camera jitter of frame `index` (CameraNode::NextJitter): Halton(2,3). */
 void     luzscene_jitter(uint32_t index, uint32_t width, uint32_t height, float jitter[2]);
 /* Seeded rigid motion of every mesh node for animation frame `frame`. */
@@ -83,9 +90,10 @@ luzscene* luzscene_load(const char* path, const char* bin_path, uint32_t* missin
 /* Writes the scene as a .luz project and its .luzbin blob. */
 int       luzscene_save(const luzscene* s, const char* path, const char* bin_path, char* error,
                         uint32_t error_len);
-/* Camera (eye, center, fov passed as fovy like CameraNode) and image size. */
+/* Camera (eye, center, fov passed as fovy like CameraNode), image size, and
+ * near/far distances (0 keeps the current ones). */
 int       luzscene_set_camera(luzscene* s, const float eye[3], const float center[3], float fov_degrees,
-                              uint32_t width, uint32_t height);
+                              uint32_t width, uint32_t height, float near_distance, float far_distance);
 /* Adds a light (LUZRT_LIGHT_*; direction may be NULL for point lights). */
 int       luzscene_add_light(luzscene* s, int type, const float position[3], const float direction[3],
                              const float color[3], float intensity, float radius);

@@ -28,6 +28,7 @@ MAX_INSTANCES = 8192
 LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
 SCENE_CUBE, SCENE_SPHERES, SCENE_TERRAIN, SCENE_RANDOM = 1, 2, 4, 5
 PRECISION_PLAIN, PRECISION_COMPACT = 0, 1
+SHADOW_RAYTRACED, SHADOW_MAP = 1, 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "device error", -4: "in use",
           -5: "limit exceeded", -6: "no device", -7: "unsupported"}
@@ -78,6 +79,22 @@ class Material(C.Structure):
 class Instance(C.Structure):
     _fields_ = [("blas", C.c_void_p), ("object_to_world", (C.c_float * 4) * 3),
                 ("custom_index", C.c_uint32), ("mask", C.c_uint32)]
+
+
+class ShadowMap(C.Structure):
+    _fields_ = [("light_type", C.c_int32), ("resolution", C.c_uint32), ("view_proj", C.c_float * 16),
+                ("inv_view_proj", C.c_float * 16), ("light_pos", C.c_float * 3), ("z_far", C.c_float),
+                ("depth", C.c_void_p)]
+
+    def as_dict(self) -> dict:
+        return {"light_type": self.light_type, "resolution": self.resolution,
+                "view_proj": np.array(self.view_proj[:], np.float32),
+                "inv_view_proj": np.array(self.inv_view_proj[:], np.float32),
+                "light_pos": np.array(self.light_pos[:], np.float32), "z_far": self.z_far}
+
+    @property
+    def texels(self) -> int:
+        return (6 if self.light_type == LIGHT_POINT else 1) * self.resolution ** 2
 
 
 class TaaDesc(C.Structure):
@@ -139,6 +156,10 @@ _sigs = {
     "luzrt_light_pass": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P]),
     "luzrt_shade_fractions": (C.c_int, [_P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P, _P]),
     "luzrt_taa": (C.c_int, [_P, C.POINTER(TaaDesc)]),
+    "luzrt_shadow_map_build": (C.c_int, [_P, _P, C.POINTER(ShadowMap)]),
+    "luzrt_shadow_map_visibility": (C.c_int, [_P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, C.c_uint32, _P]),
+    "luzrt_light_pass_shadow_maps": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, C.c_uint32,
+                                               _P]),
     "luzrt_compose": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, _P]),
     "luzrt_primary_gbuffer": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(Material), C.c_uint32,
                                         C.POINTER(GBuffer)]),
@@ -154,9 +175,11 @@ _sigs = {
     "luzscene_animate": (C.c_int, [_P, C.c_uint32]),
     "luzscene_view_proj": (C.c_int, [_P, _P, _P, _P]),
     "luzscene_jitter": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "luzscene_shadow_map": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(ShadowMap)]),
+    "luzscene_set_shadow_type": (C.c_int, [_P, C.c_int]),
     "luzscene_load": (_P, [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint32), C.c_char_p, C.c_uint32]),
     "luzscene_save": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32]),
-    "luzscene_set_camera": (C.c_int, [_P, _P, _P, C.c_float, C.c_uint32, C.c_uint32]),
+    "luzscene_set_camera": (C.c_int, [_P, _P, _P, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
     "luzscene_add_light": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_float, C.c_float]),
     "luzscene_set_samples": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_float]),
     "luzscene_instanced_triangles": (C.c_uint64, [_P]),
@@ -296,6 +319,20 @@ class Context:
         self._check(lib.luzrt_shade_fractions(self.handle, C.byref(g), C.byref(scene),
                                               visibility.ptr if scene.num_lights else None, ao.ptr, out.ptr))
 
+    def shadow_map_build(self, tlas: "Tlas", m: ShadowMap) -> None:
+        """Renders m.depth (device, m.texels floats) by closest-hit rays."""
+        self._check(lib.luzrt_shadow_map_build(self.handle, tlas.handle, C.byref(m)))
+
+    def shadow_map_visibility(self, g: GBuffer, scene: SceneBlock, maps, out: DeviceBuffer) -> None:
+        arr = (ShadowMap * max(len(maps), 1))(*maps)
+        self._check(lib.luzrt_shadow_map_visibility(self.handle, C.byref(g), C.byref(scene),
+                                                    C.cast(arr, C.c_void_p), len(maps), out.ptr))
+
+    def light_pass_shadow_maps(self, tlas: "Tlas", g: GBuffer, scene: SceneBlock, maps, out: DeviceBuffer) -> None:
+        arr = (ShadowMap * max(len(maps), 1))(*maps)
+        self._check(lib.luzrt_light_pass_shadow_maps(self.handle, tlas.handle, C.byref(g), C.byref(scene),
+                                                     C.cast(arr, C.c_void_p), len(maps), out.ptr))
+
     def taa(self, desc: TaaDesc) -> None:
         """Temporal accumulation of desc.light with desc.history into desc.out."""
         self._check(lib.luzrt_taa(self.handle, C.byref(desc)))
@@ -429,9 +466,10 @@ class HostScene:
         if lib.luzscene_save(self.handle, str(path).encode(), str(bin_path).encode() if bin_path else None, err, 512):
             raise LuzError(-1, err.value.decode())
 
-    def set_camera(self, eye, center, fov_degrees: float, width: int, height: int) -> None:
+    def set_camera(self, eye, center, fov_degrees: float, width: int, height: int, near: float = 0.0,
+                   far: float = 0.0) -> None:
         e, c = np.asarray(eye, np.float32), np.asarray(center, np.float32)
-        if lib.luzscene_set_camera(self.handle, _ptr(e), _ptr(c), fov_degrees, width, height):
+        if lib.luzscene_set_camera(self.handle, _ptr(e), _ptr(c), fov_degrees, width, height, near, far):
             raise LuzError(-1, "bad camera")
 
     def add_light(self, kind: int, position, direction=None, color=(1, 1, 1), intensity: float = 10.0,
@@ -442,6 +480,17 @@ class HostScene:
         if lib.luzscene_add_light(self.handle, kind, _ptr(p), _ptr(d) if d is not None else None, _ptr(c),
                                   intensity, radius):
             raise LuzError(-5, "bad light or more than 64 lights")
+
+    def shadow_map(self, light: int, resolution: int) -> ShadowMap:
+        """The light's shadow-map descriptor (depth left unset)."""
+        m = ShadowMap()
+        if lib.luzscene_shadow_map(self.handle, light, resolution, C.byref(m)) != 0:
+            raise LuzError(-1, f"no shadow map for light {light}")
+        return m
+
+    def set_shadow_type(self, kind: int) -> None:
+        if lib.luzscene_set_shadow_type(self.handle, kind):
+            raise LuzError(-1, "shadow type must be SHADOW_RAYTRACED or SHADOW_MAP")
 
     def set_samples(self, light_samples: int, ao_samples: int, ao_min: float = 1e-4, ao_max: float = 1.0) -> None:
         if lib.luzscene_set_samples(self.handle, light_samples, ao_samples, ao_min, ao_max):

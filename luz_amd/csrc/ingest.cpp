@@ -788,6 +788,8 @@ struct LuzLoad {
             if (const JVal* v = j.get("innerAngle")) L.innerAngle = (float)v->num();
             if (const JVal* v = j.get("outerAngle")) L.outerAngle = (float)v->num();
             if (const JVal* v = j.get("radius")) L.radius = (float)v->num();
+            if (const JVal* v = j.get("shadowMapRange")) L.shadowMapRange = (float)v->num();
+            if (const JVal* v = j.get("shadowMapFar")) L.shadowMapFar = (float)v->num();
             s->lights.push_back(L);
         } else if (type == kCameraNode) {
             uint64_t uuid = j.get("uuid") ? j.get("uuid")->u64() : 0;
@@ -973,7 +975,8 @@ bool save_luz(const luzscene& s, const std::string& path, const std::string& bin
         o.raw(",\"color\":["); for (int k = 0; k < 3; ++k) { if (k) o.raw(","); o.f(L.color[k]); }
         o.raw("],\"intensity\":"); o.f(L.intensity); o.raw(",\"lightType\":"); o.i((int)L.lightType);
         o.raw(",\"innerAngle\":"); o.f(L.innerAngle); o.raw(",\"outerAngle\":"); o.f(L.outerAngle);
-        o.raw(",\"radius\":"); o.f(L.radius); o.raw(",\"children\":[]}");
+        o.raw(",\"radius\":"); o.f(L.radius); o.raw(",\"shadowMapRange\":"); o.f(L.shadowMapRange);
+        o.raw(",\"shadowMapFar\":"); o.f(L.shadowMapFar); o.raw(",\"children\":[]}");
         if (wrapped) o.raw("]}");
     }
     if (!fn) o.raw(",");
@@ -1026,8 +1029,11 @@ int luzscene_save(const luzscene* s, const char* path, const char* bin_path, cha
 }
 
 int luzscene_set_camera(luzscene* s, const float eye[3], const float center[3], float fov_degrees, uint32_t width,
-                        uint32_t height) {
+                        uint32_t height, float near_distance, float far_distance) {
     if (!s || !eye || !center || width == 0 || height == 0 || !(fov_degrees > 0.0f && fov_degrees < 180.0f)) return -1;
+    if (near_distance < 0.0f || far_distance < 0.0f || (far_distance > 0.0f && far_distance <= near_distance)) return -1;
+    if (near_distance > 0.0f) s->camera.nearDistance = near_distance;
+    if (far_distance > 0.0f) s->camera.farDistance = far_distance;
     s->camera.eye = d3(eye[0], eye[1], eye[2]);
     s->camera.center = d3(center[0], center[1], center[2]);
     s->camera.horizontalFov = fov_degrees;

@@ -473,6 +473,143 @@ int luzscene_camera(const luzscene* s, uint32_t* w, uint32_t* h, luzrt_camera* c
     return 0;
 }
 
+}  // extern "C"
+
+namespace luz {
+namespace {
+
+// glm::lookAt (right-handed), column-major.
+void look_at(D3 eye, D3 center, D3 up, double V[16]) {
+    D3 f = normalize(center - eye);
+    D3 r = normalize(cross(f, up));
+    D3 u = cross(r, f);
+    const double v[16] = {r.x, u.x, -f.x, 0, r.y, u.y, -f.y, 0, r.z, u.z, -f.z, 0,
+                          -dot(r, eye), -dot(u, eye), dot(f, eye), 1};
+    std::memcpy(V, v, sizeof v);
+}
+
+void mat4_mul(const double A[16], const double B[16], double C[16]) {
+    for (int col = 0; col < 4; ++col)
+        for (int row = 0; row < 4; ++row) {
+            double acc = 0;
+            for (int k = 0; k < 4; ++k) acc += A[k * 4 + row] * B[col * 4 + k];
+            C[col * 4 + row] = acc;
+        }
+}
+
+// Gauss-Jordan inverse with partial pivoting; false when singular.
+bool mat4_inverse(const double M[16], double out[16]) {
+    double a[4][8];
+    for (int row = 0; row < 4; ++row)
+        for (int col = 0; col < 4; ++col) {
+            a[row][col] = M[col * 4 + row];
+            a[row][4 + col] = row == col ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int row = col + 1; row < 4; ++row)
+            if (std::fabs(a[row][col]) > std::fabs(a[piv][col])) piv = row;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int k = 0; k < 8; ++k) std::swap(a[piv][k], a[col][k]);
+        double d = a[col][col];
+        for (int k = 0; k < 8; ++k) a[col][k] /= d;
+        for (int row = 0; row < 4; ++row) {
+            if (row == col) continue;
+            double m = a[row][col];
+            for (int k = 0; k < 8; ++k) a[row][k] -= m * a[col][k];
+        }
+    }
+    for (int row = 0; row < 4; ++row)
+        for (int col = 0; col < 4; ++col) out[col * 4 + row] = a[row][4 + col];
+    return true;
+}
+
+// CameraNode::GetProj(zNear, zFar): perspective(radians(horizontalFov) as fovy,
+// W/H) in [0, 1] depth with proj[1][1] negated (AssetManager.cpp:363-377).
+void camera_proj(const luzscene* s, double n, double fa, double P[16]) {
+    const double th = std::tan(s->camera.horizontalFov * kPiD / 180.0 * 0.5);
+    const double aspect = (double)s->width / (double)s->height;
+    for (int i = 0; i < 16; ++i) P[i] = 0.0;
+    P[0] = 1.0 / (aspect * th);
+    P[5] = -1.0 / th;
+    P[10] = fa / (n - fa);
+    P[11] = -1.0;
+    P[14] = -(fa * n) / (fa - n);
+}
+
+}  // namespace
+}  // namespace luz
+
+extern "C" {
+
+int luzscene_shadow_map(const luzscene* s, uint32_t light, uint32_t resolution, luzrt_shadow_map* out) {
+    if (!s || !out || light >= s->lights.size() || resolution == 0) return -1;
+    const LightNode& L = s->lights[light];
+    std::memset(out, 0, sizeof(*out));
+    out->light_type = (int32_t)L.lightType;
+    out->resolution = resolution;
+    M34 W = L.GetWorldTransform();
+    D3 pos = W.point(d3(0, 0, 0));
+    out->light_pos[0] = (float)pos.x; out->light_pos[1] = (float)pos.y; out->light_pos[2] = (float)pos.z;
+    out->z_far = L.shadowMapFar;
+    for (int i = 0; i < 16; ++i) out->view_proj[i] = out->inv_view_proj[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    if (L.lightType == LightNode::Point) return 0;   // cube faces use the light position and z_far
+    // GPUScene::UpdateResources (GPUScene.cpp:278-311): an orthographic box
+    // around the camera frustum cut at farDistance / shadowMapRange, seen
+    // from the light's front direction.
+    const CameraNode& c = s->camera;
+    double V[16], P[16], PV[16], inv[16];
+    look_at(c.eye, c.center, d3(0, 1, 0), V);
+    camera_proj(s, c.nearDistance, (double)c.farDistance / L.shadowMapRange, P);
+    mat4_mul(P, V, PV);
+    if (!mat4_inverse(PV, inv)) return -2;
+    D3 corners[8];
+    D3 center = d3(0, 0, 0);
+    int k8 = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k) {
+                double x = 2.0 * i - 1.0, y = 2.0 * j - 1.0, z = 2.0 * k - 1.0, q[4];
+                for (int r = 0; r < 4; ++r) q[r] = inv[r] * x + inv[4 + r] * y + inv[8 + r] * z + inv[12 + r];
+                corners[k8] = d3(q[0] / q[3], q[1] / q[3], q[2] / q[3]);
+                center = center + corners[k8];
+                ++k8;
+            }
+    center = center * (1.0 / 8.0);
+    D3 front = W.vec(d3(0, -1, 0));   // Node::GetWorldFront
+    double LV[16];
+    look_at(center + front, center, d3(0, 1, 0), LV);
+    D3 lo = d3(1e300, 1e300, 1e300), hi = d3(-1e300, -1e300, -1e300);
+    for (const D3& p : corners) {
+        D3 v = d3(LV[0] * p.x + LV[4] * p.y + LV[8] * p.z + LV[12], LV[1] * p.x + LV[5] * p.y + LV[9] * p.z + LV[13],
+                  LV[2] * p.x + LV[6] * p.y + LV[10] * p.z + LV[14]);
+        lo = d3(std::min(lo.x, v.x), std::min(lo.y, v.y), std::min(lo.z, v.z));
+        hi = d3(std::max(hi.x, v.x), std::max(hi.y, v.y), std::max(hi.z, v.z));
+    }
+    const double r = L.shadowMapRange;
+    lo.z = lo.z < 0 ? lo.z * r : lo.z / r;
+    hi.z = hi.z < 0 ? hi.z / r : hi.z * r;
+    // glm::ortho(lo.x, hi.x, lo.y, hi.y, zNear = hi.z, zFar = lo.z), [0, 1] depth
+    double O[16] = {0};
+    const double zn = hi.z, zf = lo.z;
+    O[0] = 2.0 / (hi.x - lo.x);
+    O[5] = 2.0 / (hi.y - lo.y);
+    O[10] = -1.0 / (zf - zn);
+    O[12] = -(hi.x + lo.x) / (hi.x - lo.x);
+    O[13] = -(hi.y + lo.y) / (hi.y - lo.y);
+    O[14] = -zn / (zf - zn);
+    O[15] = 1.0;
+    double VP[16], IVP[16];
+    mat4_mul(O, LV, VP);
+    if (!mat4_inverse(VP, IVP)) return -2;
+    for (int i = 0; i < 16; ++i) {
+        out->view_proj[i] = (float)VP[i];
+        out->inv_view_proj[i] = (float)IVP[i];
+    }
+    return 0;
+}
+
 int luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj[16], float inv_view_proj[16]) {
     if (!s || !view_proj) return -1;
     // CameraNode::GetView (glm::lookAt, right-handed) and GetProjJittered =
@@ -551,6 +688,12 @@ void luzscene_jitter(uint32_t index, uint32_t width, uint32_t height, float jitt
     uint32_t k = index % 16 + 1;
     jitter[0] = (float)((2.0 * halton(k, 2) - 1.0) / width);
     jitter[1] = (float)((2.0 * halton(k, 3) - 1.0) / height);
+}
+
+int luzscene_set_shadow_type(luzscene* s, int type) {
+    if (!s || (type != LUZRT_SHADOW_RAYTRACED && type != LUZRT_SHADOW_MAP)) return -1;
+    s->shadowType = type;
+    return 0;
 }
 
 int luzscene_animate(luzscene* s, uint32_t frame) {

@@ -177,6 +177,8 @@ struct LightNode : Node {
     float radius = 2.0f;
     float innerAngle = 60.f;
     float outerAngle = 50.f;
+    float shadowMapRange = 3.0f;    // AssetManager.hpp:208-209
+    float shadowMapFar = 2000.0f;
     // Orient the node so that W*(0,-1,0,0) points along d (shortest arc).
     void SetDirection(D3 d) {
         D3 a = d3(0, -1, 0), b = normalize(d);
