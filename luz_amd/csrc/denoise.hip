@@ -52,6 +52,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kIn = LUZRT_DN_FEATURES, kHid = LUZRT_DN_HIDDEN, kOut = 2;
 constexpr int kBlock = 256;
 constexpr uint32_t kChunkRows = 1024;
+constexpr int kInferTiles = 2;   // 16-sample tiles per wave in inference
 
 constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn;
 constexpr int oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
@@ -186,10 +187,11 @@ __device__ inline void dn_hidden_epilogue(const f32x4 (&acc)[4], const float* bi
 }
 
 // Layer (k >= 2) with 2 K-steps: acc[q][t] = sum_s frag(t, s) * hb[q][s].
-__device__ inline void dn_layer64(const uint4* sfrag, int fbase, int lane, const uint4 (&hb)[4][2],
-                                  f32x4 (&acc)[4][4]) {
+template <int Q>
+__device__ inline void dn_layer64(const uint4* sfrag, int fbase, int lane, const uint4 (&hb)[Q][2],
+                                  f32x4 (&acc)[Q][4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < Q; ++q)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -198,7 +200,7 @@ __device__ inline void dn_layer64(const uint4* sfrag, int fbase, int lane, const
         for (int t = 0; t < 4; ++t) {
             bf16x8 a = as_bf16x8(sfrag[(fbase + t * 2 + s) * 64 + lane]);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q][t] = mfma(a, as_bf16x8(hb[q][s]), acc[q][t]);
+            for (int q = 0; q < Q; ++q) acc[q][t] = mfma(a, as_bf16x8(hb[q][s]), acc[q][t]);
         }
 }
 
@@ -222,7 +224,9 @@ __device__ inline void dn_relu_back(const f32x4 (&dh)[4], uint32_t mask, int g, 
     zb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
 }
 
-template <bool TRAIN>
+// Q: 16-sample tiles per wave and iteration (each weight fragment read from
+// LDS feeds Q MFMAs; fewer tiles hold fewer registers).
+template <bool TRAIN, int Q>
 __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
     constexpr int NFRAG = TRAIN ? kFragAll : kFragFwd;
     __shared__ uint4 sfrag[NFRAG * 64];
@@ -236,14 +240,14 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
     const uint32_t nwaves = gridDim.x * (kBlock / 64);
     const DnFrameDev& F = A.F;
     const uint32_t npix = F.width * F.height;
-    for (uint32_t base = (blockIdx.x * (kBlock / 64) + wave) * 64u; base < A.nsamples;
-         base += nwaves * 64u) {
-        uint32_t px[4], py[4], pl[4], pp[4], row[4];
-        bool valid[4], inb[4];
-        float dc[4];
+    for (uint32_t base = (blockIdx.x * (kBlock / 64) + wave) * (16u * Q); base < A.nsamples;
+         base += nwaves * (16u * Q)) {
+        uint32_t px[Q], py[Q], pl[Q], pp[Q], row[Q];
+        bool valid[Q], inb[Q];
+        float dc[Q];
         bool any = false;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < Q; ++q) {
             uint32_t i = base + 16u * q + (uint32_t)j;
             uint32_t p = 0, l = 0;
             bool in = i < A.nsamples, v = false;
@@ -278,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         if (!TRAIN && __ballot(any) == 0ull) {
             // background only: AO and visibility 0
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < Q; ++q)
                 if (g == 0 && inb[q]) {
                     A.ao_part[(size_t)pl[q] * npix + pp[q]] = 0.0f;
                     if (F.num_lights) A.vis_out[(size_t)pl[q] * npix + pp[q]] = 0.0f;
@@ -286,16 +290,16 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
             continue;
         }
         // ---- layer 1: 150 (160) -> 64
-        f32x4 acc[4][4];   // [q][t]
+        f32x4 acc[Q][4];   // [q][t]
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
-            bf16x8 b[4];
+            bf16x8 b[Q];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < Q; ++q) {
                 uint4 fb = dn_features(F, valid[q], px[q], py[q], pl[q], dc[q], 32 * s + 8 * g);
                 b[q] = as_bf16x8(fb);
                 if (TRAIN && inb[q]) {
@@ -310,37 +314,37 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
             for (int t = 0; t < 4; ++t) {
                 bf16x8 a = as_bf16x8(sfrag[(kF1 + t * 5 + s) * 64 + lane]);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q][t] = mfma(a, b[q], acc[q][t]);
+                for (int q = 0; q < Q; ++q) acc[q][t] = mfma(a, b[q], acc[q][t]);
             }
         }
-        uint4 hb[4][2];
-        uint32_t mask[4][4];   // [layer][q]
+        uint4 hb[Q][2];
+        uint32_t mask[4][Q];   // [layer][q]
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < Q; ++q)
             dn_hidden_epilogue<TRAIN>(acc[q], sbias, g, hb[q], mask[0][q], A.HT, A.ldn, row[q], inb[q]);
         // ---- layers 2..4: 64 -> 64
 #pragma unroll
         for (int k = 1; k < 4; ++k) {
-            dn_layer64(sfrag, k == 1 ? kF2 : (k == 2 ? kF3 : kF4), lane, hb, acc);
+            dn_layer64<Q>(sfrag, k == 1 ? kF2 : (k == 2 ? kF3 : kF4), lane, hb, acc);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < Q; ++q)
                 dn_hidden_epilogue<TRAIN>(acc[q], sbias + k * kHid, g, hb[q], mask[k][q],
                                           A.HT + (size_t)k * kHid * A.ldn, A.ldn, row[q], inb[q]);
         }
         // ---- layer 5: 64 -> 2 (one 16-row tile, rows 0 and 1 used)
-        f32x4 out[4];
+        f32x4 out[Q];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) out[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int q = 0; q < Q; ++q) out[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             bf16x8 a = as_bf16x8(sfrag[(kF5 + s) * 64 + lane]);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) out[q] = mfma(a, as_bf16x8(hb[q][s]), out[q]);
+            for (int q = 0; q < Q; ++q) out[q] = mfma(a, as_bf16x8(hb[q][s]), out[q]);
         }
         const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
         if (!TRAIN) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < Q; ++q)
                 if (g == 0 && inb[q]) {
                     float yao = valid[q] ? out[q][0] + b50 : 0.0f;
                     float yvis = valid[q] ? clamp01f(out[q][1] + b51) : 0.0f;
@@ -351,9 +355,9 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         }
         // ---- loss and backward
         float loss = 0.0f;
-        uint4 db[4];
+        uint4 db[Q];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < Q; ++q) {
             float d0 = 0.0f, d1 = 0.0f;
             if (g == 0 && valid[q]) {
                 float e0 = out[q][0] + b50 - A.target_ao[pp[q]];
@@ -371,21 +375,21 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         }
         // dh4 = W5^T dy
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int t = 0; t < 4; ++t)
                 acc[q][t] = mfma(as_bf16x8(sfrag[(kG5 + t) * 64 + lane]), as_bf16x8(db[q]),
                                  f32x4{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
         for (int k = 3; k >= 0; --k) {
-            uint4 zb[4][2];
+            uint4 zb[Q][2];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < Q; ++q)
                 dn_relu_back(acc[q], mask[k][q], g, zb[q], A.ZT + (size_t)k * kHid * A.ldn, A.ldn, row[q],
                              inb[q]);
             if (k == 0) break;
             // dh_{k} = W_{k+1}^T dz_{k+1}  (fragments G2..G4 hold W2^T..W4^T)
-            dn_layer64(sfrag, k == 3 ? kG4 : (k == 2 ? kG3 : kG2), lane, zb, acc);
+            dn_layer64<Q>(sfrag, k == 3 ? kG4 : (k == 2 ? kG3 : kG2), lane, zb, acc);
         }
         // per-wave loss sum (lanes g == 0 hold the outputs)
         for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
@@ -700,7 +704,7 @@ luzrt_status luzrt_denoise(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_fra
     const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * 8;
     if (groups > maxg) groups = maxg;
     stage_begin(ctx, kStageDenoise);
-    k_dn_forward<false><<<groups, kBlock, 0, ctx->stream>>>(A);
+    k_dn_forward<false, kInferTiles><<<groups, kBlock, 0, ctx->stream>>>(A);
     k_dn_ao_mean<<<cdiv(npix, 256), 256, 0, ctx->stream>>>((const float*)d->ao_part.ptr, lights, npix, ao_out);
     stage_end(ctx, kStageDenoise);
     LUZ_HIP(ctx, hipGetLastError());
@@ -751,7 +755,7 @@ luzrt_status luzrt_denoiser_accumulate(luzrt_ctx* ctx, luzrt_denoiser* d, const 
     uint32_t groups = cdiv(count, kBlock);
     const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * 4;
     if (groups > maxg) groups = maxg;
-    k_dn_forward<true><<<groups, kBlock, 0, ctx->stream>>>(A);
+    k_dn_forward<true, 4><<<groups, kBlock, 0, ctx->stream>>>(A);
     LUZ_HIP(ctx, hipGetLastError());
     d->rows += count;
     return LUZRT_OK;
