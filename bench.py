@@ -404,7 +404,9 @@ def bench_denoise(ctx, args) -> int:
            "vis_rmse": round(rmse(dn_vis, ref_vis, fg), 6),
            "raw_1spp_ao_rmse": round(rmse(raw_ao, ref_ao, fg), 6),
            "raw_1spp_vis_rmse": round(rmse(raw_vis, ref_vis, fg), 6)}
-    match = next((r for r in rows if r["ao_rmse"] <= den["ao_rmse"]), None)
+    # equal accuracy on soft-shadow visibility: AO is 1 on every pixel of this workload (the
+    # AO origin offset 0.01 x camera distance reaches the AO radius), so its RMSE cannot rank
+    match = next((r for r in rows if r["vis_rmse"] <= den["vis_rmse"]), None)
     spp64 = next(r for r in rows if r["spp"] == 64)
     flops = float(fg.sum()) * max(L, 1) * 44032.0
     achieved = flops / (td * 1e-3) / 1e12
@@ -419,6 +421,8 @@ def bench_denoise(ctx, args) -> int:
                    "denoised": den,
                    "plain": [{k: v for k, v in r.items() if k != "_ao"} for r in rows],
                    "equal_accuracy_spp": match["spp"] if match else None,
+                   "equal_accuracy_on": "visibility RMSE (AO is 1 on every pixel at this camera distance)",
+                   "ao_fraction_below_1_ref256": round(float(np.mean(ref_ao[fg] < 1.0)), 6),
                    "speedup_vs_equal_accuracy": round(match["frame_ms"] / den["frame_ms"], 2) if match else None,
                    "speedup_vs_64spp": round(spp64["frame_ms"] / den["frame_ms"], 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
