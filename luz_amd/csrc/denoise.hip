@@ -42,6 +42,10 @@ struct DnFrameDev {
     float cam[3];
     float shadow_n, ao_n;
     float pad2;
+    // per-pixel planes of the prepass: (camera distance or -1 on background,
+    // normal.xyz) and the AO fraction
+    const float4* pix;
+    const float* ao_frac;
 };
 
 namespace {
@@ -108,22 +112,38 @@ struct Nb { float c0, c1, c2, c3, c4, c5; };
 
 __device__ inline Nb dn_neighbour(const DnFrameDev& F, uint32_t qx, uint32_t qy, uint32_t light, float dc) {
     const size_t q = (size_t)qy * F.width + qx;
-    const float* n = F.normal + q * 3;
+    const float4 px = F.pix[q];
     Nb r;
-    float nx = n[0], ny = n[1], nz = n[2];
-    if (vlen(v3(nx, ny, nz)) == 0.0f) {   // background
+    if (px.x < 0.0f) {   // background
         r.c0 = 0.0f; r.c1 = 0.0f; r.c2 = 1.0f; r.c3 = 0.0f; r.c4 = 0.0f; r.c5 = 0.0f;
         return r;
     }
-    const float* p = F.position + q * 3;
-    float d = vlen(v3(p[0] - F.cam[0], p[1] - F.cam[1], p[2] - F.cam[2]));
-    r.c0 = (float)F.ao[q] / F.ao_n;
+    const float d = px.x;
+    r.c0 = F.ao_frac[q];
     r.c1 = F.num_lights ? 1.0f - (float)F.shadow[(size_t)light * F.width * F.height + q] / F.shadow_n
                         : 0.0f;
     float rel = dc > 0.0f ? (d - dc) / dc * 16.0f : 0.0f;
     r.c2 = rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel);
-    r.c3 = nx; r.c4 = ny; r.c5 = nz;
+    r.c3 = px.y; r.c4 = px.z; r.c5 = px.w;
     return r;
+}
+
+// Prepass: camera distance (-1 on background pixels, whose normal is zero),
+// normal and AO fraction of every pixel, each computed once per frame.
+__global__ __launch_bounds__(256) void k_dn_prep(DnFrameDev F, float4* pix, float* ao_frac) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= F.width * F.height) return;
+    const float* n = F.normal + (size_t)p * 3;
+    const float nx = n[0], ny = n[1], nz = n[2];
+    if (vlen(v3(nx, ny, nz)) == 0.0f) {
+        pix[p] = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+        ao_frac[p] = 0.0f;
+        return;
+    }
+    const float* P = F.position + (size_t)p * 3;
+    const float d = vlen(v3(P[0] - F.cam[0], P[1] - F.cam[1], P[2] - F.cam[2]));
+    pix[p] = make_float4(d, nx, ny, nz);
+    ao_frac[p] = (float)F.ao[p] / F.ao_n;
 }
 
 __device__ inline float nb_channel(const Nb& b, int c) {
@@ -250,6 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         for (int q = 0; q < Q; ++q) {
             uint32_t i = base + 16u * q + (uint32_t)j;
             uint32_t p = 0, l = 0;
+            float dcv = 0.0f;
             bool in = i < A.nsamples, v = false;
             if (in) {
                 if (TRAIN) {
@@ -263,18 +284,14 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
                 // caller-supplied samples outside the frame contribute nothing
                 if (p >= npix || l >= A.lights) { p = 0; l = 0; in = TRAIN ? in : false; }
                 else {
-                    const float* n = F.normal + (size_t)p * 3;
-                    v = vlen(v3(n[0], n[1], n[2])) != 0.0f;
+                    dcv = F.pix[p].x;
+                    v = dcv >= 0.0f;
                 }
             }
             pp[q] = p; pl[q] = l;
             px[q] = p % F.width; py[q] = p / F.width;
             row[q] = A.row_base + i;
-            dc[q] = 0.0f;
-            if (v) {
-                const float* P = F.position + (size_t)p * 3;
-                dc[q] = vlen(v3(P[0] - F.cam[0], P[1] - F.cam[1], P[2] - F.cam[2]));
-            }
+            dc[q] = v ? dcv : 0.0f;
             valid[q] = v;
             inb[q] = in;
             any |= v;
@@ -518,13 +535,9 @@ __global__ __launch_bounds__(kBlock) void k_dn_features(DnFrameDev F, const uint
     const uint32_t lights = F.num_lights ? F.num_lights : 1u;
     bool valid = p < F.width * F.height && l < lights;
     if (!valid) { p = 0; l = 0; }
-    const float* n = F.normal + (size_t)p * 3;
-    valid = valid && vlen(v3(n[0], n[1], n[2])) != 0.0f;
-    float dc = 0.0f;
-    if (valid) {
-        const float* P = F.position + (size_t)p * 3;
-        dc = vlen(v3(P[0] - F.cam[0], P[1] - F.cam[1], P[2] - F.cam[2]));
-    }
+    const float dcv = F.pix[p].x;
+    valid = valid && dcv >= 0.0f;
+    const float dc = valid ? dcv : 0.0f;
     uint4 fb = dn_features(F, valid, p % F.width, p / F.width, l, dc, 8 * (int)grp);
     const uint32_t ws[4] = {fb.x, fb.y, fb.z, fb.w};
     for (int e = 0; e < 8; ++e) {
@@ -556,6 +569,7 @@ struct luzrt_denoiser {
     uint16_t* ZT = nullptr;
     uint16_t* YT = nullptr;
     DevBuf ao_part;
+    DevBuf pix, ao_frac;    // prepass planes of the current frame
 };
 
 namespace {
@@ -563,7 +577,7 @@ namespace {
 void dn_free(luzrt_denoiser* d) {
     for (void* p : {(void*)d->params, (void*)d->grad, (void*)d->adam_m, (void*)d->adam_v,
                     (void*)d->partial, (void*)d->loss, (void*)d->frags, (void*)d->XT, (void*)d->HT,
-                    (void*)d->ZT, (void*)d->YT, d->ao_part.ptr})
+                    (void*)d->ZT, (void*)d->YT, d->ao_part.ptr, d->pix.ptr, d->ao_frac.ptr})
         if (p) (void)hipFree(p);
     delete d;
 }
@@ -586,6 +600,19 @@ luzrt_status frame_dev(luzrt_ctx* ctx, const luzrt_dn_frame* f, DnFrameDev& F) {
     for (int k = 0; k < 3; ++k) F.cam[k] = f->cam_pos[k];
     F.shadow_n = (float)(f->shadow_samples ? f->shadow_samples : 1u);
     F.ao_n = (float)f->ao_samples;
+    return LUZRT_OK;
+}
+
+// Runs the per-pixel prepass of a frame into the given planes.
+luzrt_status prep_frame(luzrt_ctx* ctx, DnFrameDev& F, DevBuf& pix, DevBuf& ao_frac) {
+    const uint32_t npix = F.width * F.height;
+    luzrt_status s;
+    if ((s = ensure(ctx, pix, (size_t)npix * 16)) != LUZRT_OK) return s;
+    if ((s = ensure(ctx, ao_frac, (size_t)npix * 4)) != LUZRT_OK) return s;
+    k_dn_prep<<<cdiv(npix, 256), 256, 0, ctx->stream>>>(F, (float4*)pix.ptr, (float*)ao_frac.ptr);
+    LUZ_HIP(ctx, hipGetLastError());
+    F.pix = (const float4*)pix.ptr;
+    F.ao_frac = (const float*)ao_frac.ptr;
     return LUZRT_OK;
 }
 
@@ -694,6 +721,8 @@ luzrt_status luzrt_denoise(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_fra
     const uint32_t npix = f->width * f->height;
     const uint32_t lights = f->num_lights ? f->num_lights : 1u;
     if ((s = ensure(ctx, d->ao_part, (size_t)lights * npix * 4)) != LUZRT_OK) return s;
+    stage_begin(ctx, kStageDenoise);
+    if ((s = prep_frame(ctx, A.F, d->pix, d->ao_frac)) != LUZRT_OK) return s;
     A.frags = reinterpret_cast<const uint4*>(d->frags);
     A.params = d->params;
     A.nsamples = npix * lights;
@@ -703,7 +732,6 @@ luzrt_status luzrt_denoise(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_fra
     uint32_t groups = cdiv(A.nsamples, kBlock);
     const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * 8;
     if (groups > maxg) groups = maxg;
-    stage_begin(ctx, kStageDenoise);
     k_dn_forward<false, kInferTiles><<<groups, kBlock, 0, ctx->stream>>>(A);
     k_dn_ao_mean<<<cdiv(npix, 256), 256, 0, ctx->stream>>>((const float*)d->ao_part.ptr, lights, npix, ao_out);
     stage_end(ctx, kStageDenoise);
@@ -719,9 +747,18 @@ luzrt_status luzrt_denoiser_features(luzrt_ctx* ctx, const luzrt_dn_frame* f, co
     luzrt_status s = frame_dev(ctx, f, F);
     if (s != LUZRT_OK) return s;
     if (count == 0) return LUZRT_OK;
-    k_dn_features<<<cdiv((uint64_t)count * 19, kBlock), kBlock, 0, ctx->stream>>>(F, samples, count, out);
-    LUZ_HIP(ctx, hipGetLastError());
-    return LUZRT_OK;
+    DevBuf pix, ao_frac;
+    s = prep_frame(ctx, F, pix, ao_frac);
+    if (s == LUZRT_OK) {
+        k_dn_features<<<cdiv((uint64_t)count * 19, kBlock), kBlock, 0, ctx->stream>>>(F, samples, count, out);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) s = hip_fail(ctx, e, "k_dn_features");
+    }
+    // the planes are released once the kernel has read them
+    (void)hipStreamSynchronize(ctx->stream);
+    if (pix.ptr) (void)hipFree(pix.ptr);
+    if (ao_frac.ptr) (void)hipFree(ao_frac.ptr);
+    return s;
 }
 
 luzrt_status luzrt_denoiser_accumulate(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_frame* f,
@@ -737,6 +774,7 @@ luzrt_status luzrt_denoiser_accumulate(luzrt_ctx* ctx, luzrt_denoiser* d, const 
     luzrt_status s = frame_dev(ctx, f, A.F);
     if (s != LUZRT_OK) return s;
     if (count == 0) return LUZRT_OK;
+    if ((s = prep_frame(ctx, A.F, d->pix, d->ao_frac)) != LUZRT_OK) return s;
     A.frags = reinterpret_cast<const uint4*>(d->frags);
     A.params = d->params;
     A.nsamples = count;

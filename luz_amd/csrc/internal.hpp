@@ -36,8 +36,8 @@ struct luzrt_ctx {
     bool ev_used[luz::kStageCount] = {};
     uint64_t last_rays = 0, last_active = 0;
     // traversal launch configuration
-    uint32_t groups_per_cu = 10;
-    uint32_t refill_threshold = 32;
+    uint32_t groups_per_cu = 6;      // launch shape chosen by the autotuner on MI355X
+    uint32_t refill_threshold = 48;
     uint32_t stack_entries = 16;   // LDS stack entries per lane: 8, 16 or 32
     uint32_t tile_width = 8;       // pixel tiles of 64: 8x8, 16x4 or 32x2
     // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
@@ -188,6 +188,7 @@ struct TraceLaunch {
     // scene_ray_base[s] <= i < scene_ray_base[s + 1] and is traced against
     // scene_tlas[s]; tlas_nodes and recs are then unused.
     uint32_t nscenes;
+    uint32_t tlas_nodes_count, recs_count;   // single-scene TLAS size (staged in LDS when small)
     const uint32_t* scene_ray_base;   // device, nscenes + 1 entries
     const struct BatchTlas* scene_tlas;
 };

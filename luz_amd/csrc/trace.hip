@@ -97,6 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const float* position, const 
 // Rays are claimed in chunks of kChunk per wave (one device-scope atomic per
 // chunk): a single counter hit by every refill serialises across the 8 XCDs.
 constexpr uint32_t kChunk = 2048;
+constexpr uint32_t kTlasLdsEntries = 240;   // TLAS nodes + instance records staged in LDS
 
 // Scene of ray `rid` in a multi-scene launch: the last s < n with base[s] <= rid.
 __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32_t rid) {
@@ -115,6 +116,10 @@ template <bool WIDE, int SHORT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t refill) {
     __shared__ uint32_t lds[SHORT * kBlock];
     __shared__ uint32_t sbase[BATCH ? kMaxBatch + 1 : 1];
+    // A small TLAS (nodes + instance records, 64 B each) is staged in LDS: every
+    // ray starts there, so its reads stop competing with BLAS data in L1.  15 KB
+    // keeps five 256-lane workgroups (16-entry stacks) resident per CU.
+    __shared__ float4 stlas[BATCH ? 4 : kTlasLdsEntries * 4];
     const uint32_t tid = threadIdx.x;
     if (BATCH) {
         for (uint32_t i = tid; i <= L.nscenes; i += kBlock) sbase[i] = L.scene_ray_base[i];
@@ -122,6 +127,16 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
     }
     const Node* tnodes = L.tlas_nodes;
     const InstRec* trecs = L.recs;
+    if (!BATCH && L.tlas_nodes_count + L.recs_count <= kTlasLdsEntries) {
+        const float4* gn = reinterpret_cast<const float4*>(L.tlas_nodes);
+        const float4* gr = reinterpret_cast<const float4*>(L.recs);
+        const uint32_t nn = L.tlas_nodes_count * 4, nr = L.recs_count * 4;
+        for (uint32_t i = tid; i < nn; i += kBlock) stlas[i] = gn[i];
+        for (uint32_t i = tid; i < nr; i += kBlock) stlas[nn + i] = gr[i];
+        __syncthreads();
+        tnodes = reinterpret_cast<const Node*>(stlas);
+        trecs = reinterpret_cast<const InstRec*>(stlas + nn);
+    }
     const uint32_t lane = tid & 63u;
     const uint32_t total = gridDim.x * kBlock;
     StackT<SHORT> st;
@@ -291,6 +306,8 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
         TraceLaunch L{};
         L.tlas_nodes = tlas->nodes;
         L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
+        L.tlas_nodes_count = tlas->node_count;
+        L.recs_count = tlas->count;
         L.ray_o = (const float4*)ctx->rays_o.ptr;
         L.ray_d = (const float4*)ctx->rays_d.ptr;
         L.ray_stride = 1;
@@ -336,6 +353,8 @@ extern "C" luzrt_status luzrt_trace_rays(luzrt_ctx* ctx, const luzrt_tlas* tlas,
         TraceLaunch L{};
         L.tlas_nodes = tlas->nodes;
         L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
+        L.tlas_nodes_count = tlas->node_count;
+        L.recs_count = tlas->count;
         L.ray_o = (const float4*)rays;
         L.ray_d = (const float4*)(rays + 4);
         L.ray_stride = 2;
