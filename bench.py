@@ -408,7 +408,11 @@ def bench_denoise(ctx, args) -> int:
     # AO origin offset 0.01 x camera distance reaches the AO radius), so its RMSE cannot rank
     match = next((r for r in rows if r["vis_rmse"] <= den["vis_rmse"]), None)
     spp64 = next(r for r in rows if r["spp"] == 64)
-    flops = float(fg.sum()) * max(L, 1) * 44032.0
+    # algorithmic FLOPs of k_dn_infer per foreground pixel: the 125 light-independent
+    # inputs of layer 1 once, then per light the 25 visibility inputs of layer 1,
+    # layers 2..4 (64x64) and layer 5 (64x2)
+    flops_px = 2.0 * 125 * 64 + max(L, 1) * 2.0 * (25 * 64 + 3 * 64 * 64 + 64 * 2)
+    flops = float(fg.sum()) * flops_px
     achieved = flops / (td * 1e-3) / 1e12
     record = {
         "metric": "AO RMSE vs FP64 256-spp oracle and ms/frame: 1-spp + learned denoiser vs plain multi-spp",
@@ -426,8 +430,8 @@ def bench_denoise(ctx, args) -> int:
                    "speedup_vs_equal_accuracy": round(match["frame_ms"] / den["frame_ms"], 2) if match else None,
                    "speedup_vs_64spp": round(spp64["frame_ms"] / den["frame_ms"], 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(achieved / 2500.0, 4), "traffic": None, "kernel": "k_dn_forward",
-                     "flops_per_sample": 44032},
+                     "frac": round(achieved / 2500.0, 4), "traffic": None, "kernel": "k_dn_infer",
+                     "flops_per_pixel": flops_px, "timed": "denoise stage: per-pixel prepass + k_dn_infer"},
     }
     if not args.no_cpu:
         from oracle import oracle as orc

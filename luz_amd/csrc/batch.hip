@@ -242,8 +242,7 @@ struct luzrt_batch {
     uint32_t max_m = 1;
     std::vector<luzrt_blas*> refit_blas;
     std::vector<const void*> refit_verts;
-    float* refit_box = nullptr;
-    uint32_t refit_tris = 0, refit_jobs = 0;
+    uint32_t refit_chunks = 0, refit_jobs = 0;
     bool refits_dirty = true;
     // rays
     uint64_t max_rays = 0;
@@ -308,33 +307,21 @@ luzrt_status upload_tables(luzrt_ctx* ctx, luzrt_batch* b) {
 
 luzrt_status upload_refits(luzrt_ctx* ctx, luzrt_batch* b) {
     std::vector<RefitJob> jobs;
-    uint32_t tris = 0;
+    uint32_t chunks = 0;
     for (size_t i = 0; i < b->refit_blas.size(); ++i) {
         luzrt_blas* bl = b->refit_blas[i];
         if (bl->tri_count < 2) continue;   // a single-triangle root is rebuilt by luzrt_blas_refit only
-        RefitJob J;
-        J.nodes = bl->nodes;
-        J.leaf_parent = bl->leaf_parent;
-        J.counters = bl->counters;
-        J.slot_idx = bl->slot_idx;
-        J.tris = bl->tris;
-        J.verts = (const uint8_t*)b->refit_verts[i];
-        J.stride = bl->stride;
-        J.n = bl->tri_count;
-        J.tri_offset = tris;
-        J.pad = 0;
-        tris += bl->tri_count;
+        RefitJob J = refit_job(bl, b->refit_verts[i], nullptr);
+        J.chunk_offset = chunks;
+        chunks += refit_chunks(bl->tri_count);
         jobs.push_back(J);
     }
     if (b->dev_refits) LUZ_HIP(ctx, hipFree(b->dev_refits));
-    if (b->refit_box) LUZ_HIP(ctx, hipFree(b->refit_box));
     b->dev_refits = nullptr;
-    b->refit_box = nullptr;
     b->refit_jobs = (uint32_t)jobs.size();
-    b->refit_tris = tris;
+    b->refit_chunks = chunks;
     if (!jobs.empty()) {
         LUZ_HIP(ctx, hipMalloc((void**)&b->dev_refits, jobs.size() * sizeof(RefitJob)));
-        LUZ_HIP(ctx, hipMalloc((void**)&b->refit_box, (size_t)tris * 24));
         LUZ_HIP(ctx, hipMemcpy(b->dev_refits, jobs.data(), jobs.size() * sizeof(RefitJob),
                                hipMemcpyHostToDevice));
     }
@@ -350,7 +337,7 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     LUZ_HIP(ctx, hipMemcpyAsync(b->dev_stage, b->host_stage, b->stage_bytes, hipMemcpyHostToDevice, st));
     luzrt_status s;
     if (b->refit_jobs &&
-        (s = launch_refit_batched(ctx, b->dev_refits, b->refit_jobs, b->refit_tris, b->refit_box)) !=
+        (s = launch_refit_batched(ctx, b->dev_refits, b->refit_jobs, b->refit_chunks)) !=
             LUZRT_OK)
         return s;
     if ((s = launch_tlas_batched(ctx, b->dev_jobs, b->n, b->max_m)) != LUZRT_OK) return s;
@@ -427,7 +414,7 @@ void batch_free(luzrt_batch* b) {
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->host_stage) (void)hipHostFree(b->host_stage);
     for (void* p : {(void*)b->dev_stage, (void*)b->dev_work, (void*)b->dev_tables, (void*)b->dev_mats,
-                    (void*)b->dev_tlas, (void*)b->dev_jobs, (void*)b->dev_refits, (void*)b->refit_box,
+                    (void*)b->dev_tlas, (void*)b->dev_jobs, (void*)b->dev_refits,
                     b->rays_o.ptr, b->rays_d.ptr, b->hits.ptr})
         if (p) (void)hipFree(p);
     delete b;

@@ -258,7 +258,7 @@ __device__ inline int delta_k(const uint32_t* k, int n, int i, int j) {
 
 // Karras split for internal node i; writes children and parents.
 __device__ inline void karras_node(const uint32_t* keys, int n, int i, Node* nodes,
-                                   uint32_t* leaf_parent) {
+                                   uint32_t* leaf_parent, uint32_t* node_end) {
     int d = (delta_k(keys, n, i, i + 1) - delta_k(keys, n, i, i - 1)) >= 0 ? 1 : -1;
     int dmin = delta_k(keys, n, i, i - d);
     int lmax = 2;
@@ -276,6 +276,7 @@ __device__ inline void karras_node(const uint32_t* keys, int n, int i, Node* nod
     } while (t > 1);
     int gamma = i + s * d + (d < 0 ? d : 0);
     int lo = i < j ? i : j, hi = i < j ? j : i;
+    if (node_end) node_end[i] = (uint32_t)j;   // node i covers leaf slots lo..hi
     uint32_t left = (lo == gamma) ? (kLeaf | (uint32_t)gamma) : (uint32_t)gamma;
     uint32_t right = (hi == gamma + 1) ? (kLeaf | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
     Node& nd = nodes[i];
@@ -288,10 +289,11 @@ __device__ inline void karras_node(const uint32_t* keys, int n, int i, Node* nod
 }
 
 __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, uint32_t n, Node* nodes,
-                                                   uint32_t* leaf_parent, uint32_t* counters) {
+                                                   uint32_t* leaf_parent, uint32_t* node_end,
+                                                   uint32_t* counters) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
-    karras_node(keys, (int)n, (int)i, nodes, leaf_parent);
+    karras_node(keys, (int)n, (int)i, nodes, leaf_parent, node_end);
     counters[i] = 0;
 }
 
@@ -363,32 +365,128 @@ __device__ inline void refit_climb(Node* nodes, const uint32_t* leaf_parent, uin
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_refit(Node* nodes, const uint32_t* leaf_parent,
-                                                  uint32_t* counters, uint32_t n,
-                                                  const float* slot_box, uint32_t* error) {
-    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    refit_climb(nodes, leaf_parent, counters, s, slot_box + (size_t)s * 6, error);
+constexpr int kRefitChunk = 256;    // leaf slots per workgroup of the chunk pass
+constexpr int kSpanThreads = 512;   // threads per BLAS of the span pass
+
+// Union of a node's two child boxes, child 0 first (the operand order of
+// fmin_/fmax_ matters for signed zeros).
+__device__ inline void child_union(int side, const float* mine, const float* other, float* out) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const float b0 = side == 0 ? mine[k] : other[k], b1 = side == 0 ? other[k] : mine[k];
+        out[k] = k < 3 ? fmin_(b0, b1) : fmax_(b0, b1);
+    }
 }
 
-// Recompute slot boxes and triangle records from moved vertices.
-__global__ __launch_bounds__(kBlock) void k_refit_boxes(const uint8_t* verts, uint32_t stride,
-                                                        const uint32_t* slot_idx, uint32_t n,
-                                                        float* tris, float* slot_box,
-                                                        uint32_t* counters, uint32_t ncounters) {
-    // slot_idx was validated by the build
-    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s < ncounters) counters[s] = 0;
-    if (s >= n) return;
-    V3 v[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        v[k] = load_pos(verts, stride, slot_idx[3 * s + k]);
-        tris[(size_t)s * 9 + k * 3 + 0] = v[k].x;
-        tris[(size_t)s * 9 + k * 3 + 1] = v[k].y;
-        tris[(size_t)s * 9 + k * 3 + 2] = v[k].z;
+__device__ inline uint32_t chunk_job_of(const RefitJob* jobs, uint32_t njobs, uint32_t c) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].chunk_offset <= c) lo = mid; else hi = mid;
     }
-    tri_box(v[0], v[1], v[2], slot_box + (size_t)s * 6);
+    return lo;
+}
+
+// Chunk pass of a refit (or of a build): one workgroup per 256 consecutive
+// leaf slots of a BLAS.  Every internal node covers a contiguous range of
+// slots (Karras), so a node whose range lies inside the chunk has both
+// children completed by this workgroup: the second child to arrive forms the
+// node's box, with arrival counters and child boxes in LDS.  A node whose
+// parent's range leaves the chunk stores its box in the parent and hands the
+// parent to the span pass through the BLAS's span list.  Also zeroes the span
+// pass's arrival counters of the chunk's node ids.  jobs == nullptr: `single`.
+template <bool FROM_VERTS>
+__global__ __launch_bounds__(kRefitChunk) void k_refit_chunks(const RefitJob* jobs, uint32_t njobs,
+                                                              RefitJob single, uint32_t* error) {
+    __shared__ uint32_t scnt[kRefitChunk];
+    __shared__ float sbox[kRefitChunk * 12];
+    const RefitJob& J = jobs ? jobs[chunk_job_of(jobs, njobs, blockIdx.x)] : single;
+    const uint32_t c0 = (blockIdx.x - J.chunk_offset) * kRefitChunk;
+    const uint32_t c1 = (c0 + kRefitChunk < J.n ? c0 + kRefitChunk : J.n) - 1;   // last slot
+    const uint32_t s = c0 + threadIdx.x;
+    scnt[threadIdx.x] = 0;
+    if (s < J.n - 1) J.counters[s] = 0;
+    __syncthreads();
+    if (s >= J.n) return;
+    float box[6];
+    if (FROM_VERTS) {
+        // slot_idx was validated by the build
+        V3 v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[k] = load_pos(J.verts, J.stride, J.slot_idx[3 * s + k]);
+            J.tris[(size_t)s * 9 + k * 3 + 0] = v[k].x;
+            J.tris[(size_t)s * 9 + k * 3 + 1] = v[k].y;
+            J.tris[(size_t)s * 9 + k * 3 + 2] = v[k].z;
+        }
+        tri_box(v[0], v[1], v[2], box);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) box[k] = J.leaf_box[(size_t)s * 6 + k];
+    }
+    uint32_t c = kLeaf | s;
+    uint32_t p = J.leaf_parent[s];
+    for (int level = 0; p != kNone; ++level) {
+        if (level > 80) { atomicOr(error, 1u); return; }
+        Node* nd = &J.nodes[p];
+        const int side = nd->child[0] == c ? 0 : 1;
+        store_child_box(nd, side, box);
+        const uint32_t e = J.node_end[p];
+        const uint32_t lo = p < e ? p : e, hi = p < e ? e : p;
+        if (lo < c0 || hi > c1) {
+            const uint32_t k = atomicAdd(&J.span[0], 1u);
+            J.span[1 + k] = p;
+            return;
+        }
+        float* sb = &sbox[(p - c0) * 12];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sb[side * 6 + k] = box[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (atomicAdd(&scnt[p - c0], 1u) == 0) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        float other[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) other[k] = sb[(1 - side) * 6 + k];
+        float u[6];
+        child_union(side, box, other, u);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) box[k] = u[k];
+        c = p;
+        p = nd->parent;
+    }
+}
+
+// Span pass: one workgroup per BLAS completes the nodes whose slot range
+// crosses a chunk boundary, climbing from the span list (each entry is one
+// arrival at a node whose child box is already stored in it).  Every arrival
+// at these nodes comes from this workgroup, so arrival counters in global
+// memory with workgroup-scope ordering suffice.  Leaves the list empty.
+__global__ __launch_bounds__(kSpanThreads) void k_refit_spans(const RefitJob* jobs, RefitJob single,
+                                                              uint32_t* error) {
+    const RefitJob& J = jobs ? jobs[blockIdx.x] : single;
+    const uint32_t count = J.span[0];
+    for (uint32_t i = threadIdx.x; i < count; i += kSpanThreads) {
+        uint32_t p = J.span[1 + i];
+        for (int level = 0;; ++level) {
+            if (level > 80) { atomicOr(error, 1u); break; }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (__hip_atomic_fetch_add(&J.counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const Node* nd = &J.nodes[p];
+            const float b0[6] = {nd->lox[0], nd->loy[0], nd->loz[0], nd->hix[0], nd->hiy[0], nd->hiz[0]};
+            const float b1[6] = {nd->lox[1], nd->loy[1], nd->loz[1], nd->hix[1], nd->hiy[1], nd->hiz[1]};
+            float u[6];
+            child_union(0, b0, b1, u);
+            const uint32_t q = nd->parent;
+            if (q == kNone) break;
+            Node* nq = &J.nodes[q];
+            store_child_box(nq, nq->child[0] == p ? 0 : 1, u);
+            p = q;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) J.span[0] = 0;
 }
 
 // BLAS of 0 or 1 triangles: a single root with an empty second child.
@@ -780,42 +878,6 @@ __global__ __launch_bounds__(kTlasThreads) void k_tlas_build_batched(const TlasJ
 }
 
 // Batched refit: global triangle t belongs to the job whose tri_offset range holds it.
-__device__ inline uint32_t refit_job_of(const RefitJob* jobs, uint32_t njobs, uint32_t t) {
-    uint32_t lo = 0, hi = njobs;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].tri_offset <= t) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-__global__ __launch_bounds__(kBlock) void k_refit_boxes_batched(const RefitJob* jobs, uint32_t njobs,
-                                                                uint32_t total, float* slot_box) {
-    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= total) return;
-    const RefitJob& J = jobs[refit_job_of(jobs, njobs, t)];
-    uint32_t s = t - J.tri_offset;
-    if (s < J.n - 1) J.counters[s] = 0;
-    V3 v[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        v[k] = load_pos(J.verts, J.stride, J.slot_idx[3 * s + k]);
-        J.tris[(size_t)s * 9 + k * 3 + 0] = v[k].x;
-        J.tris[(size_t)s * 9 + k * 3 + 1] = v[k].y;
-        J.tris[(size_t)s * 9 + k * 3 + 2] = v[k].z;
-    }
-    tri_box(v[0], v[1], v[2], slot_box + (size_t)t * 6);
-}
-
-__global__ __launch_bounds__(kBlock) void k_refit_batched(const RefitJob* jobs, uint32_t njobs,
-                                                          uint32_t total, const float* slot_box,
-                                                          uint32_t* error) {
-    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= total) return;
-    const RefitJob& J = jobs[refit_job_of(jobs, njobs, t)];
-    refit_climb(J.nodes, J.leaf_parent, J.counters, t - J.tri_offset, slot_box + (size_t)t * 6, error);
-}
-
 inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
@@ -873,10 +935,13 @@ luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    k_karras<<<cdiv(n - 1, kBlock), kBlock, 0, st>>>(kin, n, b->nodes, b->leaf_parent, b->counters);
+    k_karras<<<cdiv(n - 1, kBlock), kBlock, 0, st>>>(kin, n, b->nodes, b->leaf_parent, b->node_end,
+                                                      b->counters);
     k_gather<<<nb, kBlock, 0, st>>>(verts, b->stride, b->vertex_count, indices, vin, n, boxes,
                                     b->tris, b->nrms, b->prims, b->slot_idx, slot_box, error);
-    k_refit<<<nb, kBlock, 0, st>>>(b->nodes, b->leaf_parent, b->counters, n, slot_box, error);
+    const RefitJob J = refit_job(b, nullptr, slot_box);
+    k_refit_chunks<false><<<refit_chunks(n), kRefitChunk, 0, st>>>(nullptr, 1, J, error);
+    k_refit_spans<<<1, kSpanThreads, 0, st>>>(nullptr, J, error);
     LUZ_HIP(ctx, hipGetLastError());
     LUZ_HIP(ctx, hipFreeAsync(scratch, st));
     return LUZRT_OK;
@@ -895,14 +960,10 @@ luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
         LUZ_HIP(ctx, hipGetLastError());
         return LUZRT_OK;
     }
-    float* slot_box = nullptr;
-    LUZ_HIP(ctx, hipMallocAsync((void**)&slot_box, (size_t)n * 24, st));
-    const uint32_t nb = cdiv(n, kBlock);
-    k_refit_boxes<<<nb, kBlock, 0, st>>>((const uint8_t*)vertices, b->stride, b->slot_idx, n, b->tris,
-                                         slot_box, b->counters, n - 1);
-    k_refit<<<nb, kBlock, 0, st>>>(b->nodes, b->leaf_parent, b->counters, n, slot_box, error);
+    const RefitJob J = refit_job(b, vertices, nullptr);
+    k_refit_chunks<true><<<refit_chunks(n), kRefitChunk, 0, st>>>(nullptr, 1, J, error);
+    k_refit_spans<<<1, kSpanThreads, 0, st>>>(nullptr, J, error);
     LUZ_HIP(ctx, hipGetLastError());
-    LUZ_HIP(ctx, hipFreeAsync(slot_box, st));
     return LUZRT_OK;
 }
 
@@ -977,15 +1038,34 @@ luzrt_status launch_tlas_batched(luzrt_ctx* ctx, const TlasJob* dev_jobs, uint32
     return LUZRT_OK;
 }
 
+uint32_t refit_chunks(uint32_t tris) { return cdiv(tris, kRefitChunk); }
+
+RefitJob refit_job(luzrt_blas* b, const void* vertices, const float* leaf_box) {
+    RefitJob J;
+    J.nodes = b->nodes;
+    J.leaf_parent = b->leaf_parent;
+    J.counters = b->counters;
+    J.node_end = b->node_end;
+    J.span = b->span;
+    J.slot_idx = b->slot_idx;
+    J.tris = b->tris;
+    J.verts = (const uint8_t*)vertices;
+    J.leaf_box = leaf_box;
+    J.stride = b->stride;
+    J.n = b->tri_count;
+    J.chunk_offset = 0;
+    J.pad = 0;
+    return J;
+}
+
 luzrt_status launch_refit_batched(luzrt_ctx* ctx, const RefitJob* dev_jobs, uint32_t njobs,
-                                  uint32_t total_tris, float* slot_box) {
+                                  uint32_t total_chunks) {
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
-    if (njobs == 0 || total_tris == 0) return LUZRT_OK;
-    k_refit_boxes_batched<<<cdiv(total_tris, kBlock), kBlock, 0, ctx->stream>>>(dev_jobs, njobs,
-                                                                              total_tris, slot_box);
-    k_refit_batched<<<cdiv(total_tris, kBlock), kBlock, 0, ctx->stream>>>(dev_jobs, njobs, total_tris,
-                                                                        slot_box, error);
+    if (njobs == 0 || total_chunks == 0) return LUZRT_OK;
+    RefitJob none{};
+    k_refit_chunks<true><<<total_chunks, kRefitChunk, 0, ctx->stream>>>(dev_jobs, njobs, none, error);
+    k_refit_spans<<<njobs, kSpanThreads, 0, ctx->stream>>>(dev_jobs, none, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
 }

@@ -19,7 +19,11 @@
 // 8g + e.  Any K order works as long as A and B agree, because the hardware
 // pairs element e of lane group g of A with the same element of B.
 //
-// Training: k_dn_forward<true> runs forward, loss and the backward pass to
+// Inference (k_dn_infer) takes 16-pixel tiles: the 125 features that do not
+// depend on the light enter layer 1 once per pixel, the 25 visibilities of a
+// light as one more K-step per light, and layers 2..5 run per light.
+//
+// Training: k_dn_forward runs forward, loss and the backward pass to
 // every layer's pre-activation gradient dz, and stores x, h and dz (BF16)
 // sample-minor ([feature][sample]); k_dn_wgrad forms dW = dz^T h per chunk of
 // 1024 samples with MFMA (K = samples) and bias sums; k_dn_reduce adds the
@@ -56,7 +60,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kIn = LUZRT_DN_FEATURES, kHid = LUZRT_DN_HIDDEN, kOut = 2;
 constexpr int kBlock = 256;
 constexpr uint32_t kChunkRows = 1024;
-constexpr int kInferTiles = 2;   // 16-sample tiles per wave in inference
+// inference: 16-pixel tiles per wave, and 8 waves per group sharing one copy
+// of the weights in LDS (measured on the 1080p frame: 1 tile x 512 threads
+// 0.93 ms, 1 x 256 1.01 ms, 2 x 512 1.22 ms, 2 x 256 1.49 ms, 4 x 256 2.15 ms)
+constexpr int kInferTiles = 1;
+constexpr int kInferBlock = 512;
 
 constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn;
 constexpr int oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
@@ -70,15 +78,49 @@ constexpr int kBOff[5] = {oB1, oB2, oB3, oB4, oB5};
 
 // Fragment table (one fragment = 64 lanes x 8 BF16 = 1 KB):
 //   F1(t,s) t<4 s<5 | F2..F4(t,s) t<4 s<2 | F5(s) s<2 | G2..G4(t,s) | G5(t) t<4
+//   | FG(t,s) t<4 s<4 | FV(t) t<4
+// FG and FV hold layer 1 again in the inference order, which splits the
+// features of a (pixel, light) sample into the 125 that do not depend on the
+// light (one product per pixel) and the 25 visibilities (one K-step per light):
+//   FG, K-steps s < 3, slot k = 4s + g: element e < 4 is channel (AO, n.x, n.y,
+//       n.z)[e] of neighbour 2k, e >= 4 the same of neighbour 2k + 1;
+//   FG, K-step 3: element e is the relative depth of neighbour 8g + e (< 25);
+//       lane group 3 also holds (AO, n.x, n.y, n.z) of neighbour 24 in e = 1..4;
+//   FV: element e is the visibility of neighbour 8g + e (< 25).
 constexpr int kF1 = 0, kF2 = 20, kF3 = 28, kF4 = 36, kF5 = 44, kG2 = 46, kG3 = 54, kG4 = 62,
-              kG5 = 70, kFragFwd = 46, kFragAll = 74;
+              kG5 = 70, kFG = 74, kFV = 90, kFragTrain = 74, kFragAll = 94;
+// inference fragments in LDS: F2..F5 (26), then FG and FV (20)
+constexpr int kIF2 = 0, kIF3 = 8, kIF4 = 16, kIF5 = 24, kIFG = 26, kIFV = 42, kFragInfer = 46;
+
+// Original feature index (6 n + channel) of element e of lane group g in
+// inference K-step s (s == 4: the visibility step), or -1 for padding.
+__host__ __device__ inline int infer_feature(int s, int g, int e) {
+    if (s < 3) {
+        const int n = 2 * (4 * s + g) + (e >> 2), ch = e & 3;
+        return 6 * n + (ch == 0 ? 0 : ch + 2);
+    }
+    const int n = 8 * g + e;
+    if (s == 4) return n < 25 ? 6 * n + 1 : -1;
+    if (n < 25) return 6 * n + 2;
+    if (g == 3 && e >= 1 && e <= 4) return 6 * 24 + (e == 1 ? 0 : e + 1);
+    return -1;
+}
 
 __host__ __device__ inline int khid(int s, int g, int e) { return 16 * (2 * s + (e >> 2)) + 4 * g + (e & 3); }
 
-__device__ inline uint32_t bf16_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
-    return u >> 16;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// BF16 of two finite floats, round to nearest even (v_cvt_pk_bf16_f32): a in
+// the low half, b in the high half.
+__device__ inline uint32_t bf16_pack2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2{a, b}), bf16x2));
+}
+__device__ inline uint32_t bf16_bits(float f) { return bf16_pack2(f, 0.0f) & 0xFFFFu; }
+// max(z, 0) for finite z on the bit pattern (negative floats are negative integers)
+__device__ inline float relu_bits(float z) {
+    const int b = __float_as_int(z);
+    return __int_as_float(b > 0 ? b : 0);
 }
 __device__ inline float bf16_val(uint32_t h) { return __uint_as_float(h << 16); }
 __device__ inline float clamp01f(float a) { return a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a); }
@@ -92,12 +134,10 @@ struct DnArgs {
     const uint4* frags;
     const float* params;
     uint32_t nsamples;
-    uint32_t lights;           // samples per pixel in inference (max(L, 1))
-    float* ao_part;            // inference: [lights][npix] AO output per sample
-    float* vis_out;            // inference: [L][npix]
-    const uint32_t* samples;   // training: pixel | light << 24
-    const float* target_ao;    // training: [npix]
-    const float* target_vis;   // training: [L][npix]
+    uint32_t lights;           // max(L, 1)
+    const uint32_t* samples;   // pixel | light << 24
+    const float* target_ao;    // [npix]
+    const float* target_vis;   // [L][npix]
     uint32_t row_base, ldn;    // training rows of this call, leading dimension
     uint16_t* XT;              // [160][ldn]
     uint16_t* HT;              // [4][64][ldn]
@@ -178,30 +218,48 @@ __device__ inline uint4 dn_features(const DnFrameDev& F, bool valid, uint32_t x,
     return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-// Bias + ReLU of a hidden layer's 4x4 tiles of one sample column; the next
-// layer's B fragments, the ReLU mask (bit t*4+i) and, when training, h stored.
-template <bool TRAIN>
+// Bias + ReLU of a hidden layer's 4x4 tiles of one sample column (training);
+// the next layer's B fragments, the ReLU mask (bit t*4+i) and h stored.
 __device__ inline void dn_hidden_epilogue(const f32x4 (&acc)[4], const float* bias, int g, uint4 (&hb)[2],
                                           uint32_t& mask, uint16_t* HTl, uint32_t ldn, uint32_t row,
                                           bool store) {
-    uint32_t w[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
+    uint32_t w[2][4];
     mask = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
+        float h[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            int o = 16 * t + 4 * g + i;
-            float z = acc[t][i] + bias[o];
-            bool on = z > 0.0f;
-            float h = on ? z : 0.0f;
+            const float z = acc[t][i] + bias[16 * t + 4 * g + i];
+            const bool on = z > 0.0f;
+            h[i] = on ? z : 0.0f;
             mask |= (on ? 1u : 0u) << (t * 4 + i);
-            uint32_t hbits = bf16_bits(h);
-            // element e = (t & 1) * 4 + i of K-step s = t / 2
-            int s = t >> 1, e = (t & 1) * 4 + i;
-            w[s][e >> 1] |= hbits << (16 * (e & 1));
-            if (TRAIN && store) HTl[(size_t)o * ldn + row] = (uint16_t)hbits;
+        }
+        // elements e = (t & 1) * 4 + i of K-step s = t / 2
+#pragma unroll
+        for (int i = 0; i < 4; i += 2) {
+            const uint32_t bits = bf16_pack2(h[i], h[i + 1]);
+            w[t >> 1][((t & 1) * 4 + i) >> 1] = bits;
+            if (store) {
+                const int o = 16 * t + 4 * g + i;
+                HTl[(size_t)o * ldn + row] = (uint16_t)bits;
+                HTl[(size_t)(o + 1) * ldn + row] = (uint16_t)(bits >> 16);
+            }
         }
     }
+    hb[0] = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
+    hb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
+}
+
+// ReLU of a hidden layer's 4x4 tiles (bias already in the accumulators) into
+// the next layer's B fragments (inference).
+__device__ inline void dn_relu_pack(const f32x4 (&acc)[4], uint4 (&hb)[2]) {
+    uint32_t w[2][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; i += 2)
+            w[t >> 1][((t & 1) * 4 + i) >> 1] = bf16_pack2(relu_bits(acc[t][i]), relu_bits(acc[t][i + 1]));
     hb[0] = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
     hb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
 }
@@ -228,30 +286,33 @@ __device__ inline void dn_layer64(const uint4* sfrag, int fbase, int lane, const
 // when training rows are valid, and builds the B fragments of dz.
 __device__ inline void dn_relu_back(const f32x4 (&dh)[4], uint32_t mask, int g, uint4 (&zb)[2],
                                     uint16_t* ZTl, uint32_t ldn, uint32_t row, bool store) {
-    uint32_t w[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
+    uint32_t w[2][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            int o = 16 * t + 4 * g + i;
-            float dz = (mask >> (t * 4 + i)) & 1u ? dh[t][i] : 0.0f;
-            uint32_t bits = bf16_bits(dz);
-            int s = t >> 1, e = (t & 1) * 4 + i;
-            w[s][e >> 1] |= bits << (16 * (e & 1));
-            if (store) ZTl[(size_t)o * ldn + row] = (uint16_t)bits;
+        for (int i = 0; i < 4; i += 2) {
+            const float d0 = (mask >> (t * 4 + i)) & 1u ? dh[t][i] : 0.0f;
+            const float d1 = (mask >> (t * 4 + i + 1)) & 1u ? dh[t][i + 1] : 0.0f;
+            const uint32_t bits = bf16_pack2(d0, d1);
+            w[t >> 1][((t & 1) * 4 + i) >> 1] = bits;
+            if (store) {
+                const int o = 16 * t + 4 * g + i;
+                ZTl[(size_t)o * ldn + row] = (uint16_t)bits;
+                ZTl[(size_t)(o + 1) * ldn + row] = (uint16_t)(bits >> 16);
+            }
         }
     zb[0] = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
     zb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
 }
 
-// Q: 16-sample tiles per wave and iteration (each weight fragment read from
-// LDS feeds Q MFMAs; fewer tiles hold fewer registers).
-template <bool TRAIN, int Q>
+// Training forward, loss and backward of explicit samples.  Q: 16-sample
+// tiles per wave and iteration (each weight fragment read from LDS feeds Q
+// MFMAs).
+template <int Q>
 __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
-    constexpr int NFRAG = TRAIN ? kFragAll : kFragFwd;
-    __shared__ uint4 sfrag[NFRAG * 64];
+    __shared__ uint4 sfrag[kFragTrain * 64];
     __shared__ float sbias[4 * kHid + kOut];
-    for (int i = threadIdx.x; i < NFRAG * 64; i += kBlock) sfrag[i] = A.frags[i];
+    for (int i = threadIdx.x; i < kFragTrain * 64; i += kBlock) sfrag[i] = A.frags[i];
     for (int i = threadIdx.x; i < 4 * kHid + kOut; i += kBlock)
         sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
     __syncthreads();
@@ -265,7 +326,6 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         uint32_t px[Q], py[Q], pl[Q], pp[Q], row[Q];
         bool valid[Q], inb[Q];
         float dc[Q];
-        bool any = false;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             uint32_t i = base + 16u * q + (uint32_t)j;
@@ -273,16 +333,11 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
             float dcv = 0.0f;
             bool in = i < A.nsamples, v = false;
             if (in) {
-                if (TRAIN) {
-                    uint32_t sm = A.samples[i];
-                    p = sm & 0xFFFFFFu;
-                    l = sm >> 24;
-                } else {
-                    p = i / A.lights;
-                    l = i - p * A.lights;
-                }
+                uint32_t sm = A.samples[i];
+                p = sm & 0xFFFFFFu;
+                l = sm >> 24;
                 // caller-supplied samples outside the frame contribute nothing
-                if (p >= npix || l >= A.lights) { p = 0; l = 0; in = TRAIN ? in : false; }
+                if (p >= npix || l >= A.lights) { p = 0; l = 0; }
                 else {
                     dcv = F.pix[p].x;
                     v = dcv >= 0.0f;
@@ -294,17 +349,6 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
             dc[q] = v ? dcv : 0.0f;
             valid[q] = v;
             inb[q] = in;
-            any |= v;
-        }
-        if (!TRAIN && __ballot(any) == 0ull) {
-            // background only: AO and visibility 0
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (g == 0 && inb[q]) {
-                    A.ao_part[(size_t)pl[q] * npix + pp[q]] = 0.0f;
-                    if (F.num_lights) A.vis_out[(size_t)pl[q] * npix + pp[q]] = 0.0f;
-                }
-            continue;
         }
         // ---- layer 1: 150 (160) -> 64
         f32x4 acc[Q][4];   // [q][t]
@@ -319,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
             for (int q = 0; q < Q; ++q) {
                 uint4 fb = dn_features(F, valid[q], px[q], py[q], pl[q], dc[q], 32 * s + 8 * g);
                 b[q] = as_bf16x8(fb);
-                if (TRAIN && inb[q]) {
+                if (inb[q]) {
                     const uint32_t ws[4] = {fb.x, fb.y, fb.z, fb.w};
 #pragma unroll
                     for (int e = 0; e < 8; ++e)
@@ -338,15 +382,15 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         uint32_t mask[4][Q];   // [layer][q]
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            dn_hidden_epilogue<TRAIN>(acc[q], sbias, g, hb[q], mask[0][q], A.HT, A.ldn, row[q], inb[q]);
+            dn_hidden_epilogue(acc[q], sbias, g, hb[q], mask[0][q], A.HT, A.ldn, row[q], inb[q]);
         // ---- layers 2..4: 64 -> 64
 #pragma unroll
         for (int k = 1; k < 4; ++k) {
             dn_layer64<Q>(sfrag, k == 1 ? kF2 : (k == 2 ? kF3 : kF4), lane, hb, acc);
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                dn_hidden_epilogue<TRAIN>(acc[q], sbias + k * kHid, g, hb[q], mask[k][q],
-                                          A.HT + (size_t)k * kHid * A.ldn, A.ldn, row[q], inb[q]);
+                dn_hidden_epilogue(acc[q], sbias + k * kHid, g, hb[q], mask[k][q],
+                                   A.HT + (size_t)k * kHid * A.ldn, A.ldn, row[q], inb[q]);
         }
         // ---- layer 5: 64 -> 2 (one 16-row tile, rows 0 and 1 used)
         f32x4 out[Q];
@@ -359,17 +403,6 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
             for (int q = 0; q < Q; ++q) out[q] = mfma(a, as_bf16x8(hb[q][s]), out[q]);
         }
         const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
-        if (!TRAIN) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (g == 0 && inb[q]) {
-                    float yao = valid[q] ? out[q][0] + b50 : 0.0f;
-                    float yvis = valid[q] ? clamp01f(out[q][1] + b51) : 0.0f;
-                    A.ao_part[(size_t)pl[q] * npix + pp[q]] = yao;
-                    if (F.num_lights) A.vis_out[(size_t)pl[q] * npix + pp[q]] = yvis;
-                }
-            continue;
-        }
         // ---- loss and backward
         float loss = 0.0f;
         uint4 db[Q];
@@ -383,12 +416,12 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
                 d0 = A.grad_scale * e0;
                 d1 = A.grad_scale * e1;
             }
-            uint32_t b0 = bf16_bits(d0), b1 = bf16_bits(d1);
+            const uint32_t b01 = bf16_pack2(d0, d1);
             if (g == 0 && inb[q]) {
-                A.YT[row[q]] = (uint16_t)b0;
-                A.YT[(size_t)A.ldn + row[q]] = (uint16_t)b1;
+                A.YT[row[q]] = (uint16_t)b01;
+                A.YT[(size_t)A.ldn + row[q]] = (uint16_t)(b01 >> 16);
             }
-            db[q] = make_uint4(g == 0 ? (b0 | b1 << 16) : 0u, 0u, 0u, 0u);
+            db[q] = make_uint4(g == 0 ? b01 : 0u, 0u, 0u, 0u);
         }
         // dh4 = W5^T dy
 #pragma unroll
@@ -411,6 +444,227 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         // per-wave loss sum (lanes g == 0 hold the outputs)
         for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
         if (lane == 0) atomicAdd(A.loss_sum, loss);
+    }
+}
+
+struct InferArgs {
+    DnFrameDev F;
+    const uint4* frags;
+    const float* params;
+    uint32_t lights;    // max(L, 1)
+    uint32_t ntiles;    // 16-pixel tiles of the frame
+    float* ao_out;      // [npix]
+    float* vis_out;     // [L][npix]
+};
+
+// Index of neighbour n (< 25) of pixel (x, y), clamped to the image.
+__device__ inline uint32_t dn_nb_index(const DnFrameDev& F, uint32_t x, uint32_t y, int n) {
+    int qx = (int)x + n % 5 - 2, qy = (int)y + n / 5 - 2;
+    qx = qx < 0 ? 0 : (qx >= (int)F.width ? (int)F.width - 1 : qx);
+    qy = qy < 0 ? 0 : (qy >= (int)F.height ? (int)F.height - 1 : qy);
+    return (uint32_t)qy * F.width + (uint32_t)qx;
+}
+
+// Inference, one 16-pixel tile per lane column j, Q tiles per wave: the
+// light-independent part of layer 1 once per pixel (K-steps FG, bias folded
+// into the accumulators), then for every light the visibility K-step FV on
+// top of it and layers 2..5.  A pixel's AO is the mean of its lights' AO
+// outputs, summed in light order; background pixels give 0.
+template <int Q>
+__global__ __launch_bounds__(kInferBlock) void k_dn_infer(InferArgs A) {
+    __shared__ uint4 sfrag[kFragInfer * 64];
+    __shared__ __align__(16) float sbias[4 * kHid + 4];
+    __shared__ uint16_t svis[256];   // BF16 of 1 - s / shadow_n
+    for (int i = threadIdx.x; i < 26 * 64; i += kInferBlock) sfrag[kIF2 * 64 + i] = A.frags[kF2 * 64 + i];
+    for (int i = threadIdx.x; i < 20 * 64; i += kInferBlock) sfrag[kIFG * 64 + i] = A.frags[kFG * 64 + i];
+    for (int i = threadIdx.x; i < 4 * kHid + kOut; i += kInferBlock)
+        sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
+    const DnFrameDev& F = A.F;
+    for (int i = threadIdx.x; i < 256; i += kInferBlock) svis[i] = (uint16_t)bf16_bits(1.0f - (float)i / F.shadow_n);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const uint32_t npix = F.width * F.height;
+    const uint32_t L = F.num_lights;
+    // blocks of one XCD (dispatched round-robin over 8) take neighbouring tiles
+    const uint32_t nb = gridDim.x;
+    const uint32_t lb = nb % 8 == 0 ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+    const uint32_t wave = lb * (kInferBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t nwaves = nb * (kInferBlock / 64);
+    const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
+    for (uint32_t tile0 = wave * Q; tile0 < A.ntiles; tile0 += nwaves * Q) {
+        uint32_t pp[Q], px[Q], py[Q];
+        bool valid[Q], inb[Q];
+        float dc[Q];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint32_t p = (tile0 + q) * 16u + (uint32_t)j;
+            inb[q] = p < npix;
+            const uint32_t pc = inb[q] ? p : 0u;
+            const float d = F.pix[pc].x;
+            valid[q] = inb[q] && d >= 0.0f;
+            dc[q] = valid[q] ? d : 0.0f;
+            pp[q] = pc; px[q] = pc % F.width; py[q] = pc / F.width;
+            any |= valid[q];
+        }
+        if (__ballot(any) == 0ull) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (g == 0 && inb[q]) {
+                    A.ao_out[pp[q]] = 0.0f;
+                    for (uint32_t l = 0; l < L; ++l) A.vis_out[(size_t)l * npix + pp[q]] = 0.0f;
+                }
+            continue;
+        }
+        // ---- layer 1, light-independent features: K-steps 0..2 (AO, normal)
+        f32x4 ag[Q][4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const f32x4 b1 = *reinterpret_cast<const f32x4*>(&sbias[16 * t + 4 * g]);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) ag[q][t] = b1;
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            uint4 fb[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                uint32_t w[4] = {0u, 0u, 0u, 0u};
+                if (valid[q]) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const uint32_t nq = dn_nb_index(F, px[q], py[q], 2 * (4 * s + g) + h);
+                        const float4 v = F.pix[nq];
+                        const bool fg = v.x >= 0.0f;
+                        const float ao = fg ? F.ao_frac[nq] : 0.0f;
+                        w[2 * h] = bf16_pack2(ao, v.y);     // background normals are zero
+                        w[2 * h + 1] = bf16_pack2(v.z, v.w);
+                    }
+                }
+                fb[q] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bf16x8 a = as_bf16x8(sfrag[(kIFG + t * 4 + s) * 64 + lane]);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) ag[q][t] = mfma(a, as_bf16x8(fb[q]), ag[q][t]);
+            }
+        }
+        // ---- K-step 3: relative depth of neighbours 8g + e, and the neighbour
+        // indices and foreground bits that the visibility step reads per light
+        uint32_t vq[Q][8];
+        uint32_t vfg[Q];
+        {
+            uint4 fb[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                float f[8];
+                vfg[q] = 0u;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int n = 8 * g + e;
+                    f[e] = 0.0f;
+                    vq[q][e] = 0u;
+                    if (valid[q] && n < 25) {
+                        const uint32_t nq = dn_nb_index(F, px[q], py[q], n);
+                        const float d = F.pix[nq].x;
+                        vq[q][e] = nq;
+                        if (d >= 0.0f) {
+                            vfg[q] |= 1u << e;
+                            const float rel = dc[q] > 0.0f ? (d - dc[q]) / dc[q] * 16.0f : 0.0f;
+                            f[e] = rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel);
+                        } else {
+                            f[e] = 1.0f;
+                        }
+                    }
+                }
+                if (g == 3 && valid[q]) {   // (AO, normal) of neighbour 24 in elements 1..4
+                    const uint32_t nq = vq[q][0];
+                    const float4 v = F.pix[nq];
+                    const bool fg = (vfg[q] & 1u) != 0u;
+                    f[1] = fg ? F.ao_frac[nq] : 0.0f;
+                    f[2] = v.y; f[3] = v.z; f[4] = v.w;
+                }
+                fb[q] = make_uint4(bf16_pack2(f[0], f[1]), bf16_pack2(f[2], f[3]), bf16_pack2(f[4], f[5]),
+                                   bf16_pack2(f[6], f[7]));
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bf16x8 a = as_bf16x8(sfrag[(kIFG + t * 4 + 3) * 64 + lane]);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) ag[q][t] = mfma(a, as_bf16x8(fb[q]), ag[q][t]);
+            }
+        }
+        // ---- per light: visibility K-step, layers 2..5
+        float aosum[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) aosum[q] = 0.0f;
+        const uint32_t nl = A.lights;
+        for (uint32_t l = 0; l < nl; ++l) {
+            f32x4 acc[Q][4];
+            uint4 hb[Q][2];
+            // the weight fragments are read from LDS at every use: an opaque
+            // lane offset keeps the compiler from holding all 30 of them in
+            // registers across the light loop (which would allow one wave per SIMD)
+            uint32_t ln = (uint32_t)lane;
+            asm volatile("" : "+v"(ln));
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                uint32_t w[4] = {0u, 0u, 0u, 0u};
+                if (L) {
+                    const uint8_t* sh = F.shadow + (size_t)l * npix;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const uint32_t bits = (vfg[q] >> e) & 1u ? (uint32_t)svis[sh[vq[q][e]]] : 0u;
+                        w[e >> 1] |= bits << (16 * (e & 1));
+                    }
+                }
+                const bf16x8 bv = as_bf16x8(make_uint4(w[0], w[1], w[2], w[3]));
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[q][t] = mfma(as_bf16x8(sfrag[(kIFV + t) * 64 + ln]), bv, ag[q][t]);
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) dn_relu_pack(acc[q], hb[q]);
+#pragma unroll
+            for (int k = 1; k < 4; ++k) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const f32x4 bk = *reinterpret_cast<const f32x4*>(&sbias[k * kHid + 16 * t + 4 * g]);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) acc[q][t] = bk;
+                }
+                const int fb = k == 1 ? kIF2 : (k == 2 ? kIF3 : kIF4);
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const bf16x8 a = as_bf16x8(sfrag[(fb + t * 2 + s) * 64 + ln]);
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) acc[q][t] = mfma(a, as_bf16x8(hb[q][s]), acc[q][t]);
+                    }
+#pragma unroll
+                for (int q = 0; q < Q; ++q) dn_relu_pack(acc[q], hb[q]);
+            }
+            f32x4 out[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) out[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const bf16x8 a = as_bf16x8(sfrag[(kIF5 + s) * 64 + ln]);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) out[q] = mfma(a, as_bf16x8(hb[q][s]), out[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (g == 0 && inb[q]) {
+                    aosum[q] += valid[q] ? out[q][0] + b50 : 0.0f;
+                    if (L) A.vis_out[(size_t)l * npix + pp[q]] = valid[q] ? clamp01f(out[q][1] + b51) : 0.0f;
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (g == 0 && inb[q]) A.ao_out[pp[q]] = clamp01f(aosum[q] / (float)nl);
     }
 }
 
@@ -509,20 +763,15 @@ __global__ void k_dn_pack(const float* P, uint16_t* frags) {
     } else if (f < kG5) {
         const int layer = 1 + (f - kG2) / 8, r = (f - kG2) % 8, t = r / 2, s = r % 2;
         val = P[kWOff[layer] + khid(s, g, e) * kHid + (16 * t + j)];
-    } else {
+    } else if (f < kFG) {
         const int t = f - kG5, o = khid(0, g, e);
         if (o < kOut) val = P[oW5 + o * kHid + (16 * t + j)];
+    } else {
+        const int t = f < kFV ? (f - kFG) / 4 : f - kFV, s = f < kFV ? (f - kFG) % 4 : 4;
+        const int k = infer_feature(s, g, e);
+        if (k >= 0) val = P[oW1 + (16 * t + j) * kIn + k];
     }
     frags[id] = (uint16_t)bf16_bits(val);
-}
-
-// AO of a pixel: mean over its samples' AO outputs, clamped to [0, 1].
-__global__ void k_dn_ao_mean(const float* ao_part, uint32_t lights, uint32_t npix, float* ao_out) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npix) return;
-    float s = 0.0f;
-    for (uint32_t l = 0; l < lights; ++l) s += ao_part[(size_t)l * npix + p];
-    ao_out[p] = clamp01f(s / (float)lights);
 }
 
 // Input features of explicit samples, [count][150] BF16 bits (inspection).
@@ -568,7 +817,6 @@ struct luzrt_denoiser {
     uint16_t* HT = nullptr;
     uint16_t* ZT = nullptr;
     uint16_t* YT = nullptr;
-    DevBuf ao_part;
     DevBuf pix, ao_frac;    // prepass planes of the current frame
 };
 
@@ -577,7 +825,7 @@ namespace {
 void dn_free(luzrt_denoiser* d) {
     for (void* p : {(void*)d->params, (void*)d->grad, (void*)d->adam_m, (void*)d->adam_v,
                     (void*)d->partial, (void*)d->loss, (void*)d->frags, (void*)d->XT, (void*)d->HT,
-                    (void*)d->ZT, (void*)d->YT, d->ao_part.ptr, d->pix.ptr, d->ao_frac.ptr})
+                    (void*)d->ZT, (void*)d->YT, d->pix.ptr, d->ao_frac.ptr})
         if (p) (void)hipFree(p);
     delete d;
 }
@@ -715,25 +963,28 @@ luzrt_status luzrt_denoise(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_fra
     if (!ctx) return LUZRT_E_INVALID_ARG;
     if (!d || !ao_out || (f && f->num_lights && !vis_out))
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_denoise: null argument");
-    DnArgs A{};
+    InferArgs A{};
     luzrt_status s = frame_dev(ctx, f, A.F);
     if (s != LUZRT_OK) return s;
     const uint32_t npix = f->width * f->height;
-    const uint32_t lights = f->num_lights ? f->num_lights : 1u;
-    if ((s = ensure(ctx, d->ao_part, (size_t)lights * npix * 4)) != LUZRT_OK) return s;
     stage_begin(ctx, kStageDenoise);
     if ((s = prep_frame(ctx, A.F, d->pix, d->ao_frac)) != LUZRT_OK) return s;
     A.frags = reinterpret_cast<const uint4*>(d->frags);
     A.params = d->params;
-    A.nsamples = npix * lights;
-    A.lights = lights;
-    A.ao_part = (float*)d->ao_part.ptr;
+    A.lights = f->num_lights ? f->num_lights : 1u;
+    A.ntiles = cdiv(npix, 16);
+    A.ao_out = ao_out;
     A.vis_out = vis_out;
-    uint32_t groups = cdiv(A.nsamples, kBlock);
-    const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * 8;
+    // as many groups as are resident at once (48 KB of LDS and the registers
+    // of 8 waves each), so that no group waits for another to finish
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dn_infer<kInferTiles>, kInferBlock, 0) !=
+            hipSuccess || per_cu < 1)
+        per_cu = 1;
+    uint32_t groups = cdiv(A.ntiles, (kInferBlock / 64) * kInferTiles);
+    const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * (uint32_t)per_cu;
     if (groups > maxg) groups = maxg;
-    k_dn_forward<false, kInferTiles><<<groups, kBlock, 0, ctx->stream>>>(A);
-    k_dn_ao_mean<<<cdiv(npix, 256), 256, 0, ctx->stream>>>((const float*)d->ao_part.ptr, lights, npix, ao_out);
+    k_dn_infer<kInferTiles><<<groups, kInferBlock, 0, ctx->stream>>>(A);
     stage_end(ctx, kStageDenoise);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
@@ -793,7 +1044,7 @@ luzrt_status luzrt_denoiser_accumulate(luzrt_ctx* ctx, luzrt_denoiser* d, const 
     uint32_t groups = cdiv(count, kBlock);
     const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * 4;
     if (groups > maxg) groups = maxg;
-    k_dn_forward<true, 4><<<groups, kBlock, 0, ctx->stream>>>(A);
+    k_dn_forward<4><<<groups, kBlock, 0, ctx->stream>>>(A);
     LUZ_HIP(ctx, hipGetLastError());
     d->rows += count;
     return LUZRT_OK;

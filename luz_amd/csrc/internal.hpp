@@ -56,6 +56,8 @@ struct luzrt_blas {
     uint32_t* slot_idx = nullptr;   // 3 vertex indices per slot (refit)
     uint32_t* leaf_parent = nullptr;
     uint32_t* counters = nullptr;   // refit arrival counters (node_count)
+    uint32_t* node_end = nullptr;   // per internal node: the far end of its leaf range
+    uint32_t* span = nullptr;       // refit hand-over list: [0] = count, then node ids (tri_count)
     int refs = 0;                   // live TLASes built over this BLAS
     // precision form: 8-wide quantised nodes, built on demand from the binary tree
     luz::Node8* wide = nullptr;
@@ -143,20 +145,27 @@ luzrt_status launch_tlas_batched(luzrt_ctx* ctx, const TlasJob* dev_jobs, uint32
                                  uint32_t max_m);
 
 // One BLAS refit job of a batched refit (binary form only).
+// One BLAS of a refit: the boxes of every node from the leaf boxes, either
+// recomputed from moved vertices (refit) or given per slot (build).
 struct RefitJob {
     Node* nodes;
     uint32_t* leaf_parent;
     uint32_t* counters;
+    const uint32_t* node_end;
+    uint32_t* span;
     const uint32_t* slot_idx;
     float* tris;
-    const uint8_t* verts;
+    const uint8_t* verts;     // refit: moved vertices
+    const float* leaf_box;    // build: 6 floats per slot
     uint32_t stride;
-    uint32_t n;             // triangles (>= 2)
-    uint32_t tri_offset;    // first global triangle of this job
+    uint32_t n;               // triangles (>= 2)
+    uint32_t chunk_offset;    // first leaf chunk of this job among all jobs of the launch
     uint32_t pad;
 };
 luzrt_status launch_refit_batched(luzrt_ctx* ctx, const RefitJob* dev_jobs, uint32_t njobs,
-                                  uint32_t total_tris, float* slot_box);
+                                  uint32_t total_chunks);
+RefitJob refit_job(luzrt_blas* b, const void* vertices, const float* leaf_box);
+uint32_t refit_chunks(uint32_t tris);
 
 // Scene block as uploaded for one visibility call: the caller's luzrt_scene
 // plus the ray-slot layout.  Slot j of active pixel a is ray j*A + a, where A

@@ -349,6 +349,9 @@ luzrt_status luzrt_blas_build(luzrt_ctx* ctx, const void* vertices, uint32_t str
     if (e == hipSuccess) e = alloc((void**)&b->slot_idx, n * 12);
     if (e == hipSuccess) e = alloc((void**)&b->leaf_parent, n * 4);
     if (e == hipSuccess) e = alloc((void**)&b->counters, (size_t)b->node_count * 4);
+    if (e == hipSuccess) e = alloc((void**)&b->node_end, (size_t)b->node_count * 4);
+    if (e == hipSuccess) e = alloc((void**)&b->span, (n + 1) * 4);
+    if (e == hipSuccess) e = hipMemset(b->span, 0, 4);   // refit hand-over list starts empty
     if (e != hipSuccess) {
         luzrt_blas_destroy(ctx, b);
         return hip_fail(ctx, e, "hipMalloc(BLAS)");
@@ -385,7 +388,7 @@ luzrt_status luzrt_blas_destroy(luzrt_ctx* ctx, luzrt_blas* b) {
         return fail(ctx, LUZRT_E_IN_USE, "BLAS is referenced by %d TLAS(es)", b->refs);
     hipStreamSynchronize(ctx->stream);
     void* ptrs[] = {b->nodes, b->tris, b->nrms, b->prims, b->slot_idx, b->leaf_parent, b->counters,
-                    b->wide, b->tris8, b->prims8};
+                    b->node_end, b->span, b->wide, b->tris8, b->prims8};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete b;
