@@ -198,6 +198,10 @@ def main() -> int:
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_trace_any",
+            # the BVH (about 11 MB) is served from L2 and the Infinity Cache, so the
+            # algorithmic bytes can exceed the HBM rate; the measured HBM bytes per
+            # launch over the same time give the HBM fraction actually drawn
+            "traffic_frac": round(traffic / (trace_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
             "algorithmic_bytes_per_ray": round(per_ray + 33.0, 2),
             "per_ray_counts": {"node": round(st.node_visits / max(st.rays, 1), 3),
                                "tri": round(st.tri_tests / max(st.rays, 1), 3),

@@ -77,7 +77,7 @@ def _store(cache, key, result):
         cache.write_text(json.dumps(known, indent=1))
 
 
-START = {"groups_per_cu": 6, "refill": 48, "stack_entries": 16, "tile_width": 8}
+START = {"groups_per_cu": 6, "refill": 48, "stack_entries": 16, "tile_width": 32}
 
 
 def tune_visibility(ctx: api.Context, tlas: api.Tlas, g: api.GBuffer, sb: api.SceneBlock,

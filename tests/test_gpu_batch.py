@@ -51,7 +51,7 @@ def workload(ctx):
     wl = BatchWorkload(ctx, SCENES)
     yield wl
     wl.close()
-    ctx.set_trace_config(6, 48, 16, 8)   # the library defaults
+    ctx.set_trace_config(6, 48, 16, 32)   # the library defaults
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
@@ -91,7 +91,7 @@ def test_batch_visibility_matches_oracle(ctx, workload):
 
 def test_batch_recaptures_on_config_change(ctx, workload):
     want = None
-    for cfg in [(6, 48, 16, 8), (6, 16, 8, 16), (8, 32, 32, 32)]:
+    for cfg in [(6, 48, 16, 32), (6, 16, 8, 16), (8, 32, 32, 8)]:
         ctx.set_trace_config(*cfg)
         workload.frame(2, True)
         got = [workload.outputs(i) for i in range(len(SCENES))]
