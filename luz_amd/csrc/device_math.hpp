@@ -239,14 +239,10 @@ __device__ inline bool box_hit(const BoxRay& r, float lox, float hix, float loy,
     float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
     float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
     float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
-    float tn = r.tlo;
-    tn = tnx > tn ? tnx : tn;
-    tn = tny > tn ? tny : tn;
-    tn = tnz > tn ? tnz : tn;
-    float tf = r.thi;
-    tf = tfx < tf ? tfx : tf;
-    tf = tfy < tf ? tfy : tf;
-    tf = tfz < tf ? tfz : tf;
+    // max/min ignore a NaN slab distance (a zero direction component on a slab
+    // plane) as the comparisons of the CPU restatement do
+    const float tn = fmaxf(fmaxf(fmaxf(r.tlo, tnx), tny), tnz);
+    float tf = fminf(fminf(fminf(r.thi, tfx), tfy), tfz);
     tf = tf + fabsf(tf) * 0x1p-20f;
     tnear = tn;
     return tn <= tf;

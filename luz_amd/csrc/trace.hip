@@ -110,6 +110,7 @@ __device__ inline uint32_t claim_chunk(uint32_t nrays, uint32_t nwaves) {
     return c < kChunkMin ? kChunkMin : (c > kChunkMax ? kChunkMax : c);
 }
 constexpr uint32_t kTlasLdsEntries = 240;   // TLAS nodes + instance records staged in LDS
+constexpr bool kStageTlas = true;
 
 // Scene of ray `rid` in a multi-scene launch: the last s < n with base[s] <= rid.
 __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32_t rid) {
@@ -139,7 +140,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
     }
     const Node* tnodes = L.tlas_nodes;
     const InstRec* trecs = L.recs;
-    if (!BATCH && L.tlas_nodes_count + L.recs_count <= kTlasLdsEntries) {
+    bool tlas_lds = false;
+    if (kStageTlas && !BATCH && L.tlas_nodes_count + L.recs_count <= kTlasLdsEntries) {
         const float4* gn = reinterpret_cast<const float4*>(L.tlas_nodes);
         const float4* gr = reinterpret_cast<const float4*>(L.recs);
         const uint32_t nn = L.tlas_nodes_count * 4, nr = L.recs_count * 4;
@@ -148,6 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
         __syncthreads();
         tnodes = reinterpret_cast<const Node*>(stlas);
         trecs = reinterpret_cast<const InstRec*>(stlas + nn);
+        tlas_lds = true;
     }
     const uint32_t lane = tid & 63u;
     const uint32_t total = gridDim.x * kBlock;
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
         }
         if (ray != kNone) {
             for (;;) {
-                int res = ray_step<false, WIDE>(r, st, tnodes, trecs, nullptr, nullptr);
+                int res = ray_step<false, WIDE>(r, st, tnodes, trecs, nullptr, nullptr, tlas_lds);
                 if (res != kStepContinue) {
                     L.hits[ray] = res == kStepHit ? 1 : 0;
                     if (res == kStepOverflow) atomicOr(&L.counters[kCntError], 1u);

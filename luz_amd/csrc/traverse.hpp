@@ -55,9 +55,10 @@ struct Best {
 };
 
 struct RayState {
-    V3 wo, wd;
+    V3 wd;         // world direction (the world origin is wbr.o)
     float tmin, tmax;
-    BoxRay wbr, obr;
+    BoxRay br;     // the slab-test ray of the tree being walked (object space inside a BLAS)
+    BoxRay wbr;    // the world-space one, restored on return to the TLAS
     TriRay tr;
     const Node* bnodes;
     const float* btris;
@@ -75,19 +76,36 @@ enum StepResult { kStepContinue = 0, kStepHit = 1, kStepDone = 2, kStepOverflow 
 
 template <typename STACK>
 __device__ inline void ray_begin(RayState& r, STACK& st, V3 o, V3 d, float tmin, float tmax) {
-    r.wo = o; r.wd = d; r.tmin = tmin; r.tmax = tmax;
+    r.wd = d; r.tmin = tmin; r.tmax = tmax;
     r.wbr = box_setup(o, d, tmin, tmax);
+    r.br = r.wbr;
     r.next = 0;
     r.steps = 0;
     r.in_blas = false;
     st.sp = 0; st.lo = 0;
 }
 
-__device__ inline void load_node(const Node* base, uint32_t i, float4& qx, float4& qy, float4& qz,
-                                 uint4& qc) {
-    const float4* p = reinterpret_cast<const float4*>(base + i);
-    qx = p[0]; qy = p[1]; qz = p[2];
-    qc = reinterpret_cast<const uint4*>(p)[3];
+typedef float Row4 __attribute__((ext_vector_type(4)));
+typedef const Row4 __attribute__((address_space(1)))* GlobalRow;
+typedef const Row4 __attribute__((address_space(3)))* LdsRow;
+
+// Four 16-byte rows of a node or instance record, from global memory or, when
+// `lds`, from a TLAS staged in LDS.  Naming the address space lets the
+// compiler issue four 16-byte loads instead of generic per-dword loads.
+__device__ inline void load_rows(const void* row0, bool lds, float4& a, float4& b, float4& c, uint4& d) {
+    Row4 r[4];
+    if (lds) {
+        LdsRow p = (LdsRow)row0;
+        r[0] = p[0]; r[1] = p[1]; r[2] = p[2]; r[3] = p[3];
+    } else {
+        GlobalRow p = (GlobalRow)row0;
+        r[0] = p[0]; r[1] = p[1]; r[2] = p[2]; r[3] = p[3];
+    }
+    a = make_float4(r[0].x, r[0].y, r[0].z, r[0].w);
+    b = make_float4(r[1].x, r[1].y, r[1].z, r[1].w);
+    c = make_float4(r[2].x, r[2].y, r[2].z, r[2].w);
+    d = make_uint4(__float_as_uint(r[3].x), __float_as_uint(r[3].y), __float_as_uint(r[3].z),
+                   __float_as_uint(r[3].w));
 }
 
 __device__ inline void load_tri(const float* tris, uint32_t slot, V3& a, V3& b, V3& c) {
@@ -131,7 +149,7 @@ __device__ inline int wide_visit(RayState& r, STACK& st) {
         float hy = oy + (float)byte_of(qc.x, qc.y, k) * sy;
         float hz = oz + (float)byte_of(qc.z, qc.w, k) * sz;
         float tn;
-        if (k < nchild && box_hit(r.obr, lx, hx, ly, hy, lz, hz, tn)) mask |= 1u << k;
+        if (k < nchild && box_hit(r.br, lx, hx, ly, hy, lz, hz, tn)) mask |= 1u << k;
     }
     const uint64_t meta = ((uint64_t)h1.w << 32) | h1.z;
     r.next = kNone;
@@ -147,35 +165,37 @@ __device__ inline int wide_visit(RayState& r, STACK& st) {
             V3 a, b, c;
             load_tri(r.btris, h1.y + m, a, b, c);
             float T, det, U, V, W;
-            if (tri_test(r.tr, r.obr.o, r.tmin, r.tmax, a, b, c, T, det, U, V, W)) return kStepHit;
+            if (tri_test(r.tr, r.br.o, r.tmin, r.tmax, a, b, c, T, det, U, V, W)) return kStepHit;
         }
     }
     return kStepContinue;
 }
 
+// tlas_lds: tlas_nodes and recs point into LDS (a staged TLAS).
 template <bool CLOSEST, bool WIDE = false, typename STACK = Stack>
 __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, const InstRec* recs,
-                               const InstAux* aux, Best* best) {
+                               const InstAux* aux, Best* best, bool tlas_lds = false) {
     if (++r.steps > kMaxSteps) return kStepOverflow;
     if (r.next == kNone) {
         if (st.sp == 0) return kStepDone;
         r.next = st.pop();
         if (r.next == kSentinel) {
             r.in_blas = false;
+            r.br = r.wbr;
             r.next = kNone;
             return kStepContinue;
         }
     }
     if (!r.in_blas && (r.next & kLeaf)) {
         uint32_t slot = r.next & ~kLeaf;
-        const float4* rp = reinterpret_cast<const float4*>(recs + slot);
-        float4 w0 = rp[0], w1 = rp[1], w2 = rp[2];
-        uint4 pp = reinterpret_cast<const uint4*>(rp)[3];
+        float4 w0, w1, w2;
+        uint4 pp;
+        load_rows(recs + slot, tlas_lds, w0, w1, w2, pp);
         float w[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
-        V3 oo = xform_point(w, r.wo);
+        V3 oo = xform_point(w, r.wbr.o);
         V3 od = xform_vec(w, r.wd);
         float thi = CLOSEST ? best->t : r.tmax;
-        r.obr = box_setup(oo, od, r.tmin, thi);
+        r.br = box_setup(oo, od, r.tmin, thi);
         r.tr = tri_setup(od);
         r.bnodes = reinterpret_cast<const Node*>(((uint64_t)pp.y << 32) | pp.x);
         r.btris = reinterpret_cast<const float*>(((uint64_t)pp.w << 32) | pp.z);
@@ -188,11 +208,12 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
     if (WIDE && r.in_blas) return wide_visit(r, st);
     float4 qx, qy, qz;
     uint4 qc;
-    load_node(r.in_blas ? r.bnodes : tlas_nodes, r.next, qx, qy, qz, qc);
-    const BoxRay& br = r.in_blas ? r.obr : r.wbr;
+    load_rows(r.in_blas ? r.bnodes + r.next : tlas_nodes + r.next, tlas_lds && !r.in_blas, qx, qy, qz, qc);
     float tn0, tn1;
-    bool h0 = qc.x != kEmpty && box_hit(br, qx.x, qx.z, qy.x, qy.z, qz.x, qz.z, tn0);
-    bool h1 = qc.y != kEmpty && box_hit(br, qx.y, qx.w, qy.y, qy.w, qz.y, qz.w, tn1);
+    const bool b0 = box_hit(r.br, qx.x, qx.z, qy.x, qy.z, qz.x, qz.z, tn0);
+    const bool b1 = box_hit(r.br, qx.y, qx.w, qy.y, qy.w, qz.y, qz.w, tn1);
+    const bool h0 = (qc.x != kEmpty) & b0;
+    const bool h1 = (qc.y != kEmpty) & b1;
     int first = (h0 && h1) ? (tn1 < tn0 ? 1 : 0) : (h0 ? 0 : 1);
     r.next = kNone;
 #pragma unroll
@@ -206,7 +227,7 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
             V3 a, b, cc;
             load_tri(r.btris, slot, a, b, cc);
             float T, det, U, V, W;
-            if (tri_test(r.tr, r.obr.o, r.tmin, r.tmax, a, b, cc, T, det, U, V, W)) {
+            if (tri_test(r.tr, r.br.o, r.tmin, r.tmax, a, b, cc, T, det, U, V, W)) {
                 if (!CLOSEST) return kStepHit;
                 float t = __fdiv_rn(T, det);
                 if (better_hit(t, r.inst_slot, slot, *best, aux)) {
@@ -216,7 +237,7 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
                     best->tri_slot = slot;
                     best->T = T; best->det = det; best->U = U; best->V = V; best->W = W;
                     float m = t_margin(t);
-                    r.obr.thi = t + m;
+                    r.br.thi = t + m;
                     r.wbr.thi = t + m;
                 }
             }
