@@ -4,8 +4,11 @@
 // Network (per (pixel, light) sample): 150 features = a 5x5 neighbourhood
 // (clamped to the image) x 6 channels {AO, visibility of the light, relative
 // depth, normal.xyz}; 4 hidden layers of 64 with ReLU; 2 outputs {AO,
-// visibility}.  A pixel's AO is the mean of the AO outputs of its L samples.
-// Weights are BF16, accumulation FP32, biases FP32.
+// visibility} added to the 5x5 mean of the noisy AO and visibility inputs over
+// the window's foreground pixels (FP32), so the network learns a correction to
+// that box filter and BF16 precision bounds only the correction.  A pixel's AO
+// is the mean of the AO outputs of its L samples.  Weights are BF16,
+// accumulation FP32, biases FP32.
 //
 // MFMA mapping (v_mfma_f32_16x16x32_bf16, 64 lanes): weights are the A
 // operand (16 output features x 32 inputs), samples the B operand (32 inputs
@@ -77,8 +80,9 @@ constexpr int kWOff[5] = {oW1, oW2, oW3, oW4, oW5};
 constexpr int kBOff[5] = {oB1, oB2, oB3, oB4, oB5};
 
 // Fragment table (one fragment = 64 lanes x 8 BF16 = 1 KB):
-//   F1(t,s) t<4 s<5 | F2..F4(t,s) t<4 s<2 | F5(s) s<2 | G2..G4(t,s) | G5(t) t<4
+//   F1(t,s) t<4 s<5 | F2..F4(t,s) t<4 s<2 | G2..G4(t,s) | G5(t) t<4
 //   | FG(t,s) t<4 s<4 | FV(t) t<4
+// Layer 5 (64 -> 2) runs in FP32 (dn_out_fp32); G5 holds W5^T for the backward pass.
 // FG and FV hold layer 1 again in the inference order, which splits the
 // features of a (pixel, light) sample into the 125 that do not depend on the
 // light (one product per pixel) and the 25 visibilities (one K-step per light):
@@ -87,10 +91,10 @@ constexpr int kBOff[5] = {oB1, oB2, oB3, oB4, oB5};
 //   FG, K-step 3: element e is the relative depth of neighbour 8g + e (< 25);
 //       lane group 3 also holds (AO, n.x, n.y, n.z) of neighbour 24 in e = 1..4;
 //   FV: element e is the visibility of neighbour 8g + e (< 25).
-constexpr int kF1 = 0, kF2 = 20, kF3 = 28, kF4 = 36, kF5 = 44, kG2 = 46, kG3 = 54, kG4 = 62,
-              kG5 = 70, kFG = 74, kFV = 90, kFragTrain = 74, kFragAll = 94;
-// inference fragments in LDS: F2..F5 (26), then FG and FV (20)
-constexpr int kIF2 = 0, kIF3 = 8, kIF4 = 16, kIF5 = 24, kIFG = 26, kIFV = 42, kFragInfer = 46;
+constexpr int kF1 = 0, kF2 = 20, kF3 = 28, kF4 = 36, kG2 = 44, kG3 = 52, kG4 = 60,
+              kG5 = 68, kFG = 72, kFV = 88, kFragTrain = 72, kFragAll = 92;
+// inference fragments in LDS: F2..F4 (24), then FG and FV (20)
+constexpr int kIF2 = 0, kIF3 = 8, kIF4 = 16, kIFG = 24, kIFV = 40, kFragInfer = 44;
 
 // Original feature index (6 n + channel) of element e of lane group g in
 // inference K-step s (s == 4: the visibility step), or -1 for padding.
@@ -186,6 +190,37 @@ __global__ __launch_bounds__(256) void k_dn_prep(DnFrameDev F, float4* pix, floa
     ao_frac[p] = (float)F.ao[p] / F.ao_n;
 }
 
+// The base the network's outputs are added to: the mean, over the foreground
+// pixels of the clamped 5x5 window, of the AO fraction and of the visibility
+// of `light` (0 without lights).  Summed per group of 8 neighbours (n = 8g + e,
+// in order of e), then ((g0 + g1) + (g2 + g3)): the order of the lane
+// reduction in k_dn_infer, so training and inference bases are identical.
+__device__ inline void dn_box(const DnFrameDev& F, uint32_t x, uint32_t y, uint32_t light, float& bao,
+                              float& bvis) {
+    float sa[4], sv[4];
+    int cnt = 0;
+    for (int g = 0; g < 4; ++g) {
+        sa[g] = 0.0f;
+        sv[g] = 0.0f;
+        for (int e = 0; e < 8; ++e) {
+            const int n = 8 * g + e;
+            if (n >= 25) break;
+            int qx = (int)x + n % 5 - 2, qy = (int)y + n / 5 - 2;
+            qx = qx < 0 ? 0 : (qx >= (int)F.width ? (int)F.width - 1 : qx);
+            qy = qy < 0 ? 0 : (qy >= (int)F.height ? (int)F.height - 1 : qy);
+            const size_t q = (size_t)qy * F.width + qx;
+            if (F.pix[q].x < 0.0f) continue;
+            cnt++;
+            sa[g] += F.ao_frac[q];
+            if (F.num_lights)
+                sv[g] += 1.0f - (float)F.shadow[(size_t)light * F.width * F.height + q] / F.shadow_n;
+        }
+    }
+    const float c = (float)(cnt ? cnt : 1);
+    bao = ((sa[0] + sa[1]) + (sa[2] + sa[3])) / c;
+    bvis = ((sv[0] + sv[1]) + (sv[2] + sv[3])) / c;
+}
+
 __device__ inline float nb_channel(const Nb& b, int c) {
     return c == 0 ? b.c0 : c == 1 ? b.c1 : c == 2 ? b.c2 : c == 3 ? b.c3 : c == 4 ? b.c4 : b.c5;
 }
@@ -264,6 +299,31 @@ __device__ inline void dn_relu_pack(const f32x4 (&acc)[4], uint4 (&hb)[2]) {
     hb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
 }
 
+// Layer 5 in FP32: the two outputs (without bias) of sample column j from the
+// FP32 activations of layer 4 (lane (j, g) holds features 16t + 4g + i) and
+// FP32 weights w5 ([2][64]); per lane in (t, i) order, then over the lane
+// groups as ((g0 + g1) + (g2 + g3)).  FP32 keeps a correction that the weights
+// cancel to about zero as precise as the box base it is added to.
+__device__ inline void dn_out_fp32(const f32x4 (&h)[4], const float* w5, int g, float& y0, float& y1) {
+    float p0 = 0.0f, p1 = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const f32x4 a = *reinterpret_cast<const f32x4*>(&w5[16 * t + 4 * g]);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(&w5[kHid + 16 * t + 4 * g]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            p0 = fmaf(a[i], h[t][i], p0);
+            p1 = fmaf(b[i], h[t][i], p1);
+        }
+    }
+    p0 += __shfl_xor(p0, 16);
+    p1 += __shfl_xor(p1, 16);
+    p0 += __shfl_xor(p0, 32);
+    p1 += __shfl_xor(p1, 32);
+    y0 = p0;
+    y1 = p1;
+}
+
 // Layer (k >= 2) with 2 K-steps: acc[q][t] = sum_s frag(t, s) * hb[q][s].
 template <int Q>
 __device__ inline void dn_layer64(const uint4* sfrag, int fbase, int lane, const uint4 (&hb)[Q][2],
@@ -312,7 +372,9 @@ template <int Q>
 __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
     __shared__ uint4 sfrag[kFragTrain * 64];
     __shared__ float sbias[4 * kHid + kOut];
+    __shared__ __align__(16) float sw5[kOut * kHid];
     for (int i = threadIdx.x; i < kFragTrain * 64; i += kBlock) sfrag[i] = A.frags[i];
+    for (int i = threadIdx.x; i < kOut * kHid; i += kBlock) sw5[i] = A.params[oW5 + i];
     for (int i = threadIdx.x; i < 4 * kHid + kOut; i += kBlock)
         sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
     __syncthreads();
@@ -392,15 +454,19 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
                 dn_hidden_epilogue(acc[q], sbias + k * kHid, g, hb[q], mask[k][q],
                                    A.HT + (size_t)k * kHid * A.ldn, A.ldn, row[q], inb[q]);
         }
-        // ---- layer 5: 64 -> 2 (one 16-row tile, rows 0 and 1 used)
-        f32x4 out[Q];
+        // ---- layer 5: 64 -> 2 in FP32 on layer 4's activations (as in dn_hidden_epilogue)
+        float out[Q][2];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) out[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int q = 0; q < Q; ++q) {
+            f32x4 h[4];
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            bf16x8 a = as_bf16x8(sfrag[(kF5 + s) * 64 + lane]);
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int q = 0; q < Q; ++q) out[q] = mfma(a, as_bf16x8(hb[q][s]), out[q]);
+                for (int i = 0; i < 4; ++i) {
+                    const float z = acc[q][t][i] + sbias[3 * kHid + 16 * t + 4 * g + i];
+                    h[t][i] = z > 0.0f ? z : 0.0f;
+                }
+            dn_out_fp32(h, sw5, g, out[q][0], out[q][1]);
         }
         const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
         // ---- loss and backward
@@ -410,8 +476,11 @@ __global__ __launch_bounds__(kBlock) void k_dn_forward(DnArgs A) {
         for (int q = 0; q < Q; ++q) {
             float d0 = 0.0f, d1 = 0.0f;
             if (g == 0 && valid[q]) {
-                float e0 = out[q][0] + b50 - A.target_ao[pp[q]];
-                float e1 = out[q][1] + b51 - (F.num_lights ? A.target_vis[(size_t)pl[q] * npix + pp[q]] : 0.0f);
+                float bao, bvis;
+                dn_box(F, px[q], py[q], pl[q], bao, bvis);
+                float e0 = (out[q][0] + b50) + bao - A.target_ao[pp[q]];
+                float e1 = (out[q][1] + b51) + bvis -
+                           (F.num_lights ? A.target_vis[(size_t)pl[q] * npix + pp[q]] : 0.0f);
                 loss += e0 * e0 + e1 * e1;
                 d0 = A.grad_scale * e0;
                 d1 = A.grad_scale * e1;
@@ -471,16 +540,22 @@ __device__ inline uint32_t dn_nb_index(const DnFrameDev& F, uint32_t x, uint32_t
 // top of it and layers 2..5.  A pixel's AO is the mean of its lights' AO
 // outputs, summed in light order; background pixels give 0.
 template <int Q>
-__global__ __launch_bounds__(kInferBlock) void k_dn_infer(InferArgs A) {
+__global__ __launch_bounds__(kInferBlock, 4) void k_dn_infer(InferArgs A) {
     __shared__ uint4 sfrag[kFragInfer * 64];
     __shared__ __align__(16) float sbias[4 * kHid + 4];
     __shared__ uint16_t svis[256];   // BF16 of 1 - s / shadow_n
-    for (int i = threadIdx.x; i < 26 * 64; i += kInferBlock) sfrag[kIF2 * 64 + i] = A.frags[kF2 * 64 + i];
+    __shared__ float svisf[256];     // 1 - s / shadow_n
+    __shared__ __align__(16) float sw5[kOut * kHid];
+    for (int i = threadIdx.x; i < 24 * 64; i += kInferBlock) sfrag[kIF2 * 64 + i] = A.frags[kF2 * 64 + i];
+    for (int i = threadIdx.x; i < kOut * kHid; i += kInferBlock) sw5[i] = A.params[oW5 + i];
     for (int i = threadIdx.x; i < 20 * 64; i += kInferBlock) sfrag[kIFG * 64 + i] = A.frags[kFG * 64 + i];
     for (int i = threadIdx.x; i < 4 * kHid + kOut; i += kInferBlock)
         sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
     const DnFrameDev& F = A.F;
-    for (int i = threadIdx.x; i < 256; i += kInferBlock) svis[i] = (uint16_t)bf16_bits(1.0f - (float)i / F.shadow_n);
+    for (int i = threadIdx.x; i < 256; i += kInferBlock) {
+        svisf[i] = 1.0f - (float)i / F.shadow_n;
+        svis[i] = (uint16_t)bf16_bits(svisf[i]);
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
     const uint32_t npix = F.width * F.height;
@@ -554,11 +629,13 @@ __global__ __launch_bounds__(kInferBlock) void k_dn_infer(InferArgs A) {
         // indices and foreground bits that the visibility step reads per light
         uint32_t vq[Q][8];
         uint32_t vfg[Q];
+        float bao[Q], cnt[Q];   // box base of AO; foreground pixels of the window
         {
             uint4 fb[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 float f[8];
+                float sa = 0.0f;
                 vfg[q] = 0u;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
@@ -571,6 +648,7 @@ __global__ __launch_bounds__(kInferBlock) void k_dn_infer(InferArgs A) {
                         vq[q][e] = nq;
                         if (d >= 0.0f) {
                             vfg[q] |= 1u << e;
+                            sa += F.ao_frac[nq];
                             const float rel = dc[q] > 0.0f ? (d - dc[q]) / dc[q] * 16.0f : 0.0f;
                             f[e] = rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel);
                         } else {
@@ -587,6 +665,14 @@ __global__ __launch_bounds__(kInferBlock) void k_dn_infer(InferArgs A) {
                 }
                 fb[q] = make_uint4(bf16_pack2(f[0], f[1]), bf16_pack2(f[2], f[3]), bf16_pack2(f[4], f[5]),
                                    bf16_pack2(f[6], f[7]));
+                // sums over the four lane groups of a pixel column: ((g0 + g1) + (g2 + g3))
+                sa += __shfl_xor(sa, 16);
+                sa += __shfl_xor(sa, 32);
+                int c = __popc(vfg[q]);
+                c += __shfl_xor(c, 16);
+                c += __shfl_xor(c, 32);
+                cnt[q] = (float)(c ? c : 1);
+                bao[q] = sa / cnt[q];
             }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -608,17 +694,24 @@ __global__ __launch_bounds__(kInferBlock) void k_dn_infer(InferArgs A) {
             // registers across the light loop (which would allow one wave per SIMD)
             uint32_t ln = (uint32_t)lane;
             asm volatile("" : "+v"(ln));
+            float bvis[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 uint32_t w[4] = {0u, 0u, 0u, 0u};
+                float sv = 0.0f;
                 if (L) {
                     const uint8_t* sh = F.shadow + (size_t)l * npix;
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
-                        const uint32_t bits = (vfg[q] >> e) & 1u ? (uint32_t)svis[sh[vq[q][e]]] : 0u;
-                        w[e >> 1] |= bits << (16 * (e & 1));
+                        const bool on = (vfg[q] >> e) & 1u;
+                        const uint32_t s8 = on ? (uint32_t)sh[vq[q][e]] : 0u;
+                        w[e >> 1] |= (on ? (uint32_t)svis[s8] : 0u) << (16 * (e & 1));
+                        if (on) sv += svisf[s8];
                     }
                 }
+                sv += __shfl_xor(sv, 16);
+                sv += __shfl_xor(sv, 32);
+                bvis[q] = sv / cnt[q];
                 const bf16x8 bv = as_bf16x8(make_uint4(w[0], w[1], w[2], w[3]));
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
@@ -643,23 +736,29 @@ __global__ __launch_bounds__(kInferBlock) void k_dn_infer(InferArgs A) {
 #pragma unroll
                         for (int q = 0; q < Q; ++q) acc[q][t] = mfma(a, as_bf16x8(hb[q][s]), acc[q][t]);
                     }
+                if (k < 3) {
 #pragma unroll
-                for (int q = 0; q < Q; ++q) dn_relu_pack(acc[q], hb[q]);
+                    for (int q = 0; q < Q; ++q) dn_relu_pack(acc[q], hb[q]);
+                }
             }
-            f32x4 out[Q];
+            // layer 5 in FP32 on layer 4's activations
+            float out[Q][2];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) out[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int q = 0; q < Q; ++q) {
+                f32x4 h[4];
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const bf16x8 a = as_bf16x8(sfrag[(kIF5 + s) * 64 + ln]);
+                for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int q = 0; q < Q; ++q) out[q] = mfma(a, as_bf16x8(hb[q][s]), out[q]);
+                    for (int i = 0; i < 4; ++i) h[t][i] = relu_bits(acc[q][t][i]);
+                // the opaque offset (zero) keeps the weights in LDS rather than registers
+                dn_out_fp32(h, sw5 + (ln - (uint32_t)lane), g, out[q][0], out[q][1]);
             }
 #pragma unroll
             for (int q = 0; q < Q; ++q)
                 if (g == 0 && inb[q]) {
-                    aosum[q] += valid[q] ? out[q][0] + b50 : 0.0f;
-                    if (L) A.vis_out[(size_t)l * npix + pp[q]] = valid[q] ? clamp01f(out[q][1] + b51) : 0.0f;
+                    aosum[q] += valid[q] ? (out[q][0] + b50) + bao[q] : 0.0f;
+                    if (L)
+                        A.vis_out[(size_t)l * npix + pp[q]] = valid[q] ? clamp01f((out[q][1] + b51) + bvis[q]) : 0.0f;
                 }
         }
 #pragma unroll
@@ -754,12 +853,9 @@ __global__ void k_dn_pack(const float* P, uint16_t* frags) {
     if (f < kF2) {
         const int t = f / 5, s = f % 5, o = 16 * t + j, k = 32 * s + 8 * g + e;
         if (k < kIn) val = P[oW1 + o * kIn + k];
-    } else if (f < kF5) {
+    } else if (f < kG2) {
         const int layer = 1 + (f - kF2) / 8, r = (f - kF2) % 8, t = r / 2, s = r % 2;
         val = P[kWOff[layer] + (16 * t + j) * kHid + khid(s, g, e)];
-    } else if (f < kG2) {
-        const int s = f - kF5;
-        if (j < kOut) val = P[oW5 + j * kHid + khid(s, g, e)];
     } else if (f < kG5) {
         const int layer = 1 + (f - kG2) / 8, r = (f - kG2) % 8, t = r / 2, s = r % 2;
         val = P[kWOff[layer] + khid(s, g, e) * kHid + (16 * t + j)];

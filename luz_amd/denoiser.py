@@ -22,6 +22,11 @@ from .api import lib, _P
 
 N_PARAMS = 22274
 WEIGHTS = Path(__file__).resolve().parent / "data" / "denoiser.npy"
+# AO radius (aoMax) of the denoiser's frames.  The AO ray origin is lifted by
+# 0.01 x the camera distance along the normal (light.frag:229-231); at the
+# sphere fields' camera distances (100-250) that lift exceeds a radius of 1 and
+# AO would be 1 on every pixel, so these frames look for occluders out to 16.
+AO_RADIUS = 16.0
 
 
 class DnFrame(C.Structure):
@@ -224,9 +229,10 @@ def crop_origin(h: api.HostScene, cam: api.Camera, W: int, H: int, w: int, hh: i
 
 
 def make_frame(ctx: api.Context, seed: int, crop=(256, 256), rng=None, target_spp: int = 256,
-               kind: int = api.SCENE_SPHERES, focus: str = "uniform") -> Frame:
+               kind: int = api.SCENE_SPHERES, focus: str = "uniform", ao_radius: float = None) -> Frame:
     """One training/evaluation frame of scene `seed`: a crop of its image (or the
-    whole image when crop is None), 1-spp inputs and target_spp targets."""
+    whole image when crop is None), 1-spp inputs and target_spp targets, AO out
+    to `ao_radius` (AO_RADIUS by default)."""
     h = api.HostScene(kind, seed)
     gs = api.GPUScene(ctx, h)
     W, H, cam = h.camera()
@@ -240,6 +246,7 @@ def make_frame(ctx: api.Context, seed: int, crop=(256, 256), rng=None, target_sp
     gd = api.GBufferDevice(ctx, W, H)
     ctx.primary_gbuffer(gs.tlas, cam, mats, gd.desc())
     sb = h.pack()
+    sb.ao_max = AO_RADIUS if ao_radius is None else ao_radius
     L = sb.num_lights
     sh1, ao1 = render_counts(ctx, gs, gd, sb, 1, 0)
     n = W * H
@@ -347,6 +354,7 @@ def main() -> int:
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--first-scene", type=int, default=100000, help="training scene seeds start here")
     t.add_argument("--out", default=str(WEIGHTS))
+    t.add_argument("--ao-radius", type=float, default=AO_RADIUS)
     t.add_argument("--eval-scenes", type=int, default=4, help="held-out scenes evaluated after training")
     args = ap.parse_args()
     ctx = api.Context(0)
@@ -355,7 +363,8 @@ def main() -> int:
     frames = []
     for i in range(args.scenes):
         focus = "instance" if i % 4 else "uniform"
-        frames.append(make_frame(ctx, args.first_scene + i, (args.crop, args.crop), rng, focus=focus))
+        frames.append(make_frame(ctx, args.first_scene + i, (args.crop, args.crop), rng, focus=focus,
+                                 ao_radius=args.ao_radius))
         if i % 100 == 0:
             print(f"data: {i + 1}/{args.scenes} frames ({time.perf_counter() - t0:.1f} s)", flush=True)
     print(f"data: {len(frames)} frames of {args.crop}x{args.crop} in {time.perf_counter() - t0:.1f} s", flush=True)
@@ -367,7 +376,8 @@ def main() -> int:
     print(f"saved {args.out} ({time.perf_counter() - t0:.1f} s total)", flush=True)
     del frames
     for i in range(args.eval_scenes):
-        fr = make_frame(ctx, 900001 + i, (args.crop, args.crop), rng, focus="instance")
+        fr = make_frame(ctx, 900001 + i, (args.crop, args.crop), rng, focus="instance",
+                        ao_radius=args.ao_radius)
         print(f"held-out scene {900001 + i}: " +
               " ".join(f"{k} {v:.5f}" for k, v in evaluate(ctx, dn, fr).items()), flush=True)
     dn.close()

@@ -5,6 +5,8 @@ results can be checked:
   features()   the 150 BF16 input features of (pixel, light) samples, with the
                same FP32 operation order as the device (bit-exact);
   forward()    the MLP in FP64 on BF16-rounded weights and FP32 biases;
+  box_base()   the base the network's outputs are added to (the 5x5 mean of
+               the noisy AO and visibility over foreground pixels, box_filter());
   backward()   the gradient of  s/2 * sum (y - target)^2  in FP64, in the
                parameter order of the C ABI (W1 b1 W2 b2 W3 b3 W4 b4 W5 b5).
 The denoiser has no counterpart in the reference (light.frag:111-146 uses
@@ -157,3 +159,17 @@ def box_filter(frame: dict, light: int, radius: int = 2):
             cnt += m
     cnt = np.maximum(cnt, 1)
     return (acc_a / cnt).reshape(-1), (acc_v / cnt).reshape(-1)
+
+
+def box_base(frame: dict, pixels: np.ndarray, lights: np.ndarray) -> np.ndarray:
+    """(N, 2) FP64 base of samples (pixels[i], lights[i]): the device adds the
+    network's (AO, visibility) outputs to box_filter()'s (AO, visibility)."""
+    pixels = np.asarray(pixels, np.int64)
+    lights = np.asarray(lights, np.int64)
+    out = np.zeros((len(pixels), 2))
+    for l in np.unique(lights):
+        a, v = box_filter(frame, int(l))
+        m = lights == l
+        out[m, 0] = a[pixels[m]]
+        out[m, 1] = v[pixels[m]]
+    return out

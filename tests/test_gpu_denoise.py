@@ -1,7 +1,8 @@
 """GPU parity of the learned denoiser: BF16 input features bit-exact with the
 oracle, MFMA forward and weight gradients against the FP64 oracle, gradients
 deterministic, training on tracer-generated pairs reduces the error, and the
-trained network beats the raw 1-spp estimate and a non-learned box filter."""
+trained network beats the raw 1-spp estimate (and, for the soft shadows, the
+non-learned 5x5 box filter it adds its correction to)."""
 import numpy as np
 import pytest
 
@@ -56,7 +57,9 @@ def test_forward_matches_fp64_oracle(ctx, frame):
     ao = ao_o.download(np.float32, n).astype(np.float64)
     vis = vis_o.download(np.float32, (L, n)).astype(np.float64)
     px, li = _all_samples(frame)
-    y, _, _ = odn.forward(odn.device_weights(dn.params), odn.features(frame.host(), px, li))
+    host = frame.host()
+    y, _, _ = odn.forward(odn.device_weights(dn.params), odn.features(host, px, li))
+    y = y + odn.box_base(host, px, li)
     want_vis = np.clip(y[:, 1], 0, 1)
     want_ao = np.clip(y[:, 0].reshape(-1, L).mean(1), 0, 1)
     fg = frame.foreground
@@ -88,8 +91,11 @@ def test_gradient_matches_fp64_oracle_and_is_deterministic(ctx, frame):
         grads.append(g)
     np.testing.assert_array_equal(grads[0].view(np.uint32), grads[1].view(np.uint32))
     w = odn.device_weights(dn.params)
-    X = odn.features(frame.host(), px, li)
+    host = frame.host()
+    X = odn.features(host, px, li)
+    # the network's outputs are added to the box base: its targets are the residuals
     T = np.stack([frame.target_ao_host[px], frame.target_vis_host[li, px]], 1).astype(np.float64)
+    T = T - odn.box_base(host, px, li)
     want = odn.backward(w, X, T, scale=1.0 / N)
     y, _, _ = odn.forward(w, X)
     want_sse = float(np.sum((y - T) ** 2))
@@ -118,10 +124,10 @@ def test_training_on_tracer_pairs_reduces_error(ctx):
     after = denoise_rmse(ctx, dn, held)
     print(f"held-out before {before}\nafter {after}", flush=True)
     assert hist[-1][1] < 0.5 * hist[0][1]
-    # soft shadows carry the 1-spp noise; AO at this camera distance is 1 everywhere (the AO
-    # origin offset 0.01 x distance reaches the AO radius), so its error only has to stay small
     assert after["vis"]["mlp"] < after["vis"]["raw"] and after["vis"]["mlp"] < 0.6 * before["vis"]["mlp"]
-    assert after["ao"]["mlp"] < 0.05
+    assert after["ao"]["mlp"] < before["ao"]["mlp"]
+    # a crop without occluders in AO range has an exact 1-spp AO (raw error 0)
+    assert after["ao"]["mlp"] < (after["ao"]["raw"] if after["ao"]["raw"] > 0.005 else 0.05)
     dn.close()
 
 
@@ -134,7 +140,9 @@ def test_trained_network_beats_unlearned_estimates(ctx):
         r = denoise_rmse(ctx, dn, fr)
         print(f"scene {seed}: {r}", flush=True)
         assert r["vis"]["mlp"] < r["vis"]["box"] and r["vis"]["mlp"] < r["vis"]["raw"]
-        assert r["ao"]["mlp"] < 0.02      # AO is 1 on every pixel of these scenes
+        # AO: below the raw estimate and within the 0.01 target; the box filter it corrects can
+        # be closer on a crop with little occlusion (on the whole 1080p frame the network wins)
+        assert r["ao"]["mlp"] < r["ao"]["raw"] and r["ao"]["mlp"] < 0.01
     dn.close()
 
 
