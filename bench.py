@@ -440,6 +440,13 @@ def bench_denoise(ctx, args) -> int:
     flops_px = 2.0 * 125 * 64 + max(L, 1) * 2.0 * (25 * 64 + 3 * 64 * 64 + 64 * 2)
     flops = float(fg.sum()) * flops_px
     achieved = flops / (td * 1e-3) / 1e12
+    dn_traffic = None
+    tpath = ROOT / "profiles" / "k_dn_infer_traffic.json"
+    if tpath.exists():
+        try:
+            dn_traffic = json.loads(tpath.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            dn_traffic = None
     record = {
         "metric": "AO RMSE vs FP64 256-spp oracle and ms/frame: 1-spp + learned denoiser vs plain multi-spp",
         "value": den["frame_ms"], "unit": "ms/frame", "n_gpus": 1, "steps": reps, "warmup": 1,
@@ -460,7 +467,7 @@ def bench_denoise(ctx, args) -> int:
                        round(match_vis["frame_ms"] / den["frame_ms"], 2) if match_vis else None,
                    "speedup_vs_64spp": round(spp64["frame_ms"] / den["frame_ms"], 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(achieved / 2500.0, 4), "traffic": None, "kernel": "k_dn_infer",
+                     "frac": round(achieved / 2500.0, 4), "traffic": dn_traffic, "kernel": "k_dn_infer",
                      "flops_per_pixel": flops_px, "timed": "denoise stage: per-pixel prepass + k_dn_infer"},
     }
     if not args.no_cpu:
