@@ -89,16 +89,21 @@ typedef float Row4 __attribute__((ext_vector_type(4)));
 typedef const Row4 __attribute__((address_space(1)))* GlobalRow;
 typedef const Row4 __attribute__((address_space(3)))* LdsRow;
 
-// Four 16-byte rows of a node or instance record, from global memory or, when
-// `lds`, from a TLAS staged in LDS.  Naming the address space lets the
-// compiler issue four 16-byte loads instead of generic per-dword loads.
-__device__ inline void load_rows(const void* row0, bool lds, float4& a, float4& b, float4& c, uint4& d) {
+// Four 16-byte rows of entry i (a node or an instance record, 64 bytes) of an
+// array in global memory or, when `lds`, of a TLAS staged in LDS.  Naming the
+// address space lets the compiler issue four 16-byte loads instead of generic
+// per-dword loads; passing base and index (not a selected pointer) keeps the
+// LDS base an LDS address.
+template <typename T>
+__device__ inline void load_rows(const T* base, uint32_t i, bool lds, float4& a, float4& b, float4& c,
+                                 uint4& d) {
+    static_assert(sizeof(T) == 64, "64-byte entries");
     Row4 r[4];
     if (lds) {
-        LdsRow p = (LdsRow)row0;
+        LdsRow p = (LdsRow)(base + i);
         r[0] = p[0]; r[1] = p[1]; r[2] = p[2]; r[3] = p[3];
     } else {
-        GlobalRow p = (GlobalRow)row0;
+        GlobalRow p = (GlobalRow)(base + i);
         r[0] = p[0]; r[1] = p[1]; r[2] = p[2]; r[3] = p[3];
     }
     a = make_float4(r[0].x, r[0].y, r[0].z, r[0].w);
@@ -190,7 +195,7 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
         uint32_t slot = r.next & ~kLeaf;
         float4 w0, w1, w2;
         uint4 pp;
-        load_rows(recs + slot, tlas_lds, w0, w1, w2, pp);
+        load_rows(recs, slot, tlas_lds, w0, w1, w2, pp);
         float w[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
         V3 oo = xform_point(w, r.wbr.o);
         V3 od = xform_vec(w, r.wd);
@@ -208,7 +213,7 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
     if (WIDE && r.in_blas) return wide_visit(r, st);
     float4 qx, qy, qz;
     uint4 qc;
-    load_rows(r.in_blas ? r.bnodes + r.next : tlas_nodes + r.next, tlas_lds && !r.in_blas, qx, qy, qz, qc);
+    load_rows(r.in_blas ? r.bnodes : tlas_nodes, r.next, tlas_lds && !r.in_blas, qx, qy, qz, qc);
     float tn0, tn1;
     const bool b0 = box_hit(r.br, qx.x, qx.z, qy.x, qy.z, qz.x, qz.z, tn0);
     const bool b1 = box_hit(r.br, qx.y, qx.w, qy.y, qy.w, qz.y, qz.w, tn1);
