@@ -88,6 +88,9 @@ namespace luz {
 luzrt_status fail(luzrt_ctx* ctx, luzrt_status s, const char* fmt, ...);
 luzrt_status hip_fail(luzrt_ctx* ctx, hipError_t e, const char* what);
 luzrt_status ensure(luzrt_ctx* ctx, DevBuf& b, size_t bytes);
+// Zeroes `bytes` bytes at `p` on the context's stream with a kernel rather
+// than a memset node (calls on this path may be captured into a graph).
+luzrt_status zero_fill(luzrt_ctx* ctx, void* p, size_t bytes);
 void stage_begin(luzrt_ctx* ctx, Stage s);
 void stage_end(luzrt_ctx* ctx, Stage s);
 
@@ -181,6 +184,7 @@ struct DevScene {
 
 // Device counters of one visibility call (ctx->counters).
 enum Counter { kCntFetch = 0, kCntActive = 1, kCntError = 2, kCntRays = 3, kCntBuildError = 4,
+               kCntPrimFetch = 5,
                kCntCount = 8 };
 
 struct TraceLaunch {

@@ -6,7 +6,7 @@
 //   shadow_setup / raygen_pixel               light.frag:86-146, 111-135, 229-231
 //   resolve_pixel                             occluded / unoccluded sample counts
 //   shade_pixel                               light.frag:17-49, 171-235 (FP32)
-//   primary_pixel                             G-buffer texel by a closest-hit primary ray
+//   primary_pixel / primary_persistent        G-buffer texels by closest-hit primary rays
 #pragma once
 #include "internal.hpp"
 #include "traverse.hpp"
@@ -279,36 +279,30 @@ __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds,
 // position, normalised interpolated vertex normal through the instance's
 // inverse-transpose, and the material of the hit instance (background: all
 // zero).  The stack must be empty; it is empty again on return.
-template <typename STACK>
-__device__ inline void primary_pixel(const Node* tlas_nodes, const InstRec* recs, const InstAux* aux,
-                                     const luzrt_camera& cam, const luzrt_material* mats,
-                                     uint32_t nmats, const luzrt_gbuffer_out& out, uint32_t x,
-                                     uint32_t y, STACK& st, uint32_t* error_flag, bool empty) {
-    const uint32_t W = out.width, H = out.height;
-    const V3 eye = v3(cam.eye[0], cam.eye[1], cam.eye[2]);
-    Best best;
-    best.found = false;
-    best.t = cam.tmax;
+// Primary ray of pixel (x, y) through its centre (opaque.vert's camera,
+// y flipped as the Vulkan projection does).
+__device__ inline V3 primary_dir(const luzrt_camera& cam, uint32_t x, uint32_t y, uint32_t W, uint32_t H) {
     float nx = ((float)x + 0.5f) / (float)W * 2.0f - 1.0f;
     float ny = ((float)y + 0.5f) / (float)H * 2.0f - 1.0f;
-    V3 d = v3(cam.forward[0] + nx * cam.right[0] + ny * cam.up[0],
+    return v3(cam.forward[0] + nx * cam.right[0] + ny * cam.up[0],
               cam.forward[1] + nx * cam.right[1] + ny * cam.up[1],
               cam.forward[2] + nx * cam.right[2] + ny * cam.up[2]);
-    if (!empty) {
-        RayState r;
-        ray_begin(r, st, eye, d, cam.tmin, cam.tmax);
-        for (;;) {
-            int res = ray_step<true>(r, st, tlas_nodes, recs, aux, &best);
-            if (res == kStepDone) break;
-            if (res == kStepOverflow) { atomicOr(error_flag, 1u); st.sp = st.lo = 0; break; }
-        }
-    }
-    size_t p = (size_t)y * W + x;
+}
+
+// G-buffer texel (x, y) from the closest hit `best` of the primary ray `d`
+// (opaque.frag:21-58): world position, normalised interpolated vertex normal
+// through the instance's inverse-transpose, the instance's material; zeros on
+// a miss (background).
+__device__ inline void primary_texel(const luzrt_camera& cam, V3 d, const Best& best, const InstAux* aux,
+                                     const luzrt_material* mats, uint32_t nmats,
+                                     const luzrt_gbuffer_out& out, uint32_t x, uint32_t y) {
+    const size_t p = (size_t)y * out.width + x;
     if (!best.found) {
         for (int k = 0; k < 3; ++k) { out.position[p * 3 + k] = 0.0f; out.normal[p * 3 + k] = 0.0f; }
         out.albedo[p] = 0; out.material[p] = 0; out.emission[p] = 0;
         return;
     }
+    const V3 eye = v3(cam.eye[0], cam.eye[1], cam.eye[2]);
     const InstAux& ax = aux[best.inst_slot];
     V3 P = vadd(eye, vmul(d, best.t));
     float bA = __fdiv_rn(best.U, best.det), bB = __fdiv_rn(best.V, best.det),
@@ -333,6 +327,119 @@ __device__ inline void primary_pixel(const Node* tlas_nodes, const InstRec* recs
         out.albedo[p] = 0xFFFFFFFFu;
         out.material[p] = 0xFFFF0080u;  // roughness 0.5, metallic 0, occlusion 1
         out.emission[p] = 0xFF000000u;
+    }
+}
+
+// G-buffer texel (x, y) by a closest-hit primary ray run to completion.
+template <typename STACK>
+__device__ inline void primary_pixel(const Node* tlas_nodes, const InstRec* recs, const InstAux* aux,
+                                     const luzrt_camera& cam, const luzrt_material* mats,
+                                     uint32_t nmats, const luzrt_gbuffer_out& out, uint32_t x,
+                                     uint32_t y, STACK& st, uint32_t* error_flag, bool empty) {
+    Best best;
+    best.found = false;
+    best.t = cam.tmax;
+    const V3 d = primary_dir(cam, x, y, out.width, out.height);
+    if (!empty) {
+        RayState r;
+        ray_begin(r, st, v3(cam.eye[0], cam.eye[1], cam.eye[2]), d, cam.tmin, cam.tmax);
+        for (;;) {
+            int res = ray_step<true>(r, st, tlas_nodes, recs, aux, &best);
+            if (res == kStepDone) break;
+            if (res == kStepOverflow) { atomicOr(error_flag, 1u); st.sp = st.lo = 0; break; }
+        }
+    }
+    primary_texel(cam, d, best, aux, mats, nmats, out, x, y);
+}
+
+// One primary-ray camera and G-buffer: the TLAS, materials and output of a scene.
+struct PrimaryView {
+    const Node* tlas_nodes;   // null: empty TLAS, every pixel is background
+    const InstRec* recs;
+    const InstAux* aux;
+    const luzrt_material* mats;
+    uint32_t nmats;
+    luzrt_camera cam;
+    luzrt_gbuffer_out out;
+};
+
+constexpr uint32_t kPrimaryChunk = 256;   // tile-ordered pixels per claim (4 tiles of 8x8)
+
+// Persistent closest-hit primary rays.  Lanes take pixels of the 8x8-tile
+// order from a claim counter (`fetch`, zero at launch) in chunks of
+// kPrimaryChunk per wave; a wave refills its idle lanes when at least `refill`
+// are idle, so a lane that finishes a short ray starts the next pixel instead
+// of waiting for the slowest lane of its wave.  VIEWS maps a pixel index of
+// [0, npx) to its view and the pixel's index within that view's tile order
+// (find) and gives a view (operator[]).  Used for batches, whose scenes' rays
+// differ in cost; one camera's coherent rays run to completion in primary_pixel.
+template <typename VIEWS, typename STACK>
+__device__ inline void primary_persistent(const VIEWS& views, uint32_t npx, uint32_t* fetch, uint32_t refill,
+                                          STACK& st, uint32_t* error_flag) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    RayState r;
+    Best best;
+    uint32_t px = 0, py = 0, vi = 0;
+    bool busy = false, open = true;
+    uint32_t chunk_next = 0, chunk_end = 0;   // wave-uniform
+    for (;;) {
+        if (open) {
+            const uint64_t idle = __ballot(!busy);
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            if (nidle >= refill || nidle == 64u) {
+                if (chunk_next >= chunk_end) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(fetch, kPrimaryChunk);
+                    base = __shfl(base, 0);
+                    chunk_next = base;
+                    chunk_end = base < npx ? (npx - base > kPrimaryChunk ? base + kPrimaryChunk : npx) : base;
+                    if (base >= npx) open = false;
+                }
+                const uint32_t avail = chunk_end > chunk_next ? chunk_end - chunk_next : 0u;
+                const uint32_t take = nidle < avail ? nidle : avail;
+                if (!busy) {
+                    const uint32_t rank = (uint32_t)__popcll(idle & lt);
+                    if (rank < take) {
+                        uint32_t local = 0;
+                        vi = views.find(chunk_next + rank, local);
+                        const PrimaryView& v = views[vi];
+                        const uint32_t W = v.out.width, H = v.out.height, tiles_x = (W + 7) / 8;
+                        const uint32_t tile = local >> 6, t = local & 63u;
+                        px = (tile % tiles_x) * 8 + (t & 7u);
+                        py = (tile / tiles_x) * 8 + (t >> 3);
+                        if (px < W && py < H) {
+                            best.found = false;
+                            best.t = v.cam.tmax;
+                            ray_begin(r, st, v3(v.cam.eye[0], v.cam.eye[1], v.cam.eye[2]),
+                                      primary_dir(v.cam, px, py, W, H), v.cam.tmin, v.cam.tmax);
+                            // an empty TLAS: no root to visit, the first step ends the ray
+                            if (!v.tlas_nodes) r.next = kNone;
+                            busy = true;
+                        }
+                    }
+                }
+                chunk_next += take;
+            }
+        }
+        if (__ballot(busy) == 0ull) {
+            if (!open) break;
+            continue;
+        }
+        if (busy) {
+            const PrimaryView& v = views[vi];
+            for (;;) {
+                const int res = ray_step<true>(r, st, v.tlas_nodes, v.recs, v.aux, &best);
+                if (res == kStepDone || res == kStepOverflow) {
+                    if (res == kStepOverflow) { atomicOr(error_flag, 1u); st.sp = st.lo = 0; }
+                    primary_texel(v.cam, primary_dir(v.cam, px, py, v.out.width, v.out.height), best, v.aux,
+                                  v.mats, v.nmats, v.out, px, py);
+                    busy = false;
+                    break;
+                }
+                if (open && 64u - (uint32_t)__popcll(__ballot(true)) >= refill) break;
+            }
+        }
     }
 }
 

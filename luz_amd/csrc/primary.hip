@@ -15,6 +15,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Clears the error flag of the call with a kernel (a caller may capture the
+// call in a graph, where memset nodes are avoided).
+__global__ void k_primary_init(uint32_t* error_flag) {
+    if (threadIdx.x == 0) *error_flag = 0u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_primary(const Node* tlas_nodes, const InstRec* recs,
                                                     const InstAux* aux, luzrt_camera cam,
                                                     const luzrt_material* mats, uint32_t nmats,
@@ -71,8 +77,8 @@ extern "C" luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* 
     if (groups == 0) groups = 1;
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * 256 * kSpill * 4)) != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
-    LUZ_HIP(ctx, hipMemsetAsync(counters + kCntError, 0, 4, ctx->stream));
     stage_begin(ctx, kStagePrimary);
+    k_primary_init<<<1, 64, 0, ctx->stream>>>(counters + kCntError);
     k_primary<<<groups, 256, 0, ctx->stream>>>(tlas->nodes, tlas->recs, tlas->aux, *cam, dmats, nmats,
                                               *out, (uint32_t*)ctx->spill.ptr, counters + kCntError,
                                               tlas->count == 0);

@@ -49,6 +49,27 @@ luzrt_status ensure(luzrt_ctx* ctx, DevBuf& b, size_t bytes) {
     return LUZRT_OK;
 }
 
+namespace {
+__global__ void k_zero_fill(uint8_t* p, size_t bytes) {
+    const size_t words = bytes / 4;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        reinterpret_cast<uint32_t*>(p)[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < (bytes & 3)) p[words * 4 + threadIdx.x] = 0;
+}
+}  // namespace
+
+luzrt_status zero_fill(luzrt_ctx* ctx, void* p, size_t bytes) {
+    if (bytes == 0) return LUZRT_OK;
+    if (((uintptr_t)p & 3) != 0) return fail(ctx, LUZRT_E_INVALID_ARG, "zero_fill: unaligned pointer");
+    size_t words = bytes / 4;
+    uint32_t blocks = (uint32_t)((words + 255) / 256);
+    if (blocks == 0) blocks = 1;
+    if (blocks > 4096) blocks = 4096;
+    k_zero_fill<<<blocks, 256, 0, ctx->stream>>>((uint8_t*)p, bytes);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
+}
+
 void stage_begin(luzrt_ctx* ctx, Stage s) {
     hipEventRecord(ctx->ev[s][0], ctx->stream);
     ctx->ev_used[s] = true;

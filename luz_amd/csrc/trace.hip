@@ -333,7 +333,7 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
         if ((s = launch_trace(ctx, L)) != LUZRT_OK) return s;
     } else if (ds.r_total > 0) {
         // empty TLAS: nothing can be hit
-        LUZ_HIP(ctx, hipMemsetAsync(ctx->hits.ptr, 0, max_rays, st));
+        if ((s = zero_fill(ctx, ctx->hits.ptr, max_rays)) != LUZRT_OK) return s;
     }
     stage_end(ctx, kStageTrace);
 
@@ -364,7 +364,7 @@ extern "C" luzrt_status luzrt_trace_rays(luzrt_ctx* ctx, const luzrt_tlas* tlas,
     stage_begin(ctx, kStageTrace);
     k_vis_init<<<1, 64, 0, ctx->stream>>>(counters, count);
     if (tlas->count == 0) {
-        LUZ_HIP(ctx, hipMemsetAsync(hits, 0, count, ctx->stream));
+        if ((s = zero_fill(ctx, hits, count)) != LUZRT_OK) return s;
     } else {
         TraceLaunch L{};
         L.tlas_nodes = tlas->nodes;
