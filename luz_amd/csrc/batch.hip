@@ -82,50 +82,33 @@ __global__ void k_batch_init(uint32_t* counters) {
     if (threadIdx.x < kCntCount && threadIdx.x != kCntBuildError) counters[threadIdx.x] = 0u;
 }
 
-// The views of a batch: every scene with a camera, its pixels one after the
-// other in 8x8-tile order (first pixel of scene s at ppx[s], staged in LDS).
-struct BatchViews {
-    const BatchSceneDev* sc;
-    const uint32_t* ppx;
-    uint32_t n;
-    __device__ uint32_t find(uint32_t g, uint32_t& local) const {
-        const uint32_t s = find_scene(ppx, n, g);
-        local = g - ppx[s];
-        return s;
-    }
-    __device__ PrimaryView operator[](uint32_t s) const {
-        const BatchSceneDev& S = sc[s];
-        PrimaryView v;
-        v.tlas_nodes = S.tlas_nodes;
-        v.recs = S.recs;
-        v.aux = S.aux;
-        v.mats = S.mats;
-        v.nmats = S.nmats;
-        v.cam = S.cam;
-        v.out = S.gout;
-        return v;
-    }
-};
-
 // Persistent closest-hit primary rays over the 8x8-tile pixel slots of every
-// scene with a camera, in one flat index space, with the refill of
-// primary_persistent (claims from counters[kCntPrimFetch]).
-__global__ __launch_bounds__(kBlock) void k_bprimary(BatchTables T, uint32_t* spill, uint32_t* counters,
-                                                     uint32_t refill) {
+// scene with a camera, in one flat index space.
+__global__ __launch_bounds__(kBlock) void k_bprimary(BatchTables T, uint32_t* spill,
+                                                     uint32_t* error_flag) {
     __shared__ uint32_t lds[kShort * kBlock];
-    __shared__ uint32_t sppx[kMaxBatch + 1];
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i <= T.n; i += kBlock) sppx[i] = T.ppx[i];
-    __syncthreads();
+    const uint32_t total = gridDim.x * kBlock;
     Stack st;
     st.ls = lds + tid;
     st.lds_stride = kBlock;
     st.gs = spill + (size_t)blockIdx.x * kBlock + tid;
-    st.g_stride = gridDim.x * kBlock;
+    st.g_stride = total;
     st.sp = 0;
     st.lo = 0;
-    const BatchViews views{T.sc, sppx, T.n};
-    primary_persistent(views, sppx[T.n], counters + kCntPrimFetch, refill, st, counters + kCntError);
+    const uint32_t end = T.ppx[T.n];
+    for (uint32_t gg = blockIdx.x * kBlock + tid; gg < end; gg += total) {
+        uint32_t s = find_scene(T.ppx, T.n, gg);
+        const BatchSceneDev& S = T.sc[s];
+        uint32_t g = gg - T.ppx[s];
+        const uint32_t W = S.gout.width, H = S.gout.height;
+        const uint32_t tiles_x = (W + 7) / 8;
+        uint32_t tile = g >> 6, t = g & 63u;
+        uint32_t x = (tile % tiles_x) * 8 + (t & 7u), y = (tile / tiles_x) * 8 + (t >> 3);
+        if (x < W && y < H)
+            primary_pixel(S.tlas_nodes, S.recs, S.aux, S.cam, S.mats, S.nmats, S.gout, x, y, st,
+                          error_flag, S.tlas_nodes == nullptr);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_bcompact_count(BatchTables T, uint32_t tw) {
@@ -362,8 +345,8 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     if (b->total_ppx) {
         uint32_t groups = (uint32_t)ctx->num_cus * 4;
         if (groups == 0) groups = 1;
-        const uint32_t refill = ctx->refill_threshold ? ctx->refill_threshold : 64u;
-        k_bprimary<<<groups, kBlock, 0, st>>>(b->tables, (uint32_t*)ctx->spill.ptr, b->counters, refill);
+        k_bprimary<<<groups, kBlock, 0, st>>>(b->tables, (uint32_t*)ctx->spill.ptr,
+                                              b->counters + kCntError);
     }
     const uint32_t tw = ctx->tile_width;
     k_bcompact_count<<<b->total_cblk, kBlock, 0, st>>>(b->tables, tw);

@@ -184,7 +184,6 @@ struct DevScene {
 
 // Device counters of one visibility call (ctx->counters).
 enum Counter { kCntFetch = 0, kCntActive = 1, kCntError = 2, kCntRays = 3, kCntBuildError = 4,
-               kCntPrimFetch = 5,
                kCntCount = 8 };
 
 struct TraceLaunch {

@@ -6,7 +6,7 @@
 //   shadow_setup / raygen_pixel               light.frag:86-146, 111-135, 229-231
 //   resolve_pixel                             occluded / unoccluded sample counts
 //   shade_pixel                               light.frag:17-49, 171-235 (FP32)
-//   primary_pixel / primary_persistent        G-buffer texels by closest-hit primary rays
+//   primary_pixel / primary_texel             G-buffer texels by closest-hit primary rays
 #pragma once
 #include "internal.hpp"
 #include "traverse.hpp"
@@ -350,97 +350,6 @@ __device__ inline void primary_pixel(const Node* tlas_nodes, const InstRec* recs
         }
     }
     primary_texel(cam, d, best, aux, mats, nmats, out, x, y);
-}
-
-// One primary-ray camera and G-buffer: the TLAS, materials and output of a scene.
-struct PrimaryView {
-    const Node* tlas_nodes;   // null: empty TLAS, every pixel is background
-    const InstRec* recs;
-    const InstAux* aux;
-    const luzrt_material* mats;
-    uint32_t nmats;
-    luzrt_camera cam;
-    luzrt_gbuffer_out out;
-};
-
-constexpr uint32_t kPrimaryChunk = 256;   // tile-ordered pixels per claim (4 tiles of 8x8)
-
-// Persistent closest-hit primary rays.  Lanes take pixels of the 8x8-tile
-// order from a claim counter (`fetch`, zero at launch) in chunks of
-// kPrimaryChunk per wave; a wave refills its idle lanes when at least `refill`
-// are idle, so a lane that finishes a short ray starts the next pixel instead
-// of waiting for the slowest lane of its wave.  VIEWS maps a pixel index of
-// [0, npx) to its view and the pixel's index within that view's tile order
-// (find) and gives a view (operator[]).  Used for batches, whose scenes' rays
-// differ in cost; one camera's coherent rays run to completion in primary_pixel.
-template <typename VIEWS, typename STACK>
-__device__ inline void primary_persistent(const VIEWS& views, uint32_t npx, uint32_t* fetch, uint32_t refill,
-                                          STACK& st, uint32_t* error_flag) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    RayState r;
-    Best best;
-    uint32_t px = 0, py = 0, vi = 0;
-    bool busy = false, open = true;
-    uint32_t chunk_next = 0, chunk_end = 0;   // wave-uniform
-    for (;;) {
-        if (open) {
-            const uint64_t idle = __ballot(!busy);
-            const uint32_t nidle = (uint32_t)__popcll(idle);
-            if (nidle >= refill || nidle == 64u) {
-                if (chunk_next >= chunk_end) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(fetch, kPrimaryChunk);
-                    base = __shfl(base, 0);
-                    chunk_next = base;
-                    chunk_end = base < npx ? (npx - base > kPrimaryChunk ? base + kPrimaryChunk : npx) : base;
-                    if (base >= npx) open = false;
-                }
-                const uint32_t avail = chunk_end > chunk_next ? chunk_end - chunk_next : 0u;
-                const uint32_t take = nidle < avail ? nidle : avail;
-                if (!busy) {
-                    const uint32_t rank = (uint32_t)__popcll(idle & lt);
-                    if (rank < take) {
-                        uint32_t local = 0;
-                        vi = views.find(chunk_next + rank, local);
-                        const PrimaryView& v = views[vi];
-                        const uint32_t W = v.out.width, H = v.out.height, tiles_x = (W + 7) / 8;
-                        const uint32_t tile = local >> 6, t = local & 63u;
-                        px = (tile % tiles_x) * 8 + (t & 7u);
-                        py = (tile / tiles_x) * 8 + (t >> 3);
-                        if (px < W && py < H) {
-                            best.found = false;
-                            best.t = v.cam.tmax;
-                            ray_begin(r, st, v3(v.cam.eye[0], v.cam.eye[1], v.cam.eye[2]),
-                                      primary_dir(v.cam, px, py, W, H), v.cam.tmin, v.cam.tmax);
-                            // an empty TLAS: no root to visit, the first step ends the ray
-                            if (!v.tlas_nodes) r.next = kNone;
-                            busy = true;
-                        }
-                    }
-                }
-                chunk_next += take;
-            }
-        }
-        if (__ballot(busy) == 0ull) {
-            if (!open) break;
-            continue;
-        }
-        if (busy) {
-            const PrimaryView& v = views[vi];
-            for (;;) {
-                const int res = ray_step<true>(r, st, v.tlas_nodes, v.recs, v.aux, &best);
-                if (res == kStepDone || res == kStepOverflow) {
-                    if (res == kStepOverflow) { atomicOr(error_flag, 1u); st.sp = st.lo = 0; }
-                    primary_texel(v.cam, primary_dir(v.cam, px, py, v.out.width, v.out.height), best, v.aux,
-                                  v.mats, v.nmats, v.out, px, py);
-                    busy = false;
-                    break;
-                }
-                if (open && 64u - (uint32_t)__popcll(__ballot(true)) >= refill) break;
-            }
-        }
-    }
 }
 
 }  // namespace luz

@@ -352,6 +352,10 @@ luzrt_status luzrt_blas_build(luzrt_ctx* ctx, const void* vertices, uint32_t str
                     stride);
     if ((index_count && !indices) || (vertex_count && !vertices))
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_blas_build: null buffer");
+    // the build replaces an out-of-range index by vertex 0 (and reports it), so
+    // triangles need at least one vertex
+    if (index_count && vertex_count == 0)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_blas_build: %u indices but no vertices", index_count);
     if (index_count / 3 >= 0x7FFFFFFFu)
         return fail(ctx, LUZRT_E_LIMIT, "too many triangles");
     LUZ_HIP(ctx, hipSetDevice(ctx->device));
