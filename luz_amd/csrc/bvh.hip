@@ -324,46 +324,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint8_t* verts, uint32_
     for (int k = 0; k < 6; ++k) slot_box[(size_t)s * 6 + k] = boxes[(size_t)p * 6 + k];
 }
 
-__device__ inline float ld_agent(const float* p) {
-    uint32_t u = __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __uint_as_float(u);
-}
-
 __device__ inline void store_child_box(Node* nd, int side, const float* b) {
     nd->lox[side] = b[0]; nd->loy[side] = b[1]; nd->loz[side] = b[2];
     nd->hix[side] = b[3]; nd->hiy[side] = b[4]; nd->hiz[side] = b[5];
 }
 
-// Climb from leaf slot s; the second child to arrive at a node computes its box.
-// Karras trees are at most 64 levels deep (the augmented key has 64 bits),
-// so the climb is bounded; exceeding the bound flags a malformed tree.
-__device__ inline void refit_climb(Node* nodes, const uint32_t* leaf_parent, uint32_t* counters,
-                                   uint32_t s, const float* leaf_box, uint32_t* error) {
-    float box[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) box[k] = leaf_box[k];
-    uint32_t c = kLeaf | s;
-    uint32_t p = leaf_parent[s];
-    for (int level = 0; p != kNone; ++level) {
-        if (level > 80) { atomicOr(error, 1u); return; }
-        Node* nd = &nodes[p];
-        int side = nd->child[0] == c ? 0 : 1;
-        store_child_box(nd, side, box);
-        __threadfence();
-        uint32_t old = atomicAdd(&counters[p], 1u);
-        if (old == 0) return;
-        __threadfence();
-        float b0[6], b1[6];
-        b0[0] = ld_agent(&nd->lox[0]); b0[1] = ld_agent(&nd->loy[0]); b0[2] = ld_agent(&nd->loz[0]);
-        b0[3] = ld_agent(&nd->hix[0]); b0[4] = ld_agent(&nd->hiy[0]); b0[5] = ld_agent(&nd->hiz[0]);
-        b1[0] = ld_agent(&nd->lox[1]); b1[1] = ld_agent(&nd->loy[1]); b1[2] = ld_agent(&nd->loz[1]);
-        b1[3] = ld_agent(&nd->hix[1]); b1[4] = ld_agent(&nd->hiy[1]); b1[5] = ld_agent(&nd->hiz[1]);
-        box[0] = fmin_(b0[0], b1[0]); box[1] = fmin_(b0[1], b1[1]); box[2] = fmin_(b0[2], b1[2]);
-        box[3] = fmax_(b0[3], b1[3]); box[4] = fmax_(b0[4], b1[4]); box[5] = fmax_(b0[5], b1[5]);
-        c = p;
-        p = nd->parent;
-    }
-}
 
 constexpr int kRefitChunk = 256;    // leaf slots per workgroup of the chunk pass
 constexpr int kSpanThreads = 512;   // threads per BLAS of the span pass
@@ -666,22 +631,36 @@ __global__ __launch_bounds__(kBlock) void k_wide_emit(const WRef* slots, const u
 
 // ---------------------------------------------------------------- TLAS
 constexpr int kTlasThreads = 1024;
+constexpr int kPlocRadius = 16;                              // clustering search radius (slots)
+constexpr int kPlocTile = kTlasThreads + 2 * kPlocRadius;    // staged boxes per neighbour pass
+
+// Merge cost of a cluster box (lo xyz, hi xyz) with box t of a staged tile
+// (six planes of `plane` floats): half the surface area of the union, FP32 as
+// written, NaN as +inf.
+__device__ inline float merge_cost(const float* a, const float* tile, int plane, int t) {
+    const float lx = fmin_(a[0], tile[t]), ly = fmin_(a[1], tile[plane + t]),
+                lz = fmin_(a[2], tile[2 * plane + t]);
+    const float hx = fmax_(a[3], tile[3 * plane + t]), hy = fmax_(a[4], tile[4 * plane + t]),
+                hz = fmax_(a[5], tile[5 * plane + t]);
+    const float ex = hx - lx, ey = hy - ly, ez = hz - lz;
+    const float c = ex * ey + ey * ez + ez * ex;
+    return c == c ? c : INFINITY;
+}
 
 __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     const InstIn* in = J.in;
     const uint32_t n = J.n, m = J.m;
     float* wbox = J.wbox;
-    uint32_t* leaf_parent = J.leaf_parent;
-    uint32_t* counters = J.counters;
     Node* nodes = J.nodes;
     InstRec* recs = J.recs;
     InstRec* recs_w = J.recs_w;
     InstAux* aux = J.aux;
     float* w2o_out = J.w2o_out;
     uint32_t* order_out = J.order_out;
-    // max(m, 1024) 8-byte entries, m a power of two >= n.  The reductions of
-    // steps 1-2 use the same memory before the sort keys are written, which
-    // keeps the kernel within 64 KB of LDS for 8192 instances.
+    // tlas_lds_bytes(m) bytes, m a power of two >= n: the reductions of steps
+    // 1-2, then the sort keys, then the clustering's box tile and neighbour
+    // array, one after the other in the same memory (within 64 KB for 8192
+    // instances).
     extern __shared__ uint64_t skeys[];
     float* shf = reinterpret_cast<float*>(skeys);
     double* shd = reinterpret_cast<double*>(skeys);
@@ -781,7 +760,11 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
             __syncthreads();
         }
     }
-    // 4. hierarchy over the sorted codes (high halves), leaf records
+    // 4. leaf records in slot order; the clusters start as the leaves
+    float* cb = J.cbox[0];
+    float* cb2 = J.cbox[1];
+    uint32_t* cr = J.cref[0];
+    uint32_t* cr2 = J.cref[1];
     for (uint32_t s = tid; s < n; s += kTlasThreads) {
         uint32_t i = (uint32_t)(skeys[s] & 0xFFFFFFFFu);
         const InstIn& src = in[i];
@@ -805,53 +788,107 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         aux[s] = ax;
         for (int k = 0; k < 12; ++k) w2o_out[s * 12 + k] = src.w2o[k];
         order_out[s] = src.input_index;
+        for (int k = 0; k < 6; ++k) cb[s * 6 + k] = wbox[i * 6 + k];
+        cr[s] = kLeaf | s;
     }
+    __syncthreads();   // the sort keys are dead from here on
     if (n >= 2) {
-        // keys for delta: the Morton code of slot s is skeys[s] >> 32
-        for (uint32_t i = tid; i < n - 1; i += kTlasThreads) {
-            // delta on the code array: build a local view through a lambda-free copy
-            const int nn = (int)n;
-            auto code = [&](int j) { return (uint32_t)(skeys[j] >> 32); };
-            auto dlt = [&](int a, int b) -> int {
-                if (b < 0 || b > nn - 1) return -1;
-                uint32_t ka = code(a), kb = code(b);
-                if (ka == kb) return 32 + __clz((int)((uint32_t)a ^ (uint32_t)b));
-                return __clz((int)(ka ^ kb));
-            };
-            int ii = (int)i;
-            int d = (dlt(ii, ii + 1) - dlt(ii, ii - 1)) >= 0 ? 1 : -1;
-            int dmin = dlt(ii, ii - d);
-            int lmax = 2;
-            while (dlt(ii, ii + lmax * d) > dmin) lmax *= 2;
-            int l = 0;
-            for (int t = lmax / 2; t >= 1; t /= 2)
-                if (dlt(ii, ii + (l + t) * d) > dmin) l += t;
-            int j = ii + l * d;
-            int dnode = dlt(ii, j);
-            int s = 0;
-            int t = l;
-            do {
-                t = (t + 1) >> 1;
-                if (dlt(ii, ii + (s + t) * d) > dnode) s += t;
-            } while (t > 1);
-            int gamma = ii + s * d + (d < 0 ? d : 0);
-            int lo = ii < j ? ii : j, hi = ii < j ? j : ii;
-            uint32_t left = (lo == gamma) ? (kLeaf | (uint32_t)gamma) : (uint32_t)gamma;
-            uint32_t right = (hi == gamma + 1) ? (kLeaf | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
-            nodes[i].child[0] = left;
-            nodes[i].child[1] = right;
-            nodes[i].pad = 0;
-            if (i == 0) nodes[0].parent = kNone;
-            if (left & kLeaf) leaf_parent[left & ~kLeaf] = i; else nodes[left].parent = i;
-            if (right & kLeaf) leaf_parent[right & ~kLeaf] = i; else nodes[right].parent = i;
-            counters[i] = 0;
-        }
-        __threadfence();
-        __syncthreads();
-        // 5. refit
-        for (uint32_t s = tid; s < n; s += kTlasThreads) {
-            uint32_t i = (uint32_t)(skeys[s] & 0xFFFFFFFFu);
-            refit_climb(nodes, leaf_parent, counters, s, wbox + i * 6, error);
+        // 5. locally-ordered clustering (PLOC, Meister & Bittner 2018): each
+        // round every cluster finds the neighbour within kPlocRadius slots of
+        // least merge cost, mutual pairs merge into a new node at the lower
+        // position, and the survivors are compacted in order.  New nodes are
+        // numbered downwards from n - 2 in merge order, so the root is node 0.
+        float* tile = reinterpret_cast<float*>(skeys);          // 6 planes of kPlocTile boxes
+        uint32_t* snn = reinterpret_cast<uint32_t*>(tile + 6 * kPlocTile);
+        __shared__ uint32_t s_wave[2][kTlasThreads / 64];
+        const uint32_t lane = tid & 63u, wave = tid >> 6;
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        uint32_t count = n, next = n - 1;
+        while (count > 1) {
+            for (uint32_t base = 0; base < count; base += kTlasThreads) {
+                for (int t = tid; t < kPlocTile; t += kTlasThreads) {
+                    const int c = (int)base - kPlocRadius + t;
+                    if (c >= 0 && c < (int)count)
+                        for (int k = 0; k < 6; ++k) tile[k * kPlocTile + t] = cb[c * 6 + k];
+                }
+                __syncthreads();
+                const uint32_t i = base + tid;
+                if (i < count) {
+                    const int ti = tid + kPlocRadius;
+                    float box[6];
+                    for (int k = 0; k < 6; ++k) box[k] = tile[k * kPlocTile + ti];
+                    const int j0 = (int)i - kPlocRadius < 0 ? 0 : (int)i - kPlocRadius;
+                    const int j1 = (int)i + kPlocRadius > (int)count - 1 ? (int)count - 1 : (int)i + kPlocRadius;
+                    float best = INFINITY;
+                    uint32_t bj = kNone;
+                    for (int j = j0; j <= j1; ++j) {
+                        if (j == (int)i) continue;
+                        const float c = merge_cost(box, tile, kPlocTile, j - (int)base + kPlocRadius);
+                        if (bj == kNone || c < best) { best = c; bj = (uint32_t)j; }
+                    }
+                    snn[i] = bj;
+                }
+                __syncthreads();
+            }
+            uint32_t heads = 0, kept = 0;
+            for (uint32_t base = 0; base < count; base += kTlasThreads) {
+                const uint32_t i = base + tid;
+                bool head = false, keep = false;
+                uint32_t j = 0;
+                if (i < count) {
+                    j = snn[i];
+                    const bool mutual = snn[j] == i;
+                    head = mutual && i < j;
+                    keep = !(mutual && i > j);
+                }
+                const uint64_t bh = __ballot(head), bk = __ballot(keep);
+                if (lane == 0) {
+                    s_wave[0][wave] = (uint32_t)__popcll(bh);
+                    s_wave[1][wave] = (uint32_t)__popcll(bk);
+                }
+                __syncthreads();
+                uint32_t oh = 0, ok = 0, th = 0, tk = 0;
+                for (uint32_t w = 0; w < kTlasThreads / 64; ++w) {
+                    const uint32_t ch = s_wave[0][w], ck = s_wave[1][w];
+                    if (w < wave) { oh += ch; ok += ck; }
+                    th += ch;
+                    tk += ck;
+                }
+                const uint32_t hr = heads + oh + (uint32_t)__popcll(bh & below);
+                const uint32_t kr = kept + ok + (uint32_t)__popcll(bk & below);
+                if (head) {
+                    const uint32_t idx = next - 1 - hr;
+                    Node nd;
+                    store_child_box(&nd, 0, cb + (size_t)i * 6);
+                    store_child_box(&nd, 1, cb + (size_t)j * 6);
+                    nd.child[0] = cr[i];
+                    nd.child[1] = cr[j];
+                    nd.parent = kNone;
+                    nd.pad = 0;
+                    nodes[idx] = nd;
+                    if (!(nd.child[0] & kLeaf)) nodes[nd.child[0]].parent = idx;
+                    if (!(nd.child[1] & kLeaf)) nodes[nd.child[1]].parent = idx;
+                    cr2[kr] = idx;
+                    for (int k = 0; k < 3; ++k) {
+                        cb2[kr * 6 + k] = fmin_(cb[i * 6 + k], cb[j * 6 + k]);
+                        cb2[kr * 6 + 3 + k] = fmax_(cb[i * 6 + 3 + k], cb[j * 6 + 3 + k]);
+                    }
+                } else if (keep) {
+                    cr2[kr] = cr[i];
+                    for (int k = 0; k < 6; ++k) cb2[kr * 6 + k] = cb[i * 6 + k];
+                }
+                heads += th;
+                kept += tk;
+                __syncthreads();
+            }
+            if (heads == 0) {   // cannot happen: the least-cost pair is mutual
+                if (tid == 0) atomicOr(error, 1u);
+                break;
+            }
+            count = kept;
+            next -= heads;
+            float* tb = cb; cb = cb2; cb2 = tb;
+            uint32_t* tr = cr; cr = cr2; cr2 = tr;
         }
     } else if (tid == 0) {
         Node nd;
@@ -1032,7 +1069,7 @@ luzrt_status launch_tlas_batched(luzrt_ctx* ctx, const TlasJob* dev_jobs, uint32
                                  uint32_t max_m) {
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
-    size_t lds = (size_t)(max_m > 1024 ? max_m : 1024) * sizeof(uint64_t);
+    const size_t lds = tlas_lds_bytes(max_m);
     k_tlas_build_batched<<<njobs, kTlasThreads, lds, ctx->stream>>>(dev_jobs, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
@@ -1070,6 +1107,16 @@ luzrt_status launch_refit_batched(luzrt_ctx* ctx, const RefitJob* dev_jobs, uint
     return LUZRT_OK;
 }
 
+size_t tlas_scratch_bytes(uint32_t capacity) {
+    return (size_t)capacity * (3 * 6 * sizeof(float) + 2 * sizeof(uint32_t));
+}
+
+size_t tlas_lds_bytes(uint32_t m) {
+    const size_t sort = (size_t)(m > 1024 ? m : 1024) * sizeof(uint64_t);
+    const size_t ploc = (size_t)6 * kPlocTile * sizeof(float) + (size_t)m * sizeof(uint32_t);
+    return sort > ploc ? sort : ploc;
+}
+
 TlasJob tlas_job(luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
     uint32_t m = 1;
     while (m < n) m <<= 1;
@@ -1077,9 +1124,12 @@ TlasJob tlas_job(luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
     J.in = dev_in;
     J.n = n;
     J.m = m;
+    const size_t cap = t->capacity;
     J.wbox = (float*)t->scratch;
-    J.leaf_parent = (uint32_t*)(J.wbox + (size_t)t->capacity * 6);
-    J.counters = J.leaf_parent + t->capacity;
+    J.cbox[0] = J.wbox + cap * 6;
+    J.cbox[1] = J.cbox[0] + cap * 6;
+    J.cref[0] = (uint32_t*)(J.cbox[1] + cap * 6);
+    J.cref[1] = J.cref[0] + cap;
     J.nodes = t->nodes;
     J.recs = t->recs;
     J.recs_w = t->wide ? t->recs_w : nullptr;
@@ -1093,7 +1143,7 @@ luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
     TlasJob J = tlas_job(t, dev_in, n);
-    size_t lds = (size_t)(J.m > 1024 ? J.m : 1024) * sizeof(uint64_t);
+    const size_t lds = tlas_lds_bytes(J.m);
     k_tlas_build<<<1, kTlasThreads, lds, ctx->stream>>>(J, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;

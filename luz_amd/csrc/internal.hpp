@@ -133,9 +133,9 @@ luzrt_status pack_instances(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instance*
 struct TlasJob {
     const InstIn* in;
     uint32_t n, m;          // instances, power of two >= n
-    float* wbox;
-    uint32_t* leaf_parent;
-    uint32_t* counters;
+    float* wbox;            // instance world boxes (6 floats each, input order)
+    float* cbox[2];         // cluster boxes, ping-pong between clustering rounds
+    uint32_t* cref[2];      // cluster node references, likewise
     Node* nodes;
     InstRec* recs;
     InstRec* recs_w;        // null unless the compact form is built
@@ -144,6 +144,10 @@ struct TlasJob {
     uint32_t* order_out;
 };
 TlasJob tlas_job(luzrt_tlas* t, const InstIn* dev_in, uint32_t n);
+// Scratch bytes of a TLAS with room for `capacity` instances, and the dynamic
+// LDS bytes of its build kernel for m (a power of two >= the instance count).
+size_t tlas_scratch_bytes(uint32_t capacity);
+size_t tlas_lds_bytes(uint32_t m);
 luzrt_status launch_tlas_batched(luzrt_ctx* ctx, const TlasJob* dev_jobs, uint32_t njobs,
                                  uint32_t max_m);
 

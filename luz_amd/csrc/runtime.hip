@@ -449,7 +449,7 @@ luzrt_status luzrt_tlas_create(luzrt_ctx* ctx, uint32_t max_instances, luzrt_tla
     if (e == hipSuccess) e = hipMalloc((void**)&t->w2o, n * 48);
     if (e == hipSuccess) e = hipMalloc((void**)&t->order, n * 4);
     if (e == hipSuccess) e = hipMalloc(&t->input, n * sizeof(InstIn));
-    if (e == hipSuccess) e = hipMalloc(&t->scratch, n * 24 + n * 8 + 256);
+    if (e == hipSuccess) e = hipMalloc(&t->scratch, tlas_scratch_bytes((uint32_t)n) + 256);
     if (e != hipSuccess) {
         luzrt_tlas_destroy(ctx, t);
         return hip_fail(ctx, e, "hipMalloc(TLAS)");

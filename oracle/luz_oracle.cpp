@@ -29,12 +29,17 @@
  *       rounded to FP32; instance world boxes from the 8 transformed corners
  *       of the BLAS root box, widened by (max |coord| over all instances) *
  *       2^-14.
+ *   S7  TLAS hierarchy: the S5 Morton slot order, then locally-ordered
+ *       clustering (PLOC, Meister & Bittner 2018) with radius 16 and the
+ *       FP32 union-box half-area as merge cost (build_ploc below).
  *
  * Parity pins (tests/test_oracle_*.py): Morton and PCG are checked against
  * independent brute-force/bit-level restatements and statistical properties;
  * the triangle test against closed-form barycentric cases and watertightness
  * on shared edges and vertices; the BVH against brute force over all
- * triangles; sincos against numpy; the instance inverse against the identity;
+ * triangles; the TLAS clustering against exhaustive greedy agglomerative
+ * clustering (which PLOC equals when the radius spans every cluster) and a
+ * closed-form merge order; sincos against numpy; the instance inverse against the identity;
  * the GGX terms against closed forms (Walter et al. 2007, Karis 2013).
  * The Vulkan driver's own builder, traversal order and interval closure
  * cannot be run here (no Vulkan, closed driver): parity unpinned for those.
@@ -367,23 +372,25 @@ inline int delta(const std::vector<uint32_t>& k, int n, int i, int j) {
     return clz32(a ^ b);
 }
 
-// Builds nodes over primitive boxes.  order[slot] = primitive index.
-void build_lbvh(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::vector<uint32_t>& order) {
+// The one-node tree of zero or one primitive.
+void build_single(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::vector<uint32_t>& order) {
     const int n = (int)boxes.size();
-    nodes.clear();
-    order.clear();
-    if (n <= 1) {
-        Node root;
-        std::memset(&root, 0, sizeof(root));
-        set_child(root, 0, n == 1 ? boxes[0] : empty_box());
-        set_child(root, 1, empty_box());
-        root.child[0] = n == 1 ? (kLeaf | 0u) : kEmpty;
-        root.child[1] = kEmpty;
-        root.parent = kNone;
-        nodes.push_back(root);
-        if (n == 1) order.push_back(0);
-        return;
-    }
+    Node root;
+    std::memset(&root, 0, sizeof(root));
+    set_child(root, 0, n == 1 ? boxes[0] : empty_box());
+    set_child(root, 1, empty_box());
+    root.child[0] = n == 1 ? (kLeaf | 0u) : kEmpty;
+    root.child[1] = kEmpty;
+    root.parent = kNone;
+    nodes.push_back(root);
+    if (n == 1) order.push_back(0);
+}
+
+// Slot order of n >= 2 primitive boxes (S5): stable sort by the Morton code of
+// the box centroid in the FP64 centroid frame.  order[slot] = primitive index,
+// keys[slot] = its code.
+void morton_order(const std::vector<Box>& boxes, std::vector<uint32_t>& order, std::vector<uint32_t>& keys) {
+    const int n = (int)boxes.size();
     // FP64 centroid frame.
     std::vector<double> cx(n), cy(n), cz(n);
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -419,9 +426,18 @@ void build_lbvh(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::ve
     for (int i = 0; i < n; ++i) order[i] = (uint32_t)i;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
-    std::vector<uint32_t> keys(n);
+    keys.resize(n);
     for (int i = 0; i < n; ++i) keys[i] = code[order[i]];
+}
 
+// Builds nodes over primitive boxes (S5).  order[slot] = primitive index.
+void build_lbvh(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::vector<uint32_t>& order) {
+    const int n = (int)boxes.size();
+    nodes.clear();
+    order.clear();
+    if (n <= 1) return build_single(boxes, nodes, order);
+    std::vector<uint32_t> keys;
+    morton_order(boxes, order, keys);
     nodes.assign(n - 1, Node{});
     for (auto& nd : nodes) std::memset(&nd, 0, sizeof(Node));
     std::vector<uint32_t> leaf_parent(n, kNone);
@@ -472,6 +488,87 @@ void build_lbvh(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::ve
         }
         nb[ni] = box_union(cb[0], cb[1]);
         st.pop_back();
+    }
+}
+
+// S7: TLAS hierarchy by locally-ordered clustering (PLOC, Meister & Bittner
+// 2018, IEEE TVCG 24(3)) over the Morton slot order.  Clusters start as the
+// leaves in slot order.  Each round, cluster i takes as nearest neighbour the
+// j within kPlocRadius positions minimising the merge cost (half the surface
+// area of the union box, FP32, evaluated as written; NaN counts as +inf),
+// the smaller j on equal cost.  Mutual nearest neighbours i < j merge into a
+// new node (child 0 = cluster i, child 1 = cluster j) that takes position i;
+// position j drops out; the survivors keep their relative order.  Nodes are
+// numbered downwards from n - 2 in the order they are made (merges of one
+// round in increasing i), so the last merge, the root, is node 0.  The pair
+// of least (cost, i, j) is always mutual, so every round merges.
+constexpr int kPlocRadius = 16;
+
+inline float merge_cost(const Box& a, const Box& b) {
+    const Box u = box_union(a, b);
+    const float ex = u.hi[0] - u.lo[0], ey = u.hi[1] - u.lo[1], ez = u.hi[2] - u.lo[2];
+    const float c = ex * ey + ey * ez + ez * ex;
+    return c == c ? c : INFINITY;
+}
+
+void build_ploc(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::vector<uint32_t>& order) {
+    const int n = (int)boxes.size();
+    nodes.clear();
+    order.clear();
+    if (n <= 1) return build_single(boxes, nodes, order);
+    std::vector<uint32_t> keys;
+    morton_order(boxes, order, keys);
+    nodes.assign(n - 1, Node{});
+    for (auto& nd : nodes) std::memset(&nd, 0, sizeof(Node));
+    std::vector<uint32_t> ref(n), nn(n), ref2(n);
+    std::vector<Box> cb(n), cb2(n);
+    for (int s = 0; s < n; ++s) {
+        ref[s] = kLeaf | (uint32_t)s;
+        cb[s] = boxes[order[s]];
+    }
+    int count = n;
+    uint32_t next = (uint32_t)n - 1;
+    while (count > 1) {
+        for (int i = 0; i < count; ++i) {
+            float best = INFINITY;
+            uint32_t bj = kNone;
+            const int j0 = i - kPlocRadius < 0 ? 0 : i - kPlocRadius;
+            const int j1 = i + kPlocRadius > count - 1 ? count - 1 : i + kPlocRadius;
+            for (int j = j0; j <= j1; ++j) {
+                if (j == i) continue;
+                const float c = merge_cost(cb[i], cb[j]);
+                if (bj == kNone || c < best) { best = c; bj = (uint32_t)j; }
+            }
+            nn[i] = bj;
+        }
+        int w = 0;
+        uint32_t made = 0;
+        for (int i = 0; i < count; ++i) {
+            const uint32_t j = nn[i];
+            const bool mutual = nn[j] == (uint32_t)i;
+            if (mutual && (uint32_t)i > j) continue;
+            if (mutual) {
+                const uint32_t idx = next - 1 - made++;
+                Node& nd = nodes[idx];
+                nd.child[0] = ref[i];
+                nd.child[1] = ref[j];
+                set_child(nd, 0, cb[i]);
+                set_child(nd, 1, cb[j]);
+                nd.parent = kNone;
+                for (int side = 0; side < 2; ++side)
+                    if (!(nd.child[side] & kLeaf)) nodes[nd.child[side]].parent = idx;
+                ref2[w] = idx;
+                cb2[w] = box_union(cb[i], cb[j]);
+            } else {
+                ref2[w] = ref[i];
+                cb2[w] = cb[i];
+            }
+            ++w;
+        }
+        count = w;
+        next -= made;
+        std::swap(ref, ref2);
+        std::swap(cb, cb2);
     }
 }
 
@@ -805,7 +902,7 @@ void tlas_build(orc_scene& s) {
     float pad = scene_m * 0x1p-14f;
     for (uint32_t i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) { boxes[i].lo[k] = boxes[i].lo[k] - pad; boxes[i].hi[k] = boxes[i].hi[k] + pad; }
-    build_lbvh(boxes, s.tlas_nodes, s.tlas_order);
+    build_ploc(boxes, s.tlas_nodes, s.tlas_order);
     s.tlas_w2o.assign((size_t)n * 12, 0.0f);
     for (uint32_t slot = 0; slot < n; ++slot)
         std::memcpy(&s.tlas_w2o[(size_t)slot * 12], s.tlas_inst[s.tlas_order[slot]].w2o, 48);

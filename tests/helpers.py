@@ -67,3 +67,21 @@ def denoise_rmse(ctx, dn, fr) -> dict:
     ta, tv = fr.target_ao_host.astype(np.float64), fr.target_vis_host.astype(np.float64)
     return {"ao": {"raw": r(ao_raw, ta), "box": r(box_ao, ta), "mlp": r(ao_d, ta)},
             "vis": {"raw": r(vis_raw, tv), "box": r(box_vis, tv), "mlp": r(vis_d, tv)}}
+
+
+def cube_mesh():
+    """Unit cube [0,1]^3: 8 vertices (48-byte MeshVertex rows), 12 triangles."""
+    v = np.zeros((8, 12), np.float32)
+    for k in range(8):
+        v[k, :3] = (k & 1, (k >> 1) & 1, (k >> 2) & 1)
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    idx = [i for a, b, c, d in quads for i in (a, b, c, a, c, d)]
+    return v, np.array(idx, np.uint32)
+
+
+def scaled_instances(xyz, scale):
+    """Object-to-world rows (12 floats) of uniform scale `scale[i]` and translation `xyz[i]`."""
+    m = np.zeros((len(xyz), 12), np.float32)
+    for i, ((x, y, z), s) in enumerate(zip(xyz, scale)):
+        m[i] = [s, 0, 0, x, 0, s, 0, y, 0, 0, s, z]
+    return m

@@ -8,7 +8,7 @@ import pytest
 
 from luz_amd import api
 from oracle import oracle as orc
-from tests.helpers import oracle_scene, random_rays, set_light_direction
+from tests.helpers import cube_mesh, oracle_scene, random_rays, scaled_instances, set_light_direction
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +45,34 @@ def test_tlas_bytes_match_oracle(ctx):
     np.testing.assert_array_equal(gw.view(np.uint32), ow.view(np.uint32))
     np.testing.assert_array_equal(gn, on)
     g.close()
+
+
+@pytest.mark.parametrize("count,ties", [(2, False), (3, False), (17, False), (40, True), (1500, False),
+                                         (4100, False), (8192, False)])
+def test_tlas_clustering_bytes_match_oracle(ctx, count, ties):
+    """TLAS hierarchy by clustering rounds: one tile of clusters up to 1024,
+    several above, the 8192-instance maximum, and all-equal boxes (every merge
+    decided by the index tie-break)."""
+    rng = np.random.default_rng(count)
+    v, idx = cube_mesh()
+    blas = ctx.create_blas(ctx.to_device(v), 48, len(v), ctx.to_device(idx), len(idx))
+    if ties:
+        xf = scaled_instances(np.zeros((count, 3)), np.ones(count))
+    else:
+        xf = scaled_instances(rng.uniform(-200, 200, (count, 3)), rng.uniform(0.2, 5.0, count))
+    tlas = ctx.create_tlas(count)
+    print(f"TLAS of {count} instances (ties {ties})", flush=True)
+    tlas.build([api.make_instance(blas, xf[i], i) for i in range(count)])
+    gn, gw, go = tlas.arrays()
+    o = orc.OracleScene()
+    m = o.add_mesh(v, idx)
+    o.set_instances(xf, [m] * count)
+    on, ow, oo = o.tlas_arrays()
+    np.testing.assert_array_equal(go, oo)
+    np.testing.assert_array_equal(gw.view(np.uint32), ow.view(np.uint32))
+    np.testing.assert_array_equal(gn, on)
+    tlas.destroy()
+    blas.destroy()
 
 
 @pytest.mark.parametrize("kind", [api.SCENE_CUBE, api.SCENE_SPHERES])

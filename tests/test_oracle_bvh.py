@@ -5,7 +5,7 @@ import pytest
 
 from luz_amd import api
 from oracle import oracle as orc
-from tests.helpers import oracle_scene, random_rays
+from tests.helpers import cube_mesh, oracle_scene, random_rays, scaled_instances
 
 
 def _check_invariants(nodes_u8: np.ndarray, n_prims: int):
@@ -112,3 +112,96 @@ def test_degenerate_sizes():
     rays = np.array([[0.2, 0.2, 1, 0, 0, 0, -1, 5], [2, 2, 1, 0, 0, 0, -1, 5]], np.float32)
     h, _ = o.any_hit(rays)
     assert h.tolist() == [1, 0]
+
+
+def _leaf_sets(nodes_u8):
+    """Frozen set of leaf slots under every internal node, and the leaf boxes."""
+    u = nodes_u8.view(np.uint32).reshape(-1, 16)
+    f = nodes_u8.view(np.float32).reshape(-1, 16)
+    boxes = {}
+
+    def under(i):
+        out = set()
+        for side in range(2):
+            c = int(u[i, 12 + side])
+            if c >> 31:
+                out.add(c & 0x7FFFFFFF)
+                boxes[c & 0x7FFFFFFF] = (f[i][[0 + side, 4 + side, 8 + side]].astype(np.float64),
+                                         f[i][[2 + side, 6 + side, 10 + side]].astype(np.float64))
+            else:
+                out |= under(c)
+        sets.add(frozenset(out))
+        return out
+
+    sets = set()
+    under(0)
+    return sets, boxes
+
+
+def test_tlas_clustering_closed_form():
+    """Two tight pairs far apart: the pairs merge first (nodes 2 and 1, in
+    slot order), then the root joins them."""
+    v, idx = cube_mesh()
+    o = orc.OracleScene()
+    m = o.add_mesh(v, idx)
+    o.set_instances(scaled_instances([(11.5, 0, 0), (0, 0, 0), (10, 0, 0), (1.5, 0, 0)], [1] * 4), [m] * 4)
+    n, _, order = o.tlas_arrays()
+    u = n.view(np.uint32).reshape(-1, 16)
+    assert order.tolist() == [1, 3, 2, 0]                 # Morton order along x
+    assert u[0, 12:15].tolist() == [2, 1, 0xFFFFFFFF]
+    assert u[2, 12:15].tolist() == [0x80000000, 0x80000001, 0]
+    assert u[1, 12:15].tolist() == [0x80000002, 0x80000003, 0]
+    _check_invariants(n, 4)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tlas_clustering_equals_greedy_agglomerative(seed):
+    """With at most 17 instances the clustering radius (16) spans every
+    cluster, and merging all mutual nearest neighbours yields the same tree as
+    greedy agglomerative clustering (repeatedly merge the pair of least union
+    area), because the union area never shrinks under a merge (Walter et al.
+    2008).  The greedy tree is built here by exhaustive search in FP64."""
+    rng = np.random.default_rng(seed)
+    k = 14
+    v, idx = cube_mesh()
+    o = orc.OracleScene()
+    m = o.add_mesh(v, idx)
+    o.set_instances(scaled_instances(rng.uniform(-20, 20, (k, 3)), rng.uniform(0.3, 4.0, k)), [m] * k)
+    n, _, order = o.tlas_arrays()
+    _check_invariants(n, k)
+    sets, boxes = _leaf_sets(n)
+    clusters = [(frozenset([s]), boxes[s][0], boxes[s][1]) for s in range(k)]
+    greedy = set()
+
+    def area(lo, hi):
+        e = hi - lo
+        return e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+
+    while len(clusters) > 1:
+        best = None
+        for a in range(len(clusters)):
+            for b in range(a + 1, len(clusters)):
+                c = area(np.minimum(clusters[a][1], clusters[b][1]), np.maximum(clusters[a][2], clusters[b][2]))
+                if best is None or c < best[0]:
+                    best = (c, a, b)
+        _, a, b = best
+        merged = (clusters[a][0] | clusters[b][0], np.minimum(clusters[a][1], clusters[b][1]),
+                  np.maximum(clusters[a][2], clusters[b][2]))
+        greedy.add(merged[0])
+        clusters = [c for i, c in enumerate(clusters) if i not in (a, b)] + [merged]
+    assert sets == greedy
+
+
+def test_tlas_puts_the_ground_next_to_the_root():
+    """The sphere-field scene's ground quad spans the whole scene: clustering
+    merges it last, so it is a child of the root and no other subtree's box is
+    inflated by it (a Morton-order hierarchy buries it among the spheres)."""
+    s = api.HostScene(api.SCENE_SPHERES, 0)
+    o = oracle_scene(s)
+    n, _, order = o.tlas_arrays()
+    f = n.view(np.float32).reshape(-1, 16)
+    u = n.view(np.uint32).reshape(-1, 16)
+    ext = [f[0][2 + c] - f[0][0 + c] for c in range(2)]                 # x extent of each root child
+    big = int(np.argmax(ext))
+    assert u[0, 12 + big] >> 31 == 1                                    # a leaf: the ground instance
+    assert ext[big] > 4 * ext[1 - big]
