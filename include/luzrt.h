@@ -134,7 +134,9 @@ typedef struct luzrt_material {
 } luzrt_material;
 
 /* GPU time of the stages of the last call, from HIP events on the
- * context's stream (the reference's CmdTimeStamp, VulkanWrapper.cpp:1276-1296). */
+ * context's stream (the reference's CmdTimeStamp, VulkanWrapper.cpp:1276-1296).
+ * Visibility calls generate their rays inside the traversal: trace_ms
+ * includes ray generation and raygen_ms covers only the ray count. */
 typedef struct luzrt_timings {
     float build_ms, compact_ms, raygen_ms, trace_ms, resolve_ms, shade_ms, primary_ms, denoise_ms,
           post_ms;

@@ -29,7 +29,7 @@ struct luzrt_ctx {
     std::string error;
     int num_cus = 0;
     // workspace
-    luz::DevBuf rays_o, rays_d, hits, active, act_index, counters, spill, lut, scene, vis_shadow,
+    luz::DevBuf hits, active, act_index, counters, spill, lut, scene, vis_shadow,
         vis_ao;
     // timing
     hipEvent_t ev[luz::kStageCount][2] = {};
@@ -207,6 +207,15 @@ struct TraceLaunch {
     uint32_t tlas_nodes_count, recs_count;   // single-scene TLAS size (staged in LDS when small)
     const uint32_t* scene_ray_base;   // device, nscenes + 1 entries
     const struct BatchTlas* scene_tlas;
+    // Visibility launch with rays generated in place (gen): ray j * A + a is
+    // computed from active pixel a of the G-buffer and slot j of the scene
+    // block (A = counters[kCntActive]); ray_o and ray_d are then unused.
+    bool gen;
+    const struct DevScene* ds;
+    const float* position;
+    const float* normal;
+    const uint32_t* act_list;
+    uint32_t width;
 };
 
 // TLAS of one scene of a multi-scene launch (nodes null: empty TLAS).

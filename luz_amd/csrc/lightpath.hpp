@@ -3,7 +3,7 @@
 // kernels (batch.hip), so both evaluate identical arithmetic.
 //
 //   tile_pixel / pixel_active / block_prefix  compaction of non-background pixels
-//   shadow_setup / raygen_pixel               light.frag:86-146, 111-135, 229-231
+//   shadow_setup / raygen_pixel / gen_ray     light.frag:86-146, 111-135, 229-231
 //   resolve_pixel                             occluded / unoccluded sample counts
 //   shade_pixel                               light.frag:17-49, 171-235 (FP32)
 //   primary_pixel / primary_texel             G-buffer texels by closest-hit primary rays
@@ -83,6 +83,72 @@ __device__ inline ShadowSetup shadow_setup(const luzrt_light& L, V3 P) {
     return r;
 }
 
+// A G-buffer texel as the ray generators see it: pixel coordinates, position,
+// normal and distance to the camera.
+struct PixelFrame {
+    uint32_t x, y;
+    V3 P, N;
+    float lc;
+};
+
+__device__ inline PixelFrame pixel_frame(const float* position, const float* normal, uint32_t W,
+                                         const luzrt_scene& S, uint32_t p) {
+    PixelFrame f;
+    f.x = p % W;
+    f.y = p / W;
+    f.P = v3(position[(size_t)p * 3], position[(size_t)p * 3 + 1], position[(size_t)p * 3 + 2]);
+    f.N = v3(normal[(size_t)p * 3], normal[(size_t)p * 3 + 1], normal[(size_t)p * 3 + 2]);
+    f.lc = vlen(vsub(f.P, v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2])));
+    return f;
+}
+
+// Shadow-ray origin: P lifted along N by max(0.01 x camera distance, 0.05)
+// (light.frag:137-146).
+__device__ inline V3 shadow_origin(const PixelFrame& f) {
+    const float sb = fmax_(f.lc * 0.01f, 0.05f);
+    return vadd(f.P, vmul(f.N, sb));
+}
+
+// Direction of shadow sample s of light l: the light vector offset by a
+// disk sample of the light's radius in its tangent frame (light.frag:111-135).
+__device__ inline V3 shadow_dir(const luzrt_light& L, const ShadowSetup& ss, const PixelFrame& f,
+                                uint32_t frame, int l, int s) {
+    float u, v;
+    sample2(f.x, f.y, frame, (uint32_t)(l * 256 + s), u, v);
+    const float pr = L.radius * sqrtf(u);
+    float sn, cs;
+    sincos_turns(v, sn, cs);
+    const float dx = pr * cs, dy = pr * sn;
+    const V3 d = v3(ss.Lvec.x + dx * ss.T.x + dy * ss.B.x,
+                    ss.Lvec.y + dx * ss.T.y + dy * ss.B.y,
+                    ss.Lvec.z + dx * ss.T.z + dy * ss.B.z);
+    return vnorm(d);
+}
+
+// AO-ray origin: P lifted along N by 0.01 x camera distance (light.frag:229-231).
+__device__ inline V3 ao_origin(const PixelFrame& f) {
+    const float ab = f.lc * 0.01f;
+    return vadd(f.P, vmul(f.N, ab));
+}
+
+// Cosine-weighted hemisphere sample s around N (light.frag:86-97, 118-126).
+__device__ inline V3 ao_dir(const PixelFrame& f, uint32_t frame, int s) {
+    const V3 N = f.N;
+    const V3 T = fabsf(N.z) > 0.5f ? v3(0.0f, -N.z, N.y) : v3(-N.y, N.x, 0.0f);
+    const V3 B = vcross(N, T);
+    float u, v;
+    sample2(f.x, f.y, frame, (uint32_t)(16384 + s), u, v);
+    const float r = sqrtf(u);
+    float sn, cs;
+    sincos_turns(v * kHemiTurns, sn, cs);
+    const float hx = r * cs, hy = r * sn;
+    const float om = 1.0f - u;
+    const float hz = sqrtf(om > 0.0f ? om : 0.0f);
+    return v3(T.x * hx + B.x * hy + N.x * hz,
+              T.y * hx + B.y * hy + N.y * hz,
+              T.z * hx + B.z * hy + N.z * hz);
+}
+
 // Writes the shadow and AO rays of active pixel a (pixel p of a W-wide
 // image) as ray (slot * A + a), slot-major, into the structure-of-arrays
 // buffers ray_o (origin, tmin) and ray_d (direction, tmax).
@@ -90,59 +156,62 @@ __device__ inline void raygen_pixel(const float* position, const float* normal, 
                                     const DevScene* ds, uint32_t p, uint32_t a, uint32_t A,
                                     float4* ray_o, float4* ray_d) {
     const luzrt_scene& S = ds->s;
-    uint32_t x = p % W, y = p / W;
-    V3 P = v3(position[(size_t)p * 3], position[(size_t)p * 3 + 1], position[(size_t)p * 3 + 2]);
-    V3 N = v3(normal[(size_t)p * 3], normal[(size_t)p * 3 + 1], normal[(size_t)p * 3 + 2]);
-    V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
-    float lc = vlen(vsub(P, cam));
-    float sb = fmax_(lc * 0.01f, 0.05f);
-    V3 so = vadd(P, vmul(N, sb));
+    const PixelFrame f = pixel_frame(position, normal, W, S, p);
+    const V3 so = shadow_origin(f);
     const uint32_t frame = (uint32_t)S.frame;
     for (int l = 0; l < S.num_lights; ++l) {
         const luzrt_light& L = S.lights[l];
         int n = L.num_shadow_samples;
         if (n <= 0) continue;
-        ShadowSetup ss = shadow_setup(L, P);
-        uint32_t j0 = ds->slot_base[l];
+        const ShadowSetup ss = shadow_setup(L, f.P);
+        const uint32_t j0 = ds->slot_base[l];
         for (int s = 0; s < n; ++s) {
-            float u, v;
-            sample2(x, y, frame, (uint32_t)(l * 256 + s), u, v);
-            float pr = L.radius * sqrtf(u);
-            float sn, cs;
-            sincos_turns(v, sn, cs);
-            float dx = pr * cs, dy = pr * sn;
-            V3 d = v3(ss.Lvec.x + dx * ss.T.x + dy * ss.B.x,
-                      ss.Lvec.y + dx * ss.T.y + dy * ss.B.y,
-                      ss.Lvec.z + dx * ss.T.z + dy * ss.B.z);
-            d = vnorm(d);
-            size_t idx = (size_t)(j0 + s) * A + a;
+            const V3 d = shadow_dir(L, ss, f, frame, l, s);
+            const size_t idx = (size_t)(j0 + s) * A + a;
             ray_o[idx] = make_float4(so.x, so.y, so.z, 0.001f);
             ray_d[idx] = make_float4(d.x, d.y, d.z, ss.tmax);
         }
     }
-    int na = S.ao_num_samples;
+    const int na = S.ao_num_samples;
     if (na > 0) {
-        float ab = lc * 0.01f;
-        V3 ao = vadd(P, vmul(N, ab));
-        V3 T = fabsf(N.z) > 0.5f ? v3(0.0f, -N.z, N.y) : v3(-N.y, N.x, 0.0f);
-        V3 B = vcross(N, T);
-        uint32_t j0 = ds->r_shadow;
+        const V3 ao = ao_origin(f);
+        const uint32_t j0 = ds->r_shadow;
         for (int s = 0; s < na; ++s) {
-            float u, v;
-            sample2(x, y, frame, (uint32_t)(16384 + s), u, v);
-            float r = sqrtf(u);
-            float sn, cs;
-            sincos_turns(v * kHemiTurns, sn, cs);
-            float hx = r * cs, hy = r * sn;
-            float om = 1.0f - u;
-            float hz = sqrtf(om > 0.0f ? om : 0.0f);
-            V3 d = v3(T.x * hx + B.x * hy + N.x * hz,
-                      T.y * hx + B.y * hy + N.y * hz,
-                      T.z * hx + B.z * hy + N.z * hz);
-            size_t idx = (size_t)(j0 + s) * A + a;
+            const V3 d = ao_dir(f, frame, s);
+            const size_t idx = (size_t)(j0 + s) * A + a;
             ray_o[idx] = make_float4(ao.x, ao.y, ao.z, S.ao_min);
             ray_d[idx] = make_float4(d.x, d.y, d.z, S.ao_max);
         }
+    }
+}
+
+// Ray j * A + a of a visibility launch computed in place, the same values
+// raygen_pixel writes for it: slot j < r_shadow is shadow sample
+// j - slot_base[l] of the last light l with slot_base[l] <= j (lights without
+// samples own no slots), and slot j >= r_shadow is AO sample j - r_shadow.
+__device__ inline void gen_ray(const float* position, const float* normal, uint32_t W,
+                               const DevScene* ds, uint32_t p, uint32_t j, V3& o, V3& d, float& tmin,
+                               float& tmax) {
+    const luzrt_scene& S = ds->s;
+    const PixelFrame f = pixel_frame(position, normal, W, S, p);
+    const uint32_t frame = (uint32_t)S.frame;
+    if (j < ds->r_shadow) {
+        int lo = 0, hi = S.num_lights;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (ds->slot_base[mid] <= j) lo = mid; else hi = mid;
+        }
+        const luzrt_light& L = S.lights[lo];
+        const ShadowSetup ss = shadow_setup(L, f.P);
+        o = shadow_origin(f);
+        d = shadow_dir(L, ss, f, frame, lo, (int)(j - ds->slot_base[lo]));
+        tmin = 0.001f;
+        tmax = ss.tmax;
+    } else {
+        o = ao_origin(f);
+        d = ao_dir(f, frame, (int)(j - ds->r_shadow));
+        tmin = S.ao_min;
+        tmax = S.ao_max;
     }
 }
 

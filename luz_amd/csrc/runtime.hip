@@ -218,7 +218,7 @@ luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->rays_o, &ctx->rays_d, &ctx->hits, &ctx->active, &ctx->act_index,
+    DevBuf* bufs[] = {&ctx->hits, &ctx->active, &ctx->act_index,
                       &ctx->counters, &ctx->spill, &ctx->lut, &ctx->scene, &ctx->vis_shadow,
                       &ctx->vis_ao};
     for (DevBuf* b : bufs)

@@ -2,16 +2,19 @@
 //
 // Replaces the ray queries of light.frag (TraceShadowRay :86-109,
 // TraceAORays :111-135, EvaluateShadow :137-146, AO origin :229-231) with a
-// wavefront pipeline over coalesced structure-of-arrays buffers:
+// persistent traversal over the active pixels:
 //   k_vis_init  zero the call's device counters (graph-safe, no memset);
-//   k_compact_* non-background pixels in 8x8-tile order -> active list
+//   k_compact_* non-background pixels in 64-pixel-tile order -> active list
 //               (count, scan, write: deterministic, no atomics);
-//   k_raygen    one thread per active pixel writes its shadow and AO rays,
-//               slot-major (ray j*A + a), so a wave holds 64 neighbouring
-//               pixels of the same light sample or AO sample;
+//   k_gen_count the ray count, A active pixels x slots;
 //   k_trace_any persistent any-hit traversal: a wave refills its idle lanes
 //               with one atomic when at least `refill` lanes are idle
-//               (ballot-based compaction of the wave's work);
+//               (ballot-based compaction of the wave's work); ray j*A + a
+//               (slot-major, so a wave holds neighbouring pixels of the same
+//               light sample or AO sample) is generated at claim time from
+//               the G-buffer texel of active pixel a (gen_ray), so no ray
+//               records go through memory (ray records remain for
+//               luzrt_trace_rays and batched frames);
 //   k_resolve   per pixel, occluded shadow samples per light and unoccluded
 //               AO samples, written as u8 planes.
 #include "internal.hpp"
@@ -82,16 +85,9 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const float* normal, u
     if (inside) act_index[p] = active ? a : kNone;
 }
 
-__global__ __launch_bounds__(kBlock) void k_raygen(const float* position, const float* normal,
-                                                   uint32_t W, const DevScene* ds,
-                                                   const uint32_t* act_list,
-                                                   uint32_t* counters, float4* ray_o,
-                                                   float4* ray_d) {
-    const uint32_t A = counters[kCntActive];
-    uint32_t a = blockIdx.x * kBlock + threadIdx.x;
-    if (a == 0) counters[kCntRays] = A * ds->r_total;   // read by the trace kernel
-    if (a >= A) return;
-    raygen_pixel(position, normal, W, ds, act_list[a], a, A, ray_o, ray_d);
+// Ray count of a launch whose rays are generated in place.
+__global__ void k_gen_count(const DevScene* ds, uint32_t* counters) {
+    if (threadIdx.x == 0) counters[kCntRays] = counters[kCntActive] * ds->r_total;
 }
 
 // Rays are claimed in chunks per wave (one device-scope atomic on a single
@@ -122,11 +118,17 @@ __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32
     return lo;
 }
 
-// BATCH: rays of many scenes in one launch, each traced against its scene's
-// TLAS (found from the ray index at claim time; the per-lane TLAS pointers
-// then live in vector registers).
-template <bool WIDE, int SHORT, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t refill) {
+// Where a launch's rays come from.  kRays: ray records (origin, tmin;
+// direction, tmax).  kBatch: ray records of many scenes in one launch, each
+// traced against its scene's TLAS (found from the ray index at claim time;
+// the per-lane TLAS pointers then live in vector registers).  kGen: computed
+// at claim time from the G-buffer and the scene block (gen_ray), the values
+// k_raygen would have written.
+enum RaySource { kRays = 0, kBatch = 1, kGen = 2 };
+
+template <bool WIDE, int SHORT, int SRC>
+__global__ __launch_bounds__(kBlock, (WIDE || SHORT > 16) ? 1 : 5) void k_trace_any(TraceLaunch L, uint32_t refill) {
+    constexpr bool BATCH = SRC == kBatch;
     __shared__ uint32_t lds[SHORT * kBlock];
     __shared__ uint32_t sbase[BATCH ? kMaxBatch + 1 : 1];
     // A small TLAS (nodes + instance records, 64 B each) is staged in LDS: every
@@ -162,6 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
     st.sp = 0;
     st.lo = 0;
     const uint32_t nrays = L.counters[kCntRays];
+    const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // rays per slot
     const uint32_t chunk = claim_chunk(nrays, total / 64u);
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     RayState r;
@@ -198,9 +201,18 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(TraceLaunch L, uint32_t re
                             }
                         }
                         if (ray != kNone) {
-                            float4 o = L.ray_o[(size_t)rid * L.ray_stride];
-                            float4 d = L.ray_d[(size_t)rid * L.ray_stride];
-                            ray_begin(r, st, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, d.w);
+                            if (SRC == kGen) {
+                                const uint32_t slot = rid / A, a = rid - slot * A;
+                                V3 o, d;
+                                float tmin, tmax;
+                                gen_ray(L.position, L.normal, L.width, L.ds, L.act_list[a], slot, o, d,
+                                        tmin, tmax);
+                                ray_begin(r, st, o, d, tmin, tmax);
+                            } else {
+                                float4 o = L.ray_o[(size_t)rid * L.ray_stride];
+                                float4 d = L.ray_d[(size_t)rid * L.ray_stride];
+                                ray_begin(r, st, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, d.w);
+                            }
                         }
                     }
                 }
@@ -253,17 +265,27 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
         if (L.wide || L.nscenes > kMaxBatch)
             return fail(ctx, LUZRT_E_UNSUPPORTED, "multi-scene traversal: binary BVHs, <= %u scenes",
                         kMaxBatch);
-        if (sd <= 8) launch(k_trace_any<false, 8, true>);
-        else if (sd <= 16) launch(k_trace_any<false, 16, true>);
-        else launch(k_trace_any<false, 32, true>);
+        if (sd <= 8) launch(k_trace_any<false, 8, kBatch>);
+        else if (sd <= 16) launch(k_trace_any<false, 16, kBatch>);
+        else launch(k_trace_any<false, 32, kBatch>);
+    } else if (L.gen) {
+        if (L.wide) {
+            if (sd <= 8) launch(k_trace_any<true, 8, kGen>);
+            else if (sd <= 16) launch(k_trace_any<true, 16, kGen>);
+            else launch(k_trace_any<true, 32, kGen>);
+        } else {
+            if (sd <= 8) launch(k_trace_any<false, 8, kGen>);
+            else if (sd <= 16) launch(k_trace_any<false, 16, kGen>);
+            else launch(k_trace_any<false, 32, kGen>);
+        }
     } else if (L.wide) {
-        if (sd <= 8) launch(k_trace_any<true, 8, false>);
-        else if (sd <= 16) launch(k_trace_any<true, 16, false>);
-        else launch(k_trace_any<true, 32, false>);
+        if (sd <= 8) launch(k_trace_any<true, 8, kRays>);
+        else if (sd <= 16) launch(k_trace_any<true, 16, kRays>);
+        else launch(k_trace_any<true, 32, kRays>);
     } else {
-        if (sd <= 8) launch(k_trace_any<false, 8, false>);
-        else if (sd <= 16) launch(k_trace_any<false, 16, false>);
-        else launch(k_trace_any<false, 32, false>);
+        if (sd <= 8) launch(k_trace_any<false, 8, kRays>);
+        else if (sd <= 16) launch(k_trace_any<false, 16, kRays>);
+        else launch(k_trace_any<false, 32, kRays>);
     }
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
@@ -288,8 +310,6 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
     const size_t ntiles_px_sz = (size_t)((W + tw - 1) / tw) * ((H + th - 1) / th) * 64;
     if ((s = ensure(ctx, ctx->act_index, npix * 4 + (ntiles_px_sz / kBlock + 2) * 4)) != LUZRT_OK)
         return s;
-    if ((s = ensure(ctx, ctx->rays_o, max_rays * 16 + 16)) != LUZRT_OK) return s;
-    if ((s = ensure(ctx, ctx->rays_d, max_rays * 16 + 16)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->hits, max_rays + 16)) != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
     const DevScene* dsd = (const DevScene*)ctx->scene.ptr;
@@ -308,12 +328,9 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
+    // rays are generated inside the traversal; this stage only sets their count
     stage_begin(ctx, kStageRaygen);
-    if (ds.r_total > 0)
-        k_raygen<<<cdiv(npix, kBlock), kBlock, 0, st>>>(g->position, g->normal, W, dsd,
-                                                        (const uint32_t*)ctx->active.ptr, counters,
-                                                        (float4*)ctx->rays_o.ptr,
-                                                        (float4*)ctx->rays_d.ptr);
+    if (ds.r_total > 0) k_gen_count<<<1, 64, 0, st>>>(dsd, counters);
     stage_end(ctx, kStageRaygen);
     LUZ_HIP(ctx, hipGetLastError());
 
@@ -324,12 +341,15 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
         L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
         L.tlas_nodes_count = tlas->node_count;
         L.recs_count = tlas->count;
-        L.ray_o = (const float4*)ctx->rays_o.ptr;
-        L.ray_d = (const float4*)ctx->rays_d.ptr;
-        L.ray_stride = 1;
         L.hits = (uint8_t*)ctx->hits.ptr;
         L.counters = counters;
         L.wide = tlas->wide;
+        L.gen = true;
+        L.ds = dsd;
+        L.position = g->position;
+        L.normal = g->normal;
+        L.act_list = (const uint32_t*)ctx->active.ptr;
+        L.width = W;
         if ((s = launch_trace(ctx, L)) != LUZRT_OK) return s;
     } else if (ds.r_total > 0) {
         // empty TLAS: nothing can be hit
