@@ -7,9 +7,10 @@
 //   k_tlas_build_batched  one workgroup per scene rebuilds its TLAS
 //   k_bprimary        persistent closest-hit G-buffer of every scene with a camera
 //   k_bcompact_*      per-scene non-background pixel lists (flat grid of tile blocks)
-//   k_bray_base       per-scene ray ranges in one global ray buffer
-//   k_braygen         shadow and AO rays of every scene (slot-major per scene)
-//   k_trace_any<..., BATCH>  ONE persistent traversal over the rays of all scenes
+//   k_bray_base       per-scene ray ranges in one global ray index space
+//   k_trace_any<..., kBatch>  ONE persistent traversal over the rays of all
+//                     scenes, each generated at claim time from its scene's
+//                     G-buffer (slot-major per scene)
 //   k_bresolve, k_bshade     per-pixel counts and FP32 GGX radiance
 // Flat grids find their scene from per-scene block bases (binary search on
 // values that are uniform across the workgroup).  The sequence is captured
@@ -183,17 +184,6 @@ __global__ __launch_bounds__(kBlock) void k_bcompact_write(BatchTables T, uint32
     if (inside) S.act_index[p] = active ? a : kNone;
 }
 
-__global__ __launch_bounds__(kBlock) void k_braygen(BatchTables T, float4* ray_o, float4* ray_d) {
-    const uint32_t s = find_scene(T.pblk, T.n, blockIdx.x);
-    const BatchSceneDev& S = T.sc[s];
-    const uint32_t A = T.active[s];
-    const uint32_t a = (blockIdx.x - T.pblk[s]) * kBlock + threadIdx.x;
-    if (a >= A) return;
-    const size_t base = T.ray_base[s];
-    raygen_pixel(S.g.position, S.g.normal, S.g.width, S.ds, S.act_list[a], a, A, ray_o + base,
-                 ray_d + base);
-}
-
 __global__ __launch_bounds__(kBlock) void k_bresolve(BatchTables T, const uint8_t* hits) {
     const uint32_t s = find_scene(T.pblk, T.n, blockIdx.x);
     const BatchSceneDev& S = T.sc[s];
@@ -247,7 +237,7 @@ struct luzrt_batch {
     // rays
     uint64_t max_rays = 0;
     uint32_t total_cblk = 0, total_pblk = 0, total_ppx = 0;
-    DevBuf rays_o, rays_d, hits;
+    DevBuf hits;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -288,6 +278,12 @@ luzrt_status upload_tables(luzrt_ctx* ctx, luzrt_batch* b) {
         b->max_m = std::max(b->max_m, jobs[s].m);
         bt[s].nodes = b->inst_n[s] ? t->nodes : nullptr;
         bt[s].recs = t->recs;
+        bt[s].position = b->sc[s].g.position;
+        bt[s].normal = b->sc[s].g.normal;
+        bt[s].ds = b->sc[s].ds;
+        bt[s].act_list = b->sc[s].act_list;
+        bt[s].width = b->sc[s].g.width;
+        bt[s].pad = 0;
     }
     LUZ_HIP(ctx, hipMemcpyAsync(b->dev_tlas, bt.data(), n * sizeof(BatchTlas), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -353,19 +349,15 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     k_bcompact_scan<<<b->n, 1024, 0, st>>>(b->tables);
     k_bray_base<<<1, 64, 0, st>>>(b->tables, b->counters);
     k_bcompact_write<<<b->total_cblk, kBlock, 0, st>>>(b->tables, tw);
-    k_braygen<<<b->total_pblk, kBlock, 0, st>>>(b->tables, (float4*)b->rays_o.ptr,
-                                                (float4*)b->rays_d.ptr);
     LUZ_HIP(ctx, hipGetLastError());
     TraceLaunch L{};
-    L.ray_o = (const float4*)b->rays_o.ptr;
-    L.ray_d = (const float4*)b->rays_d.ptr;
-    L.ray_stride = 1;
     L.hits = (uint8_t*)b->hits.ptr;
     L.counters = b->counters;
     L.wide = false;
     L.nscenes = b->n;
     L.scene_ray_base = b->tables.ray_base;
     L.scene_tlas = b->dev_tlas;
+    L.scene_active = b->tables.active;
     if ((s = launch_trace(ctx, L)) != LUZRT_OK) return s;
     k_bresolve<<<b->total_pblk, kBlock, 0, st>>>(b->tables, (const uint8_t*)b->hits.ptr);
     k_bshade<<<b->total_pblk, kBlock, 0, st>>>(b->tables, (const float*)ctx->lut.ptr);
@@ -381,8 +373,6 @@ luzrt_status prepare_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     uint32_t groups = (uint32_t)ctx->num_cus * std::max<uint32_t>(ctx->groups_per_cu, 4u);
     if (groups == 0) groups = 1;
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * kBlock * kSpill * 4)) != LUZRT_OK) return s;
-    if ((s = ensure(ctx, b->rays_o, b->max_rays * 16 + 16)) != LUZRT_OK) return s;
-    if ((s = ensure(ctx, b->rays_d, b->max_rays * 16 + 16)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, b->hits, b->max_rays + 16)) != LUZRT_OK) return s;
     return LUZRT_OK;
 }
@@ -415,7 +405,7 @@ void batch_free(luzrt_batch* b) {
     if (b->host_stage) (void)hipHostFree(b->host_stage);
     for (void* p : {(void*)b->dev_stage, (void*)b->dev_work, (void*)b->dev_tables, (void*)b->dev_mats,
                     (void*)b->dev_tlas, (void*)b->dev_jobs, (void*)b->dev_refits,
-                    b->rays_o.ptr, b->rays_d.ptr, b->hits.ptr})
+                    b->hits.ptr})
         if (p) (void)hipFree(p);
     delete b;
 }

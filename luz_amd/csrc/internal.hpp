@@ -201,12 +201,15 @@ struct TraceLaunch {
     uint32_t* spill;       // per-lane overflow stack
     bool wide;             // instance records point to 8-wide BLASes
     // Multi-scene launch (nscenes > 0): ray i belongs to the scene s with
-    // scene_ray_base[s] <= i < scene_ray_base[s + 1] and is traced against
-    // scene_tlas[s]; tlas_nodes and recs are then unused.
+    // scene_ray_base[s] <= i < scene_ray_base[s + 1], is generated from that
+    // scene's G-buffer (its ray i - scene_ray_base[s], with
+    // A = scene_active[s]) and traced against scene_tlas[s]; tlas_nodes,
+    // recs and the ray records are then unused.
     uint32_t nscenes;
     uint32_t tlas_nodes_count, recs_count;   // single-scene TLAS size (staged in LDS when small)
     const uint32_t* scene_ray_base;   // device, nscenes + 1 entries
     const struct BatchTlas* scene_tlas;
+    const uint32_t* scene_active;     // device, nscenes entries
     // Visibility launch with rays generated in place (gen): ray j * A + a is
     // computed from active pixel a of the G-buffer and slot j of the scene
     // block (A = counters[kCntActive]); ray_o and ray_d are then unused.
@@ -218,10 +221,17 @@ struct TraceLaunch {
     uint32_t width;
 };
 
-// TLAS of one scene of a multi-scene launch (nodes null: empty TLAS).
+// One scene of a multi-scene launch: its TLAS (nodes null: empty TLAS) and
+// the G-buffer, scene block and active-pixel list its rays are generated from.
 struct BatchTlas {
     const Node* nodes;
     const InstRec* recs;
+    const float* position;
+    const float* normal;
+    const struct DevScene* ds;
+    const uint32_t* act_list;
+    uint32_t width;
+    uint32_t pad;
 };
 constexpr uint32_t kMaxBatch = LUZRT_MAX_BATCH;
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);

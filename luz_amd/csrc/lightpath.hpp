@@ -3,7 +3,7 @@
 // kernels (batch.hip), so both evaluate identical arithmetic.
 //
 //   tile_pixel / pixel_active / block_prefix  compaction of non-background pixels
-//   shadow_setup / raygen_pixel / gen_ray     light.frag:86-146, 111-135, 229-231
+//   shadow_setup / gen_ray                    light.frag:86-146, 111-135, 229-231
 //   resolve_pixel                             occluded / unoccluded sample counts
 //   shade_pixel                               light.frag:17-49, 171-235 (FP32)
 //   primary_pixel / primary_texel             G-buffer texels by closest-hit primary rays
@@ -149,46 +149,13 @@ __device__ inline V3 ao_dir(const PixelFrame& f, uint32_t frame, int s) {
               T.z * hx + B.z * hy + N.z * hz);
 }
 
-// Writes the shadow and AO rays of active pixel a (pixel p of a W-wide
-// image) as ray (slot * A + a), slot-major, into the structure-of-arrays
-// buffers ray_o (origin, tmin) and ray_d (direction, tmax).
-__device__ inline void raygen_pixel(const float* position, const float* normal, uint32_t W,
-                                    const DevScene* ds, uint32_t p, uint32_t a, uint32_t A,
-                                    float4* ray_o, float4* ray_d) {
-    const luzrt_scene& S = ds->s;
-    const PixelFrame f = pixel_frame(position, normal, W, S, p);
-    const V3 so = shadow_origin(f);
-    const uint32_t frame = (uint32_t)S.frame;
-    for (int l = 0; l < S.num_lights; ++l) {
-        const luzrt_light& L = S.lights[l];
-        int n = L.num_shadow_samples;
-        if (n <= 0) continue;
-        const ShadowSetup ss = shadow_setup(L, f.P);
-        const uint32_t j0 = ds->slot_base[l];
-        for (int s = 0; s < n; ++s) {
-            const V3 d = shadow_dir(L, ss, f, frame, l, s);
-            const size_t idx = (size_t)(j0 + s) * A + a;
-            ray_o[idx] = make_float4(so.x, so.y, so.z, 0.001f);
-            ray_d[idx] = make_float4(d.x, d.y, d.z, ss.tmax);
-        }
-    }
-    const int na = S.ao_num_samples;
-    if (na > 0) {
-        const V3 ao = ao_origin(f);
-        const uint32_t j0 = ds->r_shadow;
-        for (int s = 0; s < na; ++s) {
-            const V3 d = ao_dir(f, frame, s);
-            const size_t idx = (size_t)(j0 + s) * A + a;
-            ray_o[idx] = make_float4(ao.x, ao.y, ao.z, S.ao_min);
-            ray_d[idx] = make_float4(d.x, d.y, d.z, S.ao_max);
-        }
-    }
-}
-
-// Ray j * A + a of a visibility launch computed in place, the same values
-// raygen_pixel writes for it: slot j < r_shadow is shadow sample
-// j - slot_base[l] of the last light l with slot_base[l] <= j (lights without
-// samples own no slots), and slot j >= r_shadow is AO sample j - r_shadow.
+// Ray j * A + a of a visibility launch over A active pixels (slot-major):
+// the ray of slot j from active pixel a (G-buffer pixel p).  Slot j < r_shadow
+// is shadow sample j - slot_base[l] of the last light l with
+// slot_base[l] <= j (lights without samples own no slots); slot j >= r_shadow
+// is AO sample j - r_shadow.  Shadow rays start at shadow_origin with
+// tmin 0.001 and end at the light (tmax = |light vector|); AO rays span
+// [ao_min, ao_max] from ao_origin.
 __device__ inline void gen_ray(const float* position, const float* normal, uint32_t W,
                                const DevScene* ds, uint32_t p, uint32_t j, V3& o, V3& d, float& tmin,
                                float& tmax) {

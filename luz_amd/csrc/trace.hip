@@ -119,11 +119,11 @@ __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32
 }
 
 // Where a launch's rays come from.  kRays: ray records (origin, tmin;
-// direction, tmax).  kBatch: ray records of many scenes in one launch, each
-// traced against its scene's TLAS (found from the ray index at claim time;
-// the per-lane TLAS pointers then live in vector registers).  kGen: computed
-// at claim time from the G-buffer and the scene block (gen_ray), the values
-// k_raygen would have written.
+// direction, tmax).  kGen: computed at claim time from the G-buffer and the
+// scene block (gen_ray).  kBatch: likewise for many scenes in one launch,
+// each ray generated from and traced against its own scene (found from the
+// ray index at claim time; the per-lane TLAS pointers then live in vector
+// registers).
 enum RaySource { kRays = 0, kBatch = 1, kGen = 2 };
 
 template <bool WIDE, int SHORT, int SRC>
@@ -192,15 +192,23 @@ __global__ __launch_bounds__(kBlock, (WIDE || SHORT > 16) ? 1 : 5) void k_trace_
                         uint32_t rid = chunk_next + rank;
                         ray = rid;
                         if (BATCH) {
-                            const BatchTlas bt = L.scene_tlas[scene_of_ray(sbase, L.nscenes, rid)];
+                            const uint32_t sc = scene_of_ray(sbase, L.nscenes, rid);
+                            const BatchTlas& bt = L.scene_tlas[sc];
                             tnodes = bt.nodes;
                             trecs = bt.recs;
                             if (!tnodes) {   // empty scene: nothing can be hit
                                 L.hits[rid] = 0;
                                 ray = kNone;
+                            } else {
+                                const uint32_t As = L.scene_active[sc], local = rid - sbase[sc];
+                                const uint32_t slot = local / As, a = local - slot * As;
+                                V3 o, d;
+                                float tmin, tmax;
+                                gen_ray(bt.position, bt.normal, bt.width, bt.ds, bt.act_list[a], slot, o,
+                                        d, tmin, tmax);
+                                ray_begin(r, st, o, d, tmin, tmax);
                             }
-                        }
-                        if (ray != kNone) {
+                        } else {
                             if (SRC == kGen) {
                                 const uint32_t slot = rid / A, a = rid - slot * A;
                                 V3 o, d;
