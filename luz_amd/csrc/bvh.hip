@@ -647,6 +647,66 @@ __device__ inline float merge_cost(const float* a, const float* tile, int plane,
     return c == c ? c : INFINITY;
 }
 
+// Nearest neighbour of cluster i among clusters i - kPlocRadius .. i + kPlocRadius
+// (within 0 .. count - 1), from a tile that holds cluster c at c - base +
+// kPlocRadius: least merge cost, the smaller index on equal cost.
+__device__ inline uint32_t ploc_nearest(const float* tile, uint32_t i, uint32_t base, uint32_t count) {
+    const int ti = (int)(i - base) + kPlocRadius;
+    float box[6];
+    for (int k = 0; k < 6; ++k) box[k] = tile[k * kPlocTile + ti];
+    const int j0 = (int)i - kPlocRadius < 0 ? 0 : (int)i - kPlocRadius;
+    const int j1 = (int)i + kPlocRadius > (int)count - 1 ? (int)count - 1 : (int)i + kPlocRadius;
+    float best = INFINITY;
+    uint32_t bj = kNone;
+    for (int j = j0; j <= j1; ++j) {
+        if (j == (int)i) continue;
+        const float c = merge_cost(box, tile, kPlocTile, j - (int)base + kPlocRadius);
+        if (bj == kNone || c < best) { best = c; bj = (uint32_t)j; }
+    }
+    return bj;
+}
+
+// Exclusive ranks of this thread's head and keep flags among the workgroup's
+// threads (thread order), and their totals.  All threads must call it.
+__device__ inline void block_ranks(bool head, bool keep, uint32_t (*s_wave)[kTlasThreads / 64],
+                                   uint32_t& hr, uint32_t& kr, uint32_t& th, uint32_t& tk) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint64_t bh = __ballot(head), bk = __ballot(keep);
+    if (lane == 0) {
+        s_wave[0][wave] = (uint32_t)__popcll(bh);
+        s_wave[1][wave] = (uint32_t)__popcll(bk);
+    }
+    __syncthreads();
+    uint32_t oh = 0, ok = 0;
+    th = 0;
+    tk = 0;
+    for (uint32_t w = 0; w < kTlasThreads / 64; ++w) {
+        const uint32_t ch = s_wave[0][w], ck = s_wave[1][w];
+        if (w < wave) { oh += ch; ok += ck; }
+        th += ch;
+        tk += ck;
+    }
+    hr = oh + (uint32_t)__popcll(bh & below);
+    kr = ok + (uint32_t)__popcll(bk & below);
+}
+
+// Node idx joining clusters a (child 0) and b (child 1); the children that are
+// nodes get idx as their parent.
+__device__ inline void ploc_node(Node* nodes, uint32_t idx, const float* a, const float* b, uint32_t ra,
+                                 uint32_t rb) {
+    Node nd;
+    store_child_box(&nd, 0, a);
+    store_child_box(&nd, 1, b);
+    nd.child[0] = ra;
+    nd.child[1] = rb;
+    nd.parent = kNone;
+    nd.pad = 0;
+    nodes[idx] = nd;
+    if (!(ra & kLeaf)) nodes[ra].parent = idx;
+    if (!(rb & kLeaf)) nodes[rb].parent = idx;
+}
+
 __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     const InstIn* in = J.in;
     const uint32_t n = J.n, m = J.m;
@@ -799,12 +859,13 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         // position, and the survivors are compacted in order.  New nodes are
         // numbered downwards from n - 2 in merge order, so the root is node 0.
         float* tile = reinterpret_cast<float*>(skeys);          // 6 planes of kPlocTile boxes
-        uint32_t* snn = reinterpret_cast<uint32_t*>(tile + 6 * kPlocTile);
+        uint32_t* snn = reinterpret_cast<uint32_t*>(tile + 6 * kPlocTile);   // m entries
+        uint32_t* sref = snn + m;                                              // kTlasThreads entries
         __shared__ uint32_t s_wave[2][kTlasThreads / 64];
-        const uint32_t lane = tid & 63u, wave = tid >> 6;
-        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
         uint32_t count = n, next = n - 1;
-        while (count > 1) {
+        bool failed = false;
+        // rounds over more clusters than threads: cluster arrays in global memory
+        while (count > kTlasThreads && !failed) {
             for (uint32_t base = 0; base < count; base += kTlasThreads) {
                 for (int t = tid; t < kPlocTile; t += kTlasThreads) {
                     const int c = (int)base - kPlocRadius + t;
@@ -813,21 +874,7 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
                 }
                 __syncthreads();
                 const uint32_t i = base + tid;
-                if (i < count) {
-                    const int ti = tid + kPlocRadius;
-                    float box[6];
-                    for (int k = 0; k < 6; ++k) box[k] = tile[k * kPlocTile + ti];
-                    const int j0 = (int)i - kPlocRadius < 0 ? 0 : (int)i - kPlocRadius;
-                    const int j1 = (int)i + kPlocRadius > (int)count - 1 ? (int)count - 1 : (int)i + kPlocRadius;
-                    float best = INFINITY;
-                    uint32_t bj = kNone;
-                    for (int j = j0; j <= j1; ++j) {
-                        if (j == (int)i) continue;
-                        const float c = merge_cost(box, tile, kPlocTile, j - (int)base + kPlocRadius);
-                        if (bj == kNone || c < best) { best = c; bj = (uint32_t)j; }
-                    }
-                    snn[i] = bj;
-                }
+                if (i < count) snn[i] = ploc_nearest(tile, i, base, count);
                 __syncthreads();
             }
             uint32_t heads = 0, kept = 0;
@@ -841,33 +888,13 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
                     head = mutual && i < j;
                     keep = !(mutual && i > j);
                 }
-                const uint64_t bh = __ballot(head), bk = __ballot(keep);
-                if (lane == 0) {
-                    s_wave[0][wave] = (uint32_t)__popcll(bh);
-                    s_wave[1][wave] = (uint32_t)__popcll(bk);
-                }
-                __syncthreads();
-                uint32_t oh = 0, ok = 0, th = 0, tk = 0;
-                for (uint32_t w = 0; w < kTlasThreads / 64; ++w) {
-                    const uint32_t ch = s_wave[0][w], ck = s_wave[1][w];
-                    if (w < wave) { oh += ch; ok += ck; }
-                    th += ch;
-                    tk += ck;
-                }
-                const uint32_t hr = heads + oh + (uint32_t)__popcll(bh & below);
-                const uint32_t kr = kept + ok + (uint32_t)__popcll(bk & below);
+                uint32_t hr, kr, th, tk;
+                block_ranks(head, keep, s_wave, hr, kr, th, tk);
+                hr += heads;
+                kr += kept;
                 if (head) {
                     const uint32_t idx = next - 1 - hr;
-                    Node nd;
-                    store_child_box(&nd, 0, cb + (size_t)i * 6);
-                    store_child_box(&nd, 1, cb + (size_t)j * 6);
-                    nd.child[0] = cr[i];
-                    nd.child[1] = cr[j];
-                    nd.parent = kNone;
-                    nd.pad = 0;
-                    nodes[idx] = nd;
-                    if (!(nd.child[0] & kLeaf)) nodes[nd.child[0]].parent = idx;
-                    if (!(nd.child[1] & kLeaf)) nodes[nd.child[1]].parent = idx;
+                    ploc_node(nodes, idx, cb + (size_t)i * 6, cb + (size_t)j * 6, cr[i], cr[j]);
                     cr2[kr] = idx;
                     for (int k = 0; k < 3; ++k) {
                         cb2[kr * 6 + k] = fmin_(cb[i * 6 + k], cb[j * 6 + k]);
@@ -881,15 +908,64 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
                 kept += tk;
                 __syncthreads();
             }
-            if (heads == 0) {   // cannot happen: the least-cost pair is mutual
-                if (tid == 0) atomicOr(error, 1u);
-                break;
-            }
+            failed = heads == 0;   // cannot happen: the least-cost pair is mutual
             count = kept;
             next -= heads;
             float* tb = cb; cb = cb2; cb2 = tb;
             uint32_t* tr = cr; cr = cr2; cr2 = tr;
         }
+        // rounds over at most one cluster per thread: cluster i lives in thread
+        // i's registers and in the LDS tile (box at i + kPlocRadius) and sref
+        float box[6];
+        uint32_t ref = 0;
+        if (tid < count) {
+            for (int k = 0; k < 6; ++k) {
+                box[k] = cb[tid * 6 + k];
+                tile[k * kPlocTile + tid + kPlocRadius] = box[k];
+            }
+            ref = cr[tid];
+            sref[tid] = ref;
+        }
+        __syncthreads();
+        while (count > 1 && !failed) {
+            if (tid < count) snn[tid] = ploc_nearest(tile, tid, 0, count);
+            __syncthreads();
+            bool head = false, keep = false;
+            uint32_t j = 0;
+            if (tid < count) {
+                j = snn[tid];
+                const bool mutual = snn[j] == tid;
+                head = mutual && tid < j;
+                keep = !(mutual && tid > j);
+            }
+            uint32_t hr, kr, heads, kept;
+            block_ranks(head, keep, s_wave, hr, kr, heads, kept);
+            if (head) {
+                const uint32_t idx = next - 1 - hr;
+                float other[6];
+                for (int k = 0; k < 6; ++k) other[k] = tile[k * kPlocTile + j + kPlocRadius];
+                ploc_node(nodes, idx, box, other, ref, sref[j]);
+                for (int k = 0; k < 3; ++k) {
+                    box[k] = fmin_(box[k], other[k]);
+                    box[3 + k] = fmax_(box[3 + k], other[3 + k]);
+                }
+                ref = idx;
+            }
+            __syncthreads();   // every read of this round's tile and sref is done
+            if (keep) {
+                for (int k = 0; k < 6; ++k) tile[k * kPlocTile + kr + kPlocRadius] = box[k];
+                sref[kr] = ref;
+            }
+            __syncthreads();
+            failed = heads == 0;
+            count = kept;
+            next -= heads;
+            if (tid < count) {
+                for (int k = 0; k < 6; ++k) box[k] = tile[k * kPlocTile + tid + kPlocRadius];
+                ref = sref[tid];
+            }
+        }
+        if (failed && tid == 0) atomicOr(error, 1u);
     } else if (tid == 0) {
         Node nd;
         float empty[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
@@ -1113,7 +1189,7 @@ size_t tlas_scratch_bytes(uint32_t capacity) {
 
 size_t tlas_lds_bytes(uint32_t m) {
     const size_t sort = (size_t)(m > 1024 ? m : 1024) * sizeof(uint64_t);
-    const size_t ploc = (size_t)6 * kPlocTile * sizeof(float) + (size_t)m * sizeof(uint32_t);
+    const size_t ploc = (size_t)6 * kPlocTile * sizeof(float) + ((size_t)m + kTlasThreads) * sizeof(uint32_t);
     return sort > ploc ? sort : ploc;
 }
 
