@@ -91,12 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_bprimary(BatchTables T, uint32_t* sp
     const uint32_t tid = threadIdx.x;
     const uint32_t total = gridDim.x * kBlock;
     Stack st;
-    st.ls = lds + tid;
-    st.lds_stride = kBlock;
-    st.gs = spill + (size_t)blockIdx.x * kBlock + tid;
-    st.g_stride = total;
-    st.sp = 0;
-    st.lo = 0;
+    st.init(lds + tid, kBlock, spill + (size_t)blockIdx.x * kBlock + tid, total);
     const uint32_t end = T.ppx[T.n];
     for (uint32_t gg = blockIdx.x * kBlock + tid; gg < end; gg += total) {
         uint32_t s = find_scene(T.ppx, T.n, gg);

@@ -30,12 +30,7 @@ __global__ __launch_bounds__(kBlock) void k_primary(const Node* tlas_nodes, cons
     const uint32_t tid = threadIdx.x;
     const uint32_t total = gridDim.x * kBlock;
     Stack st;
-    st.ls = lds + tid;
-    st.lds_stride = kBlock;
-    st.gs = spill + (size_t)blockIdx.x * kBlock + tid;
-    st.g_stride = total;
-    st.sp = 0;
-    st.lo = 0;
+    st.init(lds + tid, kBlock, spill + (size_t)blockIdx.x * kBlock + tid, total);
     const uint32_t W = out.width, H = out.height;
     const uint32_t tiles_x = (W + 7) / 8;
     const uint32_t ntiles_px = tiles_x * ((H + 7) / 8) * 64;

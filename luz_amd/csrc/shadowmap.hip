@@ -56,12 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_map(const Node* tlas_nodes, c
     const uint32_t tid = threadIdx.x;
     const uint32_t total = gridDim.x * kBlock;
     Stack st;
-    st.ls = lds + tid;
-    st.lds_stride = kBlock;
-    st.gs = spill + (size_t)blockIdx.x * kBlock + tid;
-    st.g_stride = total;
-    st.sp = 0;
-    st.lo = 0;
+    st.init(lds + tid, kBlock, spill + (size_t)blockIdx.x * kBlock + tid, total);
     const uint32_t res = m.resolution;
     const bool point = m.light_type == LUZRT_LIGHT_POINT;
     const uint32_t n = (point ? 6u : 1u) * res * res;

@@ -157,12 +157,7 @@ __global__ __launch_bounds__(kBlock, (WIDE || SHORT > 16) ? 1 : 5) void k_trace_
     const uint32_t lane = tid & 63u;
     const uint32_t total = gridDim.x * kBlock;
     StackT<SHORT> st;
-    st.ls = lds + tid;
-    st.lds_stride = kBlock;
-    st.gs = L.spill + (size_t)blockIdx.x * kBlock + tid;
-    st.g_stride = total;
-    st.sp = 0;
-    st.lo = 0;
+    st.init(lds + tid, kBlock, L.spill + (size_t)blockIdx.x * kBlock + tid, total);
     const uint32_t nrays = L.counters[kCntRays];
     const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // rays per slot
     const uint32_t chunk = claim_chunk(nrays, total / 64u);

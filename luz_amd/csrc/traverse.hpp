@@ -17,14 +17,30 @@ namespace luz {
 constexpr int kShort = 16;    // default LDS stack entries per lane (power of two)
 constexpr int kSpill = 160;   // global overflow entries per lane (>= TLAS+BLAS depth + 1)
 
-// SHORT: LDS entries per lane (a power of two, chosen by the autotuner).
+typedef uint32_t __attribute__((address_space(3))) LdsWord;
+typedef uint32_t __attribute__((address_space(1))) GlobalWord;
+
+// SHORT: LDS entries per lane (a power of two, chosen by the autotuner).  The
+// two columns carry their address spaces, so a pop reads LDS with an LDS load
+// and only an overflowed entry with a global load (a generic pointer would
+// turn every pop into a flat load that waits on both memory counters).
 template <int SHORT>
 struct StackT {
-    uint32_t* ls;     // this lane's LDS column; entry e at ls[e * lds_stride]
-    uint32_t* gs;     // this lane's spill column; entry e at gs[e * g_stride]
+    LdsWord* ls;      // this lane's LDS column; entry e at ls[e * lds_stride]
+    GlobalWord* gs;   // this lane's spill column; entry e at gs[e * g_stride]
     uint32_t lds_stride;
     uint32_t g_stride;
     int sp, lo;
+
+    __device__ inline void init(uint32_t* lds_column, uint32_t lds_step, uint32_t* spill_column,
+                                uint32_t spill_step) {
+        ls = (LdsWord*)lds_column;
+        gs = (GlobalWord*)spill_column;
+        lds_stride = lds_step;
+        g_stride = spill_step;
+        sp = 0;
+        lo = 0;
+    }
 
     __device__ inline bool push(uint32_t x) {
         if (sp - lo == SHORT) {
@@ -113,8 +129,10 @@ __device__ inline void load_rows(const T* base, uint32_t i, bool lds, float4& a,
                    __float_as_uint(r[3].w));
 }
 
+// Triangle slot of a BLAS (9 floats, global memory: global loads, not flat).
 __device__ inline void load_tri(const float* tris, uint32_t slot, V3& a, V3& b, V3& c) {
-    const float* t = tris + (size_t)slot * 9;
+    typedef const float __attribute__((address_space(1)))* GlobalFloat;
+    const GlobalFloat t = (GlobalFloat)(tris + (size_t)slot * 9);
     a = v3(t[0], t[1], t[2]); b = v3(t[3], t[4], t[5]); c = v3(t[6], t[7], t[8]);
 }
 
