@@ -160,6 +160,16 @@ luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
 luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill_threshold,
                                     uint32_t stack_entries, uint32_t tile_width);
 
+/* Order of a visibility call's rays (a traversal launch parameter, like the
+ * above; no counterpart in the reference, whose ray queries run per
+ * fragment).  SLOT: all pixels' first sample, then all second samples, ...;
+ * GROUPED: the samples of 64 neighbouring pixels together, one sample after
+ * the other; AUTO (default): GROUPED while the distinct BLASes of the TLAS
+ * total at most 32 MB, SLOT above.  Results are the same in every order.
+ * Batched frames use SLOT unless GROUPED is set before luzrt_batch_create. */
+enum { LUZRT_RAY_ORDER_AUTO = 0, LUZRT_RAY_ORDER_SLOT = 1, LUZRT_RAY_ORDER_GROUPED = 2 };
+luzrt_status luzrt_set_ray_order(luzrt_ctx* ctx, uint32_t order);
+
 /* Precision modes.  PLAIN: binary BVH with FP32 boxes, FP32 shading.
  * COMPACT: BLASes traversed as 8-wide nodes with 8-bit child boxes rounded
  * outward (96-byte nodes), FP32 watertight triangle tests, GGX shading with

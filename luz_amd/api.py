@@ -27,6 +27,7 @@ MAX_LIGHTS = 64
 MAX_INSTANCES = 8192
 LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
 SCENE_CUBE, SCENE_SPHERES, SCENE_TERRAIN, SCENE_RANDOM = 1, 2, 4, 5
+RAY_ORDER_AUTO, RAY_ORDER_SLOT, RAY_ORDER_GROUPED = 0, 1, 2
 PRECISION_PLAIN, PRECISION_COMPACT = 0, 1
 SHADOW_RAYTRACED, SHADOW_MAP = 1, 2
 
@@ -133,6 +134,7 @@ _sigs = {
     "luzrt_sync": (C.c_int, [_P]),
     "luzrt_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "luzrt_set_ray_order": (C.c_int, [_P, C.c_uint32]),
     "luzrt_set_precision": (C.c_int, [_P, C.c_int]),
     "luzrt_blas_wide_arrays": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
                                          C.POINTER(_P), C.POINTER(C.c_uint32)]),
@@ -275,6 +277,10 @@ class Context:
     def set_trace_config(self, groups_per_cu: int = 0, refill: int = 0, stack_entries: int = 0,
                          tile_width: int = 0) -> None:
         self._check(lib.luzrt_set_trace_config(self.handle, groups_per_cu, refill, stack_entries, tile_width))
+
+    def set_ray_order(self, order: int) -> None:
+        """RAY_ORDER_AUTO, RAY_ORDER_SLOT or RAY_ORDER_GROUPED (same results, different speed)."""
+        self._check(lib.luzrt_set_ray_order(self.handle, order))
 
     def set_precision(self, mode: int) -> None:
         """PRECISION_PLAIN or PRECISION_COMPACT (8-wide 8-bit BVH, packed-FP16 shading)."""

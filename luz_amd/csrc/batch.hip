@@ -10,7 +10,7 @@
 //   k_bray_base       per-scene ray ranges in one global ray index space
 //   k_trace_any<..., kBatch>  ONE persistent traversal over the rays of all
 //                     scenes, each generated at claim time from its scene's
-//                     G-buffer (slot-major per scene)
+//                     G-buffer (ray_row order per scene)
 //   k_bresolve, k_bshade     per-pixel counts and FP32 GGX radiance
 // Flat grids find their scene from per-scene block bases (binary search on
 // values that are uniform across the workgroup).  The sequence is captured
@@ -435,6 +435,7 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         if (!d.tlas || !d.rgba32f_out || !d.ao_counts || (d.scene.num_lights > 0 && !d.shadow_counts))
             return bad(LUZRT_E_INVALID_ARG, "null TLAS or output");
         if ((st = fill_scene(ctx, &d.scene, &ds[s])) != LUZRT_OK) { batch_free(b); return st; }
+        ds[s].ray_group = ctx->ray_order == LUZRT_RAY_ORDER_GROUPED ? kRayGroup : 0;
         const luzrt_gbuffer& g = d.gbuffer;
         if (!g.position || !g.normal || !g.albedo || !g.material || !g.emission)
             return bad(LUZRT_E_INVALID_ARG, "null G-buffer plane");

@@ -40,6 +40,7 @@ struct luzrt_ctx {
     uint32_t refill_threshold = 48;
     uint32_t stack_entries = 16;   // LDS stack entries per lane: 8, 16 or 32
     uint32_t tile_width = 32;      // pixel tiles of 64: 8x8, 16x4 or 32x2
+    uint32_t ray_order = 0;        // LUZRT_RAY_ORDER_*: automatic, slot-major or 64-pixel groups
     // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
     int precision = 0;
 };
@@ -175,15 +176,17 @@ RefitJob refit_job(luzrt_blas* b, const void* vertices, const float* leaf_box);
 uint32_t refit_chunks(uint32_t tris);
 
 // Scene block as uploaded for one visibility call: the caller's luzrt_scene
-// plus the ray-slot layout.  Slot j of active pixel a is ray j*A + a, where A
-// is the number of non-background pixels; shadow slots of light l are
-// slot_base[l] .. slot_base[l] + n_l - 1, AO slots follow at r_shadow.
+// plus the ray-slot layout.  Each active (non-background) pixel owns r_total
+// ray slots: shadow slots of light l are slot_base[l] .. slot_base[l] + n_l - 1,
+// AO slots follow at r_shadow.  The ray index of slot j of active pixel a is
+// ray_row() (lightpath.hpp).
 struct DevScene {
     luzrt_scene s;
     uint32_t slot_base[LUZRT_MAX_LIGHTS];
     uint32_t r_shadow;
     uint32_t r_total;
-    uint32_t pad[2];
+    uint32_t ray_group;   // pixels per ray group (ray_row); 0: one group of all active pixels
+    uint32_t pad;
 };
 
 // Device counters of one visibility call (ctx->counters).
@@ -210,9 +213,10 @@ struct TraceLaunch {
     const uint32_t* scene_ray_base;   // device, nscenes + 1 entries
     const struct BatchTlas* scene_tlas;
     const uint32_t* scene_active;     // device, nscenes entries
-    // Visibility launch with rays generated in place (gen): ray j * A + a is
-    // computed from active pixel a of the G-buffer and slot j of the scene
-    // block (A = counters[kCntActive]); ray_o and ray_d are then unused.
+    // Visibility launch with rays generated in place (gen): ray
+    // ray_row(a, A, r_total, ray_group) is computed from active pixel a of the
+    // G-buffer and slot j of the scene block (A = counters[kCntActive]);
+    // ray_o and ray_d are then unused.
     bool gen;
     const struct DevScene* ds;
     const float* position;
@@ -234,12 +238,16 @@ struct BatchTlas {
     uint32_t pad;
 };
 constexpr uint32_t kMaxBatch = LUZRT_MAX_BATCH;
+constexpr uint32_t kRayGroup = 64;   // pixels per ray group of the grouped ray order (ray_row)
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
 
 // Validates a scene and fills its slot layout.
 luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds);
-// fill_scene, then uploads the block to ctx->scene.
-luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy);
+// fill_scene, then uploads the block to ctx->scene with the given ray group.
+luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy,
+                          uint32_t ray_group = 0);
+// Ray group of a visibility call on `tlas` under the context's ray order.
+uint32_t ray_group_for(const luzrt_ctx* ctx, const luzrt_tlas* tlas);
 luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g);
 luzrt_status ensure_lut(luzrt_ctx* ctx);
 

@@ -149,8 +149,7 @@ __device__ inline V3 ao_dir(const PixelFrame& f, uint32_t frame, int s) {
               T.z * hx + B.z * hy + N.z * hz);
 }
 
-// Ray j * A + a of a visibility launch over A active pixels (slot-major):
-// the ray of slot j from active pixel a (G-buffer pixel p).  Slot j < r_shadow
+// The ray of slot j from active pixel a (G-buffer pixel p).  Slot j < r_shadow
 // is shadow sample j - slot_base[l] of the last light l with
 // slot_base[l] <= j (lights without samples own no slots); slot j >= r_shadow
 // is AO sample j - r_shadow.  Shadow rays start at shadow_origin with
@@ -182,24 +181,63 @@ __device__ inline void gen_ray(const float* position, const float* normal, uint3
     }
 }
 
+// Ray order of a visibility launch over A active pixels with R slots each:
+// groups of G consecutive active pixels (G = DevScene::ray_group, or one group
+// of all A pixels when that is 0), slot-major inside a group, so ray
+// (g * R + j) * G + i is slot j of active pixel g * G + i (the last, partial
+// group likewise with its own width).  A wave's 64 lanes hold one slot (one
+// light sample or AO sample) of 64 neighbouring pixels either way; with small
+// groups the R rays of a pixel are also claimed close together, so its
+// G-buffer texel comes from cache after the first of them.
+//
+// ray_row: the ray of slot 0 of active pixel a and the step between its slots.
+__device__ inline void ray_row(uint32_t a, uint32_t A, uint32_t R, uint32_t G, uint32_t& first,
+                               uint32_t& step) {
+    if (G == 0) {
+        first = a;
+        step = A;
+        return;
+    }
+    const uint32_t g = a / G, full = A / G;
+    step = g < full ? G : A - full * G;
+    first = g * G * R + (a - g * G);
+}
+
+// Inverse: slot j and active pixel a of ray rid < A * R.
+__device__ inline void ray_slot(uint32_t rid, uint32_t A, uint32_t R, uint32_t G, uint32_t& j,
+                                uint32_t& a) {
+    if (G == 0) {
+        j = rid / A;
+        a = rid - j * A;
+        return;
+    }
+    const uint32_t g = rid / (G * R), full = A / G;
+    const uint32_t w = g < full ? G : A - full * G;
+    const uint32_t within = rid - g * G * R;
+    j = within / w;
+    a = g * G + (within - j * w);
+}
+
 // Per-pixel counts from hit bytes: occluded shadow samples per light
 // (plane l of `shadow`), unoccluded AO samples; a = kNone for background.
 __device__ inline void resolve_pixel(const DevScene* ds, uint32_t a, uint32_t A, const uint8_t* hits,
                                      uint32_t npix, uint32_t p, uint8_t* shadow, uint8_t* ao) {
     const luzrt_scene& S = ds->s;
+    uint32_t first = 0, step = 0;
+    if (a != kNone) ray_row(a, A, ds->r_total, ds->ray_group, first, step);
     for (int l = 0; l < S.num_lights; ++l) {
         uint32_t cnt = 0;
         int n = S.lights[l].num_shadow_samples;
         if (a != kNone && n > 0) {
             uint32_t j0 = ds->slot_base[l];
-            for (int s = 0; s < n; ++s) cnt += hits[(size_t)(j0 + s) * A + a];
+            for (int s = 0; s < n; ++s) cnt += hits[first + (j0 + s) * step];
         }
         shadow[(size_t)l * npix + p] = (uint8_t)cnt;
     }
     uint32_t miss = 0;
     if (a != kNone) {
         uint32_t j0 = ds->r_shadow;
-        for (int s = 0; s < S.ao_num_samples; ++s) miss += 1u - hits[(size_t)(j0 + s) * A + a];
+        for (int s = 0; s < S.ao_num_samples; ++s) miss += 1u - hits[first + (j0 + s) * step];
     }
     ao[p] = (uint8_t)miss;
 }

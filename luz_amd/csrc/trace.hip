@@ -9,12 +9,11 @@
 //   k_gen_count the ray count, A active pixels x slots;
 //   k_trace_any persistent any-hit traversal: a wave refills its idle lanes
 //               with one atomic when at least `refill` lanes are idle
-//               (ballot-based compaction of the wave's work); ray j*A + a
-//               (slot-major, so a wave holds neighbouring pixels of the same
-//               light sample or AO sample) is generated at claim time from
-//               the G-buffer texel of active pixel a (gen_ray), so no ray
-//               records go through memory (ray records remain for
-//               luzrt_trace_rays and batched frames);
+//               (ballot-based compaction of the wave's work); each ray
+//               (ray_row order: a wave holds one light sample or AO sample
+//               of 64 neighbouring pixels) is generated at claim time from
+//               the G-buffer texel of its pixel (gen_ray), so no ray records
+//               go through memory (they remain for luzrt_trace_rays);
 //   k_resolve   per pixel, occluded shadow samples per light and unoccluded
 //               AO samples, written as u8 planes.
 #include "internal.hpp"
@@ -159,7 +158,9 @@ __global__ __launch_bounds__(kBlock, (WIDE || SHORT > 16) ? 1 : 5) void k_trace_
     StackT<SHORT> st;
     st.init(lds + tid, kBlock, L.spill + (size_t)blockIdx.x * kBlock + tid, total);
     const uint32_t nrays = L.counters[kCntRays];
-    const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // rays per slot
+    const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // active pixels
+    const uint32_t R = SRC == kGen ? L.ds->r_total : 1u;             // ray slots per pixel
+    const uint32_t G = SRC == kGen ? L.ds->ray_group : 0u;           // pixels per ray group
     const uint32_t chunk = claim_chunk(nrays, total / 64u);
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     RayState r;
@@ -195,8 +196,9 @@ __global__ __launch_bounds__(kBlock, (WIDE || SHORT > 16) ? 1 : 5) void k_trace_
                                 L.hits[rid] = 0;
                                 ray = kNone;
                             } else {
-                                const uint32_t As = L.scene_active[sc], local = rid - sbase[sc];
-                                const uint32_t slot = local / As, a = local - slot * As;
+                                uint32_t slot, a;
+                                ray_slot(rid - sbase[sc], L.scene_active[sc], bt.ds->r_total,
+                                         bt.ds->ray_group, slot, a);
                                 V3 o, d;
                                 float tmin, tmax;
                                 gen_ray(bt.position, bt.normal, bt.width, bt.ds, bt.act_list[a], slot, o,
@@ -205,7 +207,8 @@ __global__ __launch_bounds__(kBlock, (WIDE || SHORT > 16) ? 1 : 5) void k_trace_
                             }
                         } else {
                             if (SRC == kGen) {
-                                const uint32_t slot = rid / A, a = rid - slot * A;
+                                uint32_t slot, a;
+                                ray_slot(rid, A, R, G, slot, a);
                                 V3 o, d;
                                 float tmin, tmax;
                                 gen_ray(L.position, L.normal, L.width, L.ds, L.act_list[a], slot, o, d,
@@ -298,7 +301,7 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
 static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* g,
                                     const luzrt_scene* scene, uint8_t* shadow, uint8_t* ao) {
     DevScene ds;
-    luzrt_status s = upload_scene(ctx, scene, &ds);
+    luzrt_status s = upload_scene(ctx, scene, &ds, ray_group_for(ctx, tlas));
     if (s != LUZRT_OK) return s;
     if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
     const uint32_t W = g->width, H = g->height;

@@ -89,6 +89,33 @@ def test_batch_visibility_matches_oracle(ctx, workload):
         np.testing.assert_array_equal(ao, wao)
 
 
+def test_batch_grouped_ray_order_matches_oracle(ctx):
+    """A batch created under the grouped ray order (the samples of 64
+    neighbouring pixels claimed together in every scene) gives the oracle's
+    counts, like the default order."""
+    ctx.set_ray_order(api.RAY_ORDER_GROUPED)
+    try:
+        wl = BatchWorkload(ctx, SCENES)
+    finally:
+        ctx.set_ray_order(api.RAY_ORDER_AUTO)
+    try:
+        frame = 4
+        wl.frame(frame, True)
+        for i in (0, 3, 4):
+            h = wl.hosts[i]
+            h.animate(frame)
+            gbuf, sh, ao, _ = wl.outputs(i)
+            sb = h.pack()
+            sb.frame = frame
+            W, H, _ = h.camera()
+            wsh, wao, _ = oracle_scene(h).trace_visibility(gbuf, W, H, sb)
+            print(f"grouped order, scene {i}: {W}x{H}", flush=True)
+            np.testing.assert_array_equal(sh, wsh)
+            np.testing.assert_array_equal(ao, wao)
+    finally:
+        wl.close()
+
+
 def test_batch_recaptures_on_config_change(ctx, workload):
     want = None
     for cfg in [(6, 48, 16, 32), (6, 16, 8, 16), (8, 32, 32, 8)]:

@@ -151,6 +151,32 @@ def test_visibility_bit_exact(ctx, kind, res, light_dir):
     g.close()
 
 
+@pytest.mark.parametrize("order", [api.RAY_ORDER_SLOT, api.RAY_ORDER_GROUPED, api.RAY_ORDER_AUTO])
+def test_visibility_same_in_every_ray_order(ctx, order):
+    """The order in which a visibility call claims its rays (all pixels' first
+    sample first, or the samples of 64 neighbouring pixels together) changes
+    no count: both against the oracle, on an image whose active-pixel count
+    leaves a partial last group of pixels."""
+    s, g = _gpu_scene(ctx, api.SCENE_SPHERES)
+    o = oracle_scene(s)
+    W, H, gd, got, _ = _gbuffers(ctx, s, g, o, 331, 187)
+    active = int((np.abs(got["normal"]).sum(1) != 0).sum())
+    assert active % 64 != 0
+    sb = s.pack()
+    nl = sb.num_lights
+    dsh, dao = ctx.buffer(nl * W * H), ctx.buffer(W * H)
+    ctx.set_ray_order(order)
+    try:
+        print(f"ray order {order}: {active} active pixels", flush=True)
+        ctx.trace_visibility(g.tlas, gd.desc(), sb, dsh, dao)
+    finally:
+        ctx.set_ray_order(api.RAY_ORDER_AUTO)
+    wsh, wao, _ = o.trace_visibility(got, W, H, sb)
+    np.testing.assert_array_equal(dsh.download(np.uint8, (nl, W * H)), wsh)
+    np.testing.assert_array_equal(dao.download(np.uint8, W * H), wao)
+    g.close()
+
+
 @pytest.mark.parametrize("kind,res", [(api.SCENE_CUBE, None), (api.SCENE_SPHERES, (480, 270))])
 def test_light_pass_radiance(ctx, kind, res):
     s, g = _gpu_scene(ctx, kind)
