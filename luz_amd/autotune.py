@@ -7,10 +7,12 @@ Searches, on the live workload, the launch shape of the visibility pipeline:
                  ray chunk,
   stack_entries  LDS stack entries per lane (8, 16, 32) before spilling,
   tile_width     ray-tile shape of the pixel compaction (8x8, 16x4, 32x2
-                 pixels per 64-lane wave).
+                 pixels per 64-lane wave),
+  ray_order      order of the rays of a visibility call (single scene only):
+                 slot-major or 64-pixel groups (luzrt_set_ray_order).
 The search is a coordinate descent: the (groups_per_cu, refill) grid first,
-then the stack depth, then the tile shape, each with the others fixed at the
-best so far.  Single-scene candidates are timed with the library's own HIP
+then the stack depth, then the tile shape, then the ray order, each with the
+others fixed at the best so far.  Single-scene candidates are timed with the library's own HIP
 events around the visibility stages; batched frames by the wall-clock
 latency of whole frames (hipGraph replays).  The best median wins.
 
@@ -32,14 +34,16 @@ GROUPS_PER_CU = (4, 6, 8, 10)
 REFILL = (8, 16, 32, 48)
 STACK_ENTRIES = (8, 16, 32)
 TILE_WIDTH = (8, 16, 32)
-KEYS = ("groups_per_cu", "refill", "stack_entries", "tile_width")
+RAY_ORDER = (api.RAY_ORDER_SLOT, api.RAY_ORDER_GROUPED)
+KEYS = ("groups_per_cu", "refill", "stack_entries", "tile_width", "ray_order")
 
 
 def _apply(ctx: api.Context, cfg: dict) -> None:
     ctx.set_trace_config(cfg["groups_per_cu"], cfg["refill"], cfg["stack_entries"], cfg["tile_width"])
+    ctx.set_ray_order(cfg["ray_order"])
 
 
-def _descend(ctx: api.Context, measure, groups, refills, stacks, tiles, start: dict):
+def _descend(ctx: api.Context, measure, groups, refills, stacks, tiles, start: dict, orders=()):
     table = []
     best = dict(start)
 
@@ -56,6 +60,8 @@ def _descend(ctx: api.Context, measure, groups, refills, stacks, tiles, start: d
     run([{"groups_per_cu": g, "refill": r} for g, r in itertools.product(groups, refills)])
     run([{"stack_entries": s} for s in stacks])
     run([{"tile_width": t} for t in tiles])
+    if orders:
+        run([{"ray_order": o} for o in orders])
     _apply(ctx, best)
     top = min(table, key=lambda r: r["ms"])
     return dict(best, ms=top["ms"], table=table)
@@ -65,8 +71,9 @@ def _cached(ctx, cache, key):
     if cache is not None and key and cache.exists():
         known = json.loads(cache.read_text())
         if key in known:
-            _apply(ctx, known[key])
-            return known[key]
+            cfg = dict(START, **known[key])   # entries stored before a key existed take its start value
+            _apply(ctx, cfg)
+            return cfg
     return None
 
 
@@ -77,12 +84,14 @@ def _store(cache, key, result):
         cache.write_text(json.dumps(known, indent=1))
 
 
-START = {"groups_per_cu": 6, "refill": 48, "stack_entries": 16, "tile_width": 32}
+START = {"groups_per_cu": 6, "refill": 48, "stack_entries": 16, "tile_width": 32,
+         "ray_order": api.RAY_ORDER_AUTO}
 
 
 def tune_visibility(ctx: api.Context, tlas: api.Tlas, g: api.GBuffer, sb: api.SceneBlock,
                     reps: int = 3, groups=GROUPS_PER_CU, refills=REFILL, stacks=STACK_ENTRIES,
-                    tiles=TILE_WIDTH, cache: Path | None = None, key: str | None = None) -> dict:
+                    tiles=TILE_WIDTH, orders=RAY_ORDER, cache: Path | None = None,
+                    key: str | None = None) -> dict:
     """Returns the best configuration (KEYS, "ms" = visibility ms, "table") and applies it."""
     hit = _cached(ctx, cache, key)
     if hit is not None:
@@ -101,14 +110,15 @@ def tune_visibility(ctx: api.Context, tlas: api.Tlas, g: api.GBuffer, sb: api.Sc
             times.append(t.compact_ms + t.raygen_ms + t.trace_ms + t.resolve_ms)
         return statistics.median(times)
 
-    result = _descend(ctx, measure, groups, refills, stacks, tiles, START)
+    result = _descend(ctx, measure, groups, refills, stacks, tiles, START, orders)
     _store(cache, key, result)
     return result
 
 
 def tune_batch(ctx: api.Context, workload, reps: int = 3, groups=GROUPS_PER_CU, refills=REFILL,
                stacks=STACK_ENTRIES, tiles=TILE_WIDTH, frame: int = 0) -> dict:
-    """Tunes batched frames (luz_amd.batching.BatchWorkload) by median frame latency."""
+    """Tunes batched frames (luz_amd.batching.BatchWorkload) by median frame latency (the ray
+    order of a batch is fixed when it is created and is not searched here)."""
 
     def measure():
         workload.frame(frame, True)                       # capture + first replay
