@@ -73,8 +73,8 @@ struct Best {
 struct RayState {
     V3 wd;         // world direction (the world origin is wbr.o)
     float tmin, tmax;
-    BoxRay br;     // the slab-test ray of the tree being walked (object space inside a BLAS)
-    BoxRay wbr;    // the world-space one, restored on return to the TLAS
+    BoxRay br;     // the object-space slab-test ray of the instance being walked
+    BoxRay wbr;    // the world-space one (TLAS nodes)
     TriRay tr;
     const Node* bnodes;
     const float* btris;
@@ -204,7 +204,6 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
         r.next = st.pop();
         if (r.next == kSentinel) {
             r.in_blas = false;
-            r.br = r.wbr;
             r.next = kNone;
             return kStepContinue;
         }
@@ -233,8 +232,11 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
     uint4 qc;
     load_rows(r.in_blas ? r.bnodes : tlas_nodes, r.next, tlas_lds && !r.in_blas, qx, qy, qz, qc);
     float tn0, tn1;
-    const bool b0 = box_hit(r.br, qx.x, qx.z, qy.x, qy.z, qz.x, qz.z, tn0);
-    const bool b1 = box_hit(r.br, qx.y, qx.w, qy.y, qy.w, qz.y, qz.w, tn1);
+    // the ray of the tree being walked, selected per lane (restoring the world
+    // ray into a single slot on every return to the TLAS costs more)
+    const BoxRay cur = r.in_blas ? r.br : r.wbr;
+    const bool b0 = box_hit(cur, qx.x, qx.z, qy.x, qy.z, qz.x, qz.z, tn0);
+    const bool b1 = box_hit(cur, qx.y, qx.w, qy.y, qy.w, qz.y, qz.w, tn1);
     const bool h0 = (qc.x != kEmpty) & b0;
     const bool h1 = (qc.y != kEmpty) & b1;
     int first = (h0 && h1) ? (tn1 < tn0 ? 1 : 0) : (h0 ? 0 : 1);
