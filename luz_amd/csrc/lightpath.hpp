@@ -5,7 +5,7 @@
 //   tile_pixel / pixel_active / block_prefix  compaction of non-background pixels
 //   shadow_setup / gen_ray                    light.frag:86-146, 111-135, 229-231
 //   resolve_pixel                             occluded / unoccluded sample counts
-//   shade_pixel                               light.frag:17-49, 171-235 (FP32)
+//   shade_pixel                               light.frag:17-49, 171-235 (FP32, hardware rcp/rsq)
 //   primary_pixel / primary_texel             G-buffer texels by closest-hit primary rays
 #pragma once
 #include "internal.hpp"
@@ -274,6 +274,16 @@ struct FractionVisibility {
     }
 };
 
+// The lighting arithmetic is compared with the oracle to 1e-3 relative, not
+// bit for bit (only visibility decisions are), so it uses the hardware
+// reciprocal, reciprocal square root and square root (about 1 ulp each)
+// instead of the correctly rounded division and square-root sequences that
+// the ray set-up needs.
+__device__ inline float rcp_hw(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ inline float sqrt_hw(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ inline V3 vnorm_hw(V3 a) { return vmul(a, __builtin_amdgcn_rsqf(vdot(a, a))); }
+constexpr float kInvPiF = 0.318309886183791f;
+
 // light.frag:main in FP32; lut = pow(i/255, 2.2).
 template <typename VIS>
 __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds, const float* lut,
@@ -286,64 +296,63 @@ __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds,
                            S.ambient_color[2] * S.ambient_intensity, 1.0f);
     uint32_t alb = g.albedo[p], mat = g.material[p], emi = g.emission[p];
     float albedo[3] = {lut[alb & 0xFFu], lut[(alb >> 8) & 0xFFu], lut[(alb >> 16) & 0xFFu]};
-    float rough = (float)(mat & 0xFFu) / 255.0f;
-    float metal = (float)((mat >> 8) & 0xFFu) / 255.0f;
-    float occl = (float)((mat >> 16) & 0xFFu) / 255.0f;
+    const float inv255 = rcp_hw(255.0f);
+    float rough = (float)(mat & 0xFFu) * inv255;
+    float metal = (float)((mat >> 8) & 0xFFu) * inv255;
+    float occl = (float)((mat >> 16) & 0xFFu) * inv255;
     V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
     V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
-    V3 V = vnorm(vsub(cam, P));
+    V3 V = vnorm_hw(vsub(cam, P));
     float F0[3];
     for (int k = 0; k < 3; ++k) F0[k] = 0.04f * (1.0f - metal) + albedo[k] * metal;
     float Lo[3] = {0.0f, 0.0f, 0.0f};
     float a = rough * rough, a2 = a * a;
     float rr = rough + 1.0f, kk = (rr * rr) / 8.0f;
     float NdotV = fmax_(vdot(N, V), 0.0f);
-    float gv = NdotV / (NdotV * (1.0f - kk) + kk);
+    float gv = NdotV * rcp_hw(NdotV * (1.0f - kk) + kk);
     const uint32_t nl = (uint32_t)S.num_lights;
     for (uint32_t l = 0; l < nl; ++l) {
         const luzrt_light& Lb = S.lights[l];
         V3 Lr = vsub(v3(Lb.position[0], Lb.position[1], Lb.position[2]), P);
-        V3 L = vnorm(Lr);
+        V3 L = vnorm_hw(Lr);
         float att = 1.0f;
         V3 ndir = v3(-Lb.direction[0], -Lb.direction[1], -Lb.direction[2]);
         if (Lb.type == LUZRT_LIGHT_DIRECTIONAL) {
-            L = vnorm(ndir);
+            L = vnorm_hw(ndir);
         } else if (Lb.type == LUZRT_LIGHT_SPOT) {
-            float dist = vlen(Lr);
-            att = 1.0f / (dist * dist);
-            float theta = vdot(L, vnorm(ndir));
+            att = rcp_hw(vdot(Lr, Lr));
+            float theta = vdot(L, vnorm_hw(ndir));
             float eps = Lb.inner_angle - Lb.outer_angle;
-            att = att * clamp01((theta - Lb.outer_angle) / eps);
+            att = att * clamp01((theta - Lb.outer_angle) * rcp_hw(eps));
         } else {
-            float dist = vlen(Lr);
-            att = 1.0f / (dist * dist);
+            att = rcp_hw(vdot(Lr, Lr));
         }
         float sh = vis.shadow_frac(S, l, p);
         float scale = Lb.intensity * att * (1.0f - sh);
-        V3 H = vnorm(vadd(V, L));
+        V3 H = vnorm_hw(vadd(V, L));
         float NdotH = fmax_(vdot(N, H), 0.0f);
         float den = NdotH * NdotH * (a2 - 1.0f) + 1.0f;
-        float D = a2 / (kPiF * den * den);
+        float D = a2 * rcp_hw(kPiF * den * den);
         float NdotL = fmax_(vdot(N, L), 0.0f);
-        float G = (NdotL / (NdotL * (1.0f - kk) + kk)) * gv;
+        float G = (NdotL * rcp_hw(NdotL * (1.0f - kk) + kk)) * gv;
         float HV = clamp01(vdot(H, V));
         float om = clamp01(1.0f - HV);
         float om5 = om * om * om * om * om;
-        float denom = 4.0f * NdotV * NdotL + 0.0001f;
+        float inv_denom = rcp_hw(4.0f * NdotV * NdotL + 0.0001f);
         float DG = D * G;
         for (int k = 0; k < 3; ++k) {
             float F = F0[k] + (1.0f - F0[k]) * om5;
-            float spec = DG * F / denom;
+            float spec = DG * F * inv_denom;
             float kD = (1.0f - F) * (1.0f - metal);
             float radiance = Lb.color[k] * scale;
-            Lo[k] = Lo[k] + (kD * albedo[k] / kPiF + spec) * radiance * NdotL;
+            Lo[k] = Lo[k] + (kD * albedo[k] * kInvPiF + spec) * radiance * NdotL;
         }
     }
     float aof = vis.ao_frac(S, p);
     float o[3];
     for (int k = 0; k < 3; ++k) {
         float amb = S.ambient_color[k] * S.ambient_intensity * albedo[k] * occl * aof;
-        float e = (float)((emi >> (8 * k)) & 0xFFu) / 255.0f;
+        float e = (float)((emi >> (8 * k)) & 0xFFu) * inv255;
         o[k] = amb + Lo[k] + e;
     }
     return make_float4(o[0], o[1], o[2], 1.0f);
