@@ -717,15 +717,10 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     InstAux* aux = J.aux;
     float* w2o_out = J.w2o_out;
     uint32_t* order_out = J.order_out;
-    // tlas_lds_bytes(m) bytes, m a power of two >= n: the reductions of steps
-    // 1-2, then the sort keys, then the clustering's box tile and neighbour
-    // array, one after the other in the same memory (within 64 KB for 8192
-    // instances).
+    // tlas_lds_bytes(m) bytes, m a power of two >= n: the sort keys, then the
+    // clustering's box tile, neighbour and reference arrays in the same memory
+    // (within 64 KB for 8192 instances).
     extern __shared__ uint64_t skeys[];
-    float* shf = reinterpret_cast<float*>(skeys);
-    double* shd = reinterpret_cast<double*>(skeys);
-    __shared__ float s_pad;
-    __shared__ double s_frame[6];
     const int tid = threadIdx.x;
 
     // 1. world boxes of the BLAS root boxes, scene magnitude
@@ -749,15 +744,17 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
             wbox[i * 6 + 3 + k] = hi[k];
         }
     }
-    shf[tid] = mloc;
+    // scene magnitude: wave maxima, then the 16 wave values (min/max are exact,
+    // so the order of the reduction does not change the result)
+    __shared__ float s_red[kTlasThreads / 64];
+    __shared__ double s_redd[6][kTlasThreads / 64];
+    const uint32_t wave = (uint32_t)tid >> 6, lane = (uint32_t)tid & 63u;
+    for (int o = 32; o > 0; o >>= 1) mloc = fmax_(mloc, __shfl_xor(mloc, o));
+    if (lane == 0) s_red[wave] = mloc;
     __syncthreads();
-    for (int s = kTlasThreads / 2; s > 0; s >>= 1) {
-        if (tid < s) shf[tid] = fmax_(shf[tid], shf[tid + s]);
-        __syncthreads();
-    }
-    if (tid == 0) s_pad = shf[0] * 0x1p-14f;
-    __syncthreads();
-    const float pad = s_pad;
+    float m_all = 0.0f;
+    for (uint32_t w = 0; w < kTlasThreads / 64; ++w) m_all = fmax_(m_all, s_red[w]);
+    const float pad = m_all * 0x1p-14f;
     // 2. padded boxes, FP64 centroid frame
     double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = tid; i < n; i += kTlasThreads) {
@@ -771,43 +768,59 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         }
     }
     for (int k = 0; k < 3; ++k) {
-        shd[tid] = cmn[k];
-        __syncthreads();
-        for (int s = kTlasThreads / 2; s > 0; s >>= 1) {
-            if (tid < s) shd[tid] = shd[tid + s] < shd[tid] ? shd[tid + s] : shd[tid];
-            __syncthreads();
+        for (int o = 32; o > 0; o >>= 1) {
+            const double a = __shfl_xor(cmn[k], o), b = __shfl_xor(cmx[k], o);
+            cmn[k] = a < cmn[k] ? a : cmn[k];
+            cmx[k] = b > cmx[k] ? b : cmx[k];
         }
-        double mn = shd[0];
-        __syncthreads();
-        shd[tid] = cmx[k];
-        __syncthreads();
-        for (int s = kTlasThreads / 2; s > 0; s >>= 1) {
-            if (tid < s) shd[tid] = shd[tid + s] > shd[tid] ? shd[tid + s] : shd[tid];
-            __syncthreads();
-        }
-        double mx = shd[0];
-        __syncthreads();
-        if (tid == 0) {
-            double e = mx - mn;
-            s_frame[k] = mn;
-            s_frame[3 + k] = e > 0.0 ? 1024.0 / e : 0.0;
+        if (lane == 0) {
+            s_redd[k][wave] = cmn[k];
+            s_redd[3 + k][wave] = cmx[k];
         }
     }
     __syncthreads();
+    double s_frame[6];
+    for (int k = 0; k < 3; ++k) {
+        double mn = INFINITY, mx = -INFINITY;
+        for (uint32_t w = 0; w < kTlasThreads / 64; ++w) {
+            mn = s_redd[k][w] < mn ? s_redd[k][w] : mn;
+            mx = s_redd[3 + k][w] > mx ? s_redd[3 + k][w] : mx;
+        }
+        const double e = mx - mn;
+        s_frame[k] = mn;
+        s_frame[3 + k] = e > 0.0 ? 1024.0 / e : 0.0;
+    }
     // 3. Morton codes, bitonic sort of (code << 32 | index)
-    for (uint32_t i = tid; i < m; i += kTlasThreads) {
-        uint64_t key = ~0ull;
-        if (i < n) {
-            const float* b = wbox + i * 6;
-            double cx = 0.5 * ((double)b[0] + (double)b[3]);
-            double cy = 0.5 * ((double)b[1] + (double)b[4]);
-            double cz = 0.5 * ((double)b[2] + (double)b[5]);
-            key = ((uint64_t)morton_code(s_frame, cx, cy, cz) << 32) | i;
+    auto sort_key = [&](uint32_t i) -> uint64_t {
+        if (i >= n) return ~0ull;
+        const float* b = wbox + i * 6;
+        double cx = 0.5 * ((double)b[0] + (double)b[3]);
+        double cy = 0.5 * ((double)b[1] + (double)b[4]);
+        double cz = 0.5 * ((double)b[2] + (double)b[5]);
+        return ((uint64_t)morton_code(s_frame, cx, cy, cz) << 32) | i;
+    };
+    if (m <= 64) {
+        // one wave sorts in registers: lane i holds key i, partners by xor
+        if (tid < 64) {
+            uint64_t key = sort_key((uint32_t)tid);
+            for (uint32_t k = 2; k <= m; k <<= 1) {
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    const uint32_t lo = __shfl_xor((uint32_t)key, (int)j);
+                    const uint32_t hi = __shfl_xor((uint32_t)(key >> 32), (int)j);
+                    const uint64_t other = ((uint64_t)hi << 32) | lo;
+                    const bool lower = ((uint32_t)tid & j) == 0, up = ((uint32_t)tid & k) == 0;
+                    const uint64_t mn = other < key ? other : key, mx = other < key ? key : other;
+                    key = lower == up ? mn : mx;
+                }
+            }
+            if ((uint32_t)tid < m) skeys[tid] = key;
         }
-        skeys[i] = key;
+        __syncthreads();
+    } else {
+        for (uint32_t i = tid; i < m; i += kTlasThreads) skeys[i] = sort_key(i);
+        __syncthreads();
     }
-    __syncthreads();
-    for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t k = 2; m > 64 && k <= m; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t i = tid; i < m; i += kTlasThreads) {
                 uint32_t ixj = i ^ j;

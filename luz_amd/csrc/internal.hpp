@@ -43,6 +43,17 @@ struct luzrt_ctx {
     uint32_t ray_order = 0;        // LUZRT_RAY_ORDER_*: automatic, slot-major or 64-pixel groups
     // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
     int precision = 0;
+    // pinned staging ring of small host-to-device uploads (scene blocks, TLAS
+    // instances): stage_upload
+    struct Staging {
+        void* host = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kStagingSlots = 4;
+    Staging staging[kStagingSlots];
+    int staging_next = 0;
 };
 
 struct luzrt_blas {
@@ -243,6 +254,11 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
 
 // Validates a scene and fills its slot layout.
 luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds);
+// Copies `bytes` from host memory `src` to device memory `dst` on the
+// context's stream through a pinned staging buffer: `src` may be reused as soon
+// as the call returns, and the call does not wait for the GPU (a staging buffer
+// is reused once its previous copy has finished).
+luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes);
 // fill_scene, then uploads the block to ctx->scene with the given ray group.
 luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy,
                           uint32_t ray_group = 0);

@@ -8,6 +8,7 @@
 // as status codes with a message instead of ASSERT/abort (Base.hpp:51-59).
 #include <cmath>
 #include <cstdarg>
+#include <cstring>
 
 #include <algorithm>
 
@@ -117,10 +118,30 @@ luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds
     ds->ray_group = ray_group;
     s = ensure(ctx, ctx->scene, sizeof(DevScene));
     if (s != LUZRT_OK) return s;
-    LUZ_HIP(ctx, hipMemcpyAsync(ctx->scene.ptr, ds, sizeof(DevScene), hipMemcpyHostToDevice,
-                                ctx->stream));
-    // the host copy must outlive the asynchronous copy
-    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return stage_upload(ctx, ctx->scene.ptr, ds, sizeof(DevScene));
+}
+
+luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return LUZRT_OK;
+    luzrt_ctx::Staging& sl = ctx->staging[ctx->staging_next];
+    ctx->staging_next = (ctx->staging_next + 1) % luzrt_ctx::kStagingSlots;
+    if (sl.pending) {
+        LUZ_HIP(ctx, hipEventSynchronize(sl.done));
+        sl.pending = false;
+    }
+    if (!sl.done) LUZ_HIP(ctx, hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    if (sl.bytes < bytes) {
+        if (sl.host) LUZ_HIP(ctx, hipHostFree(sl.host));
+        sl.host = nullptr;
+        sl.bytes = 0;
+        const size_t sz = bytes < 4096 ? 4096 : bytes;
+        LUZ_HIP(ctx, hipHostMalloc(&sl.host, sz, hipHostMallocDefault));
+        sl.bytes = sz;
+    }
+    std::memcpy(sl.host, src, bytes);
+    LUZ_HIP(ctx, hipMemcpyAsync(dst, sl.host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    LUZ_HIP(ctx, hipEventRecord(sl.done, ctx->stream));
+    sl.pending = true;
     return LUZRT_OK;
 }
 
@@ -249,6 +270,10 @@ luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
         if (b->ptr) hipFree(b->ptr);
     for (int s = 0; s < kStageCount; ++s)
         for (int k = 0; k < 2; ++k) hipEventDestroy(ctx->ev[s][k]);
+    for (luzrt_ctx::Staging& sl : ctx->staging) {
+        if (sl.host) hipHostFree(sl.host);
+        if (sl.done) hipEventDestroy(sl.done);
+    }
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return LUZRT_OK;
@@ -557,16 +582,12 @@ luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instanc
     t->wide = ctx->precision == LUZRT_PRECISION_COMPACT;
     if (t->wide && !t->recs_w)
         LUZ_HIP(ctx, hipMalloc((void**)&t->recs_w, (size_t)t->capacity * sizeof(InstRec)));
-    if (!host.empty())
-        LUZ_HIP(ctx, hipMemcpyAsync(t->input, host.data(), host.size() * sizeof(InstIn),
-                                    hipMemcpyHostToDevice, ctx->stream));
+    if ((s = stage_upload(ctx, t->input, host.data(), host.size() * sizeof(InstIn))) != LUZRT_OK)
+        return s;
     stage_begin(ctx, kStageBuild);
     s = tlas_build_device(ctx, t, (const InstIn*)t->input, (uint32_t)host.size());
     stage_end(ctx, kStageBuild);
-    if (s != LUZRT_OK) return s;
-    // the staged host array must stay valid until the copy has been consumed
-    LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return LUZRT_OK;
+    return s;
 }
 
 luzrt_status luzrt_tlas_destroy(luzrt_ctx* ctx, luzrt_tlas* t) {
