@@ -126,7 +126,7 @@ __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32
 enum RaySource { kRays = 0, kBatch = 1, kGen = 2 };
 
 template <bool WIDE, int SHORT, int SRC>
-__global__ __launch_bounds__(kBlock, (WIDE || SHORT > 16) ? 1 : 5) void k_trace_any(TraceLaunch L, uint32_t refill) {
+__global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceLaunch L, uint32_t refill) {
     constexpr bool BATCH = SRC == kBatch;
     __shared__ uint32_t lds[SHORT * kBlock];
     __shared__ uint32_t sbase[BATCH ? kMaxBatch + 1 : 1];
