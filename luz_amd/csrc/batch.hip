@@ -334,7 +334,7 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     if ((s = launch_tlas_batched(ctx, b->dev_jobs, b->n, b->max_m)) != LUZRT_OK) return s;
     k_batch_init<<<1, 64, 0, st>>>(b->counters);
     if (b->total_ppx) {
-        uint32_t groups = (uint32_t)ctx->num_cus * 4;
+        uint32_t groups = (uint32_t)ctx->num_cus * kPrimaryGroupsPerCu;
         if (groups == 0) groups = 1;
         k_bprimary<<<groups, kBlock, 0, st>>>(b->tables, (uint32_t*)ctx->spill.ptr,
                                               b->counters + kCntError);
@@ -365,7 +365,7 @@ luzrt_status prepare_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     luzrt_status s;
     if ((s = ensure_lut(ctx)) != LUZRT_OK) return s;
     if (!build_error_flag(ctx)) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory");
-    uint32_t groups = (uint32_t)ctx->num_cus * std::max<uint32_t>(ctx->groups_per_cu, 4u);
+    uint32_t groups = (uint32_t)ctx->num_cus * std::max<uint32_t>(ctx->groups_per_cu, kPrimaryGroupsPerCu);
     if (groups == 0) groups = 1;
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * kBlock * kSpill * 4)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, b->hits, b->max_rays + 16)) != LUZRT_OK) return s;

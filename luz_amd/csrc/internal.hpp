@@ -250,6 +250,9 @@ struct BatchTlas {
 };
 constexpr uint32_t kMaxBatch = LUZRT_MAX_BATCH;
 constexpr uint32_t kRayGroup = 64;   // pixels per ray group of the grouped ray order (ray_row)
+// Persistent workgroups per CU of the primary-ray kernels (92 VGPRs and 16 KB
+// of LDS stacks per workgroup leave room for five waves per SIMD).
+constexpr uint32_t kPrimaryGroupsPerCu = 5;
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
 
 // Validates a scene and fills its slot layout.

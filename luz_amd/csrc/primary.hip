@@ -68,7 +68,7 @@ extern "C" luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* 
     if (nmats)
         LUZ_HIP(ctx, hipMemcpyAsync(dmats, materials, (size_t)nmats * sizeof(luzrt_material),
                                     hipMemcpyHostToDevice, ctx->stream));
-    uint32_t groups = (uint32_t)ctx->num_cus * 4;
+    uint32_t groups = (uint32_t)ctx->num_cus * kPrimaryGroupsPerCu;
     if (groups == 0) groups = 1;
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * 256 * kSpill * 4)) != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
