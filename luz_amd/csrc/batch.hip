@@ -179,12 +179,17 @@ __global__ __launch_bounds__(kBlock) void k_bcompact_write(BatchTables T, uint32
     if (inside) S.act_index[p] = active ? a : kNone;
 }
 
+// As k_resolve: thread q of scene s zeroes background pixel q and resolves
+// active pixel q.
 __global__ __launch_bounds__(kBlock) void k_bresolve(BatchTables T, const uint8_t* hits) {
     const uint32_t s = find_scene(T.pblk, T.n, blockIdx.x);
     const BatchSceneDev& S = T.sc[s];
-    const uint32_t p = (blockIdx.x - T.pblk[s]) * kBlock + threadIdx.x;
-    if (p >= S.npix) return;
-    resolve_pixel(S.ds, S.act_index[p], T.active[s], hits + T.ray_base[s], S.npix, p, S.shadow, S.ao);
+    const uint32_t q = (blockIdx.x - T.pblk[s]) * kBlock + threadIdx.x;
+    if (q >= S.npix) return;
+    const uint32_t A = T.active[s];
+    const uint8_t* h = hits + T.ray_base[s];
+    if (S.act_index[q] == kNone) resolve_pixel(S.ds, kNone, A, h, S.npix, q, S.shadow, S.ao);
+    if (q < A) resolve_pixel(S.ds, q, A, h, S.npix, S.act_list[q], S.shadow, S.ao);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bshade(BatchTables T, const float* lut) {

@@ -14,8 +14,9 @@
 //               of 64 neighbouring pixels) is generated at claim time from
 //               the G-buffer texel of its pixel (gen_ray), so no ray records
 //               go through memory (they remain for luzrt_trace_rays);
-//   k_resolve   per pixel, occluded shadow samples per light and unoccluded
-//               AO samples, written as u8 planes.
+//   k_resolve   per active pixel (in active order), occluded shadow samples
+//               per light and unoccluded AO samples, written as u8 planes;
+//               background pixels get zero counts.
 #include "internal.hpp"
 #include "lightpath.hpp"
 #include "traverse.hpp"
@@ -244,12 +245,17 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     }
 }
 
+// Thread q zeroes the counts of pixel q if it is background and resolves
+// active pixel q (in active order, so a wave's hit reads are contiguous).
 __global__ __launch_bounds__(kBlock) void k_resolve(const DevScene* ds, const uint32_t* act_index,
-                                                    const uint8_t* hits, const uint32_t* counters,
-                                                    uint32_t npix, uint8_t* shadow, uint8_t* ao) {
-    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= npix) return;
-    resolve_pixel(ds, act_index[p], counters[kCntActive], hits, npix, p, shadow, ao);
+                                                    const uint32_t* act_list, const uint8_t* hits,
+                                                    const uint32_t* counters, uint32_t npix,
+                                                    uint8_t* shadow, uint8_t* ao) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= npix) return;
+    const uint32_t A = counters[kCntActive];
+    if (act_index[q] == kNone) resolve_pixel(ds, kNone, A, hits, npix, q, shadow, ao);
+    if (q < A) resolve_pixel(ds, q, A, hits, npix, act_list[q], shadow, ao);
 }
 
 inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -365,6 +371,7 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
 
     stage_begin(ctx, kStageResolve);
     k_resolve<<<cdiv(npix, kBlock), kBlock, 0, st>>>(dsd, (const uint32_t*)ctx->act_index.ptr,
+                                                     (const uint32_t*)ctx->active.ptr,
                                                      (const uint8_t*)ctx->hits.ptr, counters,
                                                      (uint32_t)npix, shadow, ao);
     stage_end(ctx, kStageResolve);
