@@ -3,10 +3,10 @@
 // Replaces the ray queries of light.frag (TraceShadowRay :86-109,
 // TraceAORays :111-135, EvaluateShadow :137-146, AO origin :229-231) with a
 // persistent traversal over the active pixels:
-//   k_vis_init  zero the call's device counters (graph-safe, no memset);
 //   k_compact_* non-background pixels in 64-pixel-tile order -> active list
-//               (count, scan, write: deterministic, no atomics);
-//   k_gen_count the ray count, A active pixels x slots;
+//               (count, scan, write: deterministic, no atomics); the scan
+//               also sets the call's counters (active pixels, rays = active
+//               pixels x slots, claim and error counters zeroed);
 //   k_trace_any persistent any-hit traversal: a wave refills its idle lanes
 //               with one atomic when at least `refill` lanes are idle
 //               (ballot-based compaction of the wave's work); each ray
@@ -42,32 +42,44 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(const float* normal, u
     if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block counts (one workgroup); writes the total.
+// Exclusive scan of the block counts (one workgroup: each thread scans a run
+// of consecutive counts, the runs' sums are scanned by wave shuffles and one
+// exchange of wave totals).  Writes the active-pixel and ray counts and zeroes
+// the claim and error counters of the traversal that follows.
 __global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, uint32_t nblocks,
-                                                       uint32_t* counters) {
-    __shared__ uint32_t sh[1024];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < nblocks; base += 1024) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < nblocks ? block_counts[i] : 0u;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (uint32_t off = 1; off < 1024; off <<= 1) {
-            uint32_t add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
-            __syncthreads();
-            sh[threadIdx.x] += add;
-            __syncthreads();
-        }
-        uint32_t incl = sh[threadIdx.x];
-        uint32_t c = carry;
-        if (i < nblocks) block_counts[i] = c + incl - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = c + incl;
-        __syncthreads();
+                                                       uint32_t* counters, const DevScene* ds) {
+    __shared__ uint32_t wave_sum[1024 / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t per = (nblocks + 1023u) / 1024u, begin = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (begin + k < nblocks) sum += block_counts[begin + k];
+    uint32_t incl = sum;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
     }
-    if (threadIdx.x == 0) counters[kCntActive] = carry;
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t w = 0; w < 1024 / 64; ++w) {
+        if (w < wave) before += wave_sum[w];
+        total += wave_sum[w];
+    }
+    uint32_t run = before + incl - sum;
+    for (uint32_t k = 0; k < per; ++k) {
+        if (begin + k < nblocks) {
+            const uint32_t v = block_counts[begin + k];
+            block_counts[begin + k] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0) {
+        counters[kCntActive] = total;
+        counters[kCntRays] = total * ds->r_total;
+        counters[kCntFetch] = 0;
+        counters[kCntError] = 0;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_write(const float* normal, uint32_t W, uint32_t H,
@@ -83,11 +95,6 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const float* normal, u
     uint32_t a = block_offsets[blockIdx.x] + block_prefix(active, wave_tot, total);
     if (active) act_list[a] = p;
     if (inside) act_index[p] = active ? a : kNone;
-}
-
-// Ray count of a launch whose rays are generated in place.
-__global__ void k_gen_count(const DevScene* ds, uint32_t* counters) {
-    if (threadIdx.x == 0) counters[kCntRays] = counters[kCntActive] * ds->r_total;
 }
 
 // Rays are claimed in chunks per wave (one device-scope atomic on a single
@@ -328,21 +335,19 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
     hipStream_t st = ctx->stream;
 
     stage_begin(ctx, kStageCompact);
-    k_vis_init<<<1, 64, 0, st>>>(counters, 0u);
     uint32_t ntiles_px = (uint32_t)ntiles_px_sz;
     uint32_t nblocks = cdiv(ntiles_px, kBlock);
     uint32_t* block_counts = (uint32_t*)ctx->act_index.ptr + npix;
     k_compact_count<<<nblocks, kBlock, 0, st>>>(g->normal, W, H, tw, ntiles_px, block_counts);
-    k_compact_scan<<<1, 1024, 0, st>>>(block_counts, nblocks, counters);
+    k_compact_scan<<<1, 1024, 0, st>>>(block_counts, nblocks, counters, dsd);
     k_compact_write<<<nblocks, kBlock, 0, st>>>(g->normal, W, H, tw, ntiles_px, block_counts,
                                                (uint32_t*)ctx->active.ptr,
                                                (uint32_t*)ctx->act_index.ptr);
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
-    // rays are generated inside the traversal; this stage only sets their count
+    // rays are generated inside the traversal; the compaction scan set their count
     stage_begin(ctx, kStageRaygen);
-    if (ds.r_total > 0) k_gen_count<<<1, 64, 0, st>>>(dsd, counters);
     stage_end(ctx, kStageRaygen);
     LUZ_HIP(ctx, hipGetLastError());
 
