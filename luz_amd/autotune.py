@@ -84,7 +84,7 @@ def _store(cache, key, result):
         cache.write_text(json.dumps(known, indent=1))
 
 
-START = {"groups_per_cu": 6, "refill": 48, "stack_entries": 16, "tile_width": 32,
+START = {"groups_per_cu": 6, "refill": 48, "stack_entries": 16, "tile_width": 8,
          "ray_order": api.RAY_ORDER_AUTO}
 
 

@@ -39,7 +39,7 @@ struct luzrt_ctx {
     uint32_t groups_per_cu = 6;      // launch shape chosen by the autotuner on MI355X
     uint32_t refill_threshold = 48;
     uint32_t stack_entries = 16;   // LDS stack entries per lane: 8, 16 or 32
-    uint32_t tile_width = 32;      // pixel tiles of 64: 8x8, 16x4 or 32x2
+    uint32_t tile_width = 8;       // pixel tiles of 64: 8x8, 16x4 or 32x2
     uint32_t ray_order = 0;        // LUZRT_RAY_ORDER_*: automatic, slot-major or 64-pixel groups
     // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
     int precision = 0;

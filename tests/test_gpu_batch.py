@@ -51,7 +51,7 @@ def workload(ctx):
     wl = BatchWorkload(ctx, SCENES)
     yield wl
     wl.close()
-    ctx.set_trace_config(6, 48, 16, 32)   # the library defaults
+    ctx.set_trace_config(6, 48, 16, 8)   # the library defaults
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
