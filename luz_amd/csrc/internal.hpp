@@ -29,7 +29,7 @@ struct luzrt_ctx {
     std::string error;
     int num_cus = 0;
     // workspace
-    luz::DevBuf hits, active, act_index, counters, spill, lut, scene, vis_shadow,
+    luz::DevBuf hits, active, act_index, act_rec, counters, spill, lut, scene, vis_shadow,
         vis_ao;
     // timing
     hipEvent_t ev[luz::kStageCount][2] = {};
@@ -224,16 +224,13 @@ struct TraceLaunch {
     const uint32_t* scene_ray_base;   // device, nscenes + 1 entries
     const struct BatchTlas* scene_tlas;
     const uint32_t* scene_active;     // device, nscenes entries
-    // Visibility launch with rays generated in place (gen): ray
-    // ray_row(a, A, r_total, ray_group) is computed from active pixel a of the
-    // G-buffer and slot j of the scene block (A = counters[kCntActive]);
-    // ray_o and ray_d are then unused.
+    // Visibility launch with rays generated in place (gen): the ray of slot j
+    // of active pixel a (ray_row order, A = counters[kCntActive]) is computed
+    // from the pixel's record act_rec[2a], act_rec[2a + 1] (pixel_record) and
+    // the scene block; ray_o and ray_d are then unused.
     bool gen;
     const struct DevScene* ds;
-    const float* position;
-    const float* normal;
-    const uint32_t* act_list;
-    uint32_t width;
+    const float4* act_rec;
 };
 
 // One scene of a multi-scene launch: its TLAS (nodes null: empty TLAS) and

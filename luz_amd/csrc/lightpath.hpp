@@ -102,6 +102,26 @@ __device__ inline PixelFrame pixel_frame(const float* position, const float* nor
     return f;
 }
 
+// The frame of an active pixel as the compaction stores it for the traversal
+// (two 16-byte rows: position and camera distance; normal and x | y << 16),
+// so that each ray's generation reads one record instead of the active list,
+// then the position and normal planes, and does not recompute the distance.
+__device__ inline void pixel_record(const PixelFrame& f, float4& r0, float4& r1) {
+    r0 = make_float4(f.P.x, f.P.y, f.P.z, f.lc);
+    r1 = make_float4(f.N.x, f.N.y, f.N.z, __uint_as_float(f.x | (f.y << 16)));
+}
+
+__device__ inline PixelFrame pixel_from_record(float4 r0, float4 r1) {
+    PixelFrame f;
+    const uint32_t xy = __float_as_uint(r1.w);
+    f.x = xy & 0xFFFFu;
+    f.y = xy >> 16;
+    f.P = v3(r0.x, r0.y, r0.z);
+    f.N = v3(r1.x, r1.y, r1.z);
+    f.lc = r0.w;
+    return f;
+}
+
 // Shadow-ray origin: P lifted along N by max(0.01 x camera distance, 0.05)
 // (light.frag:137-146).
 __device__ inline V3 shadow_origin(const PixelFrame& f) {
@@ -149,17 +169,15 @@ __device__ inline V3 ao_dir(const PixelFrame& f, uint32_t frame, int s) {
               T.z * hx + B.z * hy + N.z * hz);
 }
 
-// The ray of slot j from active pixel a (G-buffer pixel p).  Slot j < r_shadow
+// The ray of slot j from the frame f of an active pixel.  Slot j < r_shadow
 // is shadow sample j - slot_base[l] of the last light l with
 // slot_base[l] <= j (lights without samples own no slots); slot j >= r_shadow
 // is AO sample j - r_shadow.  Shadow rays start at shadow_origin with
 // tmin 0.001 and end at the light (tmax = |light vector|); AO rays span
 // [ao_min, ao_max] from ao_origin.
-__device__ inline void gen_ray(const float* position, const float* normal, uint32_t W,
-                               const DevScene* ds, uint32_t p, uint32_t j, V3& o, V3& d, float& tmin,
-                               float& tmax) {
+__device__ inline void gen_ray(const DevScene* ds, const PixelFrame& f, uint32_t j, V3& o, V3& d,
+                               float& tmin, float& tmax) {
     const luzrt_scene& S = ds->s;
-    const PixelFrame f = pixel_frame(position, normal, W, S, p);
     const uint32_t frame = (uint32_t)S.frame;
     if (j < ds->r_shadow) {
         int lo = 0, hi = S.num_lights;

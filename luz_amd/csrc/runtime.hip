@@ -263,7 +263,7 @@ luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->hits, &ctx->active, &ctx->act_index,
+    DevBuf* bufs[] = {&ctx->hits, &ctx->active, &ctx->act_index, &ctx->act_rec,
                       &ctx->counters, &ctx->spill, &ctx->lut, &ctx->scene, &ctx->vis_shadow,
                       &ctx->vis_ao};
     for (DevBuf* b : bufs)

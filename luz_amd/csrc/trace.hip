@@ -82,10 +82,14 @@ __global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, u
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact_write(const float* normal, uint32_t W, uint32_t H,
-                                                          uint32_t tw, uint32_t ntiles_px,
+// Writes the active list, the per-pixel active index and the traversal's
+// per-active-pixel records (pixel_record).
+__global__ __launch_bounds__(kBlock) void k_compact_write(const float* position, const float* normal,
+                                                          uint32_t W, uint32_t H, uint32_t tw,
+                                                          uint32_t ntiles_px, const DevScene* ds,
                                                           const uint32_t* block_offsets,
-                                                          uint32_t* act_list, uint32_t* act_index) {
+                                                          uint32_t* act_list, uint32_t* act_index,
+                                                          float4* act_rec) {
     __shared__ uint32_t wave_tot[kBlock / 64];
     uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     uint32_t p;
@@ -93,7 +97,13 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const float* normal, u
     bool active = pixel_active(normal, g, W, H, tw, ntiles_px, p, inside);
     uint32_t total;
     uint32_t a = block_offsets[blockIdx.x] + block_prefix(active, wave_tot, total);
-    if (active) act_list[a] = p;
+    if (active) {
+        act_list[a] = p;
+        float4 r0, r1;
+        pixel_record(pixel_frame(position, normal, W, ds->s, p), r0, r1);
+        act_rec[2 * (size_t)a] = r0;
+        act_rec[2 * (size_t)a + 1] = r1;
+    }
     if (inside) act_index[p] = active ? a : kNone;
 }
 
@@ -209,18 +219,19 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                                          bt.ds->ray_group, slot, a);
                                 V3 o, d;
                                 float tmin, tmax;
-                                gen_ray(bt.position, bt.normal, bt.width, bt.ds, bt.act_list[a], slot, o,
-                                        d, tmin, tmax);
+                                gen_ray(bt.ds, pixel_frame(bt.position, bt.normal, bt.width, bt.ds->s,
+                                                           bt.act_list[a]),
+                                        slot, o, d, tmin, tmax);
                                 ray_begin(r, st, o, d, tmin, tmax);
                             }
                         } else {
                             if (SRC == kGen) {
                                 uint32_t slot, a;
                                 ray_slot(rid, A, R, G, slot, a);
+                                const float4 r0 = L.act_rec[2 * (size_t)a], r1 = L.act_rec[2 * (size_t)a + 1];
                                 V3 o, d;
                                 float tmin, tmax;
-                                gen_ray(L.position, L.normal, L.width, L.ds, L.act_list[a], slot, o, d,
-                                        tmin, tmax);
+                                gen_ray(L.ds, pixel_from_record(r0, r1), slot, o, d, tmin, tmax);
                                 ray_begin(r, st, o, d, tmin, tmax);
                             } else {
                                 float4 o = L.ray_o[(size_t)rid * L.ray_stride];
@@ -324,6 +335,7 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
         return fail(ctx, LUZRT_E_LIMIT, "%zu rays exceed the 32-bit ray index", max_rays);
     if ((s = ensure(ctx, ctx->counters, 64)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->active, npix * 4)) != LUZRT_OK) return s;
+    if ((s = ensure(ctx, ctx->act_rec, npix * 32)) != LUZRT_OK) return s;
     // act_index holds the per-pixel active index followed by the per-block counts
     const uint32_t tw = ctx->tile_width, th = 64u / tw;
     const size_t ntiles_px_sz = (size_t)((W + tw - 1) / tw) * ((H + th - 1) / th) * 64;
@@ -340,9 +352,9 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
     uint32_t* block_counts = (uint32_t*)ctx->act_index.ptr + npix;
     k_compact_count<<<nblocks, kBlock, 0, st>>>(g->normal, W, H, tw, ntiles_px, block_counts);
     k_compact_scan<<<1, 1024, 0, st>>>(block_counts, nblocks, counters, dsd);
-    k_compact_write<<<nblocks, kBlock, 0, st>>>(g->normal, W, H, tw, ntiles_px, block_counts,
-                                               (uint32_t*)ctx->active.ptr,
-                                               (uint32_t*)ctx->act_index.ptr);
+    k_compact_write<<<nblocks, kBlock, 0, st>>>(g->position, g->normal, W, H, tw, ntiles_px, dsd,
+                                               block_counts, (uint32_t*)ctx->active.ptr,
+                                               (uint32_t*)ctx->act_index.ptr, (float4*)ctx->act_rec.ptr);
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
@@ -363,10 +375,7 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
         L.wide = tlas->wide;
         L.gen = true;
         L.ds = dsd;
-        L.position = g->position;
-        L.normal = g->normal;
-        L.act_list = (const uint32_t*)ctx->active.ptr;
-        L.width = W;
+        L.act_rec = (const float4*)ctx->act_rec.ptr;
         if ((s = launch_trace(ctx, L)) != LUZRT_OK) return s;
     } else if (ds.r_total > 0) {
         // empty TLAS: nothing can be hit
