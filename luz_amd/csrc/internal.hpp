@@ -251,6 +251,12 @@ constexpr uint32_t kRayGroup = 64;   // pixels per ray group of the grouped ray 
 // of LDS stacks per workgroup leave room for five waves per SIMD).
 constexpr uint32_t kPrimaryGroupsPerCu = 5;
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
+// luzrt_trace_visibility's work: compaction, traversal with rays generated in
+// place and, if `resolve`, the count planes `shadow` and `ao`.  Without it the
+// hit bytes stay in ctx->hits (ray_row order over ctx->active, the active
+// count in ctx->counters) for a pass that reads them directly.
+luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* g,
+                             const luzrt_scene* scene, uint8_t* shadow, uint8_t* ao, bool resolve);
 
 // Validates a scene and fills its slot layout.
 luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds);

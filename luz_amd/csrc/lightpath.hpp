@@ -277,6 +277,31 @@ struct CountVisibility {
     }
 };
 
+// Visibility straight from the hit bytes of one active pixel (its slot-0 ray
+// `first` and the step between its slots, ray_row): the same fractions as
+// CountVisibility over the counts k_resolve would have written.
+struct HitVisibility {
+    const uint8_t* hits;
+    const DevScene* ds;
+    uint32_t first, step;
+    __device__ float shadow_frac(const luzrt_scene& S, uint32_t l, uint32_t) const {
+        const int ns = S.lights[l].num_shadow_samples;
+        if (ns <= 0) return 0.0f;
+        const uint32_t j0 = ds->slot_base[l];
+        uint32_t cnt = 0;
+        for (int s = 0; s < ns; ++s) cnt += hits[first + (j0 + s) * step];
+        return (float)cnt / (float)ns;
+    }
+    __device__ float ao_frac(const luzrt_scene& S, uint32_t) const {
+        const int na = S.ao_num_samples;
+        if (na <= 0) return 1.0f;
+        const uint32_t j0 = ds->r_shadow;
+        uint32_t miss = 0;
+        for (int s = 0; s < na; ++s) miss += 1u - hits[first + (j0 + s) * step];
+        return (float)miss / (float)na;
+    }
+};
+
 // Visibility from denoised fractions: visibility of light l (plane l) and
 // the AO fraction; lights without shadow samples are unshadowed, and AO is
 // 1 when the scene casts no AO rays, as with counts.

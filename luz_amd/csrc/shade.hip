@@ -28,6 +28,29 @@ __global__ __launch_bounds__(kBlock) void k_shade(const luzrt_gbuffer g, const D
     out[p] = shade_pixel(g, ds, s_lut, CountVisibility{shadow, ao, g.width * g.height}, p);
 }
 
+// Light pass without count planes: thread q shades background pixel q and
+// active pixel q (active order), the latter's visibility summed from its hit
+// bytes (HitVisibility), as k_resolve + k_shade would.
+__global__ __launch_bounds__(kBlock) void k_shade_hits(const luzrt_gbuffer g, const DevScene* ds,
+                                                       const float* lut, const uint32_t* act_index,
+                                                       const uint32_t* act_list, const uint8_t* hits,
+                                                       const uint32_t* counters, float4* out) {
+    __shared__ float s_lut[256];
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    const uint32_t npix = g.width * g.height;
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= npix) return;
+    const uint32_t A = counters[kCntActive];
+    if (act_index[q] == kNone) out[q] = shade_pixel(g, ds, s_lut, HitVisibility{hits, ds, 0u, 0u}, q);
+    if (q < A) {
+        uint32_t first, step;
+        ray_row(q, A, ds->r_total, ds->ray_group, first, step);
+        const uint32_t p = act_list[q];
+        out[p] = shade_pixel(g, ds, s_lut, HitVisibility{hits, ds, first, step}, p);
+    }
+}
+
 // The same lighting from denoised visibility and AO fractions.
 __global__ __launch_bounds__(kBlock) void k_shade_fractions(const luzrt_gbuffer g, const DevScene* ds,
                                                             const float* lut, const float* vis,
@@ -234,6 +257,19 @@ extern "C" luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas,
     luzrt_status s = check_gbuffer(ctx, g);
     if (s != LUZRT_OK) return s;
     const size_t npix = (size_t)g->width * g->height;
+    if (ctx->precision != LUZRT_PRECISION_COMPACT) {
+        // visibility without count planes, then lighting straight from the hit bytes
+        if ((s = visibility_core(ctx, tlas, g, scene, nullptr, nullptr, false)) != LUZRT_OK) return s;
+        if ((s = ensure_lut(ctx)) != LUZRT_OK) return s;
+        stage_begin(ctx, kStageShade);
+        k_shade_hits<<<cdiv(npix, kBlock), kBlock, 0, ctx->stream>>>(
+            *g, (const DevScene*)ctx->scene.ptr, (const float*)ctx->lut.ptr,
+            (const uint32_t*)ctx->act_index.ptr, (const uint32_t*)ctx->active.ptr,
+            (const uint8_t*)ctx->hits.ptr, (const uint32_t*)ctx->counters.ptr, (float4*)out);
+        stage_end(ctx, kStageShade);
+        LUZ_HIP(ctx, hipGetLastError());
+        return LUZRT_OK;
+    }
     int nl = scene->num_lights < 0 ? 0 : scene->num_lights;
     if ((s = ensure(ctx, ctx->vis_shadow, npix * (size_t)(nl > 0 ? nl : 1))) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->vis_ao, npix)) != LUZRT_OK) return s;

@@ -322,8 +322,8 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
 }
 
 // Counters are read back after the call to size the ray count.
-static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* g,
-                                    const luzrt_scene* scene, uint8_t* shadow, uint8_t* ao) {
+luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* g,
+                             const luzrt_scene* scene, uint8_t* shadow, uint8_t* ao, bool resolve) {
     DevScene ds;
     luzrt_status s = upload_scene(ctx, scene, &ds, ray_group_for(ctx, tlas));
     if (s != LUZRT_OK) return s;
@@ -383,6 +383,7 @@ static luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, cons
     }
     stage_end(ctx, kStageTrace);
 
+    if (!resolve) return LUZRT_OK;
     stage_begin(ctx, kStageResolve);
     k_resolve<<<cdiv(npix, kBlock), kBlock, 0, st>>>(dsd, (const uint32_t*)ctx->act_index.ptr,
                                                      (const uint32_t*)ctx->active.ptr,
@@ -437,5 +438,5 @@ extern "C" luzrt_status luzrt_trace_visibility(luzrt_ctx* ctx, const luzrt_tlas*
     if (!ctx) return LUZRT_E_INVALID_ARG;
     if (!tlas || !g || !scene || !ao || (scene->num_lights > 0 && !shadow))
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_trace_visibility: null argument");
-    return visibility_core(ctx, tlas, g, scene, shadow, ao);
+    return visibility_core(ctx, tlas, g, scene, shadow, ao, true);
 }
