@@ -634,6 +634,11 @@ constexpr int kTlasThreads = 1024;
 constexpr int kPlocRadius = 16;                              // clustering search radius (slots)
 constexpr int kPlocTile = kTlasThreads + 2 * kPlocRadius;    // staged boxes per neighbour pass
 
+// Threads of a TLAS build launch for m (a power of two >= the instance count):
+// one wave up to 64 instances (every step of the build then fits one wave and
+// its barriers cost little), else kTlasThreads.
+inline uint32_t tlas_threads(uint32_t m) { return m <= 64 ? 64u : (uint32_t)kTlasThreads; }
+
 // Merge cost of a cluster box (lo xyz, hi xyz) with box t of a staged tile
 // (six planes of `plane` floats): half the surface area of the union, FP32 as
 // written, NaN as +inf.
@@ -681,7 +686,7 @@ __device__ inline void block_ranks(bool head, bool keep, uint32_t (*s_wave)[kTla
     uint32_t oh = 0, ok = 0;
     th = 0;
     tk = 0;
-    for (uint32_t w = 0; w < kTlasThreads / 64; ++w) {
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
         const uint32_t ch = s_wave[0][w], ck = s_wave[1][w];
         if (w < wave) { oh += ch; ok += ck; }
         th += ch;
@@ -722,10 +727,13 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     // (within 64 KB for 8192 instances).
     extern __shared__ uint64_t skeys[];
     const int tid = threadIdx.x;
+    // launched with 64 threads (one wave, cheap barriers) up to 64 instances,
+    // else with kTlasThreads
+    const uint32_t nt = blockDim.x;
 
     // 1. world boxes of the BLAS root boxes, scene magnitude
     float mloc = 0.0f;
-    for (uint32_t i = tid; i < n; i += kTlasThreads) {
+    for (uint32_t i = tid; i < n; i += nt) {
         const Node& r = in[i].nodes[0];
         float rlo[3] = {fmin_(r.lox[0], r.lox[1]), fmin_(r.loy[0], r.loy[1]), fmin_(r.loz[0], r.loz[1])};
         float rhi[3] = {fmax_(r.hix[0], r.hix[1]), fmax_(r.hiy[0], r.hiy[1]), fmax_(r.hiz[0], r.hiz[1])};
@@ -753,11 +761,11 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     if (lane == 0) s_red[wave] = mloc;
     __syncthreads();
     float m_all = 0.0f;
-    for (uint32_t w = 0; w < kTlasThreads / 64; ++w) m_all = fmax_(m_all, s_red[w]);
+    for (uint32_t w = 0; w < nt / 64; ++w) m_all = fmax_(m_all, s_red[w]);
     const float pad = m_all * 0x1p-14f;
     // 2. padded boxes, FP64 centroid frame
     double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = tid; i < n; i += kTlasThreads) {
+    for (uint32_t i = tid; i < n; i += nt) {
         for (int k = 0; k < 3; ++k) {
             float lo = wbox[i * 6 + k] - pad, hi = wbox[i * 6 + 3 + k] + pad;
             wbox[i * 6 + k] = lo;
@@ -782,7 +790,7 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     double s_frame[6];
     for (int k = 0; k < 3; ++k) {
         double mn = INFINITY, mx = -INFINITY;
-        for (uint32_t w = 0; w < kTlasThreads / 64; ++w) {
+        for (uint32_t w = 0; w < nt / 64; ++w) {
             mn = s_redd[k][w] < mn ? s_redd[k][w] : mn;
             mx = s_redd[3 + k][w] > mx ? s_redd[3 + k][w] : mx;
         }
@@ -817,12 +825,12 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         }
         __syncthreads();
     } else {
-        for (uint32_t i = tid; i < m; i += kTlasThreads) skeys[i] = sort_key(i);
+        for (uint32_t i = tid; i < m; i += nt) skeys[i] = sort_key(i);
         __syncthreads();
     }
     for (uint32_t k = 2; m > 64 && k <= m; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < m; i += kTlasThreads) {
+            for (uint32_t i = tid; i < m; i += nt) {
                 uint32_t ixj = i ^ j;
                 if (ixj > i) {
                     uint64_t a = skeys[i], b = skeys[ixj];
@@ -838,7 +846,7 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     float* cb2 = J.cbox[1];
     uint32_t* cr = J.cref[0];
     uint32_t* cr2 = J.cref[1];
-    for (uint32_t s = tid; s < n; s += kTlasThreads) {
+    for (uint32_t s = tid; s < n; s += nt) {
         uint32_t i = (uint32_t)(skeys[s] & 0xFFFFFFFFu);
         const InstIn& src = in[i];
         InstRec rec;
@@ -878,9 +886,9 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         uint32_t count = n, next = n - 1;
         bool failed = false;
         // rounds over more clusters than threads: cluster arrays in global memory
-        while (count > kTlasThreads && !failed) {
-            for (uint32_t base = 0; base < count; base += kTlasThreads) {
-                for (int t = tid; t < kPlocTile; t += kTlasThreads) {
+        while (count > nt && !failed) {
+            for (uint32_t base = 0; base < count; base += nt) {
+                for (int t = tid; t < (int)nt + 2 * kPlocRadius; t += (int)nt) {
                     const int c = (int)base - kPlocRadius + t;
                     if (c >= 0 && c < (int)count)
                         for (int k = 0; k < 6; ++k) tile[k * kPlocTile + t] = cb[c * 6 + k];
@@ -891,7 +899,7 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
                 __syncthreads();
             }
             uint32_t heads = 0, kept = 0;
-            for (uint32_t base = 0; base < count; base += kTlasThreads) {
+            for (uint32_t base = 0; base < count; base += nt) {
                 const uint32_t i = base + tid;
                 bool head = false, keep = false;
                 uint32_t j = 0;
@@ -1159,7 +1167,7 @@ luzrt_status launch_tlas_batched(luzrt_ctx* ctx, const TlasJob* dev_jobs, uint32
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
     const size_t lds = tlas_lds_bytes(max_m);
-    k_tlas_build_batched<<<njobs, kTlasThreads, lds, ctx->stream>>>(dev_jobs, error);
+    k_tlas_build_batched<<<njobs, tlas_threads(max_m), lds, ctx->stream>>>(dev_jobs, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
 }
@@ -1233,7 +1241,7 @@ luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
     TlasJob J = tlas_job(t, dev_in, n);
     const size_t lds = tlas_lds_bytes(J.m);
-    k_tlas_build<<<1, kTlasThreads, lds, ctx->stream>>>(J, error);
+    k_tlas_build<<<1, tlas_threads(J.m), lds, ctx->stream>>>(J, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
 }
