@@ -219,10 +219,19 @@ struct BoxRay {
     float tlo, thi;
 };
 
+// Slab reciprocal by the hardware reciprocal (within 1 ulp, signed zeros to
+// signed infinities).  The box test only has to be conservative and its far
+// bound is widened by |tfar|·2^-20, eight ulps, so the reciprocal need not be
+// correctly rounded.  v_rcp_f32 flushes denormal inputs: the component is
+// scaled by 2^24 first (exact below 2^104) and the result scaled back.
+__device__ inline float slab_rcp(float d) {
+    return __builtin_amdgcn_rcpf(d * 0x1p24f) * 0x1p24f;
+}
+
 __device__ inline BoxRay box_setup(V3 o, V3 d, float tmin, float tmax) {
     BoxRay b;
     b.o = o;
-    b.inv = v3(__fdiv_rn(1.0f, d.x), __fdiv_rn(1.0f, d.y), __fdiv_rn(1.0f, d.z));
+    b.inv = v3(slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z));
     float m = t_margin(tmax);
     b.tlo = tmin - m;
     b.thi = tmax + m;
