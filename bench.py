@@ -198,9 +198,9 @@ def main() -> int:
         na = max(sb.ao_num_samples, 1)
         ao_rmse = float(np.sqrt(np.mean((gao[m64].astype(np.float64) / na - dao[m64] / na) ** 2))) if m64.any() else 0.0
         per_ray = (64.0 * st.node_visits + 36.0 * st.tri_tests + 64.0 * st.inst_visits) / max(st.rays, 1)
-        # node/tri/instance reads + the G-buffer texel each ray is generated from
-        # (active index, position, normal: 28 B) + hit write
-        trace_bytes = rays * (per_ray + 28.0 + 1.0)
+        # node/tri/instance reads + the per-pixel record each ray is generated from
+        # (position, camera distance, normal, x|y: 32 B) + hit write
+        trace_bytes = rays * (per_ray + 32.0 + 1.0)
         achieved = trace_bytes / (trace_avg * 1e-3) / 1e9
         record["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -210,7 +210,7 @@ def main() -> int:
             # algorithmic bytes can exceed the HBM rate; the measured HBM bytes per
             # launch over the same time give the HBM fraction actually drawn
             "traffic_frac": round(traffic / (trace_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-            "algorithmic_bytes_per_ray": round(per_ray + 29.0, 2),
+            "algorithmic_bytes_per_ray": round(per_ray + 33.0, 2),
             "per_ray_counts": {"node": round(st.node_visits / max(st.rays, 1), 3),
                                "tri": round(st.tri_tests / max(st.rays, 1), 3),
                                "inst": round(st.inst_visits / max(st.rays, 1), 3)},
@@ -551,11 +551,11 @@ def compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays) -> dict
     na = max(sb.ao_num_samples, 1)
     per_ray = (96.0 * st.wide_node_visits + 64.0 * st.node_visits + 36.0 * st.tri_tests +
                64.0 * st.inst_visits) / max(st.rays, 1)
-    achieved = rays * (per_ray + 29.0) / (trace_avg * 1e-3) / 1e9   # + G-buffer texel (28 B) + hit byte
+    achieved = rays * (per_ray + 33.0) / (trace_avg * 1e-3) / 1e9   # + per-pixel record (32 B) + hit byte
     return {
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace_any<wide>",
-                     "algorithmic_bytes_per_ray": round(per_ray + 29.0, 2)},
+                     "algorithmic_bytes_per_ray": round(per_ray + 33.0, 2)},
         "cpu_baseline": {"value": round(st.rays / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads,
                          "kind": "port",
                          "sample": f"oracle FP32 8-wide visibility on every {step32}th row ({st.rays} rays)"},
