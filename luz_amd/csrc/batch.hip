@@ -84,26 +84,35 @@ __global__ void k_batch_init(uint32_t* counters) {
 }
 
 // Persistent closest-hit primary rays over the 8x8-tile pixel slots of every
-// scene with a camera, in one flat index space.
+// scene with a camera, in one flat index space; each wave takes the next tile
+// (lane = pixel of the tile) from counters[kCntPrimary] until none is left.
 __global__ __launch_bounds__(kBlock) void k_bprimary(BatchTables T, uint32_t* spill,
-                                                     uint32_t* error_flag) {
+                                                     uint32_t* counters, uint32_t chunk) {
     __shared__ uint32_t lds[kShort * kBlock];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t total = gridDim.x * kBlock;
     Stack st;
     st.init(lds + tid, kBlock, spill + (size_t)blockIdx.x * kBlock + tid, total);
-    const uint32_t end = T.ppx[T.n];
-    for (uint32_t gg = blockIdx.x * kBlock + tid; gg < end; gg += total) {
-        uint32_t s = find_scene(T.ppx, T.n, gg);
-        const BatchSceneDev& S = T.sc[s];
-        uint32_t g = gg - T.ppx[s];
-        const uint32_t W = S.gout.width, H = S.gout.height;
-        const uint32_t tiles_x = (W + 7) / 8;
-        uint32_t tile = g >> 6, t = g & 63u;
-        uint32_t x = (tile % tiles_x) * 8 + (t & 7u), y = (tile / tiles_x) * 8 + (t >> 3);
-        if (x < W && y < H)
-            primary_pixel(S.tlas_nodes, S.recs, S.aux, S.cam, S.mats, S.nmats, S.gout, x, y, st,
-                          error_flag, S.tlas_nodes == nullptr);
+    const uint32_t ntiles = T.ppx[T.n] >> 6;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&counters[kCntPrimary], chunk);
+        base = __shfl(base, 0);
+        if (base >= ntiles) break;
+        const uint32_t end = min(base + chunk, ntiles);
+        for (uint32_t tile = base; tile < end; ++tile) {
+            const uint32_t gg = tile * 64u;
+            uint32_t s = find_scene(T.ppx, T.n, gg);
+            const BatchSceneDev& S = T.sc[s];
+            uint32_t g = gg - T.ppx[s];
+            const uint32_t W = S.gout.width, H = S.gout.height;
+            const uint32_t tiles_x = (W + 7) / 8;
+            uint32_t t = g >> 6;
+            uint32_t x = (t % tiles_x) * 8 + (lane & 7u), y = (t / tiles_x) * 8 + (lane >> 3);
+            if (x < W && y < H)
+                primary_pixel(S.tlas_nodes, S.recs, S.aux, S.cam, S.mats, S.nmats, S.gout, x, y,
+                              st, &counters[kCntError], S.tlas_nodes == nullptr);
+        }
     }
 }
 
@@ -341,8 +350,8 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     if (b->total_ppx) {
         uint32_t groups = (uint32_t)ctx->num_cus * kPrimaryGroupsPerCu;
         if (groups == 0) groups = 1;
-        k_bprimary<<<groups, kBlock, 0, st>>>(b->tables, (uint32_t*)ctx->spill.ptr,
-                                              b->counters + kCntError);
+        k_bprimary<<<groups, kBlock, 0, st>>>(b->tables, (uint32_t*)ctx->spill.ptr, b->counters,
+                                              primary_chunk(b->total_ppx / 64, groups));
     }
     const uint32_t tw = ctx->tile_width;
     k_bcompact_count<<<b->total_cblk, kBlock, 0, st>>>(b->tables, tw);

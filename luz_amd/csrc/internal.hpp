@@ -202,7 +202,7 @@ struct DevScene {
 
 // Device counters of one visibility call (ctx->counters).
 enum Counter { kCntFetch = 0, kCntActive = 1, kCntError = 2, kCntRays = 3, kCntBuildError = 4,
-               kCntCount = 8 };
+               kCntPrimary = 5, kCntCount = 8 };
 
 struct TraceLaunch {
     const Node* tlas_nodes;
@@ -248,8 +248,16 @@ struct BatchTlas {
 constexpr uint32_t kMaxBatch = LUZRT_MAX_BATCH;
 constexpr uint32_t kRayGroup = 64;   // pixels per ray group of the grouped ray order (ray_row)
 // Persistent workgroups per CU of the primary-ray kernels (92 VGPRs and 16 KB
-// of LDS stacks per workgroup leave room for five waves per SIMD).
+// of LDS stacks per workgroup leave room for five waves per SIMD).  Their waves
+// claim 8x8 pixel tiles from the kCntPrimary counter, `primary_chunk` tiles
+// per claim: about 16 claims per wave, so that the claims of a large batch
+// do not queue on the one counter.
 constexpr uint32_t kPrimaryGroupsPerCu = 5;
+inline uint32_t primary_chunk(uint64_t ntiles, uint32_t groups) {
+    const uint64_t waves = (uint64_t)groups * 4;
+    const uint64_t c = ntiles / (waves * 16);
+    return c < 1 ? 1u : (c > 64 ? 64u : (uint32_t)c);
+}
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
 // luzrt_trace_visibility's work: compaction, traversal with rays generated in
 // place and, if `resolve`, the count planes `shadow` and `ao`.  Without it the

@@ -15,30 +15,42 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Clears the error flag of the call with a kernel (a caller may capture the
-// call in a graph, where memset nodes are avoided).
-__global__ void k_primary_init(uint32_t* error_flag) {
-    if (threadIdx.x == 0) *error_flag = 0u;
+// Clears the error flag and the tile counter of the call with a kernel (a
+// caller may capture the call in a graph, where memset nodes are avoided).
+__global__ void k_primary_init(uint32_t* counters) {
+    if (threadIdx.x == 0) {
+        counters[kCntError] = 0u;
+        counters[kCntPrimary] = 0u;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_primary(const Node* tlas_nodes, const InstRec* recs,
                                                     const InstAux* aux, luzrt_camera cam,
                                                     const luzrt_material* mats, uint32_t nmats,
                                                     luzrt_gbuffer_out out, uint32_t* spill,
-                                                    uint32_t* error_flag, bool empty) {
+                                                    uint32_t* counters, uint32_t chunk, bool empty) {
     __shared__ uint32_t lds[kShort * kBlock];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t total = gridDim.x * kBlock;
     Stack st;
     st.init(lds + tid, kBlock, spill + (size_t)blockIdx.x * kBlock + tid, total);
     const uint32_t W = out.width, H = out.height;
     const uint32_t tiles_x = (W + 7) / 8;
-    const uint32_t ntiles_px = tiles_x * ((H + 7) / 8) * 64;
-    for (uint32_t g = blockIdx.x * kBlock + tid; g < ntiles_px; g += total) {
-        uint32_t tile = g >> 6, t = g & 63u;
-        uint32_t x = (tile % tiles_x) * 8 + (t & 7u), y = (tile / tiles_x) * 8 + (t >> 3);
-        if (x < W && y < H)
-            primary_pixel(tlas_nodes, recs, aux, cam, mats, nmats, out, x, y, st, error_flag, empty);
+    const uint32_t ntiles = tiles_x * ((H + 7) / 8);
+    // each wave takes the next `chunk` 8x8 tiles (lane = pixel of a tile) until
+    // none is left: waves that drew cheap tiles take more of them
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&counters[kCntPrimary], chunk);
+        base = __shfl(base, 0);
+        if (base >= ntiles) break;
+        const uint32_t end = min(base + chunk, ntiles);
+        for (uint32_t tile = base; tile < end; ++tile) {
+            uint32_t x = (tile % tiles_x) * 8 + (lane & 7u), y = (tile / tiles_x) * 8 + (lane >> 3);
+            if (x < W && y < H)
+                primary_pixel(tlas_nodes, recs, aux, cam, mats, nmats, out, x, y, st,
+                              &counters[kCntError], empty);
+        }
     }
 }
 
@@ -73,10 +85,11 @@ extern "C" luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* 
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * 256 * kSpill * 4)) != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
     stage_begin(ctx, kStagePrimary);
-    k_primary_init<<<1, 64, 0, ctx->stream>>>(counters + kCntError);
+    k_primary_init<<<1, 64, 0, ctx->stream>>>(counters);
+    const uint32_t ntiles = ((out->width + 7) / 8) * ((out->height + 7) / 8);
     k_primary<<<groups, 256, 0, ctx->stream>>>(tlas->nodes, tlas->recs, tlas->aux, *cam, dmats, nmats,
-                                              *out, (uint32_t*)ctx->spill.ptr, counters + kCntError,
-                                              tlas->count == 0);
+                                              *out, (uint32_t*)ctx->spill.ptr, counters,
+                                              primary_chunk(ntiles, groups), tlas->count == 0);
     stage_end(ctx, kStagePrimary);
     LUZ_HIP(ctx, hipGetLastError());
     LUZ_HIP(ctx, hipFreeAsync(dmats, ctx->stream));
