@@ -116,6 +116,49 @@ def test_batch_grouped_ray_order_matches_oracle(ctx):
         wl.close()
 
 
+def test_batch_mixes_cameras_and_caller_gbuffers(ctx):
+    """Scenes without a camera keep the G-buffer their caller wrote (no primary
+    tiles of their own in the batch's tile space, between scenes that have
+    them) and still match the single-scene calls bit for bit."""
+    from types import SimpleNamespace
+    specs = [SCENES[0], SCENES[1], SCENES[2], SCENES[4]]
+    frame = 2
+    hosts, gpus, gbufs, outs, blocks, entries = [], [], [], [], [], []
+    for i, (seed, budget, w, hgt) in enumerate(specs):
+        h = api.HostScene(api.SCENE_RANDOM, seed, budget, w, hgt)
+        g = api.GPUScene(ctx, h)
+        h.animate(frame)
+        g.update_resources_gpu(h)
+        W, H, cam = h.camera()
+        _, _, mats = h.nodes()
+        sb = h.pack()
+        gb = api.GBufferDevice(ctx, W, H)
+        caller = i in (0, 2)   # the first scene and one in the middle
+        if caller:
+            ctx.primary_gbuffer(g.tlas, cam, mats, gb.desc())
+        bufs = (ctx.buffer(max(sb.num_lights, 1) * W * H), ctx.buffer(W * H), ctx.buffer(W * H * 16))
+        hosts.append(h); gpus.append(g); gbufs.append(gb); outs.append(bufs); blocks.append(sb)
+        entries.append({"tlas": g.tlas, "gbuffer": gb.desc(), "scene": sb,
+                        "camera": None if caller else cam, "materials": [] if caller else mats,
+                        "shadow": bufs[0], "ao": bufs[1], "out": bufs[2]})
+    batch = api.Batch(ctx, entries)
+    try:
+        for i, (h, g) in enumerate(zip(hosts, gpus)):
+            arr, n = g.instances(h)
+            batch.set_instances(i, arr, n)
+        batch.frame(frame, True)
+        wl = SimpleNamespace(hosts=hosts, gpus=gpus)
+        for i in range(len(specs)):
+            n = gbufs[i].width * gbufs[i].height
+            L = blocks[i].num_lights
+            got = (gbufs[i].download(), outs[i][0].download(np.uint8, (L, n)),
+                   outs[i][1].download(np.uint8, n), outs[i][2].download(np.float32, (n, 4)))
+            print(f"mixed batch scene {i}: camera={entries[i]['camera'] is not None}", flush=True)
+            assert_same(got, single_scene(ctx, wl, i, frame), f"mixed scene {i}")
+    finally:
+        batch.close()
+
+
 def test_batch_recaptures_on_config_change(ctx, workload):
     want = None
     for cfg in [(6, 48, 16, 32), (6, 16, 8, 16), (8, 32, 32, 8)]:
