@@ -12,8 +12,9 @@
 //               (ballot-based compaction of the wave's work); each ray
 //               (ray_row order: a wave holds one light sample or AO sample
 //               of 64 neighbouring pixels) is generated at claim time from
-//               the G-buffer texel of its pixel (gen_ray), so no ray records
-//               go through memory (they remain for luzrt_trace_rays);
+//               the 32-byte record of its pixel that the compaction wrote
+//               (gen_ray), so no ray records go through memory (they remain
+//               for luzrt_trace_rays);
 //   k_resolve   per active pixel (in active order), occluded shadow samples
 //               per light and unoccluded AO samples, written as u8 planes;
 //               background pixels get zero counts.
