@@ -1669,6 +1669,17 @@ int orc_any_hit_wide(const orc_scene* s, const float* rays, uint32_t n, uint8_t*
     return 0;
 }
 
+// FP64 any-hit of caller-given rays (8 doubles each: origin, tmin, direction,
+// tmax) through traverse_d: the visibility decision the FP64 frame uses.
+int orc_any_hit_f64(const orc_scene* s, const double* rays, uint32_t n, uint8_t* hits) {
+    if (!s) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* r = rays + (size_t)i * 8;
+        hits[i] = traverse_d(*s, dk(r[0], r[1], r[2]), dk(r[4], r[5], r[6]), r[3], r[7]) ? 1 : 0;
+    }
+    return 0;
+}
+
 int orc_trace_visibility_f64(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
                              uint8_t* shadow, uint8_t* ao, int threads, uint32_t row_step) {
     if (!s || !g || !p || p->num_lights < 0 || p->num_lights > 64) return -1;

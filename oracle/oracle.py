@@ -99,6 +99,7 @@ _sigs = {
     "orc_blas_wide_arrays": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
                                        C.POINTER(_P), C.POINTER(C.c_uint32)]),
     "orc_any_hit_wide": (C.c_int, [_P, _P, C.c_uint32, _P, C.POINTER(OrcStats)]),
+    "orc_any_hit_f64": (C.c_int, [_P, _P, C.c_uint32, _P]),
     "orc_trace_visibility_wide": (C.c_int, [_P, C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P,
                                             C.POINTER(OrcStats), C.c_int, C.c_uint32]),
     "orc_visibility_rays": (C.c_int64, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, C.c_int64]),
@@ -269,6 +270,13 @@ class OracleScene:
         st = OrcStats()
         lib.orc_any_hit_wide(self.h, _p(r), len(r), _p(hits), C.byref(st))
         return hits, st
+
+    def any_hit_f64(self, rays: np.ndarray) -> np.ndarray:
+        """FP64 any-hit of rays given as 8 doubles (origin, tmin, direction, tmax)."""
+        r = np.ascontiguousarray(rays, np.float64).reshape(-1, 8)
+        hits = np.zeros(len(r), np.uint8)
+        lib.orc_any_hit_f64(self.h, _p(r), len(r), _p(hits))
+        return hits
 
     def any_hit(self, rays: np.ndarray, brute: bool = False):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
