@@ -48,6 +48,7 @@ extern "C" {
 #define LUZRT_LIGHT_SPOT 1
 #define LUZRT_LIGHT_DIRECTIONAL 2
 
+#define LUZRT_SHADOW_DISABLED 0   /* AssetManager.hpp:41 ShadowDisabled */
 #define LUZRT_SHADOW_RAYTRACED 1  /* LuzCommon.h:28 SHADOW_TYPE_RAYTRACING */
 #define LUZRT_SHADOW_MAP 2        /* LuzCommon.h:29 SHADOW_TYPE_MAP */
 

@@ -70,7 +70,8 @@ int      luzscene_view_proj(const luzscene* s, const float jitter[2], float view
  * and spot lights, the position and shadowMapFar for point lights; depth is
  * left NULL for the caller to allocate (res^2 or 6 res^2 floats). */
 int      luzscene_shadow_map(const luzscene* s, uint32_t light, uint32_t resolution, luzrt_shadow_map* out);
-/* SceneAsset::shadowType: LUZRT_SHADOW_RAYTRACED or LUZRT_SHADOW_MAP. */
+/* SceneAsset::shadowType: LUZRT_SHADOW_DISABLED, LUZRT_SHADOW_RAYTRACED or
+ * LUZRT_SHADOW_MAP (AssetManager.hpp:40-44). */
 int      luzscene_set_shadow_type(luzscene* s, int type);
 /* The camera jitter of frame `index` (CameraNode::NextJitter): Halton(2,3). */
 void     luzscene_jitter(uint32_t index, uint32_t width, uint32_t height, float jitter[2]);

@@ -29,7 +29,7 @@ LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
 SCENE_CUBE, SCENE_SPHERES, SCENE_TERRAIN, SCENE_RANDOM = 1, 2, 4, 5
 RAY_ORDER_AUTO, RAY_ORDER_SLOT, RAY_ORDER_GROUPED = 0, 1, 2
 PRECISION_PLAIN, PRECISION_COMPACT = 0, 1
-SHADOW_RAYTRACED, SHADOW_MAP = 1, 2
+SHADOW_DISABLED, SHADOW_RAYTRACED, SHADOW_MAP = 0, 1, 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "device error", -4: "in use",
           -5: "limit exceeded", -6: "no device", -7: "unsupported"}
@@ -56,6 +56,13 @@ class SceneBlock(C.Structure):
                 ("ao_min", C.c_float), ("ao_max", C.c_float),
                 ("ao_num_samples", C.c_int32), ("frame", C.c_int32),
                 ("shadow_type", C.c_int32), ("pad", C.c_int32 * 3)]
+
+    def __init__(self, *args, **kwargs):
+        # a new block is ray-traced, as a new Luz scene is (SceneAsset::shadowType);
+        # the zeroed C struct would mean ShadowDisabled
+        super().__init__(*args, **kwargs)
+        if "shadow_type" not in kwargs:
+            self.shadow_type = SHADOW_RAYTRACED
 
 
 class GBuffer(C.Structure):
@@ -496,7 +503,7 @@ class HostScene:
 
     def set_shadow_type(self, kind: int) -> None:
         if lib.luzscene_set_shadow_type(self.handle, kind):
-            raise LuzError(-1, "shadow type must be SHADOW_RAYTRACED or SHADOW_MAP")
+            raise LuzError(-1, "shadow type must be SHADOW_DISABLED, SHADOW_RAYTRACED or SHADOW_MAP")
 
     def set_samples(self, light_samples: int, ao_samples: int, ao_min: float = 1e-4, ao_max: float = 1.0) -> None:
         if lib.luzscene_set_samples(self.handle, light_samples, ao_samples, ao_min, ao_max):

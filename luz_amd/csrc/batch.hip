@@ -448,7 +448,7 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         };
         if (!d.tlas || !d.rgba32f_out || !d.ao_counts || (d.scene.num_lights > 0 && !d.shadow_counts))
             return bad(LUZRT_E_INVALID_ARG, "null TLAS or output");
-        if ((st = fill_scene(ctx, &d.scene, &ds[s])) != LUZRT_OK) { batch_free(b); return st; }
+        if ((st = fill_scene(ctx, &d.scene, &ds[s], true)) != LUZRT_OK) { batch_free(b); return st; }
         ds[s].ray_group = ctx->ray_order == LUZRT_RAY_ORDER_GROUPED ? kRayGroup : 0;
         const luzrt_gbuffer& g = d.gbuffer;
         if (!g.position || !g.normal || !g.albedo || !g.material || !g.emission)

@@ -266,8 +266,12 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
 luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* g,
                              const luzrt_scene* scene, uint8_t* shadow, uint8_t* ao, bool resolve);
 
-// Validates a scene and fills its slot layout.
-luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds);
+// Validates a scene and fills its slot layout.  For a visibility call
+// (`trace`) the lights of a scene that is not LUZRT_SHADOW_RAYTRACED own no
+// shadow slots and their sample counts read 0 in the block: the reference
+// packs numShadowSamples = 0 in those scenes (GPUScene.cpp:248) and casts
+// shadow rays only in ray-traced ones (light.frag:141).
+luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds, bool trace);
 // Copies `bytes` from host memory `src` to device memory `dst` on the
 // context's stream through a pinned staging buffer: `src` may be reused as soon
 // as the call returns, and the call does not wait for the GPU (a staging buffer
@@ -275,7 +279,7 @@ luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds);
 luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes);
 // fill_scene, then uploads the block to ctx->scene with the given ray group.
 luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy,
-                          uint32_t ray_group = 0);
+                          uint32_t ray_group = 0, bool trace = false);
 // Ray group of a visibility call on `tlas` under the context's ray order.
 uint32_t ray_group_for(const luzrt_ctx* ctx, const luzrt_tlas* tlas);
 luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g);

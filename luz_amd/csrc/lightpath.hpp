@@ -317,6 +317,23 @@ struct FractionVisibility {
     }
 };
 
+// EvaluateShadow's result by shadow type (light.frag:141-168): ray-traced
+// scenes take the traced fraction (0 without samples, TraceShadowRay
+// :87-89); shadow-map scenes take the map decision supplied as one sample
+// per light and are fully shadowed for a light without one (shadowMap ==
+// -1); every other type (ShadowDisabled) is fully shadowed.  Visibility
+// calls zero the sample counts of non-ray-traced scenes, so their light pass
+// is fully shadowed as the reference's is.
+__device__ inline bool shadow_from_samples(const luzrt_scene& S, uint32_t l) {
+    return S.shadow_type == LUZRT_SHADOW_RAYTRACED ||
+           (S.shadow_type == LUZRT_SHADOW_MAP && S.lights[l].num_shadow_samples > 0);
+}
+
+template <typename VIS>
+__device__ inline float shadow_factor(const luzrt_scene& S, const VIS& vis, uint32_t l, uint32_t p) {
+    return shadow_from_samples(S, l) ? vis.shadow_frac(S, l, p) : 1.0f;
+}
+
 // The lighting arithmetic is compared with the oracle to 1e-3 relative, not
 // bit for bit (only visibility decisions are), so it uses the hardware
 // reciprocal, reciprocal square root and square root (about 1 ulp each)
@@ -370,7 +387,7 @@ __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds,
         } else {
             att = rcp_hw(vdot(Lr, Lr));
         }
-        float sh = vis.shadow_frac(S, l, p);
+        float sh = shadow_factor(S, vis, l, p);
         float scale = Lb.intensity * att * (1.0f - sh);
         V3 H = vnorm_hw(vadd(V, L));
         float NdotH = fmax_(vdot(N, H), 0.0f);

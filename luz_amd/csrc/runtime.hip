@@ -87,13 +87,15 @@ luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g) {
     return LUZRT_OK;
 }
 
-luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds) {
+luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds, bool trace) {
     if (scene->num_lights < 0 || scene->num_lights > LUZRT_MAX_LIGHTS)
         return fail(ctx, LUZRT_E_LIMIT, "num_lights %d outside 0..%d", scene->num_lights,
                     LUZRT_MAX_LIGHTS);
     if (scene->ao_num_samples < 0 || scene->ao_num_samples > LUZRT_MAX_SAMPLES)
         return fail(ctx, LUZRT_E_LIMIT, "ao_num_samples %d outside 0..%d", scene->ao_num_samples,
                     LUZRT_MAX_SAMPLES);
+    if (scene->shadow_type < LUZRT_SHADOW_DISABLED || scene->shadow_type > LUZRT_SHADOW_MAP)
+        return fail(ctx, LUZRT_E_INVALID_ARG, "shadow_type %d is not one of 0, 1, 2", scene->shadow_type);
     std::memset(ds, 0, sizeof(*ds));
     ds->s = *scene;
     uint32_t j = 0;
@@ -104,16 +106,20 @@ luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds) 
         if (L.num_shadow_samples < 0 || L.num_shadow_samples > LUZRT_MAX_SAMPLES)
             return fail(ctx, LUZRT_E_LIMIT, "light %d num_shadow_samples %d outside 0..%d", l,
                         L.num_shadow_samples, LUZRT_MAX_SAMPLES);
+        // shadow rays only in ray-traced scenes (GPUScene.cpp:248 packs
+        // numShadowSamples = 0 otherwise; light.frag:141)
+        if (trace && scene->shadow_type != LUZRT_SHADOW_RAYTRACED) ds->s.lights[l].num_shadow_samples = 0;
         ds->slot_base[l] = j;
-        j += (uint32_t)L.num_shadow_samples;
+        j += (uint32_t)ds->s.lights[l].num_shadow_samples;
     }
     ds->r_shadow = j;
     ds->r_total = j + (uint32_t)scene->ao_num_samples;
     return LUZRT_OK;
 }
 
-luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds, uint32_t ray_group) {
-    luzrt_status s = fill_scene(ctx, scene, ds);
+luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds, uint32_t ray_group,
+                          bool trace) {
+    luzrt_status s = fill_scene(ctx, scene, ds, trace);
     if (s != LUZRT_OK) return s;
     ds->ray_group = ray_group;
     s = ensure(ctx, ctx->scene, sizeof(DevScene));

@@ -684,7 +684,7 @@ void luzscene_jitter(uint32_t index, uint32_t width, uint32_t height, float jitt
 }
 
 int luzscene_set_shadow_type(luzscene* s, int type) {
-    if (!s || (type != LUZRT_SHADOW_RAYTRACED && type != LUZRT_SHADOW_MAP)) return -1;
+    if (!s || type < LUZRT_SHADOW_DISABLED || type > LUZRT_SHADOW_MAP) return -1;
     s->shadowType = type;
     return 0;
 }

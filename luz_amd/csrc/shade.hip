@@ -143,12 +143,14 @@ __global__ __launch_bounds__(kBlock) void k_shade_compact(const luzrt_gbuffer g,
     for (int l = 0; l < nl; l += 2) {
         const bool pair = l + 1 < nl;
         int n0 = S.lights[l].num_shadow_samples;
-        float sh0 = n0 > 0 ? (float)shadow[(size_t)l * npix + p] / (float)n0 : 0.0f;
+        float sh0 = !shadow_from_samples(S, l) ? 1.0f
+                    : (n0 > 0 ? (float)shadow[(size_t)l * npix + p] / (float)n0 : 0.0f);
         LightGeo g0 = light_geo(S.lights[l], P, N, V, a2, kk, NdotV, gv, sh0);
         LightGeo g1 = g0;
         if (pair) {
             int n1 = S.lights[l + 1].num_shadow_samples;
-            float sh1 = n1 > 0 ? (float)shadow[(size_t)(l + 1) * npix + p] / (float)n1 : 0.0f;
+            float sh1 = !shadow_from_samples(S, l + 1) ? 1.0f
+                        : (n1 > 0 ? (float)shadow[(size_t)(l + 1) * npix + p] / (float)n1 : 0.0f);
             g1 = light_geo(S.lights[l + 1], P, N, V, a2, kk, NdotV, gv, sh1);
         }
         // packed FP16, lane x = light l, lane y = light l+1

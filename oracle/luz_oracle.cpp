@@ -1283,6 +1283,26 @@ inline float srgb_to_linear(uint32_t c) { return std::pow((float)c / 255.0f, 2.2
 template <typename R>
 struct Shade3 { R x, y, z; };
 
+// Shadow rays are cast only in ray-traced scenes: GPUScene.cpp:248 packs
+// numShadowSamples = lightSamples only for ShadowRayTraced, and
+// EvaluateShadow calls TraceShadowRay only then (light.frag:141).
+inline int traced_samples(const orc_params* p, int l) {
+    return p->shadow_type == 1 ? p->lights[l].num_shadow_samples : 0;
+}
+
+// EvaluateShadow's result by shadow type (light.frag:141-168): ray-traced
+// scenes take occluded/samples (0 without samples, TraceShadowRay :87-89);
+// shadow-map scenes take the map decision supplied as one sample per light,
+// and are fully shadowed for a light without one (light.shadowMap == -1);
+// every other type (ShadowDisabled) is fully shadowed.
+template <typename R>
+inline R shadow_factor(const orc_params* p, const uint8_t* shadow, int l, size_t npix, size_t idx) {
+    const int ns = p->lights[l].num_shadow_samples;
+    const bool from_counts = p->shadow_type == 1 || (p->shadow_type == 2 && ns > 0);
+    if (!from_counts) return (R)1;
+    return ns > 0 ? (R)shadow[(size_t)l * npix + idx] / (R)ns : (R)0;
+}
+
 template <typename R>
 void shade_pixel(const orc_gbuffer* g, const orc_params* p, const uint8_t* shadow,
                  const uint8_t* ao, size_t idx, size_t npix, R out[4]) {
@@ -1340,8 +1360,7 @@ void shade_pixel(const orc_gbuffer* g, const orc_params* p, const uint8_t* shado
             R dist = len3(Lr);
             att = (R)1 / (dist * dist);
         }
-        int ns = Lb.num_shadow_samples;
-        R sh = ns > 0 ? (R)shadow[(size_t)l * npix + idx] / (R)ns : (R)0;
+        R sh = shadow_factor<R>(p, shadow, l, npix, idx);
         R scale = (R)Lb.intensity * att * ((R)1 - sh);
         Shade3<R> H = nrm3(add3(V, L));
         R NdotH = mx(dot3(N, H), (R)0);
@@ -1586,7 +1605,7 @@ static int trace_visibility_impl(const orc_scene* s, const orc_gbuffer* g, const
             f3 P = load3(g->position + idx * 3);
             for (int l = 0; l < p->num_lights; ++l) {
                 const orc_light& L = p->lights[l];
-                if (L.num_shadow_samples <= 0) continue;
+                if (traced_samples(p, l) <= 0) continue;
                 ShadowSetup ss = shadow_setup(L, P);
                 int cnt = 0;
                 for (int smp = 0; smp < ss.n; ++smp) {
@@ -1701,7 +1720,7 @@ int orc_trace_visibility_f64(const orc_scene* s, const orc_gbuffer* g, const orc
             d3 aoo = dadd(P, dmul(N, lc * 0.01));
             for (int l = 0; l < p->num_lights; ++l) {
                 const orc_light& L = p->lights[l];
-                if (L.num_shadow_samples <= 0) continue;
+                if (traced_samples(p, l) <= 0) continue;
                 d3 lpos = dk(L.position[0], L.position[1], L.position[2]);
                 d3 Lraw = dsub(lpos, P);
                 double dist = dlength(Lraw);
@@ -1766,7 +1785,7 @@ int64_t orc_visibility_rays(const orc_gbuffer* g, const orc_params* p, float* ra
                 n++;
             };
             for (int l = 0; l < p->num_lights; ++l) {
-                if (p->lights[l].num_shadow_samples <= 0) continue;
+                if (traced_samples(p, l) <= 0) continue;
                 ShadowSetup ss = shadow_setup(p->lights[l], P);
                 for (int smp = 0; smp < ss.n; ++smp) {
                     float u, v;

@@ -193,3 +193,59 @@ def test_light_pass_radiance(ctx, kind, res):
     print(f"max rel err {float((err / np.maximum(np.abs(want), 1e-3)).max()):.3g}", flush=True)
     assert np.all(err <= tol)
     g.close()
+
+
+@pytest.mark.parametrize("shadow_type", [api.SHADOW_RAYTRACED, api.SHADOW_DISABLED, api.SHADOW_MAP])
+def test_spot_light_and_shadow_types(ctx, shadow_type):
+    """The cube scene with a spot light and a point light beside its
+    directional light, two soft-shadow samples per light and 8 AO samples,
+    in each shadow type: counts bit-exact and the light pass within 1e-3 of
+    the oracle.  The spot's cone (light.frag:199-204, angles in radians)
+    lights part of the frame; without ray-traced shadows no shadow ray is
+    cast and every light is fully shadowed (light.frag:141-168), so only
+    ambient and emission remain."""
+    s = api.HostScene(api.SCENE_CUBE, 0)
+    s.add_light(api.LIGHT_SPOT, (1.5, 3.0, 1.0), direction=(-0.45, -0.85, -0.3), color=(1.0, 0.8, 0.6),
+                intensity=40.0, radius=0.3)
+    s.add_light(api.LIGHT_POINT, (-2.0, 2.5, 1.5), color=(0.3, 0.5, 1.0), intensity=25.0, radius=0.2)
+    s.set_samples(2, 8)
+    s.set_shadow_type(shadow_type)
+    g = api.GPUScene(ctx, s)
+    o = oracle_scene(s)
+    W, H, gd, got, _ = _gbuffers(ctx, s, g, o)
+    sb = s.pack()
+    assert sb.num_lights == 3 and sb.lights[1].type == api.LIGHT_SPOT
+    nl = sb.num_lights
+    dsh, dao = ctx.buffer(nl * W * H), ctx.buffer(W * H)
+    ctx.trace_visibility(g.tlas, gd.desc(), sb, dsh, dao)
+    gsh, gao = dsh.download(np.uint8, (nl, W * H)), dao.download(np.uint8, W * H)
+    rays = ctx.timings().rays
+    wsh, wao, st = o.trace_visibility(got, W, H, sb)
+    active = int((np.abs(got["normal"]).sum(1) != 0).sum())
+    per_px = 8 + (2 * nl if shadow_type == api.SHADOW_RAYTRACED else 0)
+    print(f"shadow type {shadow_type}: {rays} rays, shadowed samples per light {wsh.sum(1)}", flush=True)
+    assert rays == st.rays == active * per_px
+    np.testing.assert_array_equal(gsh, wsh)
+    np.testing.assert_array_equal(gao, wao)
+    out = ctx.buffer(W * H * 16)
+    ctx.light_pass(g.tlas, gd.desc(), sb, out)
+    rad = out.download(np.float32, (W * H, 4))
+    want = orc.shade(got, W, H, sb, wsh, wao)
+    err = np.abs(rad - want) / np.maximum(np.abs(want), 1e-3)
+    print(f"max rel err {float(err.max()):.3g}", flush=True)
+    assert err.max() <= RADIANCE_RTOL
+    # the spot light's share of the frame, from the FP64 oracle with and without it
+    dark = api.SceneBlock()
+    import ctypes as C
+    C.memmove(C.byref(dark), C.byref(sb), C.sizeof(api.SceneBlock))
+    dark.lights[1].intensity = 0.0
+    spot = np.abs(orc.shade(got, W, H, sb, wsh, wao, f64=True) - orc.shade(got, W, H, dark, wsh, wao, f64=True))
+    lit_by_spot = int((spot[:, :3].max(1) > 1e-4).sum())
+    if shadow_type == api.SHADOW_RAYTRACED:
+        assert wsh.sum() > 100 and lit_by_spot > 1000
+    else:
+        assert wsh.sum() == 0 and lit_by_spot == 0
+        for l in range(nl):
+            dark.lights[l].intensity = 0.0
+        np.testing.assert_allclose(rad, orc.shade(got, W, H, dark, wsh, wao), rtol=1e-6, atol=1e-7)
+    g.close()

@@ -132,3 +132,60 @@ def test_spot_cone_uses_radian_angles():
     cone = min(max((theta - outer32) / (inner32 - outer32), 0.0), 1.0)
     want = ((1 - F) * alb / math.pi + spec) * 20.0 / 16.0 * cone + float(np.float32(0.01)) * alb
     np.testing.assert_allclose(out[:3], want, rtol=1e-9)
+
+
+def test_shadow_type_outcomes():
+    """EvaluateShadow by shadow type (light.frag:141-168): ray-traced scenes
+    use occluded/samples; ShadowDisabled, and a shadow-map scene whose light
+    has no map (no map sample), shade every light as fully shadowed; a map
+    decision given as one sample shades as that decision."""
+    g = _pixel((0, 0, 0), (0, 0, 1), emission=(255, 0, 128), occl=128)
+    lights = [(api.LIGHT_POINT, (0, 0, 4), (0, -1, 0), (1, 1, 1), 20.0, 4),
+              (api.LIGHT_SPOT, (0.5, 0, 4), (0, 0, -1), (1, 0.5, 0.2), 30.0, 4)]
+    alb = np.array([_srgb(200), _srgb(100), _srgb(50)])
+    emis = np.array([1.0, 0.0, 128 / 255.0])
+    dark = float(np.float32(0.01)) * alb * (128 / 255.0) * (3 / 8) + emis
+    counts = np.array([[1], [2]], np.uint8)
+    ao = np.array([3], np.uint8)
+    sb = _scene((0, 0, 10), lights, ao_n=8)
+    lit = orc.shade(g, 1, 1, sb, counts, ao, f64=True)[0]
+    assert np.all(lit[:3] > dark + 1e-3)
+    for kind in (api.SHADOW_DISABLED, api.SHADOW_MAP):
+        sb.shadow_type = kind
+        out = orc.shade(g, 1, 1, sb, counts, ao, f64=True)[0]
+        if kind == api.SHADOW_MAP:   # counts are map decisions out of 4 "samples" here
+            np.testing.assert_allclose(out, lit, rtol=1e-12)
+        for l in range(2):
+            sb.lights[l].num_shadow_samples = 0
+        out = orc.shade(g, 1, 1, sb, counts, ao, f64=True)[0]
+        np.testing.assert_allclose(out[:3], dark, rtol=1e-12)
+        for l in range(2):
+            sb.lights[l].num_shadow_samples = 4
+    # a map decision of "shadowed" for light 0 and "lit" for light 1
+    sb.shadow_type = api.SHADOW_MAP
+    for l in range(2):
+        sb.lights[l].num_shadow_samples = 1
+    half = orc.shade(g, 1, 1, sb, np.array([[1], [0]], np.uint8), ao, f64=True)[0]
+    sb.shadow_type = api.SHADOW_RAYTRACED
+    rt = orc.shade(g, 1, 1, sb, np.array([[1], [0]], np.uint8), ao, f64=True)[0]
+    np.testing.assert_allclose(half, rt, rtol=1e-12)
+    assert np.all(half[:3] > dark + 1e-3)
+
+
+def test_visibility_casts_shadow_rays_only_in_ray_traced_scenes():
+    from tests.helpers import oracle_scene
+    s = api.HostScene(api.SCENE_CUBE, 0)
+    W, H, cam = s.camera()
+    _, _, mats = s.nodes()
+    o = oracle_scene(s)
+    g = o.primary_gbuffer(cam, mats, 64, 64)
+    sb = s.pack()
+    active = int((np.abs(g["normal"]).sum(1) != 0).sum())
+    sh_rt, ao_rt, st_rt = o.trace_visibility(g, 64, 64, sb)
+    assert sh_rt.sum() > 0 and st_rt.rays == active * (1 + sb.ao_num_samples)
+    for kind in (api.SHADOW_DISABLED, api.SHADOW_MAP):
+        sb.shadow_type = kind
+        sh, ao, st = o.trace_visibility(g, 64, 64, sb)
+        assert sh.sum() == 0 and st.rays == active * sb.ao_num_samples
+        np.testing.assert_array_equal(ao, ao_rt)
+    s.close()
