@@ -150,6 +150,12 @@ luzrt_status luzrt_create(luzrt_ctx** out, int device);
 luzrt_status luzrt_destroy(luzrt_ctx* ctx);
 const char*  luzrt_last_error(const luzrt_ctx* ctx);
 const char*  luzrt_status_string(luzrt_status s);
+/* Waits for the context's work.  Returns LUZRT_E_LIMIT (once; the condition
+ * is then cleared) when a visibility, light-pass, ray, primary or shadow-map
+ * call since the last luzrt_sync / luzrt_get_timings had a ray that exceeded
+ * the traversal stack or step limit: such a ray was written as a miss (the
+ * device latches the condition; no later call clears it).  The reference has
+ * no such limit (the driver's traversal) and no error path (Base.hpp:51-59). */
 luzrt_status luzrt_sync(luzrt_ctx* ctx);
 /* Per-stage GPU times of the last build/trace/light/primary call (synchronises). */
 luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
@@ -160,6 +166,12 @@ luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
  * Chosen by the autotuner (luz_amd/autotune.py). */
 luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint32_t refill_threshold,
                                     uint32_t stack_entries, uint32_t tile_width);
+
+/* Traversal steps (node visits and instance entries) a ray may take before
+ * it is abandoned as a miss and the error latch reported by luzrt_sync is
+ * set; 0 restores the default 2^24, which a well-formed structure never
+ * reaches (a guard against malformed trees).  Visibility and ray calls. */
+luzrt_status luzrt_set_step_limit(luzrt_ctx* ctx, uint32_t max_steps);
 
 /* Order of a visibility call's rays (a traversal launch parameter, like the
  * above; no counterpart in the reference, whose ray queries run per

@@ -142,6 +142,7 @@ _sigs = {
     "luzrt_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "luzrt_set_ray_order": (C.c_int, [_P, C.c_uint32]),
+    "luzrt_set_step_limit": (C.c_int, [_P, C.c_uint32]),
     "luzrt_set_precision": (C.c_int, [_P, C.c_int]),
     "luzrt_blas_wide_arrays": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
                                          C.POINTER(_P), C.POINTER(C.c_uint32)]),
@@ -288,6 +289,10 @@ class Context:
     def set_ray_order(self, order: int) -> None:
         """RAY_ORDER_AUTO, RAY_ORDER_SLOT or RAY_ORDER_GROUPED (same results, different speed)."""
         self._check(lib.luzrt_set_ray_order(self.handle, order))
+
+    def set_step_limit(self, max_steps: int) -> None:
+        """Traversal steps per ray before it is abandoned (0: the default 2^24)."""
+        self._check(lib.luzrt_set_step_limit(self.handle, max_steps))
 
     def set_precision(self, mode: int) -> None:
         """PRECISION_PLAIN or PRECISION_COMPACT (8-wide 8-bit BVH, packed-FP16 shading)."""

@@ -21,6 +21,7 @@
 // so a batched frame reproduces luzrt_primary_gbuffer + luzrt_light_pass of
 // each scene bit for bit.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <vector>
 
@@ -79,8 +80,11 @@ struct BatchTables {
     uint32_t n;
 };
 
+// Per-frame counters; the build-error flag and the traversal error latch
+// kCntError stay set until luzrt_batch_rays reports (and clears) them.
 __global__ void k_batch_init(uint32_t* counters) {
-    if (threadIdx.x < kCntCount && threadIdx.x != kCntBuildError) counters[threadIdx.x] = 0u;
+    if (threadIdx.x < kCntCount && threadIdx.x != kCntBuildError && threadIdx.x != kCntError)
+        counters[threadIdx.x] = 0u;
 }
 
 // Persistent closest-hit primary rays over the 8x8-tile pixel slots of every
@@ -250,7 +254,8 @@ struct luzrt_batch {
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    uint32_t graph_cfg[4] = {0, 0, 0, 0};
+    // launch configuration and workspace a captured graph was recorded with
+    std::array<uint64_t, 7> graph_key{};
     hipEvent_t done = nullptr;
     bool pending = false;
     uint32_t frame = 0;
@@ -315,7 +320,7 @@ luzrt_status upload_refits(luzrt_ctx* ctx, luzrt_batch* b) {
     uint32_t chunks = 0;
     for (size_t i = 0; i < b->refit_blas.size(); ++i) {
         luzrt_blas* bl = b->refit_blas[i];
-        if (bl->tri_count < 2) continue;   // a single-triangle root is rebuilt by luzrt_blas_refit only
+        if (bl->tri_count < 2) continue;   // a single-triangle root is rebuilt in issue_frame
         RefitJob J = refit_job(bl, b->refit_verts[i], nullptr);
         J.chunk_offset = chunks;
         chunks += refit_chunks(bl->tri_count);
@@ -345,6 +350,12 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
         (s = launch_refit_batched(ctx, b->dev_refits, b->refit_jobs, b->refit_chunks)) !=
             LUZRT_OK)
         return s;
+    // BLASes of fewer than 2 triangles are not refit jobs: their single root
+    // is rebuilt from the moved vertices as luzrt_blas_refit does
+    for (size_t i = 0; i < b->refit_blas.size(); ++i)
+        if (b->refit_blas[i]->tri_count < 2 &&
+            (s = blas_refit_device(ctx, b->refit_blas[i], b->refit_verts[i])) != LUZRT_OK)
+            return s;
     if ((s = launch_tlas_batched(ctx, b->dev_jobs, b->n, b->max_m)) != LUZRT_OK) return s;
     k_batch_init<<<1, 64, 0, st>>>(b->counters);
     if (b->total_ppx) {
@@ -469,6 +480,26 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         return fail(ctx, LUZRT_E_LIMIT, "%llu rays per frame exceed the 32-bit ray index",
                     (unsigned long long)rays);
     }
+    // two scenes must not share a TLAS (one build job per TLAS and frame), an
+    // output, or a G-buffer plane that either of them writes (camera set)
+    for (uint32_t s = 0; s < count; ++s)
+        for (uint32_t t = 0; t < s; ++t) {
+            const luzrt_batch_scene &a = scenes[s], &c = scenes[t];
+            const char* what = nullptr;
+            if (a.tlas == c.tlas) what = "the same TLAS";
+            else if (a.rgba32f_out == c.rgba32f_out || a.ao_counts == c.ao_counts ||
+                     (a.shadow_counts && a.shadow_counts == c.shadow_counts))
+                what = "an output buffer";
+            else if ((a.camera || c.camera) &&
+                     (a.gbuffer.position == c.gbuffer.position || a.gbuffer.normal == c.gbuffer.normal ||
+                      a.gbuffer.albedo == c.gbuffer.albedo || a.gbuffer.material == c.gbuffer.material ||
+                      a.gbuffer.emission == c.gbuffer.emission))
+                what = "a G-buffer plane that a camera writes";
+            if (what) {
+                batch_free(b);
+                return fail(ctx, LUZRT_E_INVALID_ARG, "batch scenes %u and %u share %s", t, s, what);
+            }
+        }
     b->max_rays = rays;
     const size_t scene_bytes = ((size_t)count * sizeof(DevScene) + 255) & ~(size_t)255;
     b->stage_bytes = scene_bytes + inst_bytes;
@@ -501,6 +532,9 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
     B_ALLOC(b->dev_tlas, count * sizeof(BatchTlas));
     B_ALLOC(b->dev_jobs, count * sizeof(TlasJob));
 #undef B_ALLOC
+    // the error latch in the counters starts clear; frames never clear it
+    e = hipMemset(b->counters, 0, kCntCount * 4);
+    if (e != hipSuccess) { batch_free(b); return hip_fail(ctx, e, "hipMemset(batch counters)"); }
     e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
     if (e != hipSuccess) { batch_free(b); return hip_fail(ctx, e, "hipEventCreate"); }
 
@@ -616,17 +650,23 @@ luzrt_status luzrt_batch_frame(luzrt_ctx* ctx, luzrt_batch* b, uint32_t frame, i
     for (uint32_t s = 0; s < b->n; ++s)
         ((DevScene*)b->host_stage)[s].s.frame = (int32_t)(b->desc[s].scene.frame + frame);
     for (luzrt_blas* bl : b->refit_blas) bl->wide_valid = false;
-    const uint32_t cfg[4] = {ctx->groups_per_cu, ctx->refill_threshold, ctx->stack_entries,
-                             ctx->tile_width};
-    bool same = b->exec && std::memcmp(cfg, b->graph_cfg, sizeof(cfg)) == 0 && b->inst_n == b->graph_n;
+    // a graph holds the launch configuration and the context's traversal
+    // spill buffer by value: a single-scene call may reallocate that buffer,
+    // so its address is part of the key
+    auto key = [&]() {
+        return std::array<uint64_t, 7>{ctx->groups_per_cu, ctx->refill_threshold, ctx->stack_entries,
+                                       ctx->tile_width, (uint64_t)(uintptr_t)ctx->spill.ptr,
+                                       (uint64_t)ctx->spill.bytes, ctx->max_steps};
+    };
+    bool same = b->exec && key() == b->graph_key && b->inst_n == b->graph_n;
     if (!same || !use_graph) {
         graph_release(b);
-        if (cfg[3] != b->graph_cfg[3] || !b->graph_cfg[3])
+        if (ctx->tile_width != b->graph_key[3] || !b->graph_key[3])
             if ((st = layout_blocks(ctx, b)) != LUZRT_OK) return st;
         if (b->refits_dirty && (st = upload_refits(ctx, b)) != LUZRT_OK) return st;
         if ((st = upload_tables(ctx, b)) != LUZRT_OK) return st;
         if ((st = prepare_frame(ctx, b)) != LUZRT_OK) return st;
-        std::memcpy(b->graph_cfg, cfg, sizeof(cfg));
+        b->graph_key = key();
         b->graph_n = b->inst_n;
     }
     if (use_graph) {
@@ -666,8 +706,12 @@ luzrt_status luzrt_batch_rays(luzrt_ctx* ctx, luzrt_batch* b, uint64_t* total_ra
         LUZ_HIP(ctx, hipMemset(flag, 0, 4));
         return fail(ctx, LUZRT_E_DEVICE, "batched BVH build or refit reported error %u", build);
     }
-    if (c[kCntError])
-        return fail(ctx, LUZRT_E_DEVICE, "a ray exceeded the traversal stack or step limit");
+    if (c[kCntError]) {
+        LUZ_HIP(ctx, hipMemset(b->counters + kCntError, 0, 4));
+        return fail(ctx, LUZRT_E_LIMIT,
+                    "a ray exceeded the traversal stack or step limit in a frame since the last check "
+                    "(such rays are written as misses)");
+    }
     return LUZRT_OK;
 }
 

@@ -32,18 +32,29 @@ __device__ inline uint32_t expand10(uint32_t v) {
     return v;
 }
 
-__device__ inline uint32_t morton_code(const double* frame, double cx, double cy, double cz) {
-    double c[3] = {cx, cy, cz};
+// Morton code of the FP32 box centroid 0.5 (lo + hi) in the frame {origin[3],
+// scale[3]}: the FP64 centroid-bounds reduction rounded to FP32 (morton_frame32).
+// Quantisation is FP32; only the reduction that sets the frame runs in FP64.
+__device__ inline uint32_t morton_code(const float* frame, const float* box) {
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        double t = (c[k] - frame[k]) * frame[3 + k];
-        t = floor(t);
-        if (t < 0.0) t = 0.0;
-        if (t > 1023.0) t = 1023.0;
+        const float c = 0.5f * (box[k] + box[3 + k]);
+        float t = floorf((c - frame[k]) * frame[3 + k]);
+        if (!(t >= 0.0f)) t = 0.0f;
+        if (t > 1023.0f) t = 1023.0f;
         q[k] = (uint32_t)t;
     }
     return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+}
+
+// The FP32 quantisation frame from the FP64 centroid bounds mn, mx of one
+// axis: origin mn and scale 1024 / (mx - mn) (0 for a flat axis), each
+// rounded once to FP32.
+__device__ inline void morton_frame32(double mn, double mx, float& origin, float& scale) {
+    const double e = mx - mn;
+    origin = (float)mn;
+    scale = e > 0.0 ? (float)(1024.0 / e) : 0.0f;
 }
 
 // Padded triangle box (max |coord| * 2^-16 on every side).
@@ -122,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_boxes(const uint8_t* verts, uint
 
 // frame = {cmin[3], scale[3]} with scale = 1024/extent (0 for a flat axis).
 __global__ __launch_bounds__(kBlock) void k_frame(const double* partial, uint32_t nparts,
-                                                  double* frame) {
+                                                  float* frame) {
     __shared__ double sh[kBlock];
     for (int k = 0; k < 3; ++k) {
         double mn = INFINITY, mx = -INFINITY;
@@ -133,24 +144,16 @@ __global__ __launch_bounds__(kBlock) void k_frame(const double* partial, uint32_
         }
         mn = block_reduce(mn, sh, DMin());
         mx = block_reduce(mx, sh, DMax());
-        if (threadIdx.x == 0) {
-            double e = mx - mn;
-            frame[k] = mn;
-            frame[3 + k] = e > 0.0 ? 1024.0 / e : 0.0;
-        }
+        if (threadIdx.x == 0) morton_frame32(mn, mx, frame[k], frame[3 + k]);
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_morton(const float* boxes, uint32_t n,
-                                                   const double* frame, uint32_t* keys,
+                                                   const float* frame, uint32_t* keys,
                                                    uint32_t* vals) {
     uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
-    const float* b = boxes + (size_t)t * 6;
-    double cx = 0.5 * ((double)b[0] + (double)b[3]);
-    double cy = 0.5 * ((double)b[1] + (double)b[4]);
-    double cz = 0.5 * ((double)b[2] + (double)b[5]);
-    keys[t] = morton_code(frame, cx, cy, cz);
+    keys[t] = morton_code(frame, boxes + (size_t)t * 6);
     vals[t] = t;
 }
 
@@ -787,25 +790,19 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         }
     }
     __syncthreads();
-    double s_frame[6];
+    float s_frame[6];
     for (int k = 0; k < 3; ++k) {
         double mn = INFINITY, mx = -INFINITY;
         for (uint32_t w = 0; w < nt / 64; ++w) {
             mn = s_redd[k][w] < mn ? s_redd[k][w] : mn;
             mx = s_redd[3 + k][w] > mx ? s_redd[3 + k][w] : mx;
         }
-        const double e = mx - mn;
-        s_frame[k] = mn;
-        s_frame[3 + k] = e > 0.0 ? 1024.0 / e : 0.0;
+        morton_frame32(mn, mx, s_frame[k], s_frame[3 + k]);
     }
     // 3. Morton codes, bitonic sort of (code << 32 | index)
     auto sort_key = [&](uint32_t i) -> uint64_t {
         if (i >= n) return ~0ull;
-        const float* b = wbox + i * 6;
-        double cx = 0.5 * ((double)b[0] + (double)b[3]);
-        double cy = 0.5 * ((double)b[1] + (double)b[4]);
-        double cz = 0.5 * ((double)b[2] + (double)b[5]);
-        return ((uint64_t)morton_code(s_frame, cx, cy, cz) << 32) | i;
+        return ((uint64_t)morton_code(s_frame, wbox + i * 6) << 32) | i;
     };
     if (m <= 64) {
         // one wave sorts in registers: lane i holds key i, partners by xor
@@ -1047,7 +1044,7 @@ luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
     auto take = [&](size_t sz) { uint8_t* r = p; p += (sz + 255) & ~(size_t)255; return r; };
     float* boxes = (float*)take((size_t)n * 24);
     double* partial = (double*)take((size_t)nb * 48);
-    double* frame = (double*)take(64);
+    float* frame = (float*)take(64);
     uint32_t* k0 = (uint32_t*)take((size_t)n * 4);
     uint32_t* v0 = (uint32_t*)take((size_t)n * 4);
     uint32_t* k1 = (uint32_t*)take((size_t)n * 4);

@@ -173,6 +173,20 @@ __device__ inline TriRay tri_setup(V3 d) {
     return r;
 }
 
+// a*b - c*d to within 2 ulps (Kahan's algorithm with fused multiply-adds;
+// Jeannerod, Louvet and Muller, Math. Comp. 82 (2013)): e is the exact
+// rounding error of c*d, so the result is 0 only when a*b == c*d exactly and
+// otherwise has the sign of the exact difference (products away from the
+// FP32 underflow range).  The watertight test's edge functions fall back to
+// it where the plain FP32 differences give 0, in place of the published FP64
+// re-evaluation (Woop, Benthin and Wald, JCGT 2(1), 2013, section 3.3).
+__device__ inline float diff_of_products(float a, float b, float c, float d) {
+    const float w = c * d;
+    const float e = __builtin_fmaf(-c, d, w);
+    const float f = __builtin_fmaf(a, b, -w);
+    return f + e;
+}
+
 // Returns true on a hit with T/det in [tmin, tmax]; signed T, det, U, V, W out.
 __device__ inline bool tri_test(const TriRay& r, V3 o, float tmin, float tmax, V3 v0, V3 v1,
                                 V3 v2, float& T_out, float& det_out, float& U_out, float& V_out,
@@ -189,12 +203,9 @@ __device__ inline bool tri_test(const TriRay& r, V3 o, float tmin, float tmax, V
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
-        U = (float)(CxBy - CyBx);
-        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
-        V = (float)(AxCy - AyCx);
-        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
-        W = (float)(BxAy - ByAx);
+        U = diff_of_products(Cx, By, Cy, Bx);
+        V = diff_of_products(Ax, Cy, Ay, Cx);
+        W = diff_of_products(Bx, Ay, By, Ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;

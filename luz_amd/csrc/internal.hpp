@@ -41,6 +41,7 @@ struct luzrt_ctx {
     uint32_t stack_entries = 16;   // LDS stack entries per lane: 8, 16 or 32
     uint32_t tile_width = 8;       // pixel tiles of 64: 8x8, 16x4 or 32x2
     uint32_t ray_order = 0;        // LUZRT_RAY_ORDER_*: automatic, slot-major or 64-pixel groups
+    uint32_t max_steps = 1u << 24; // traversal steps per ray before it is abandoned (kMaxSteps)
     // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
     int precision = 0;
     // pinned staging ring of small host-to-device uploads (scene blocks, TLAS
@@ -213,6 +214,7 @@ struct TraceLaunch {
     uint8_t* hits;
     uint32_t* counters;    // device Counter array; kCntRays read, kCntFetch zeroed before
     uint32_t* spill;       // per-lane overflow stack
+    uint32_t max_steps;    // traversal steps per ray before it is abandoned (the context's limit)
     bool wide;             // instance records point to 8-wide BLASes
     // Multi-scene launch (nscenes > 0): ray i belongs to the scene s with
     // scene_ray_base[s] <= i < scene_ray_base[s + 1], is generated from that

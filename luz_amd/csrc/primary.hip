@@ -15,13 +15,11 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Clears the error flag and the tile counter of the call with a kernel (a
-// caller may capture the call in a graph, where memset nodes are avoided).
+// Clears the tile counter of the call with a kernel (a caller may capture
+// the call in a graph, where memset nodes are avoided).  The error latch
+// kCntError is left set until the host reports it.
 __global__ void k_primary_init(uint32_t* counters) {
-    if (threadIdx.x == 0) {
-        counters[kCntError] = 0u;
-        counters[kCntPrimary] = 0u;
-    }
+    if (threadIdx.x == 0) counters[kCntPrimary] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_primary(const Node* tlas_nodes, const InstRec* recs,

@@ -285,10 +285,26 @@ luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
     return LUZRT_OK;
 }
 
+// After a stream synchronisation: reports (and clears) the traversal error
+// latch, which a ray that overflows its stack or step limit sets on the
+// device and no kernel clears.
+static luzrt_status check_trace_latch(luzrt_ctx* ctx) {
+    if (!ctx->counters.ptr) return LUZRT_OK;
+    uint32_t* d = (uint32_t*)ctx->counters.ptr + kCntError;
+    uint32_t flag = 0;
+    LUZ_HIP(ctx, hipMemcpy(&flag, d, 4, hipMemcpyDeviceToHost));
+    if (!flag) return LUZRT_OK;
+    const uint32_t zero = 0;
+    LUZ_HIP(ctx, hipMemcpy(d, &zero, 4, hipMemcpyHostToDevice));
+    return fail(ctx, LUZRT_E_LIMIT,
+                "a ray exceeded the traversal stack (%u LDS + %d spilled entries) or its step limit "
+                "since the last check; such rays were written as misses", ctx->stack_entries, kSpill);
+}
+
 luzrt_status luzrt_sync(luzrt_ctx* ctx) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return LUZRT_OK;
+    return check_trace_latch(ctx);
 }
 
 luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out) {
@@ -312,9 +328,7 @@ luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out) {
         LUZ_HIP(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
         out->rays = c[kCntRays];
         out->active_pixels = c[kCntActive];
-        if (c[kCntError])
-            return fail(ctx, LUZRT_E_LIMIT, "traversal overflow (stack depth > %d or > %u steps)",
-                        kSpill, kMaxSteps);
+        if (c[kCntError]) return check_trace_latch(ctx);
         if (c[kCntBuildError])
             return fail(ctx, LUZRT_E_INVALID_ARG, "acceleration-structure build error flag %u",
                         c[kCntBuildError]);
@@ -335,6 +349,12 @@ luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint
     if (refill) ctx->refill_threshold = refill;
     if (stack_entries) ctx->stack_entries = stack_entries;
     if (tile_width) ctx->tile_width = tile_width;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_set_step_limit(luzrt_ctx* ctx, uint32_t max_steps) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    ctx->max_steps = max_steps ? max_steps : kMaxSteps;
     return LUZRT_OK;
 }
 

@@ -183,7 +183,6 @@ extern "C" luzrt_status luzrt_shadow_map_build(luzrt_ctx* ctx, const luzrt_tlas*
     if (groups == 0) groups = 1;
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * kBlock * kSpill * 4)) != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
-    if ((s = zero_fill(ctx, counters + kCntError, 4)) != LUZRT_OK) return s;
     stage_begin(ctx, kStagePrimary);
     k_shadow_map<<<groups, kBlock, 0, ctx->stream>>>(tlas->nodes, tlas->recs, tlas->aux, *m,
                                                     (uint32_t*)ctx->spill.ptr, counters + kCntError,

@@ -27,8 +27,11 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Counters of a trace_rays call; the error latch kCntError is left as it is
+// (only the host clears it, when it reports it).
 __global__ void k_vis_init(uint32_t* counters, uint32_t ray_count) {
-    if (threadIdx.x <= kCntRays) counters[threadIdx.x] = threadIdx.x == kCntRays ? ray_count : 0u;
+    if (threadIdx.x <= kCntRays && threadIdx.x != kCntError)
+        counters[threadIdx.x] = threadIdx.x == kCntRays ? ray_count : 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_count(const float* normal, uint32_t W, uint32_t H,
@@ -79,7 +82,6 @@ __global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, u
         counters[kCntActive] = total;
         counters[kCntRays] = total * ds->r_total;
         counters[kCntFetch] = 0;
-        counters[kCntError] = 0;
     }
 }
 
@@ -251,7 +253,8 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
         }
         if (ray != kNone) {
             for (;;) {
-                int res = ray_step<false, WIDE>(r, st, tnodes, trecs, nullptr, nullptr, tlas_lds);
+                int res = ray_step<false, WIDE>(r, st, tnodes, trecs, nullptr, nullptr, tlas_lds,
+                                                L.max_steps);
                 if (res != kStepContinue) {
                     L.hits[ray] = res == kStepHit ? 1 : 0;
                     if (res == kStepOverflow) atomicOr(&L.counters[kCntError], 1u);
@@ -289,6 +292,7 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
     if (s != LUZRT_OK) return s;
     TraceLaunch l = L;
     l.spill = (uint32_t*)ctx->spill.ptr;
+    l.max_steps = ctx->max_steps;
     uint32_t refill = ctx->refill_threshold ? ctx->refill_threshold : 64u;
     auto launch = [&](auto kern) { kern<<<groups, kBlock, 0, ctx->stream>>>(l, refill); };
     const uint32_t sd = ctx->stack_entries;

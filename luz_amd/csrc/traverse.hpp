@@ -197,8 +197,9 @@ __device__ inline int wide_visit(RayState& r, STACK& st) {
 // tlas_lds: tlas_nodes and recs point into LDS (a staged TLAS).
 template <bool CLOSEST, bool WIDE = false, typename STACK = Stack>
 __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, const InstRec* recs,
-                               const InstAux* aux, Best* best, bool tlas_lds = false) {
-    if (++r.steps > kMaxSteps) return kStepOverflow;
+                               const InstAux* aux, Best* best, bool tlas_lds = false,
+                               uint32_t max_steps = kMaxSteps) {
+    if (++r.steps > max_steps) return kStepOverflow;
     if (r.next == kNone) {
         if (st.sp == 0) return kStepDone;
         r.next = st.pop();

@@ -177,6 +177,18 @@ inline TriRay tri_setup(f3 d) {
     return r;
 }
 
+// a*b - c*d to within 2 ulps by Kahan's algorithm with fused multiply-adds
+// (Jeannerod, Louvet and Muller, Math. Comp. 82, 2013): e = w - c*d exactly,
+// so the result is 0 only for an exact 0 and otherwise carries the exact
+// difference's sign.  It replaces the published FP64 re-evaluation of zero
+// edge functions (Woop, Benthin and Wald 2013, section 3.3).
+inline float diff_of_products(float a, float b, float c, float d) {
+    const float w = c * d;
+    const float e = std::fma(-c, d, w);
+    const float f = std::fma(a, b, -w);
+    return f + e;
+}
+
 // Returns true on hit with T/det in [tmin, tmax]; outputs T, det, U, V, W (signed).
 inline bool tri_test(const TriRay& r, f3 o, float tmin, float tmax, f3 v0, f3 v1, f3 v2,
                      float& T_out, float& det_out, float& U_out, float& V_out, float& W_out) {
@@ -191,12 +203,9 @@ inline bool tri_test(const TriRay& r, f3 o, float tmin, float tmax, f3 v0, f3 v1
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
-        U = (float)(CxBy - CyBx);
-        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
-        V = (float)(AxCy - AyCx);
-        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
-        W = (float)(BxAy - ByAx);
+        U = diff_of_products(Cx, By, Cy, Bx);
+        V = diff_of_products(Ax, Cy, Ay, Cx);
+        W = diff_of_products(Bx, Ay, By, Ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;
@@ -387,8 +396,9 @@ void build_single(const std::vector<Box>& boxes, std::vector<Node>& nodes, std::
 }
 
 // Slot order of n >= 2 primitive boxes (S5): stable sort by the Morton code of
-// the box centroid in the FP64 centroid frame.  order[slot] = primitive index,
-// keys[slot] = its code.
+// the FP32 box centroid in the centroid frame, whose bounds are reduced in
+// FP64 (exact centroids) and then rounded to an FP32 origin and scale.
+// order[slot] = primitive index, keys[slot] = its code.
 void morton_order(const std::vector<Box>& boxes, std::vector<uint32_t>& order, std::vector<uint32_t>& keys) {
     const int n = (int)boxes.size();
     // FP64 centroid frame.
@@ -404,20 +414,20 @@ void morton_order(const std::vector<Box>& boxes, std::vector<uint32_t>& order, s
             mx[k] = c[k] > mx[k] ? c[k] : mx[k];
         }
     }
-    double sc[3];
+    float org[3], sc[3];
     for (int k = 0; k < 3; ++k) {
         double e = mx[k] - mn[k];
-        sc[k] = e > 0.0 ? 1024.0 / e : 0.0;
+        org[k] = (float)mn[k];
+        sc[k] = e > 0.0 ? (float)(1024.0 / e) : 0.0f;
     }
     std::vector<uint32_t> code(n);
     for (int i = 0; i < n; ++i) {
-        double c[3] = {cx[i], cy[i], cz[i]};
         uint32_t q[3];
         for (int k = 0; k < 3; ++k) {
-            double t = (c[k] - mn[k]) * sc[k];
-            t = std::floor(t);
-            if (t < 0.0) t = 0.0;
-            if (t > 1023.0) t = 1023.0;
+            float c = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+            float t = std::floor((c - org[k]) * sc[k]);
+            if (!(t >= 0.0f)) t = 0.0f;
+            if (t > 1023.0f) t = 1023.0f;
             q[k] = (uint32_t)t;
         }
         code[i] = morton3(q[0], q[1], q[2]);

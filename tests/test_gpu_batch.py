@@ -179,3 +179,93 @@ def test_batch_rejects_compact_precision(ctx, workload):
             workload.batch.frame(0, True)
     finally:
         ctx.set_precision(api.PRECISION_PLAIN)
+
+
+def test_batch_graph_follows_a_reallocated_spill_buffer(ctx, workload):
+    """A graph captured at 6 groups per CU holds the context's traversal spill
+    buffer; a single-scene call at 10 groups per CU reallocates that buffer,
+    so the replay after the configuration is set back must recapture rather
+    than write the freed one: every scene still equals the single-scene calls."""
+    ctx.set_trace_config(6, 48, 16, 8)
+    workload.frame(1, True)
+    ctx.set_trace_config(10, 48, 16, 8)
+    single_scene(ctx, workload, 0, 1)
+    ctx.set_trace_config(6, 48, 16, 8)
+    frame = 3
+    workload.frame(frame, True)
+    got = [workload.outputs(i) for i in range(len(SCENES))]
+    for i in range(len(SCENES)):
+        assert_same(got[i], single_scene(ctx, workload, i, frame), f"after reallocation, scene {i}")
+
+
+def test_batch_refits_a_single_triangle_blas(ctx):
+    """A one-triangle mesh registered for refit moves with its vertices in
+    every batched frame (its single root is rebuilt), as luzrt_blas_refit
+    moves it in the single-scene path."""
+    s = api.HostScene(api.SCENE_CUBE, 0)
+    g = api.GPUScene(ctx, s)
+    W, H, cam = s.camera()
+    _, _, mats = s.nodes()
+    sb = s.pack()
+
+    def tri_at(frame):
+        v = np.zeros((3, 12), np.float32)
+        y = 1.3 + 0.2 * frame
+        v[:, :3] = [[-0.8, y, -0.8], [0.8, y, -0.6], [0.0, y + 0.6, 0.9]]
+        v[:, 4] = 1.0
+        return v
+
+    dv = ctx.to_device(tri_at(0))
+    tblas = ctx.create_blas(dv, 48, 3, ctx.to_device(np.array([0, 1, 2], np.uint32)), 3)
+    arr, n = g.instances(s)
+    insts = [arr[i] for i in range(n)] + [api.make_instance(tblas, np.eye(3, 4), n)]
+    tlas, tlas1 = ctx.create_tlas(16), ctx.create_tlas(16)
+    gb, gb1 = api.GBufferDevice(ctx, W, H), api.GBufferDevice(ctx, W, H)
+    bufs = [ctx.buffer(W * H), ctx.buffer(W * H), ctx.buffer(W * H * 16)]
+    batch = api.Batch(ctx, [{"tlas": tlas, "gbuffer": gb.desc(), "scene": sb, "camera": cam, "materials": mats,
+                             "shadow": bufs[0], "ao": bufs[1], "out": bufs[2]}])
+    try:
+        batch.add_refit(tblas, dv.ptr)
+        prev = None
+        for frame in (1, 2):
+            dv.upload(tri_at(frame))
+            batch.set_instances(0, (api.Instance * len(insts))(*insts), len(insts))
+            batch.frame(frame, True)
+            batch.rays()
+            got = gb.download()
+            # the single-scene path: refit, TLAS rebuild, primary rays
+            tblas.refit(dv)
+            tlas1.build(insts)
+            ctx.primary_gbuffer(tlas1, cam, mats, gb1.desc())
+            want = gb1.download()
+            for k in got:
+                np.testing.assert_array_equal(got[k].view(np.uint32), want[k].view(np.uint32), err_msg=k)
+            above = int((got["position"][:, 1] > 1.2).sum())
+            print(f"frame {frame}: {above} pixels see the moving triangle", flush=True)
+            assert above > 200
+            if prev is not None:
+                assert not np.array_equal(prev, got["position"])
+            prev = got["position"]
+    finally:
+        batch.close()
+        tlas.destroy()
+        tlas1.destroy()
+        tblas.destroy()
+        g.close()
+
+
+def test_batch_reports_traversal_overflow_once(ctx, workload):
+    """A frame whose rays exceed the step limit (3 steps, set from the host)
+    makes luzrt_batch_rays return LUZRT_E_LIMIT once; frames at the default
+    limit report nothing."""
+    ctx.set_step_limit(3)
+    try:
+        workload.stage(1)
+        workload.batch.frame(1, True)
+        with pytest.raises(api.LuzError) as e:
+            workload.batch.rays()
+        assert e.value.status == -5
+        workload.batch.rays()
+    finally:
+        ctx.set_step_limit(0)
+    workload.frame(2, True)

@@ -98,3 +98,60 @@ def test_denoiser_rejects_bad_parameters(ctx):
                                            short.ctypes.data_as(C.POINTER(C.c_float)), len(short))
     assert rc == E_INVALID_ARG
     dn.close()
+
+
+def test_traversal_overflow_is_reported_by_the_next_sync(ctx):
+    """A ray that exceeds its step limit (forced here from the host with a
+    limit of 3 steps; kernels still end normally) is written as a miss and
+    latched on the device: the next luzrt_sync returns LUZRT_E_LIMIT once,
+    although a further call ran in between, and the condition is then clear."""
+    s, g, gd, W, H = _cube_scene(ctx)
+    sb = s.pack()
+    set_light_direction(sb, 0, (0.8, -1.0, 0.6))
+    sh, ao = ctx.buffer(W * H), ctx.buffer(W * H)
+    ctx.sync()
+    ctx.set_step_limit(3)
+    try:
+        ctx.trace_visibility(g.tlas, gd.desc(), sb, sh, ao)
+        # a second call before the check does not clear the latch
+        ctx.trace_visibility(g.tlas, gd.desc(), sb, sh, ao)
+        assert _status(ctx.sync) == E_LIMIT
+        assert b"step limit" in api.lib.luzrt_last_error(ctx.handle)
+        ctx.sync()
+    finally:
+        ctx.set_step_limit(0)
+    ctx.trace_visibility(g.tlas, gd.desc(), sb, sh, ao)
+    ctx.sync()
+    from tests.helpers import oracle_scene
+    host = gd.download()
+    wsh, wao, _ = oracle_scene(s).trace_visibility(host, W, H, sb)
+    np.testing.assert_array_equal(sh.download(np.uint8, (1, W * H)), wsh)
+    np.testing.assert_array_equal(ao.download(np.uint8, W * H), wao)
+    g.close()
+
+
+def test_batch_rejects_shared_tlas_and_outputs(ctx):
+    s, g, gd, W, H = _cube_scene(ctx)
+    W2, H2, cam = s.camera()
+    _, _, mats = s.nodes()
+    sb = s.pack()
+    other = api.GPUScene(ctx, s)
+    gd2 = api.GBufferDevice(ctx, W, H)
+
+    def entry(tlas, gbuf, bufs, camera=None):
+        return {"tlas": tlas, "gbuffer": gbuf.desc(), "scene": sb, "camera": camera, "materials": mats,
+                "shadow": bufs[0], "ao": bufs[1], "out": bufs[2]}
+
+    def bufs():
+        return ctx.buffer(W * H), ctx.buffer(W * H), ctx.buffer(W * H * 16)
+
+    a, b = bufs(), bufs()
+    assert _status(lambda: api.Batch(ctx, [entry(g.tlas, gd, a), entry(g.tlas, gd2, b)])) == E_INVALID_ARG
+    assert b"same TLAS" in api.lib.luzrt_last_error(ctx.handle)
+    assert _status(lambda: api.Batch(ctx, [entry(g.tlas, gd, a), entry(other.tlas, gd2, a)])) == E_INVALID_ARG
+    assert _status(lambda: api.Batch(ctx, [entry(g.tlas, gd, a, cam), entry(other.tlas, gd, b)])) == E_INVALID_ARG
+    # G-buffers that no camera writes may be shared
+    ok = api.Batch(ctx, [entry(g.tlas, gd, a), entry(other.tlas, gd, b)])
+    ok.close()
+    other.close()
+    g.close()
