@@ -101,16 +101,38 @@ typedef struct luzrt_scene {
 } luzrt_scene;
 
 /* The G-buffer of DeferredRenderer::CreateImages (DeferredRenderer.cpp:175-248),
- * device pointers, tightly packed rows of width pixels.  World-space position
- * replaces the depth attachment + DepthToWorld (utils.glsl:1-7).  A zero
- * normal marks a background pixel (light.frag:178). */
+ * device pointers.  A zero normal marks a background pixel (light.frag:178).
+ *
+ * Position comes either from a world-space position plane or, with
+ * position == NULL, from the engine's D32F depth attachment
+ * (DeferredRenderer.cpp:232-238) through DepthToWorld (utils.glsl:1-7):
+ * clip = (uv * 2 - 1, depth, 1) at the pixel centre uv = ((x + 0.5) / width,
+ * (y + 0.5) / height), v = inverse_proj * clip, v /= v.w, P = (inverse_view *
+ * v).xyz, with SceneBlock's inverseProj / inverseView (LuzCommon.h:104-105,
+ * GPUScene.cpp:233-234), column-major, products summed over the columns in
+ * order and the divisions correctly rounded.
+ *
+ * Every plane is read in place with a row pitch and (position and normal)
+ * a pixel stride in bytes, so the engine's images are passed as they are:
+ * normal_stride 16 reads the RGB of the RGBA32F normal attachment
+ * (DeferredRenderer.cpp:190-196).  Zero layout fields (a zero-initialised
+ * tail) mean tightly packed rows: stride 12, pitch width * stride (width * 4
+ * for the RGBA8 planes and the depth). */
 typedef struct luzrt_gbuffer {
     uint32_t width, height;
-    const float* position;     /* float3 per pixel */
-    const float* normal;       /* float3 per pixel */
+    const float* position;     /* float3 per pixel, or NULL: positions from depth */
+    const float* normal;       /* float3 per pixel (first three floats of normal_stride bytes) */
     const uint32_t* albedo;    /* RGBA8 sRGB (light.frag:172 decodes with pow 2.2) */
     const uint32_t* material;  /* RGBA8: r roughness, g metallic, b occlusion (opaque.frag:57) */
     const uint32_t* emission;  /* RGBA8 */
+    /* layout: pixel strides (12 or 16) and row pitches in bytes, 0 = packed */
+    uint32_t position_stride, normal_stride;
+    uint32_t position_pitch, normal_pitch, albedo_pitch, material_pitch, emission_pitch;
+    /* depth mode (position == NULL): D32F depth, its row pitch (0 = width * 4) */
+    uint32_t depth_pitch;
+    const float* depth;
+    float inverse_proj[16];
+    float inverse_view[16];
 } luzrt_gbuffer;
 
 /* Writable G-buffer for the primary-ray producer. */

@@ -65,6 +65,12 @@ int      luzscene_camera(const luzscene* s, uint32_t* width, uint32_t* height, l
  * the given NDC jitter (NULL = none), and its inverse (may be NULL). */
 int      luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj[16],
                             float inv_view_proj[16]);
+/* The camera's view and (jittered) projection and their inverses, as
+ * SceneBlock carries them (GPUScene.cpp:230-234): column-major, computed in
+ * FP64 and rounded once.  inverse_view / inverse_proj are what a depth-mode
+ * luzrt_gbuffer needs for DepthToWorld.  Any output may be NULL. */
+int      luzscene_camera_matrices(const luzscene* s, const float jitter[2], float view[16], float proj[16],
+                                  float inv_view[16], float inv_proj[16]);
 /* Shadow-map descriptor of light `light` (GPUScene.cpp:266-312): the
  * orthographic view-projection fitted to the camera frustum for directional
  * and spot lights, the position and shadowMapFar for point lights; depth is

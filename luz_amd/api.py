@@ -66,12 +66,24 @@ class SceneBlock(C.Structure):
 
 
 class GBuffer(C.Structure):
+    """luzrt_gbuffer: planes with optional strides/pitches (0 = packed) and the
+    depth mode (position None, depth + inverse projection and view)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
+                ("position", C.c_void_p), ("normal", C.c_void_p), ("albedo", C.c_void_p),
+                ("material", C.c_void_p), ("emission", C.c_void_p),
+                ("position_stride", C.c_uint32), ("normal_stride", C.c_uint32),
+                ("position_pitch", C.c_uint32), ("normal_pitch", C.c_uint32),
+                ("albedo_pitch", C.c_uint32), ("material_pitch", C.c_uint32),
+                ("emission_pitch", C.c_uint32), ("depth_pitch", C.c_uint32),
+                ("depth", C.c_void_p),
+                ("inverse_proj", C.c_float * 16), ("inverse_view", C.c_float * 16)]
+
+
+class GBufferOut(C.Structure):
+    """luzrt_gbuffer_out: tightly packed planes written by the primary rays."""
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
                 ("position", C.c_void_p), ("normal", C.c_void_p), ("albedo", C.c_void_p),
                 ("material", C.c_void_p), ("emission", C.c_void_p)]
-
-
-GBufferOut = GBuffer
 
 
 class Camera(C.Structure):
@@ -184,6 +196,7 @@ _sigs = {
     "luzscene_camera": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(Camera)]),
     "luzscene_animate": (C.c_int, [_P, C.c_uint32]),
     "luzscene_view_proj": (C.c_int, [_P, _P, _P, _P]),
+    "luzscene_camera_matrices": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "luzscene_jitter": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "luzscene_shadow_map": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(ShadowMap)]),
     "luzscene_set_shadow_type": (C.c_int, [_P, C.c_int]),
@@ -564,6 +577,15 @@ class HostScene:
         if lib.luzscene_view_proj(self.handle, _ptr(j) if j is not None else None, _ptr(vp), _ptr(ivp)) != 0:
             raise LuzError(-1, "singular view-projection")
         return vp, ivp
+
+    def camera_matrices(self, jitter=None) -> dict:
+        """view, proj and their inverses (column-major float32[16]), as SceneBlock carries them."""
+        m = {k: np.zeros(16, np.float32) for k in ("view", "proj", "inverse_view", "inverse_proj")}
+        j = None if jitter is None else np.asarray(jitter, np.float32)
+        if lib.luzscene_camera_matrices(self.handle, _ptr(j) if j is not None else None, _ptr(m["view"]),
+                                        _ptr(m["proj"]), _ptr(m["inverse_view"]), _ptr(m["inverse_proj"])) != 0:
+            raise LuzError(-1, "singular camera matrix")
+        return m
 
     def animate(self, frame: int) -> None:
         lib.luzscene_animate(self.handle, frame)

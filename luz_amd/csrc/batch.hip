@@ -461,11 +461,12 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
             return bad(LUZRT_E_INVALID_ARG, "null TLAS or output");
         if ((st = fill_scene(ctx, &d.scene, &ds[s], true)) != LUZRT_OK) { batch_free(b); return st; }
         ds[s].ray_group = ctx->ray_order == LUZRT_RAY_ORDER_GROUPED ? kRayGroup : 0;
-        const luzrt_gbuffer& g = d.gbuffer;
-        if (!g.position || !g.normal || !g.albedo || !g.material || !g.emission)
-            return bad(LUZRT_E_INVALID_ARG, "null G-buffer plane");
-        if (g.width == 0 || g.height == 0 || g.width > 16384 || g.height > 16384)
-            return bad(LUZRT_E_LIMIT, "G-buffer size outside 1..16384");
+        luzrt_gbuffer g;
+        if ((st = check_gbuffer(ctx, &d.gbuffer, &g)) != LUZRT_OK) { batch_free(b); return st; }
+        // the batched kernels (and the primary rays that write the planes)
+        // address tightly packed planes with a position plane
+        if (!gbuffer_packed(g))
+            return bad(LUZRT_E_UNSUPPORTED, "batched frames take tightly packed G-buffer planes with a position plane");
         if (d.material_count && !d.materials) return bad(LUZRT_E_INVALID_ARG, "null material table");
         const uint64_t npix = (uint64_t)g.width * g.height;
         rays += npix * ds[s].r_total;
@@ -559,7 +560,7 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         const luzrt_batch_scene& d = scenes[s];
         BatchSceneDev& S = sc[s];
         std::memset(&S, 0, sizeof(S));
-        S.g = d.gbuffer;
+        (void)check_gbuffer(ctx, &d.gbuffer, &S.g);   // validated above: the layout filled in
         S.gout.width = d.gbuffer.width;
         S.gout.height = d.gbuffer.height;
         S.gout.position = const_cast<float*>(d.gbuffer.position);

@@ -284,7 +284,12 @@ luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ho
                           uint32_t ray_group = 0, bool trace = false);
 // Ray group of a visibility call on `tlas` under the context's ray order.
 uint32_t ray_group_for(const luzrt_ctx* ctx, const luzrt_tlas* tlas);
-luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g);
+// Validates a G-buffer descriptor and writes it with its layout filled in
+// (zero strides and pitches replaced by the packed defaults; the depth
+// pointer cleared when a position plane is given): the form kernels take.
+luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g, luzrt_gbuffer* out);
+// A checked descriptor of tightly packed planes with a position plane.
+bool gbuffer_packed(const luzrt_gbuffer& g);
 luzrt_status ensure_lut(luzrt_ctx* ctx);
 
 }  // namespace luz

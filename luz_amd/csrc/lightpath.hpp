@@ -17,6 +17,59 @@ constexpr float kPiF = 3.14159265359f;  // LuzCommon.h:10
 
 __device__ inline float clamp01(float a) { return a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a); }
 
+// ------------------------------------------------------------ G-buffer texels
+// A luzrt_gbuffer reaches the device with its layout filled in (the host's
+// check_gbuffer replaces zero strides and pitches by the packed defaults), so
+// texel (x, y) of a plane is at base + y * pitch + x * stride.
+__device__ inline const float* gb_f3(const float* base, uint32_t pitch, uint32_t stride, uint32_t x,
+                                     uint32_t y) {
+    return reinterpret_cast<const float*>(reinterpret_cast<const uint8_t*>(base) + (size_t)y * pitch +
+                                          (size_t)x * stride);
+}
+
+__device__ inline uint32_t gb_word(const uint32_t* base, uint32_t pitch, uint32_t x, uint32_t y) {
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(base) + (size_t)y * pitch +
+                                              (size_t)x * 4u);
+}
+
+__device__ inline V3 gb_normal(const luzrt_gbuffer& g, uint32_t x, uint32_t y) {
+    const float* n = gb_f3(g.normal, g.normal_pitch, g.normal_stride, x, y);
+    return v3(n[0], n[1], n[2]);
+}
+
+// out[r] = sum over c of m[c][r] v[c] (column-major), summed in column order.
+__device__ inline void mat4_mul(const float* m, const float v[4], float out[4]) {
+    for (int r = 0; r < 4; ++r) {
+        float a = m[r] * v[0];
+        a = a + m[4 + r] * v[1];
+        a = a + m[8 + r] * v[2];
+        a = a + m[12 + r] * v[3];
+        out[r] = a;
+    }
+}
+
+// DepthToWorld (utils.glsl:1-7) at the centre of pixel (x, y): the clip-space
+// point (uv * 2 - 1, depth, 1) through the inverse projection, divided by its
+// w (correctly rounded), then through the inverse view.
+__device__ inline V3 depth_to_world(const luzrt_gbuffer& g, uint32_t x, uint32_t y) {
+    const float d = gb_f3(g.depth, g.depth_pitch, 4u, x, y)[0];
+    const float u = __fdiv_rn((float)x + 0.5f, (float)g.width);
+    const float v = __fdiv_rn((float)y + 0.5f, (float)g.height);
+    const float clip[4] = {u * 2.0f - 1.0f, v * 2.0f - 1.0f, d, 1.0f};
+    float vs[4], ws[4];
+    mat4_mul(g.inverse_proj, clip, vs);
+    const float w = vs[3];
+    for (int k = 0; k < 4; ++k) vs[k] = __fdiv_rn(vs[k], w);
+    mat4_mul(g.inverse_view, vs, ws);
+    return v3(ws[0], ws[1], ws[2]);
+}
+
+__device__ inline V3 gb_position(const luzrt_gbuffer& g, uint32_t x, uint32_t y) {
+    if (!g.position) return depth_to_world(g, x, y);
+    const float* q = gb_f3(g.position, g.position_pitch, g.position_stride, x, y);
+    return v3(q[0], q[1], q[2]);
+}
+
 // Pixel g of the 64-pixel-tile order with tiles tw wide and 64/tw tall.
 __device__ inline void tile_pixel(uint32_t g, uint32_t W, uint32_t tw, uint32_t& x, uint32_t& y) {
     uint32_t tile = g >> 6, t = g & 63u;
@@ -40,6 +93,20 @@ __device__ inline bool pixel_active(const float* normal, uint32_t g, uint32_t W,
     p = y * W + x;
     const float* n = normal + (size_t)p * 3;
     return vlen(v3(n[0], n[1], n[2])) != 0.0f;
+}
+
+// As above, through the layout of a G-buffer descriptor.
+__device__ inline bool pixel_active(const luzrt_gbuffer& gb, uint32_t g, uint32_t tw, uint32_t ntiles_px,
+                                    uint32_t& p, bool& inside) {
+    uint32_t x = 0, y = 0;
+    p = 0;
+    inside = false;
+    if (g >= ntiles_px) return false;
+    tile_pixel(g, gb.width, tw, x, y);
+    if (x >= gb.width || y >= gb.height) return false;
+    inside = true;
+    p = y * gb.width + x;
+    return vlen(gb_normal(gb, x, y)) != 0.0f;
 }
 
 // Exclusive rank of this thread's flag within its 256-thread block; the
@@ -98,6 +165,17 @@ __device__ inline PixelFrame pixel_frame(const float* position, const float* nor
     f.y = p / W;
     f.P = v3(position[(size_t)p * 3], position[(size_t)p * 3 + 1], position[(size_t)p * 3 + 2]);
     f.N = v3(normal[(size_t)p * 3], normal[(size_t)p * 3 + 1], normal[(size_t)p * 3 + 2]);
+    f.lc = vlen(vsub(f.P, v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2])));
+    return f;
+}
+
+// The frame of pixel p through the layout of a G-buffer descriptor.
+__device__ inline PixelFrame pixel_frame(const luzrt_gbuffer& g, const luzrt_scene& S, uint32_t p) {
+    PixelFrame f;
+    f.x = p % g.width;
+    f.y = p / g.width;
+    f.P = gb_position(g, f.x, f.y);
+    f.N = gb_normal(g, f.x, f.y);
     f.lc = vlen(vsub(f.P, v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2])));
     return f;
 }
@@ -349,18 +427,20 @@ template <typename VIS>
 __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds, const float* lut,
                                      const VIS& vis, uint32_t p) {
     const luzrt_scene& S = ds->s;
-    V3 N = v3(g.normal[(size_t)p * 3], g.normal[(size_t)p * 3 + 1], g.normal[(size_t)p * 3 + 2]);
+    const uint32_t px = p % g.width, py = p / g.width;
+    V3 N = gb_normal(g, px, py);
     if (vlen(N) == 0.0f)
         return make_float4(S.ambient_color[0] * S.ambient_intensity,
                            S.ambient_color[1] * S.ambient_intensity,
                            S.ambient_color[2] * S.ambient_intensity, 1.0f);
-    uint32_t alb = g.albedo[p], mat = g.material[p], emi = g.emission[p];
+    uint32_t alb = gb_word(g.albedo, g.albedo_pitch, px, py), mat = gb_word(g.material, g.material_pitch, px, py),
+             emi = gb_word(g.emission, g.emission_pitch, px, py);
     float albedo[3] = {lut[alb & 0xFFu], lut[(alb >> 8) & 0xFFu], lut[(alb >> 16) & 0xFFu]};
     const float inv255 = rcp_hw(255.0f);
     float rough = (float)(mat & 0xFFu) * inv255;
     float metal = (float)((mat >> 8) & 0xFFu) * inv255;
     float occl = (float)((mat >> 16) & 0xFFu) * inv255;
-    V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
+    V3 P = gb_position(g, px, py);
     V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
     V3 V = vnorm_hw(vsub(cam, P));
     float F0[3];

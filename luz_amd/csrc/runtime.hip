@@ -79,12 +79,50 @@ void stage_begin(luzrt_ctx* ctx, Stage s) {
 }
 void stage_end(luzrt_ctx* ctx, Stage s) { hipEventRecord(ctx->ev[s][1], ctx->stream); }
 
-luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g) {
+luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g, luzrt_gbuffer* out) {
     if (g->width == 0 || g->height == 0 || g->width > 16384 || g->height > 16384)
         return fail(ctx, LUZRT_E_LIMIT, "G-buffer size %ux%u outside 1..16384", g->width, g->height);
-    if (!g->position || !g->normal || !g->albedo || !g->material || !g->emission)
+    if ((!g->position && !g->depth) || !g->normal || !g->albedo || !g->material || !g->emission)
         return fail(ctx, LUZRT_E_INVALID_ARG, "G-buffer plane is null");
+    luzrt_gbuffer L = *g;
+    const uint64_t W = g->width;
+    auto stride = [&](uint32_t& s, const char* name) {
+        if (s == 0) s = 12;
+        if (s != 12 && s != 16)
+            return fail(ctx, LUZRT_E_INVALID_ARG, "G-buffer %s stride %u is neither 12 nor 16", name, s);
+        return LUZRT_OK;
+    };
+    auto pitch = [&](uint32_t& p, uint64_t row, const void* base, const char* name) {
+        if (p == 0) p = (uint32_t)row;
+        if (p < row || (p & 3u) != 0 || ((uintptr_t)base & 3u) != 0)
+            return fail(ctx, LUZRT_E_INVALID_ARG,
+                        "G-buffer %s: pitch %u below the row (%llu bytes) or not 4-byte aligned", name, p,
+                        (unsigned long long)row);
+        return LUZRT_OK;
+    };
+    luzrt_status s;
+    if ((s = stride(L.position_stride, "position")) != LUZRT_OK) return s;
+    if ((s = stride(L.normal_stride, "normal")) != LUZRT_OK) return s;
+    if (W * 16 > 0xFFFFFFFFull) return fail(ctx, LUZRT_E_LIMIT, "G-buffer row too wide");
+    if (L.position) {
+        if ((s = pitch(L.position_pitch, W * L.position_stride, L.position, "position")) != LUZRT_OK) return s;
+        L.depth = nullptr;
+    } else if ((s = pitch(L.depth_pitch, W * 4, L.depth, "depth")) != LUZRT_OK) {
+        return s;
+    }
+    if ((s = pitch(L.normal_pitch, W * L.normal_stride, L.normal, "normal")) != LUZRT_OK) return s;
+    if ((s = pitch(L.albedo_pitch, W * 4, L.albedo, "albedo")) != LUZRT_OK) return s;
+    if ((s = pitch(L.material_pitch, W * 4, L.material, "material")) != LUZRT_OK) return s;
+    if ((s = pitch(L.emission_pitch, W * 4, L.emission, "emission")) != LUZRT_OK) return s;
+    *out = L;
     return LUZRT_OK;
+}
+
+bool gbuffer_packed(const luzrt_gbuffer& g) {
+    const uint32_t W = g.width;
+    return g.position && g.position_stride == 12 && g.normal_stride == 12 && g.position_pitch == W * 12 &&
+           g.normal_pitch == W * 12 && g.albedo_pitch == W * 4 && g.material_pitch == W * 4 &&
+           g.emission_pitch == W * 4;
 }
 
 luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds, bool trace) {

@@ -603,21 +603,26 @@ int luzscene_shadow_map(const luzscene* s, uint32_t light, uint32_t resolution, 
     return 0;
 }
 
-int luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj[16], float inv_view_proj[16]) {
-    if (!s || !view_proj) return -1;
-    // CameraNode::GetView (glm::lookAt, right-handed) and GetProjJittered =
-    // translate(jitter) * perspective(radians(horizontalFov) as fovy, W/H,
-    // near, far) in [0, 1] depth with proj[1][1] negated (AssetManager.cpp:334-377).
+}  // extern "C"
+
+namespace {
+
+// CameraNode::GetView (glm::lookAt, right-handed) and GetProjJittered =
+// translate(jitter) * perspective(radians(horizontalFov) as fovy, W/H, near,
+// far) in [0, 1] depth with proj[1][1] negated (AssetManager.cpp:334-377),
+// column-major FP64.
+void camera_view_proj(const luzscene* s, const float jitter[2], double V[16], double P[16]) {
     const CameraNode& c = s->camera;
     D3 f = normalize(c.center - c.eye);
     D3 r = normalize(cross(f, d3(0, 1, 0)));
     D3 u = cross(r, f);
-    double V[16] = {r.x, u.x, -f.x, 0, r.y, u.y, -f.y, 0, r.z, u.z, -f.z, 0,
-                    -dot(r, c.eye), -dot(u, c.eye), dot(f, c.eye), 1};
+    const double v[16] = {r.x, u.x, -f.x, 0, r.y, u.y, -f.y, 0, r.z, u.z, -f.z, 0,
+                          -dot(r, c.eye), -dot(u, c.eye), dot(f, c.eye), 1};
+    for (int i = 0; i < 16; ++i) V[i] = v[i];
     const double th = std::tan(c.horizontalFov * kPiD / 180.0 * 0.5);
     const double aspect = (double)s->width / (double)s->height;
     const double n = c.nearDistance, fa = c.farDistance;
-    double P[16] = {0};
+    for (int i = 0; i < 16; ++i) P[i] = 0.0;
     P[0] = 1.0 / (aspect * th);
     P[5] = -1.0 / th;
     P[10] = fa / (n - fa);
@@ -629,6 +634,45 @@ int luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj
             P[col * 4 + 1] += jitter[1] * P[col * 4 + 3];
         }
     }
+}
+
+// Inverse of a column-major 4x4 FP64 matrix by Gauss-Jordan with partial
+// pivoting, rounded once to FP32; false when singular.
+bool invert4(const double M[16], float out[16]) {
+    double a[4][8];
+    for (int row = 0; row < 4; ++row)
+        for (int col = 0; col < 4; ++col) {
+            a[row][col] = M[col * 4 + row];
+            a[row][4 + col] = row == col ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int row = col + 1; row < 4; ++row)
+            if (std::fabs(a[row][col]) > std::fabs(a[piv][col])) piv = row;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int k = 0; k < 8; ++k) std::swap(a[piv][k], a[col][k]);
+        double d = a[col][col];
+        for (int k = 0; k < 8; ++k) a[col][k] /= d;
+        for (int row = 0; row < 4; ++row) {
+            if (row == col) continue;
+            double m = a[row][col];
+            for (int k = 0; k < 8; ++k) a[row][k] -= m * a[col][k];
+        }
+    }
+    for (int row = 0; row < 4; ++row)
+        for (int col = 0; col < 4; ++col) out[col * 4 + row] = (float)a[row][4 + col];
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj[16], float inv_view_proj[16]) {
+    if (!s || !view_proj) return -1;
+    double V[16], P[16];
+    camera_view_proj(s, jitter, V, P);
     double M[16];
     for (int col = 0; col < 4; ++col)
         for (int row = 0; row < 4; ++row) {
@@ -637,32 +681,21 @@ int luzscene_view_proj(const luzscene* s, const float jitter[2], float view_proj
             M[col * 4 + row] = acc;
         }
     for (int i = 0; i < 16; ++i) view_proj[i] = (float)M[i];
-    if (inv_view_proj) {
-        // Gauss-Jordan with partial pivoting on the FP64 matrix (row-major copy)
-        double a[4][8];
-        for (int row = 0; row < 4; ++row)
-            for (int col = 0; col < 4; ++col) {
-                a[row][col] = M[col * 4 + row];
-                a[row][4 + col] = row == col ? 1.0 : 0.0;
-            }
-        for (int col = 0; col < 4; ++col) {
-            int piv = col;
-            for (int row = col + 1; row < 4; ++row)
-                if (std::fabs(a[row][col]) > std::fabs(a[piv][col])) piv = row;
-            if (a[piv][col] == 0.0) return -2;
-            if (piv != col)
-                for (int k = 0; k < 8; ++k) std::swap(a[piv][k], a[col][k]);
-            double d = a[col][col];
-            for (int k = 0; k < 8; ++k) a[col][k] /= d;
-            for (int row = 0; row < 4; ++row) {
-                if (row == col) continue;
-                double m = a[row][col];
-                for (int k = 0; k < 8; ++k) a[row][k] -= m * a[col][k];
-            }
-        }
-        for (int row = 0; row < 4; ++row)
-            for (int col = 0; col < 4; ++col) inv_view_proj[col * 4 + row] = (float)a[row][4 + col];
+    if (inv_view_proj && !invert4(M, inv_view_proj)) return -2;
+    return 0;
+}
+
+int luzscene_camera_matrices(const luzscene* s, const float jitter[2], float view[16], float proj[16],
+                             float inv_view[16], float inv_proj[16]) {
+    if (!s) return -1;
+    double V[16], P[16];
+    camera_view_proj(s, jitter, V, P);
+    for (int i = 0; i < 16; ++i) {
+        if (view) view[i] = (float)V[i];
+        if (proj) proj[i] = (float)P[i];
     }
+    if (inv_view && !invert4(V, inv_view)) return -2;
+    if (inv_proj && !invert4(P, inv_proj)) return -2;
     return 0;
 }
 

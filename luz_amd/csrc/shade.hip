@@ -113,19 +113,21 @@ __global__ __launch_bounds__(kBlock) void k_shade_compact(const luzrt_gbuffer g,
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npix) return;
     const luzrt_scene& S = ds->s;
-    V3 N = v3(g.normal[(size_t)p * 3], g.normal[(size_t)p * 3 + 1], g.normal[(size_t)p * 3 + 2]);
+    const uint32_t px = p % g.width, py = p / g.width;
+    V3 N = gb_normal(g, px, py);
     if (vlen(N) == 0.0f) {
         out[p] = make_float4(S.ambient_color[0] * S.ambient_intensity,
                              S.ambient_color[1] * S.ambient_intensity,
                              S.ambient_color[2] * S.ambient_intensity, 1.0f);
         return;
     }
-    uint32_t alb = g.albedo[p], mat = g.material[p], emi = g.emission[p];
+    uint32_t alb = gb_word(g.albedo, g.albedo_pitch, px, py), mat = gb_word(g.material, g.material_pitch, px, py),
+             emi = gb_word(g.emission, g.emission_pitch, px, py);
     float albedo[3] = {s_lut[alb & 0xFFu], s_lut[(alb >> 8) & 0xFFu], s_lut[(alb >> 16) & 0xFFu]};
     float rough = (float)(mat & 0xFFu) / 255.0f;
     float metal = (float)((mat >> 8) & 0xFFu) / 255.0f;
     float occl = (float)((mat >> 16) & 0xFFu) / 255.0f;
-    V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
+    V3 P = gb_position(g, px, py);
     V3 V = vnorm(vsub(v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]), P));
     float a = rough * rough, a2 = a * a;
     float rr = rough + 1.0f, kk = (rr * rr) / 8.0f;
@@ -191,6 +193,7 @@ luzrt_status ensure_lut(luzrt_ctx* ctx) {
     return LUZRT_OK;
 }
 
+// g: a checked descriptor (check_gbuffer).
 luzrt_status shade_device(luzrt_ctx* ctx, const luzrt_gbuffer* g, const uint8_t* shadow,
                           const uint8_t* ao, float* out) {
     luzrt_status s = ensure_lut(ctx);
@@ -222,8 +225,9 @@ extern "C" luzrt_status luzrt_shade(luzrt_ctx* ctx, const luzrt_gbuffer* g, cons
     DevScene ds;
     luzrt_status s = upload_scene(ctx, scene, &ds);
     if (s != LUZRT_OK) return s;
-    if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
-    return shade_device(ctx, g, shadow, ao, out);
+    luzrt_gbuffer gl;
+    if ((s = check_gbuffer(ctx, g, &gl)) != LUZRT_OK) return s;
+    return shade_device(ctx, &gl, shadow, ao, out);
 }
 
 extern "C" luzrt_status luzrt_shade_fractions(luzrt_ctx* ctx, const luzrt_gbuffer* g,
@@ -237,12 +241,13 @@ extern "C" luzrt_status luzrt_shade_fractions(luzrt_ctx* ctx, const luzrt_gbuffe
     DevScene ds;
     luzrt_status s = upload_scene(ctx, scene, &ds);
     if (s != LUZRT_OK) return s;
-    if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
+    luzrt_gbuffer gl;
+    if ((s = check_gbuffer(ctx, g, &gl)) != LUZRT_OK) return s;
     if ((s = ensure_lut(ctx)) != LUZRT_OK) return s;
     const uint32_t npix = g->width * g->height;
     stage_begin(ctx, kStageShade);
     k_shade_fractions<<<(npix + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(
-        *g, (const DevScene*)ctx->scene.ptr, (const float*)ctx->lut.ptr, visibility, ao, (float4*)out);
+        gl, (const DevScene*)ctx->scene.ptr, (const float*)ctx->lut.ptr, visibility, ao, (float4*)out);
     stage_end(ctx, kStageShade);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
@@ -256,7 +261,8 @@ extern "C" luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas,
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_light_pass: null argument");
     if (((uintptr_t)out & 15u) != 0)
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_light_pass: output must be 16-byte aligned");
-    luzrt_status s = check_gbuffer(ctx, g);
+    luzrt_gbuffer gl;
+    luzrt_status s = check_gbuffer(ctx, g, &gl);
     if (s != LUZRT_OK) return s;
     const size_t npix = (size_t)g->width * g->height;
     if (ctx->precision != LUZRT_PRECISION_COMPACT) {
@@ -265,7 +271,7 @@ extern "C" luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas,
         if ((s = ensure_lut(ctx)) != LUZRT_OK) return s;
         stage_begin(ctx, kStageShade);
         k_shade_hits<<<cdiv(npix, kBlock), kBlock, 0, ctx->stream>>>(
-            *g, (const DevScene*)ctx->scene.ptr, (const float*)ctx->lut.ptr,
+            gl, (const DevScene*)ctx->scene.ptr, (const float*)ctx->lut.ptr,
             (const uint32_t*)ctx->act_index.ptr, (const uint32_t*)ctx->active.ptr,
             (const uint8_t*)ctx->hits.ptr, (const uint32_t*)ctx->counters.ptr, (float4*)out);
         stage_end(ctx, kStageShade);
@@ -278,6 +284,6 @@ extern "C" luzrt_status luzrt_light_pass(luzrt_ctx* ctx, const luzrt_tlas* tlas,
     s = luzrt_trace_visibility(ctx, tlas, g, scene, (uint8_t*)ctx->vis_shadow.ptr,
                                (uint8_t*)ctx->vis_ao.ptr);
     if (s != LUZRT_OK) return s;
-    return shade_device(ctx, g, (const uint8_t*)ctx->vis_shadow.ptr, (const uint8_t*)ctx->vis_ao.ptr,
+    return shade_device(ctx, &gl, (const uint8_t*)ctx->vis_shadow.ptr, (const uint8_t*)ctx->vis_ao.ptr,
                         out);
 }

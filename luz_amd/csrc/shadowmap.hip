@@ -150,9 +150,10 @@ __global__ __launch_bounds__(kBlock) void k_map_visibility(luzrt_gbuffer g, cons
     const uint32_t npix = g.width * g.height;
     if (p >= npix) return;
     const luzrt_scene& S = ds->s;
-    V3 N = v3(g.normal[(size_t)p * 3], g.normal[(size_t)p * 3 + 1], g.normal[(size_t)p * 3 + 2]);
+    const uint32_t px = p % g.width, py = p / g.width;
+    V3 N = gb_normal(g, px, py);
     const bool fg = vlen(N) != 0.0f;
-    V3 P = v3(g.position[(size_t)p * 3], g.position[(size_t)p * 3 + 1], g.position[(size_t)p * 3 + 2]);
+    V3 P = gb_position(g, px, py);
     // shadow origin (light.frag:138-139)
     V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
     float bias = fmax_(vlen(vsub(P, cam)) * 0.01f, 0.05f);
@@ -200,7 +201,8 @@ extern "C" luzrt_status luzrt_shadow_map_visibility(luzrt_ctx* ctx, const luzrt_
     DevScene ds;
     luzrt_status s = upload_scene(ctx, scene, &ds);
     if (s != LUZRT_OK) return s;
-    if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
+    luzrt_gbuffer gl;
+    if ((s = check_gbuffer(ctx, g, &gl)) != LUZRT_OK) return s;
     if (scene->num_lights == 0) return LUZRT_OK;
     for (uint32_t l = 0; l < count; ++l)
         if (maps[l].depth && (maps[l].resolution == 0 || maps[l].light_type < 0 || maps[l].light_type > 2))
@@ -212,7 +214,7 @@ extern "C" luzrt_status luzrt_shadow_map_visibility(luzrt_ctx* ctx, const luzrt_
     if (count)
         LUZ_HIP(ctx, hipMemcpyAsync(dmaps, maps, count * sizeof(luzrt_shadow_map), hipMemcpyHostToDevice, ctx->stream));
     const uint32_t npix = g->width * g->height;
-    k_map_visibility<<<(npix + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(*g, (const DevScene*)ctx->scene.ptr,
+    k_map_visibility<<<(npix + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(gl, (const DevScene*)ctx->scene.ptr,
                                                                               dmaps, count, shadow);
     LUZ_HIP(ctx, hipGetLastError());
     LUZ_HIP(ctx, hipFreeAsync(dmaps, ctx->stream));
@@ -226,7 +228,8 @@ extern "C" luzrt_status luzrt_light_pass_shadow_maps(luzrt_ctx* ctx, const luzrt
                                                      uint32_t count, float* out) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     if (!tlas || !g || !scene || !out) return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_light_pass_shadow_maps: null argument");
-    luzrt_status s = check_gbuffer(ctx, g);
+    luzrt_gbuffer gl;
+    luzrt_status s = check_gbuffer(ctx, g, &gl);
     if (s != LUZRT_OK) return s;
     const size_t npix = (size_t)g->width * g->height;
     const int nl = scene->num_lights < 0 ? 0 : scene->num_lights;

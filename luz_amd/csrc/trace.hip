@@ -34,13 +34,12 @@ __global__ void k_vis_init(uint32_t* counters, uint32_t ray_count) {
         counters[threadIdx.x] = threadIdx.x == kCntRays ? ray_count : 0u;
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact_count(const float* normal, uint32_t W, uint32_t H,
-                                                          uint32_t tw, uint32_t ntiles_px,
-                                                          uint32_t* block_counts) {
+__global__ __launch_bounds__(kBlock) void k_compact_count(const luzrt_gbuffer gb, uint32_t tw,
+                                                          uint32_t ntiles_px, uint32_t* block_counts) {
     __shared__ uint32_t wave_tot[kBlock / 64];
     uint32_t p;
     bool inside;
-    bool active = pixel_active(normal, blockIdx.x * kBlock + threadIdx.x, W, H, tw, ntiles_px, p, inside);
+    bool active = pixel_active(gb, blockIdx.x * kBlock + threadIdx.x, tw, ntiles_px, p, inside);
     uint32_t total;
     block_prefix(active, wave_tot, total);
     if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
@@ -87,8 +86,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, u
 
 // Writes the active list, the per-pixel active index and the traversal's
 // per-active-pixel records (pixel_record).
-__global__ __launch_bounds__(kBlock) void k_compact_write(const float* position, const float* normal,
-                                                          uint32_t W, uint32_t H, uint32_t tw,
+__global__ __launch_bounds__(kBlock) void k_compact_write(const luzrt_gbuffer gb, uint32_t tw,
                                                           uint32_t ntiles_px, const DevScene* ds,
                                                           const uint32_t* block_offsets,
                                                           uint32_t* act_list, uint32_t* act_index,
@@ -97,13 +95,13 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const float* position,
     uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     uint32_t p;
     bool inside;
-    bool active = pixel_active(normal, g, W, H, tw, ntiles_px, p, inside);
+    bool active = pixel_active(gb, g, tw, ntiles_px, p, inside);
     uint32_t total;
     uint32_t a = block_offsets[blockIdx.x] + block_prefix(active, wave_tot, total);
     if (active) {
         act_list[a] = p;
         float4 r0, r1;
-        pixel_record(pixel_frame(position, normal, W, ds->s, p), r0, r1);
+        pixel_record(pixel_frame(gb, ds->s, p), r0, r1);
         act_rec[2 * (size_t)a] = r0;
         act_rec[2 * (size_t)a + 1] = r1;
     }
@@ -332,7 +330,8 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     DevScene ds;
     luzrt_status s = upload_scene(ctx, scene, &ds, ray_group_for(ctx, tlas), true);
     if (s != LUZRT_OK) return s;
-    if ((s = check_gbuffer(ctx, g)) != LUZRT_OK) return s;
+    luzrt_gbuffer gl;
+    if ((s = check_gbuffer(ctx, g, &gl)) != LUZRT_OK) return s;
     const uint32_t W = g->width, H = g->height;
     const size_t npix = (size_t)W * H;
     const size_t max_rays = npix * ds.r_total;
@@ -355,9 +354,9 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     uint32_t ntiles_px = (uint32_t)ntiles_px_sz;
     uint32_t nblocks = cdiv(ntiles_px, kBlock);
     uint32_t* block_counts = (uint32_t*)ctx->act_index.ptr + npix;
-    k_compact_count<<<nblocks, kBlock, 0, st>>>(g->normal, W, H, tw, ntiles_px, block_counts);
+    k_compact_count<<<nblocks, kBlock, 0, st>>>(gl, tw, ntiles_px, block_counts);
     k_compact_scan<<<1, 1024, 0, st>>>(block_counts, nblocks, counters, dsd);
-    k_compact_write<<<nblocks, kBlock, 0, st>>>(g->position, g->normal, W, H, tw, ntiles_px, dsd,
+    k_compact_write<<<nblocks, kBlock, 0, st>>>(gl, tw, ntiles_px, dsd,
                                                block_counts, (uint32_t*)ctx->active.ptr,
                                                (uint32_t*)ctx->act_index.ptr, (float4*)ctx->act_rec.ptr);
     stage_end(ctx, kStageCompact);

@@ -1698,6 +1698,40 @@ int orc_any_hit_wide(const orc_scene* s, const float* rays, uint32_t n, uint8_t*
     return 0;
 }
 
+// DepthToWorld (utils.glsl:1-7) for every pixel of a D32F depth image (rows
+// `pitch` bytes apart): clip = (uv * 2 - 1, depth, 1) at the pixel centre
+// uv = ((x + 0.5) / W, (y + 0.5) / H); v = inverse_proj * clip with the
+// column products summed in column order; v /= v.w; P = (inverse_view * v).xyz.
+// FP32, every operation rounded as written.
+int orc_depth_to_world(const float* depth, uint32_t W, uint32_t H, uint32_t pitch, const float inv_proj[16],
+                       const float inv_view[16], float* position) {
+    if (!depth || !position || pitch < W * 4) return -1;
+    auto mul = [](const float* m, const float v[4], float out[4]) {
+        for (int r = 0; r < 4; ++r) {
+            float a = m[r] * v[0];
+            a = a + m[4 + r] * v[1];
+            a = a + m[8 + r] * v[2];
+            a = a + m[12 + r] * v[3];
+            out[r] = a;
+        }
+    };
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            float d;
+            std::memcpy(&d, reinterpret_cast<const uint8_t*>(depth) + (size_t)y * pitch + (size_t)x * 4, 4);
+            const float u = ((float)x + 0.5f) / (float)W, v = ((float)y + 0.5f) / (float)H;
+            const float clip[4] = {u * 2.0f - 1.0f, v * 2.0f - 1.0f, d, 1.0f};
+            float vs[4], ws[4];
+            mul(inv_proj, clip, vs);
+            const float w = vs[3];
+            for (int k = 0; k < 4; ++k) vs[k] = vs[k] / w;
+            mul(inv_view, vs, ws);
+            float* P = position + ((size_t)y * W + x) * 3;
+            P[0] = ws[0]; P[1] = ws[1]; P[2] = ws[2];
+        }
+    return 0;
+}
+
 // FP64 any-hit of caller-given rays (8 doubles each: origin, tmin, direction,
 // tmax) through traverse_d: the visibility decision the FP64 frame uses.
 int orc_any_hit_f64(const orc_scene* s, const double* rays, uint32_t n, uint8_t* hits) {

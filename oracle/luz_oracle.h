@@ -123,6 +123,8 @@ int  orc_trace_visibility(const orc_scene* s, const orc_gbuffer* g, const orc_pa
                           uint8_t* shadow, uint8_t* ao, orc_stats* stats, int threads,
                           uint32_t row_step);
 /* Same rays in FP64 (ray set-up, sampling angles and intersection). */
+int  orc_depth_to_world(const float* depth, uint32_t W, uint32_t H, uint32_t pitch, const float inv_proj[16],
+                        const float inv_view[16], float* position);
 int  orc_any_hit_f64(const orc_scene* s, const double* rays, uint32_t n, uint8_t* hits);
 int  orc_trace_visibility_f64(const orc_scene* s, const orc_gbuffer* g, const orc_params* p,
                               uint8_t* shadow, uint8_t* ao, int threads, uint32_t row_step);

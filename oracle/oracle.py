@@ -100,6 +100,7 @@ _sigs = {
                                        C.POINTER(_P), C.POINTER(C.c_uint32)]),
     "orc_any_hit_wide": (C.c_int, [_P, _P, C.c_uint32, _P, C.POINTER(OrcStats)]),
     "orc_any_hit_f64": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "orc_depth_to_world": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P]),
     "orc_trace_visibility_wide": (C.c_int, [_P, C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, _P,
                                             C.POINTER(OrcStats), C.c_int, C.c_uint32]),
     "orc_visibility_rays": (C.c_int64, [C.POINTER(OrcGBuffer), C.POINTER(OrcParams), _P, C.c_int64]),
@@ -174,6 +175,18 @@ def intersect_tri(o, d, tmin, tmax, v0, v1, v2):
 def box_test(o, d, tmin, tmax, lo, hi) -> bool:
     arr = [np.asarray(x, np.float32) for x in (o, d, lo, hi)]
     return bool(lib.orc_box_test(_p(arr[0]), _p(arr[1]), tmin, tmax, _p(arr[2]), _p(arr[3])))
+
+
+def depth_to_world(depth: np.ndarray, width: int, height: int, inverse_proj, inverse_view) -> np.ndarray:
+    """World positions (float32 [H*W, 3]) of a depth image [H, pitch/4] by DepthToWorld."""
+    d = np.ascontiguousarray(depth, np.float32)
+    pitch = d.shape[-1] * 4 if d.ndim == 2 else width * 4
+    ip = np.ascontiguousarray(inverse_proj, np.float32).reshape(16)
+    iv = np.ascontiguousarray(inverse_view, np.float32).reshape(16)
+    out = np.zeros((width * height, 3), np.float32)
+    if lib.orc_depth_to_world(_p(d), width, height, pitch, _p(ip), _p(iv), _p(out)) != 0:
+        raise ValueError("bad depth image")
+    return out
 
 
 def invert_3x4(m):

@@ -52,6 +52,8 @@ int main(void) {
          offsetof(luzrt_batch_scene, rgba32f_out), sizeof(luzrt_dn_frame));
   printf("%zu %zu %zu %zu\\n", offsetof(luzrt_dn_frame, ao_samples), sizeof(luzrt_timings),
          offsetof(luzrt_timings, denoise_ms), offsetof(luzrt_timings, rays));
+  printf("%zu %zu %zu %zu %zu\\n", sizeof(luzrt_gbuffer), offsetof(luzrt_gbuffer, normal_stride),
+         offsetof(luzrt_gbuffer, depth), offsetof(luzrt_gbuffer, inverse_view), sizeof(luzrt_gbuffer_out));
   return 0;
 }
 """)
@@ -60,7 +62,9 @@ int main(void) {
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(api.BatchScene), api.BatchScene.camera.offset, api.BatchScene.rgba32f_out.offset,
             C.sizeof(denoiser.DnFrame), denoiser.DnFrame.ao_samples.offset, C.sizeof(api.Timings),
-            api.Timings.denoise_ms.offset, api.Timings.rays.offset]
+            api.Timings.denoise_ms.offset, api.Timings.rays.offset,
+            C.sizeof(api.GBuffer), api.GBuffer.normal_stride.offset, api.GBuffer.depth.offset,
+            api.GBuffer.inverse_view.offset, C.sizeof(api.GBufferOut)]
     assert got == want
 
 

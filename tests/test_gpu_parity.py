@@ -249,3 +249,89 @@ def test_spot_light_and_shadow_types(ctx, shadow_type):
             dark.lights[l].intensity = 0.0
         np.testing.assert_allclose(rad, orc.shade(got, W, H, dark, wsh, wao), rtol=1e-6, atol=1e-7)
     g.close()
+
+
+def _engine_images(ctx, s, got, W, H, pad):
+    """Luz's own G-buffer images for a frame (DeferredRenderer.cpp:175-248):
+    RGBA32F normal, D32F depth (the camera's (proj view P).z / w, 1 on the
+    background), RGBA8 albedo/material/emission, each with `pad` bytes of
+    filler (NaN / 0xAB) after every row; and the descriptor that reads them in
+    place (depth mode)."""
+    m = s.camera_matrices()
+    V = m["view"].astype(np.float64).reshape(4, 4).T
+    Pm = m["proj"].astype(np.float64).reshape(4, 4).T
+    fg = np.abs(got["normal"]).sum(1) != 0
+    P = np.concatenate([got["position"].astype(np.float64), np.ones((W * H, 1))], 1)
+    clip = P @ (Pm @ V).T
+    depth = np.where(fg, clip[:, 2] / clip[:, 3], 1.0).astype(np.float32).reshape(H, W)
+    fpad = pad // 4
+    dimg = np.full((H, W + fpad), np.nan, np.float32)
+    dimg[:, :W] = depth
+    nimg = np.full((H, W * 4 + fpad), np.nan, np.float32)
+    nimg[:, :W * 4] = np.concatenate([got["normal"], np.zeros((W * H, 1), np.float32)], 1).reshape(H, W * 4)
+    words = {}
+    for k in ("albedo", "material", "emission"):
+        a = np.full((H, W + fpad), 0xABABABAB, np.uint32)
+        a[:, :W] = got[k].reshape(H, W)
+        words[k] = a
+    bufs = {"depth": ctx.to_device(dimg), "normal": ctx.to_device(nimg)}
+    bufs.update({k: ctx.to_device(v) for k, v in words.items()})
+    desc = api.GBuffer()
+    desc.width, desc.height = W, H
+    desc.position = None
+    desc.normal, desc.albedo = bufs["normal"].ptr, bufs["albedo"].ptr
+    desc.material, desc.emission = bufs["material"].ptr, bufs["emission"].ptr
+    desc.depth = bufs["depth"].ptr
+    desc.normal_stride = 16
+    desc.normal_pitch = W * 16 + pad
+    desc.albedo_pitch = desc.material_pitch = desc.emission_pitch = W * 4 + pad
+    desc.depth_pitch = W * 4 + pad
+    for k in range(16):
+        desc.inverse_proj[k] = float(m["inverse_proj"][k])
+        desc.inverse_view[k] = float(m["inverse_view"][k])
+    # what the oracle sees: packed planes with DepthToWorld positions
+    host = {"position": orc.depth_to_world(dimg, W, H, m["inverse_proj"], m["inverse_view"]),
+            "normal": got["normal"].copy(), "albedo": got["albedo"].copy(),
+            "material": got["material"].copy(), "emission": got["emission"].copy()}
+    return desc, bufs, host
+
+
+@pytest.mark.parametrize("kind,res", [(api.SCENE_CUBE, None), (api.SCENE_SPHERES, (480, 270))])
+def test_engine_gbuffer_images_read_in_place(ctx, kind, res):
+    """The engine's own attachments passed unchanged: RGBA32F normals (16-byte
+    stride), the D32F depth with DepthToWorld on the device, RGBA8 planes,
+    every row padded.  Visibility equals the oracle's on the DepthToWorld
+    positions bit for bit, and the light pass is within 1e-3."""
+    s, g = _gpu_scene(ctx, kind)
+    o = oracle_scene(s)
+    W, H, gd, got, _ = _gbuffers(ctx, s, g, o, *(res or (None, None)))
+    desc, bufs, host = _engine_images(ctx, s, got, W, H, pad=64)
+    sb = s.pack()
+    if kind == api.SCENE_CUBE:
+        set_light_direction(sb, 0, (0.8, -1.0, 0.6))
+    nl = sb.num_lights
+    dsh, dao = ctx.buffer(nl * W * H), ctx.buffer(W * H)
+    ctx.trace_visibility(g.tlas, desc, sb, dsh, dao)
+    wsh, wao, st = o.trace_visibility(host, W, H, sb)
+    print(f"kind {kind} {W}x{H} depth mode: {st.rays} rays, shadowed {int(wsh.sum())}", flush=True)
+    np.testing.assert_array_equal(dsh.download(np.uint8, (nl, W * H)), wsh)
+    np.testing.assert_array_equal(dao.download(np.uint8, W * H), wao)
+    out = ctx.buffer(W * H * 16)
+    ctx.light_pass(g.tlas, desc, sb, out)
+    rad = out.download(np.float32, (W * H, 4))
+    want = orc.shade(host, W, H, sb, wsh, wao)
+    err = np.abs(rad - want) / np.maximum(np.abs(want), 1e-3)
+    print(f"max rel err {float(err.max()):.3g}", flush=True)
+    assert err.max() <= RADIANCE_RTOL
+    # a position plane with a 16-byte stride and padded rows reads as the packed one
+    pimg = np.full((H, W * 4 + 16), np.nan, np.float32)
+    pimg[:, :W * 4] = np.concatenate([got["position"], np.ones((W * H, 1), np.float32)], 1).reshape(H, W * 4)
+    dpos = ctx.to_device(pimg)
+    desc.position = dpos.ptr
+    desc.position_stride = 16
+    desc.position_pitch = W * 16 + 64
+    ctx.trace_visibility(g.tlas, desc, sb, dsh, dao)
+    psh, pao, _ = o.trace_visibility(got, W, H, sb)
+    np.testing.assert_array_equal(dsh.download(np.uint8, (nl, W * H)), psh)
+    np.testing.assert_array_equal(dao.download(np.uint8, W * H), pao)
+    g.close()

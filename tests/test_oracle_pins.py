@@ -214,3 +214,35 @@ def test_fp64_counts_equal_fp32_counts_on_the_cube_frame():
     assert 0 < sh32.sum() < W * H and ao32.sum() < W * H * sb.ao_num_samples
     np.testing.assert_array_equal(sh64, sh32)
     np.testing.assert_array_equal(ao64, ao32)
+
+
+def test_depth_to_world_reproduces_the_primary_positions():
+    """DepthToWorld (utils.glsl:1-7) of the depth that the camera's own
+    projection gives each primary-ray hit returns that hit: the depth image is
+    (proj view P).z / w in FP64, rounded to FP32 as a D32F attachment stores
+    it; positions come back within the FP32 depth resolution (1e-3 of the
+    camera distance on the cube workload), rows and columns in image order."""
+    s = api.HostScene(api.SCENE_CUBE, 0)
+    W, H, cam = s.camera()
+    _, _, mats = s.nodes()
+    o = oracle_scene(s)
+    g = o.primary_gbuffer(cam, mats, W, H)
+    m = s.camera_matrices()
+    V = m["view"].astype(np.float64).reshape(4, 4).T   # column-major -> row-major
+    Pm = m["proj"].astype(np.float64).reshape(4, 4).T
+    fg = np.abs(g["normal"]).sum(1) != 0
+    P = np.concatenate([g["position"].astype(np.float64), np.ones((W * H, 1))], 1)
+    clip = P @ (Pm @ V).T
+    depth = np.where(fg, clip[:, 2] / clip[:, 3], 1.0).astype(np.float32).reshape(H, W)
+    assert 0.0 < depth[fg.reshape(H, W)].min() and depth.max() <= 1.0
+    got = orc.depth_to_world(depth, W, H, m["inverse_proj"], m["inverse_view"])
+    eye = np.array(cam.eye, np.float64)
+    dist = np.linalg.norm(g["position"][fg] - eye, axis=1)
+    err = np.linalg.norm(got[fg] - g["position"][fg], axis=1)
+    assert fg.sum() > 10000 and (err <= 1e-3 * dist).all(), float((err / dist).max())
+    # a padded pitch reads the same texels
+    padded = np.full((H, W + 5), np.nan, np.float32)
+    padded[:, :W] = depth
+    np.testing.assert_array_equal(orc.depth_to_world(padded, W, H, m["inverse_proj"], m["inverse_view"])[fg],
+                                  got[fg])
+    s.close()
