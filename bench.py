@@ -9,10 +9,15 @@ The G-buffer is produced once by primary rays before timing and stays
 resident in HBM, as do the BLASes.
 
 Prints one JSON line.  `value` is shadow+AO rays traced per second over the
-whole frame (Mrays/s).  `roofline` refers to the traversal kernel, with
-algorithmic bytes from the CPU oracle's canonical traversal counts on a row
-sample of the same frame.  `cpu_baseline` is the oracle (FP32, same
-arithmetic) timed on this host on that row sample.
+whole frame (Mrays/s).  `roofline` refers to the traversal kernel: its
+algorithmic bytes per launch (per-ray node, triangle and instance counts of
+the CPU oracle's canonical traversal of the identical BVH, plus the G-buffer
+reads and count writes per pixel) over its HIP-event time, the profiled HBM
+traffic, and the profiled VALU issue fraction that binds it.  `cpu_baseline`
+is the oracle (FP32, same arithmetic) on the whole frame on this host's
+cores.  `parity` compares every pixel's counts with the FP32 oracle and a
+row sample with the FP64 oracle.  `ao_radius16` repeats the frame with an AO
+radius of 16, where occluders fall within range of the lifted AO origins.
 """
 from __future__ import annotations
 
@@ -150,93 +155,191 @@ def main() -> int:
         record["config"]["autotune"] = {k: tuned[k] for k in
                                         ("groups_per_cu", "refill", "stack_entries", "tile_width", "table")}
 
-    traffic = None
+    prof = {}
     tpath = Path(args.traffic)
     if tpath.exists():
         try:
-            traffic = json.loads(tpath.read_text()).get("hbm_bytes_per_launch")
+            prof = json.loads(tpath.read_text())
         except (ValueError, OSError):
-            traffic = None
+            prof = {}
+    traffic = prof.get("hbm_bytes_per_launch")
 
     if not args.no_cpu and compact:
         record.update(compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays))
     elif not args.no_cpu:
         from oracle import oracle as orc
-        # the same scene on the oracle: FP32 visibility of the whole frame
         host = gbuf.download()
-        gpu_sh = ctx.buffer(max(nl, 1) * W * H)
-        gpu_ao = ctx.buffer(W * H)
-        ctx.trace_visibility(gs.tlas, gbuf.desc(), sb, gpu_sh, gpu_ao)
-        gsh = gpu_sh.download(np.uint8, (nl, W * H))
-        gao = gpu_ao.download(np.uint8, W * H)
         o = orc.OracleScene()
         for i in range(scene.mesh_count):
             v, idx = scene.mesh(i)
             o.add_mesh(v, idx)
         o2w, mesh_ids, _ = scene.nodes()
         o.set_instances(o2w, mesh_ids)
-        threads = min(orc.threads_default(), 16)
+        threads, cores_avail = cpu_threads()
+        par, st = visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, args.cpu_row_step * 4)
+        # CPU baseline: the oracle's FP32 visibility of the whole frame, median of 5
         o.trace_visibility(host, W, H, sb, threads=threads)      # warm-up frame
         times = []
-        for _ in range(3):
+        for _ in range(5):
             c0 = time.perf_counter()
-            wsh, wao, st = o.trace_visibility(host, W, H, sb, threads=threads)
+            o.trace_visibility(host, W, H, sb, threads=threads)
             times.append(time.perf_counter() - c0)
         cpu_s = float(np.median(times))
-        # single core on a row sample
         c0 = time.perf_counter()
         _, _, st1 = o.trace_visibility(host, W, H, sb, threads=1, row_step=args.cpu_row_step * 2)
         cpu1_s = time.perf_counter() - c0
-        m = np.ones(W * H, bool)
-        # FP64 oracle on a sparser sample for the AO RMSE
-        f64_step = args.cpu_row_step * 4
-        rows64 = np.zeros((H, W), bool)
-        rows64[::f64_step] = True
-        act = np.abs(host["normal"]).sum(1) != 0
-        m64 = rows64.reshape(-1) & act
-        dsh, dao, _ = o.trace_visibility(host, W, H, sb, threads=threads, row_step=f64_step, f64=True)
-        na = max(sb.ao_num_samples, 1)
-        ao_rmse = float(np.sqrt(np.mean((gao[m64].astype(np.float64) / na - dao[m64] / na) ** 2))) if m64.any() else 0.0
-        per_ray = (64.0 * st.node_visits + 36.0 * st.tri_tests + 64.0 * st.inst_visits) / max(st.rays, 1)
-        # node/tri/instance reads + the per-pixel record each ray is generated from
-        # (position, camera distance, normal, x|y: 32 B) + hit write
-        trace_bytes = rays * (per_ray + 32.0 + 1.0)
-        achieved = trace_bytes / (trace_avg * 1e-3) / 1e9
-        record["roofline"] = {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_trace_any",
-            # the BVH (about 11 MB) is served from L2 and the Infinity Cache, so the
-            # algorithmic bytes can exceed the HBM rate; the measured HBM bytes per
-            # launch over the same time give the HBM fraction actually drawn
-            "traffic_frac": round(traffic / (trace_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-            "algorithmic_bytes_per_ray": round(per_ray + 33.0, 2),
-            "per_ray_counts": {"node": round(st.node_visits / max(st.rays, 1), 3),
-                               "tri": round(st.tri_tests / max(st.rays, 1), 3),
-                               "inst": round(st.inst_visits / max(st.rays, 1), 3)},
-        }
+        record["roofline"] = roofline(ctx, st, rays, W * H, nl, trace_avg, traffic, prof)
         record["cpu_baseline"] = {
             "value": round(st.rays / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle FP32 visibility of the whole frame ({st.rays} rays), median of 3 "
-                      f"after a warm-up frame: {cpu_s * 1e3:.0f} ms/frame on {threads} threads; "
+            "sample": f"oracle FP32 visibility of the whole frame ({st.rays} rays), median of 5 "
+                      f"after a warm-up frame: {cpu_s * 1e3:.0f} ms/frame on {threads} threads "
+                      f"({cores_avail} CPUs in this process's affinity mask, {cpu_model()}); "
                       f"single thread on every {args.cpu_row_step * 2}th row: "
                       f"{st1.rays / cpu1_s / 1e6:.2f} Mrays/s",
             "ms_per_frame": round(cpu_s * 1e3, 1),
             "single_thread_mrays_per_s": round(st1.rays / cpu1_s / 1e6, 3),
+            "cpus_available": cores_avail,
+            "cpu_model": cpu_model(),
         }
-        record["parity"] = {
-            "compared_pixels": int(m.sum()),
-            "shadow_mismatch": int((gsh[:, m] != wsh[:, m]).sum()),
-            "ao_mismatch": int((gao[m] != wao[m]).sum()),
-            "ao_rmse_vs_fp64": round(ao_rmse, 6),
-            "count_mismatch_vs_fp64": int((gsh[:, m64] != dsh[:, m64]).sum() + (gao[m64] != dao[m64]).sum()),
-            "fp64_sample_rows_every": f64_step,
-        }
+        record["parity"] = par
+        # the same frame with an AO radius at which occluders fall in range (16):
+        # the reference lifts the AO origin by 0.01 x the camera distance (1-2.5
+        # here), so with its radius of 1 every AO ray of this scene misses
+        sb16 = api.SceneBlock()
+        import ctypes as C
+        C.memmove(C.byref(sb16), C.byref(sb), C.sizeof(api.SceneBlock))
+        sb16.ao_max = 16.0
+        record["ao_radius16"] = ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb16, out, W, H, threads,
+                                                args, traffic)
     print(json.dumps(record), flush=True)
     gs.close()
     ctx.close()
     return 0
+
+
+def cpu_threads():
+    """Oracle threads: every CPU of this process's affinity mask, at most 16
+    (the host CPU share of one GPU on the measurement boxes)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(avail, 16)), avail
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, f64_step):
+    """GPU counts against the FP32 oracle on every pixel and the FP64 oracle
+    on every f64_step-th row; the oracle's stats (canonical traversal counts)."""
+    import numpy as np
+
+    nl = sb.num_lights
+    gpu_sh = ctx.buffer(max(nl, 1) * W * H)
+    gpu_ao = ctx.buffer(W * H)
+    ctx.trace_visibility(gs.tlas, gbuf.desc(), sb, gpu_sh, gpu_ao)
+    gsh = gpu_sh.download(np.uint8, (max(nl, 1), W * H))[:nl]
+    gao = gpu_ao.download(np.uint8, W * H)
+    wsh, wao, st = o.trace_visibility(host, W, H, sb, threads=threads)
+    rows64 = np.zeros((H, W), bool)
+    rows64[::f64_step] = True
+    act = np.abs(host["normal"]).sum(1) != 0
+    m64 = rows64.reshape(-1) & act
+    dsh, dao, _ = o.trace_visibility(host, W, H, sb, threads=threads, row_step=f64_step, f64=True)
+    na = max(sb.ao_num_samples, 1)
+    ao_rmse = float(np.sqrt(np.mean((gao[m64].astype(np.float64) / na - dao[m64] / na) ** 2))) if m64.any() else 0.0
+    par = {
+        "compared_pixels": W * H,
+        "shadow_mismatch": int((gsh != wsh).sum()),
+        "ao_mismatch": int((gao != wao).sum()),
+        "ao_rmse_vs_fp64": round(ao_rmse, 6),
+        "count_mismatch_vs_fp64": int((gsh[:, m64] != dsh[:, m64]).sum() + (gao[m64] != dao[m64]).sum()),
+        "fp64_sample_rows_every": f64_step,
+        "ao_fraction_below_1": round(float(np.mean(gao[act] < sb.ao_num_samples)), 6),
+        "mean_ao": round(float(np.mean(gao[act].astype(np.float64) / na)), 6),
+    }
+    return par, st
+
+
+def roofline(ctx, st, rays, pixels, nl, trace_ms, traffic, prof) -> dict:
+    """The traversal kernel against its rooflines.  Algorithmic bytes per launch
+    follow the visibility formula sum over rays of (64 n_node + 36 n_tri +
+    64 n_inst) + 24 P + (L + 1) P, with per-ray counts from the oracle's
+    canonical traversal of the identical BVH; the VALU issue fraction is the
+    profiled SQ_INSTS_VALU per launch over CUs x 4 SIMDs x clock / 4 (one
+    wave64 VALU instruction per SIMD every 4 cycles) over the kernel time."""
+    nr = max(st.rays, 1)
+    per_ray = (64.0 * st.node_visits + 36.0 * st.tri_tests + 64.0 * st.inst_visits) / nr
+    b_vis = rays * per_ray + 24.0 * pixels + (nl + 1) * pixels
+    t = trace_ms * 1e-3
+    achieved = b_vis / t / 1e9
+    cus, clk_khz = ctx.device_info()
+    clock_hz = max(clk_khz, 1) * 1e3
+    valu = prof.get("valu_insts_per_launch")
+    valu_frac = valu / (cus * 4 * clock_hz / 4) / t if valu else None
+    hbm_frac = traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None
+    bound = "valu" if (valu_frac or 0) > max(hbm_frac or 0, achieved / HBM_PEAK_GBS) else "hbm"
+    return {
+        "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_trace_any",
+        "algorithmic_bytes_per_launch": round(b_vis),
+        "per_ray_counts": {"node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
+                           "inst": round(st.inst_visits / nr, 3)},
+        # the BVH (about 11 MB) is served from L2 and the Infinity Cache: the
+        # measured HBM bytes per launch over the same time give the HBM fraction drawn
+        "traffic_frac": round(hbm_frac, 4) if hbm_frac else None,
+        "valu_issue_frac": round(valu_frac, 4) if valu_frac else None,
+        "valu_insts_per_launch": valu,
+        "compute_units": cus, "clock_mhz": round(clock_hz / 1e6),
+        "counters_from": prof.get("source"),
+        # node/triangle/instance bytes plus the 32-byte pixel record each ray is
+        # generated from and its hit byte, per ray
+        "bytes_per_ray_with_pixel_records": round(per_ray + 33.0, 2),
+    }
+
+
+def ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb, out, W, H, threads, args, traffic) -> dict:
+    """The headline frame with AO radius 16: timing, parity and traversal counts."""
+    import numpy as np
+
+    def frame():
+        gs.update_resources_gpu(scene)
+        ctx.light_pass(gs.tlas, gbuf.desc(), sb, out)
+
+    for _ in range(max(args.warmup, 1)):
+        frame()
+    ctx.sync()
+    steps = max(args.steps // 2, 5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        frame()
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    trace_ms = []
+    for _ in range(5):
+        frame()
+        t = ctx.timings()
+        trace_ms.append(t.trace_ms)
+    rays = int(t.rays)
+    par, st = visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, args.cpu_row_step * 4)
+    nr = max(st.rays, 1)
+    return {
+        "ao_max": float(sb.ao_max), "steps": steps, "ms_per_frame": round(elapsed / steps * 1e3, 4),
+        "mrays_per_s": round(rays * steps / elapsed / 1e6, 2), "trace_kernel_ms": round(float(np.median(trace_ms)), 4),
+        "rays_per_frame": rays,
+        "per_ray_counts": {"node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
+                           "inst": round(st.inst_visits / nr, 3)},
+        "parity": par,
+    }
 
 
 def bench_batch(ctx, args) -> int:
@@ -278,7 +381,7 @@ def bench_batch(ctx, args) -> int:
             o2w, mesh_ids, _ = h.nodes()
             o.set_instances(o2w, mesh_ids)
             W, H, _ = h.camera()
-            wsh, wao, _ = o.trace_visibility(gbuf, W, H, sb, threads=min(orc.threads_default(), 16))
+            wsh, wao, _ = o.trace_visibility(gbuf, W, H, sb, threads=cpu_threads()[0])
             mism += int((sh != wsh).sum() + (ao != wao).sum())
             checked += W * H
         parity = {"scenes_checked": len(order), "pixels_checked": checked, "count_mismatch": mism}
@@ -297,10 +400,11 @@ def bench_batch(ctx, args) -> int:
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": (f"{args.scenes} seeded sphere-field scenes, tris log-uniform in [1e3, 5e5], "
-                         "resolutions from {320x240, 640x480, 1280x720, 1920x1080}; per frame: rigid "
-                         "animation, BLAS refit, TLAS rebuild, primary G-buffer, 5 shadow + 8 AO rays/px, "
-                         "GGX shading; one hipGraph replay per frame"),
+            "workload": (f"{args.scenes} seeded sphere-field scenes from the 1080p workload's generator, "
+                         "tris log-uniform in [1e3, 5e5], resolutions from {320x240, 640x480, 1280x720, "
+                         "1920x1080}; per frame: rigid animation, BLAS refit, TLAS rebuild, primary "
+                         "G-buffer, 1 dir + 8 point lights, 9 shadow + 16 AO rays/px, GGX shading; one "
+                         "hipGraph replay per frame"),
             "scenes": args.scenes, "pixels": wl.pixels, "instanced_tris": wl.triangles,
             "resolutions": [f"{w}x{h}" for w, h in res_set],
             "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
@@ -487,7 +591,7 @@ def bench_denoise(ctx, args) -> int:
         o2w, mesh_ids, _ = scene.nodes()
         o.set_instances(o2w, mesh_ids)
         step = 16
-        threads = min(orc.threads_default(), 16)
+        threads = cpu_threads()[0]
         f_sh = np.zeros((max(L, 1), n))
         f_ao = np.zeros(n)
         c0 = time.perf_counter()
@@ -536,7 +640,7 @@ def compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays) -> dict
     o2w, mesh_ids, _ = scene.nodes()
     o.set_instances(o2w, mesh_ids)
     o.build_wide()
-    threads = min(orc.threads_default(), 16)
+    threads = cpu_threads()[0]
     step32, step64 = 16, 64
     c0 = time.perf_counter()
     wsh, wao, st = o.trace_visibility(host, W, H, sb, threads=threads, row_step=step32, wide=True)

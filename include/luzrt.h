@@ -181,6 +181,9 @@ const char*  luzrt_status_string(luzrt_status s);
 luzrt_status luzrt_sync(luzrt_ctx* ctx);
 /* Per-stage GPU times of the last build/trace/light/primary call (synchronises). */
 luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
+/* The context's device: compute units and peak engine clock (kHz), as the
+ * HIP runtime reports them (for throughput and issue-rate figures). */
+luzrt_status luzrt_device_info(const luzrt_ctx* ctx, uint32_t* compute_units, uint32_t* clock_khz);
 /* Traversal launch parameters (0 keeps the current value): persistent
  * workgroups per CU (wave occupancy), idle lanes of a wave that trigger a
  * refill, LDS stack entries per lane (8, 16 or 32; deeper entries spill to

@@ -21,8 +21,11 @@
  *   LUZSCENE_TERRAIN    2048x2048 fBm heightfield (decimated 2x) and 40
  *                       displaced icospheres, 4 directional + 32 point
  *                       lights, 3840x2160, 8 AO rays per pixel;
- *   LUZSCENE_RANDOM     a sphere-field scene with a seeded triangle budget
- *                       and resolution (batched workloads).
+ *   LUZSCENE_RANDOM     the LUZSCENE_SPHERES generator at a seeded triangle
+ *                       budget (log-uniform in [1e3, 5e5]) and resolution
+ *                       (320x240, 640x480, 1280x720 or 1920x1080): icosphere
+ *                       level and count fill the budget, same lights, samples
+ *                       and camera (batched workloads).
  */
 #ifndef LUZSCENE_H
 #define LUZSCENE_H

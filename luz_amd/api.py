@@ -152,6 +152,7 @@ _sigs = {
     "luzrt_status_string": (C.c_char_p, [C.c_int]),
     "luzrt_sync": (C.c_int, [_P]),
     "luzrt_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
+    "luzrt_device_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "luzrt_set_ray_order": (C.c_int, [_P, C.c_uint32]),
     "luzrt_set_step_limit": (C.c_int, [_P, C.c_uint32]),
@@ -289,6 +290,12 @@ class Context:
 
     def sync(self) -> None:
         self._check(lib.luzrt_sync(self.handle))
+
+    def device_info(self) -> tuple:
+        """(compute units, peak engine clock in kHz) of the context's device."""
+        cu, clk = C.c_uint32(), C.c_uint32()
+        self._check(lib.luzrt_device_info(self.handle, C.byref(cu), C.byref(clk)))
+        return cu.value, clk.value
 
     def timings(self) -> Timings:
         t = Timings()

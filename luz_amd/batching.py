@@ -1,8 +1,10 @@
 """Batched serving workload: many independent scenes rendered per frame.
 
-Each scene is a seeded LUZSCENE_RANDOM sphere field (triangle budget
-log-uniform in [1e3, 5e5], resolution drawn from {320x240, 640x480,
-1280x720, 1920x1080}) with its own BLASes, TLAS, G-buffer and light buffer.
+Each scene is a seeded LUZSCENE_RANDOM sphere field, drawn by the 1080p
+sphere field's generator (triangle budget log-uniform in [1e3, 5e5],
+resolution drawn from {320x240, 640x480, 1280x720, 1920x1080}, 1 directional
++ 8 point lights, 1 shadow ray per light + 16 AO rays per pixel) with its
+own BLASes, TLAS, G-buffer and light buffer.
 Per frame every scene's mesh nodes move (seeded rigid motion,
 luzscene_animate), their instances are staged, and one luzrt_batch_frame
 refits the BLASes, rebuilds all TLASes, renders the G-buffers by primary

@@ -390,6 +390,15 @@ luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint
     return LUZRT_OK;
 }
 
+luzrt_status luzrt_device_info(const luzrt_ctx* ctx, uint32_t* compute_units, uint32_t* clock_khz) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return LUZRT_E_DEVICE;
+    if (compute_units) *compute_units = (uint32_t)prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = (uint32_t)prop.clockRate;
+    return LUZRT_OK;
+}
+
 luzrt_status luzrt_set_step_limit(luzrt_ctx* ctx, uint32_t max_steps) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     ctx->max_steps = max_steps ? max_steps : kMaxSteps;

@@ -317,20 +317,27 @@ luzscene* gen_random(uint64_t seed, uint32_t tri_budget, uint32_t w, uint32_t h)
         h = res[k][1];
     }
     auto* s = new luzscene();
-    // ground 16x16 quads (512 triangles) + spheres filling the rest of the budget
+    // The 1080p sphere field's generator (gen_spheres) at a triangle budget:
+    // displaced icospheres (up to 5 distinct meshes, one BLAS each) at seeded
+    // positions in the same 100^3 box, uniform rotations, scales in [0.5, 3],
+    // over the same ground (a 64x64-quad grid, 16x16 below 32k triangles), with
+    // the icosphere level and instance count filling the rest of the budget;
+    // 1 directional + 8 point lights, 1 shadow ray per light + 16 AO rays.
     SplitMix64 mr = stream(seed ^ 0x3131, kMaterials);
-    uint32_t rest = tri_budget > 512 ? tri_budget - 512 : 0;
+    const uint32_t grid = tri_budget >= 32768 ? 64 : 16;
+    const uint32_t ground = 2 * grid * grid;
+    uint32_t rest = tri_budget > ground ? tri_budget - ground : 0;
     int level = rest >= 20 * 20480 ? 5 : (rest >= 20 * 5120 ? 4 : (rest >= 20 * 1280 ? 3 : 2));
     uint32_t per = 20u << (2 * level);
     uint32_t count = std::max(1u, rest / per);
-    add_sphere_field(*s, seed, level, std::min(count, 3u), count, 20.0, 0.5, 2.0);
-    add_static(*s, make_grid(16, 16, 30.0, -25.0), mr);
+    add_sphere_field(*s, seed, level, std::min(count, 5u), count, 50.0, 0.5, 3.0);
+    add_static(*s, make_grid(grid, grid, 250.0, -55.0), mr);
     SplitMix64 lr = stream(seed, kLights);
     s->lights.push_back(directional(d3(-0.3, -1.0, -0.2), 3.0f, 0.0f));
-    add_point_lights(*s, lr, 4, 20.0, 300.0f, 0.5f);
-    set_camera(*s, d3(0, 15, 60), d3(0, -5, 0), w, h);
+    add_point_lights(*s, lr, 8, 50.0, 1000.0f, 0.5f);
+    set_camera(*s, d3(0, 50, 110), d3(0, -55, -10), w, h);
     s->lightSamples = 1;
-    s->aoSamples = 8;
+    s->aoSamples = 16;
     return s;
 }
 

@@ -181,12 +181,23 @@ def test_fp64_traversal_equals_brute_force_on_the_cube_scene():
 
 
 def test_fp64_traversal_equals_brute_force_on_an_instanced_sphere_scene():
-    # 29 rotated and scaled instances of three displaced icospheres + ground grid
+    # 29 rotated and scaled instances of five displaced icospheres + ground grid:
+    # rays from around one instance towards another (or the ground below it)
     s = api.HostScene(api.SCENE_RANDOM, 3, tri_budget=10000, width=64, height=48)
     o2w, mesh, _ = s.nodes()
     assert len(mesh) > 10 and s.mesh_count >= 3
     rng = np.random.default_rng(24)
-    _check_traversal_f64(s, _rays_f64(rng, 1500, (0.0, -5.0, 0.0), 18.0, 25.0))
+    centers = o2w[:, [3, 7, 11]].astype(np.float64)
+    n = 1500
+    src = centers[rng.integers(0, len(centers), n)] + rng.normal(0.0, 5.0, (n, 3))
+    dst = centers[rng.integers(0, len(centers), n)] + rng.normal(0.0, 2.0, (n, 3))
+    dst[::4, 1] = -60.0
+    rays = np.zeros((n, 8))
+    rays[:, 0:3] = src
+    rays[:, 3] = rng.uniform(0.0, 0.01, n)
+    rays[:, 4:7] = dst - src
+    rays[:, 7] = rng.uniform(0.2, 1.5, n)
+    _check_traversal_f64(s, rays)
     s.close()
 
 
