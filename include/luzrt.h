@@ -372,7 +372,10 @@ luzrt_status luzrt_compose(luzrt_ctx* ctx, uint32_t width, uint32_t height, cons
  * into a hipGraph and replayed; it is recaptured when staged instance counts
  * or the trace configuration change.  Plain precision only.  The TLASes,
  * BLASes and buffers named here belong to the batch's frames until
- * luzrt_batch_destroy; the batch does not take ownership of them. */
+ * luzrt_batch_destroy; the batch does not take ownership of them.  Two
+ * scenes may not share a TLAS, an output buffer, or a G-buffer plane that a
+ * camera writes (LUZRT_E_INVALID_ARG); G-buffers are tightly packed planes
+ * with a position plane (LUZRT_E_UNSUPPORTED otherwise). */
 typedef struct luzrt_batch luzrt_batch;
 
 typedef struct luzrt_batch_scene {

@@ -413,13 +413,15 @@ __device__ inline float shadow_factor(const luzrt_scene& S, const VIS& vis, uint
 }
 
 // The lighting arithmetic is compared with the oracle to 1e-3 relative, not
-// bit for bit (only visibility decisions are), so it uses the hardware
-// reciprocal, reciprocal square root and square root (about 1 ulp each)
-// instead of the correctly rounded division and square-root sequences that
-// the ray set-up needs.
+// bit for bit (only visibility decisions are), so its well-conditioned terms
+// use the hardware reciprocal, reciprocal square root and square root (about
+// 1 ulp each).  The GGX distribution is not well conditioned: near a
+// highlight of a smooth surface den = NdotH^2 (a^2 - 1) + 1 cancels to ~a^2
+// (5e-5 at roughness 0.09), so a 1-ulp change of |H| moves D by ~0.5%.  The
+// chain V, L, H, NdotH, den, D is therefore evaluated with correctly rounded
+// square roots and divisions in the oracle's order, and matches it bit for bit.
 __device__ inline float rcp_hw(float a) { return __builtin_amdgcn_rcpf(a); }
 __device__ inline float sqrt_hw(float a) { return __builtin_amdgcn_sqrtf(a); }
-__device__ inline V3 vnorm_hw(V3 a) { return vmul(a, __builtin_amdgcn_rsqf(vdot(a, a))); }
 constexpr float kInvPiF = 0.318309886183791f;
 
 // light.frag:main in FP32; lut = pow(i/255, 2.2).
@@ -437,12 +439,12 @@ __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds,
              emi = gb_word(g.emission, g.emission_pitch, px, py);
     float albedo[3] = {lut[alb & 0xFFu], lut[(alb >> 8) & 0xFFu], lut[(alb >> 16) & 0xFFu]};
     const float inv255 = rcp_hw(255.0f);
-    float rough = (float)(mat & 0xFFu) * inv255;
+    float rough = __fdiv_rn((float)(mat & 0xFFu), 255.0f);   // a^2 enters D: exact, as the oracle
     float metal = (float)((mat >> 8) & 0xFFu) * inv255;
     float occl = (float)((mat >> 16) & 0xFFu) * inv255;
     V3 P = gb_position(g, px, py);
     V3 cam = v3(S.cam_pos[0], S.cam_pos[1], S.cam_pos[2]);
-    V3 V = vnorm_hw(vsub(cam, P));
+    V3 V = vnorm(vsub(cam, P));
     float F0[3];
     for (int k = 0; k < 3; ++k) F0[k] = 0.04f * (1.0f - metal) + albedo[k] * metal;
     float Lo[3] = {0.0f, 0.0f, 0.0f};
@@ -454,25 +456,24 @@ __device__ inline float4 shade_pixel(const luzrt_gbuffer& g, const DevScene* ds,
     for (uint32_t l = 0; l < nl; ++l) {
         const luzrt_light& Lb = S.lights[l];
         V3 Lr = vsub(v3(Lb.position[0], Lb.position[1], Lb.position[2]), P);
-        V3 L = vnorm_hw(Lr);
-        float att = 1.0f;
         V3 ndir = v3(-Lb.direction[0], -Lb.direction[1], -Lb.direction[2]);
+        V3 L = vnorm(Lb.type == LUZRT_LIGHT_DIRECTIONAL ? ndir : Lr);
+        float att = 1.0f;
         if (Lb.type == LUZRT_LIGHT_DIRECTIONAL) {
-            L = vnorm_hw(ndir);
         } else if (Lb.type == LUZRT_LIGHT_SPOT) {
             att = rcp_hw(vdot(Lr, Lr));
-            float theta = vdot(L, vnorm_hw(ndir));
+            float theta = vdot(L, vnorm(ndir));
             float eps = Lb.inner_angle - Lb.outer_angle;
-            att = att * clamp01((theta - Lb.outer_angle) * rcp_hw(eps));
+            att = att * clamp01(__fdiv_rn(theta - Lb.outer_angle, eps));
         } else {
             att = rcp_hw(vdot(Lr, Lr));
         }
         float sh = shadow_factor(S, vis, l, p);
         float scale = Lb.intensity * att * (1.0f - sh);
-        V3 H = vnorm_hw(vadd(V, L));
+        V3 H = vnorm(vadd(V, L));
         float NdotH = fmax_(vdot(N, H), 0.0f);
         float den = NdotH * NdotH * (a2 - 1.0f) + 1.0f;
-        float D = a2 * rcp_hw(kPiF * den * den);
+        float D = __fdiv_rn(a2, kPiF * den * den);
         float NdotL = fmax_(vdot(N, L), 0.0f);
         float G = (NdotL * rcp_hw(NdotL * (1.0f - kk) + kk)) * gv;
         float HV = clamp01(vdot(H, V));
