@@ -287,7 +287,10 @@ def roofline(ctx, st, rays, pixels, nl, trace_ms, traffic, prof) -> dict:
     valu = prof.get("valu_insts_per_launch")
     valu_frac = valu / (cus * 4 * clock_hz / 4) / t if valu else None
     hbm_frac = traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None
-    bound = "valu" if (valu_frac or 0) > max(hbm_frac or 0, achieved / HBM_PEAK_GBS) else "hbm"
+    # the binding resource is the one whose measured fraction is larger: VALU
+    # issue against the HBM traffic actually drawn (the algorithmic bytes are
+    # mostly served from the caches and bound nothing)
+    bound = "valu" if valu_frac is not None and valu_frac > (hbm_frac or 0.0) else "hbm"
     return {
         "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_trace_any",
