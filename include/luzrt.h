@@ -271,6 +271,20 @@ luzrt_status luzrt_trace_rays(luzrt_ctx* ctx, const luzrt_tlas* tlas, const floa
 luzrt_status luzrt_trace_visibility(luzrt_ctx* ctx, const luzrt_tlas* tlas,
                                     const luzrt_gbuffer* gbuffer, const luzrt_scene* scene,
                                     uint8_t* shadow_counts, uint8_t* ao_counts);
+/* luzrt_trace_visibility without any allocation on the call path: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc).  Its work buffers live in one device array of 60.1 MiB
+ * that the library holds on each GPU it runs on until the process ends.
+ * Limits: width*height <= 2^19 pixels and width*height x (shadow samples of
+ * all lights + AO samples) <= 2^24 rays (LUZRT_E_LIMIT otherwise); at most
+ * 768 persistent traversal workgroups; a ray's stack holds 48 entries (a
+ * deeper one is reported like any traversal overflow, by luzrt_sync).
+ * Results equal luzrt_trace_visibility's.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (two contexts' streams) race. */
+luzrt_status luzrt_trace_visibility_static(luzrt_ctx* ctx, const luzrt_tlas* tlas,
+                                           const luzrt_gbuffer* gbuffer, const luzrt_scene* scene,
+                                           uint8_t* shadow_counts, uint8_t* ao_counts);
 /* GGX lighting from a G-buffer and visibility counts (light.frag:171-235). */
 luzrt_status luzrt_shade(luzrt_ctx* ctx, const luzrt_gbuffer* gbuffer, const luzrt_scene* scene,
                          const uint8_t* shadow_counts, const uint8_t* ao_counts,

@@ -156,6 +156,7 @@ _sigs = {
     "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "luzrt_set_ray_order": (C.c_int, [_P, C.c_uint32]),
     "luzrt_set_step_limit": (C.c_int, [_P, C.c_uint32]),
+    "luzrt_trace_visibility_static": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P]),
     "luzrt_set_precision": (C.c_int, [_P, C.c_int]),
     "luzrt_blas_wide_arrays": (C.c_int, [_P, _P, C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P),
                                          C.POINTER(_P), C.POINTER(C.c_uint32)]),

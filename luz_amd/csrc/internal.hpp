@@ -44,6 +44,10 @@ struct luzrt_ctx {
     uint32_t max_steps = 1u << 24; // traversal steps per ray before it is abandoned (kMaxSteps)
     // LUZRT_PRECISION_* mode: plain binary FP32 path or the compact form
     int precision = 0;
+    // device address of the file-scope scratch of luzrt_trace_visibility_static
+    // on this context's device (trace.hip), found once at luzrt_create
+    void* static_scratch = nullptr;
+    hipError_t static_scratch_error = hipSuccess;
     // pinned staging ring of small host-to-device uploads (scene blocks, TLAS
     // instances): stage_upload
     struct Staging {
@@ -261,6 +265,10 @@ inline uint32_t primary_chunk(uint64_t ntiles, uint32_t groups) {
     return c < 1 ? 1u : (c > 64 ? 64u : (uint32_t)c);
 }
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
+// Finds the device address of the no-allocation entry point's scratch array
+// on the context's device (current at the call); records a failure in the
+// context for that entry point to report.
+void static_scratch_init(luzrt_ctx* ctx);
 // luzrt_trace_visibility's work: compaction, traversal with rays generated in
 // place and, if `resolve`, the count planes `shadow` and `ao`.  Without it the
 // hit bytes stay in ctx->hits (ray_row order over ctx->active, the active

@@ -299,6 +299,7 @@ luzrt_status luzrt_create(luzrt_ctx** out, int device) {
         luzrt_destroy(ctx);
         return LUZRT_E_DEVICE;
     }
+    static_scratch_init(ctx);
     *out = ctx;
     return LUZRT_OK;
 }

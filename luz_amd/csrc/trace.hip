@@ -18,6 +18,8 @@
 //   k_resolve   per active pixel (in active order), occluded shadow samples
 //               per light and unoccluded AO samples, written as u8 planes;
 //               background pixels get zero counts.
+#include <algorithm>
+
 #include "internal.hpp"
 #include "lightpath.hpp"
 #include "traverse.hpp"
@@ -144,7 +146,7 @@ __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32
 // registers).
 enum RaySource { kRays = 0, kBatch = 1, kGen = 2 };
 
-template <bool WIDE, int SHORT, int SRC>
+template <bool WIDE, int SHORT, int SRC, int DEPTH = kSpill>
 __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceLaunch L, uint32_t refill) {
     constexpr bool BATCH = SRC == kBatch;
     __shared__ uint32_t lds[SHORT * kBlock];
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     }
     const uint32_t lane = tid & 63u;
     const uint32_t total = gridDim.x * kBlock;
-    StackT<SHORT> st;
+    StackT<SHORT, DEPTH> st;
     st.init(lds + tid, kBlock, L.spill + (size_t)blockIdx.x * kBlock + tid, total);
     const uint32_t nrays = L.counters[kCntRays];
     const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // active pixels
@@ -398,6 +400,75 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     return LUZRT_OK;
 }
 
+// ------------------------------------------------ visibility without allocation
+// luzrt_trace_visibility_static runs the same kernels as luzrt_trace_visibility
+// on work buffers carved from one file-scope device array (`scratch`), so the
+// call allocates nothing.  Limits: 2^19 pixels, 2^24 rays (pixels x ray slots),
+// a traversal of at most kStaticGroups persistent workgroups whose stacks hold
+// kStaticDepth entries (16 in LDS, the rest spilled); a deeper ray is reported
+// through the error latch like any traversal overflow.
+constexpr size_t kStaticPixels = size_t(1) << 19;
+constexpr size_t kStaticRays = size_t(1) << 24;
+constexpr int kStaticDepth = 48;
+constexpr int kStaticShort = 16;
+constexpr size_t kStaticSpillBytes = size_t(24) << 20;
+constexpr uint32_t kStaticGroups =
+    (uint32_t)(kStaticSpillBytes / (kBlock * (kStaticDepth - kStaticShort) * 4));   // 768
+constexpr size_t kStaticBlockBytes = size_t(64) << 10;   // compaction block counts
+constexpr size_t kOffCounters = 0;
+constexpr size_t kOffScene = 256;
+constexpr size_t kOffActive = 16384;
+constexpr size_t kOffIndex = kOffActive + kStaticPixels * 4;
+constexpr size_t kOffBlocks = kOffIndex + kStaticPixels * 4;
+constexpr size_t kOffRec = kOffBlocks + kStaticBlockBytes;
+constexpr size_t kOffHits = kOffRec + kStaticPixels * 32;
+constexpr size_t kOffSpill = kOffHits + kStaticRays + 256;
+constexpr size_t kStaticBytes = kOffSpill + kStaticSpillBytes;   // 60.1 MiB
+static_assert(sizeof(DevScene) <= kOffActive - kOffScene, "scene block fits its slot");
+static_assert(kOffRec % 16 == 0 && kOffSpill % 256 == 0, "aligned regions");
+
+// Each GPU that loads the library holds its own copy until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns.
+static __device__ __align__(256) uint8_t scratch[kStaticBytes];
+
+void static_scratch_init(luzrt_ctx* ctx) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    ctx->static_scratch = e == hipSuccess ? p : nullptr;
+    ctx->static_scratch_error = e;
+}
+
+namespace {
+
+// The scene block reaches the scratch as kernel arguments (no host staging
+// buffer whose lifetime the asynchronous call would have to outlive).
+constexpr uint32_t kSceneChunkWords = 512;
+struct SceneChunk {
+    uint32_t w[kSceneChunkWords];
+};
+
+__global__ void k_put_scene(uint32_t* dst, SceneChunk c, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = c.w[i];
+}
+
+// Clears the call's counters (the error latch included: the last call's was
+// handed to the context) before the compaction sets them.
+__global__ void k_static_init(uint32_t* counters) {
+    if (threadIdx.x < kCntCount) counters[threadIdx.x] = 0u;
+}
+
+// Hands the call's ray and pixel counts and its traversal error to the
+// context's counters, where luzrt_get_timings and luzrt_sync read them.
+__global__ void k_static_latch(const uint32_t* counters, uint32_t* ctx_counters) {
+    if (threadIdx.x == 0) {
+        ctx_counters[kCntActive] = counters[kCntActive];
+        ctx_counters[kCntRays] = counters[kCntRays];
+        if (counters[kCntError]) atomicOr(&ctx_counters[kCntError], counters[kCntError]);
+    }
+}
+
+}  // namespace
+
 }  // namespace luz
 
 using namespace luz;
@@ -443,4 +514,100 @@ extern "C" luzrt_status luzrt_trace_visibility(luzrt_ctx* ctx, const luzrt_tlas*
     if (!tlas || !g || !scene || !ao || (scene->num_lights > 0 && !shadow))
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_trace_visibility: null argument");
     return visibility_core(ctx, tlas, g, scene, shadow, ao, true);
+}
+
+extern "C" luzrt_status luzrt_trace_visibility_static(luzrt_ctx* ctx, const luzrt_tlas* tlas,
+                                                      const luzrt_gbuffer* g, const luzrt_scene* scene,
+                                                      uint8_t* shadow, uint8_t* ao) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    if (!tlas || !g || !scene || !ao || (scene->num_lights > 0 && !shadow))
+        return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_trace_visibility_static: null argument");
+    if (!ctx->static_scratch)
+        return fail(ctx, LUZRT_E_DEVICE, "luzrt_trace_visibility_static: no scratch on this device (%s)",
+                    hipGetErrorString(ctx->static_scratch_error));
+    DevScene ds;
+    luzrt_status s = fill_scene(ctx, scene, &ds, true);
+    if (s != LUZRT_OK) return s;
+    ds.ray_group = ray_group_for(ctx, tlas);
+    luzrt_gbuffer gl;
+    if ((s = check_gbuffer(ctx, g, &gl)) != LUZRT_OK) return s;
+    const uint32_t W = g->width, H = g->height;
+    const size_t npix = (size_t)W * H;
+    const uint32_t tw = ctx->tile_width, th = 64u / tw;
+    const size_t ntiles_px_sz = (size_t)((W + tw - 1) / tw) * ((H + th - 1) / th) * 64;
+    const uint32_t nblocks = cdiv(ntiles_px_sz, kBlock);
+    if (npix > kStaticPixels)
+        return fail(ctx, LUZRT_E_LIMIT, "luzrt_trace_visibility_static: %ux%u exceeds %zu pixels", W, H,
+                    kStaticPixels);
+    if (npix * ds.r_total > kStaticRays)
+        return fail(ctx, LUZRT_E_LIMIT,
+                    "luzrt_trace_visibility_static: %zu pixels x %u ray slots exceed %zu rays", npix,
+                    ds.r_total, kStaticRays);
+    if ((size_t)(nblocks + 2) * 4 > kStaticBlockBytes)
+        return fail(ctx, LUZRT_E_LIMIT, "luzrt_trace_visibility_static: %u compaction blocks", nblocks);
+    uint8_t* base = (uint8_t*)ctx->static_scratch;
+    uint32_t* counters = (uint32_t*)(base + kOffCounters);
+    DevScene* dsd = (DevScene*)(base + kOffScene);
+    uint32_t* act_list = (uint32_t*)(base + kOffActive);
+    uint32_t* act_index = (uint32_t*)(base + kOffIndex);
+    uint32_t* block_counts = (uint32_t*)(base + kOffBlocks);
+    float4* act_rec = (float4*)(base + kOffRec);
+    uint8_t* hits = base + kOffHits;
+    uint32_t* spill = (uint32_t*)(base + kOffSpill);
+    hipStream_t st = ctx->stream;
+
+    const uint32_t words = (uint32_t)((sizeof(DevScene) + 3) / 4);
+    for (uint32_t w0 = 0; w0 < words; w0 += kSceneChunkWords) {
+        SceneChunk c;
+        const uint32_t n = words - w0 < kSceneChunkWords ? words - w0 : kSceneChunkWords;
+        std::memset(&c, 0, sizeof(c));
+        std::memcpy(c.w, reinterpret_cast<const uint8_t*>(&ds) + (size_t)w0 * 4,
+                    std::min<size_t>((size_t)n * 4, sizeof(DevScene) - (size_t)w0 * 4));
+        k_put_scene<<<1, 256, 0, st>>>(reinterpret_cast<uint32_t*>(dsd) + w0, c, n);
+    }
+    k_static_init<<<1, 64, 0, st>>>(counters);
+    stage_begin(ctx, kStageCompact);
+    const uint32_t ntiles_px = (uint32_t)ntiles_px_sz;
+    k_compact_count<<<nblocks, kBlock, 0, st>>>(gl, tw, ntiles_px, block_counts);
+    k_compact_scan<<<1, 1024, 0, st>>>(block_counts, nblocks, counters, dsd);
+    k_compact_write<<<nblocks, kBlock, 0, st>>>(gl, tw, ntiles_px, dsd, block_counts, act_list, act_index,
+                                               act_rec);
+    stage_end(ctx, kStageCompact);
+    LUZ_HIP(ctx, hipGetLastError());
+
+    stage_begin(ctx, kStageTrace);
+    if (ds.r_total > 0 && tlas->count > 0) {
+        TraceLaunch L{};
+        L.tlas_nodes = tlas->nodes;
+        L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
+        L.tlas_nodes_count = tlas->node_count;
+        L.recs_count = tlas->count;
+        L.hits = hits;
+        L.counters = counters;
+        L.spill = spill;
+        L.max_steps = ctx->max_steps;
+        L.wide = tlas->wide;
+        L.gen = true;
+        L.ds = dsd;
+        L.act_rec = act_rec;
+        uint32_t groups = (uint32_t)ctx->num_cus * ctx->groups_per_cu;
+        groups = groups == 0 ? 1u : (groups > kStaticGroups ? kStaticGroups : groups);
+        const uint32_t refill = ctx->refill_threshold ? ctx->refill_threshold : 64u;
+        if (L.wide)
+            k_trace_any<true, kStaticShort, kGen, kStaticDepth><<<groups, kBlock, 0, st>>>(L, refill);
+        else
+            k_trace_any<false, kStaticShort, kGen, kStaticDepth><<<groups, kBlock, 0, st>>>(L, refill);
+        LUZ_HIP(ctx, hipGetLastError());
+    } else if (ds.r_total > 0) {
+        if ((s = zero_fill(ctx, hits, npix * ds.r_total)) != LUZRT_OK) return s;
+    }
+    stage_end(ctx, kStageTrace);
+
+    stage_begin(ctx, kStageResolve);
+    k_resolve<<<cdiv(npix, kBlock), kBlock, 0, st>>>(dsd, act_index, act_list, hits, counters,
+                                                     (uint32_t)npix, shadow, ao);
+    k_static_latch<<<1, 64, 0, st>>>(counters, (uint32_t*)ctx->counters.ptr);
+    stage_end(ctx, kStageResolve);
+    LUZ_HIP(ctx, hipGetLastError());
+    return LUZRT_OK;
 }

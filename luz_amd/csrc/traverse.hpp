@@ -20,11 +20,13 @@ constexpr int kSpill = 160;   // global overflow entries per lane (>= TLAS+BLAS 
 typedef uint32_t __attribute__((address_space(3))) LdsWord;
 typedef uint32_t __attribute__((address_space(1))) GlobalWord;
 
-// SHORT: LDS entries per lane (a power of two, chosen by the autotuner).  The
-// two columns carry their address spaces, so a pop reads LDS with an LDS load
-// and only an overflowed entry with a global load (a generic pointer would
-// turn every pop into a flat load that waits on both memory counters).
-template <int SHORT>
+// SHORT: LDS entries per lane (a power of two, chosen by the autotuner);
+// DEPTH: the deepest stack a ray may use (its spill column holds DEPTH
+// entries).  The two columns carry their address spaces, so a pop reads LDS
+// with an LDS load and only an overflowed entry with a global load (a generic
+// pointer would turn every pop into a flat load that waits on both memory
+// counters).
+template <int SHORT, int DEPTH = kSpill>
 struct StackT {
     LdsWord* ls;      // this lane's LDS column; entry e at ls[e * lds_stride]
     GlobalWord* gs;   // this lane's spill column; entry e at gs[e * g_stride]
@@ -42,12 +44,14 @@ struct StackT {
         lo = 0;
     }
 
+    // A full stack (DEPTH entries) refuses the push before anything spills, so
+    // the spill column is written at entries 0 .. DEPTH - SHORT - 1 only.
     __device__ inline bool push(uint32_t x) {
+        if (sp >= DEPTH) return false;
         if (sp - lo == SHORT) {
             gs[(size_t)lo * g_stride] = ls[(lo & (SHORT - 1)) * lds_stride];
             lo++;
         }
-        if (sp >= kSpill) return false;
         ls[(sp & (SHORT - 1)) * lds_stride] = x;
         sp++;
         return true;
