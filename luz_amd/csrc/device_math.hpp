@@ -93,9 +93,12 @@ __host__ __device__ inline V3 vcross(V3 a, V3 b) {
     return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 __device__ inline float vlen(V3 a) { return sqrtf(vdot(a, a)); }
+// normalize() as the oracle restates GLSL's: the correctly rounded
+// reciprocal of the correctly rounded length, then three products (one
+// division instead of three).
 __device__ inline V3 vnorm(V3 a) {
-    float l = vlen(a);
-    return v3(__fdiv_rn(a.x, l), __fdiv_rn(a.y, l), __fdiv_rn(a.z, l));
+    const float r = __fdiv_rn(1.0f, vlen(a));
+    return v3(a.x * r, a.y * r, a.z * r);
 }
 __device__ inline float fmin_(float a, float b) { return a < b ? a : b; }
 __device__ inline float fmax_(float a, float b) { return a > b ? a : b; }
@@ -167,9 +170,11 @@ __device__ inline TriRay tri_setup(V3 d) {
     float dz = sel3(d, kz);
     if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
     r.kx = kx; r.ky = ky; r.kz = kz;
-    r.Sx = __fdiv_rn(sel3(d, kx), dz);
-    r.Sy = __fdiv_rn(sel3(d, ky), dz);
+    // shear constants from one correctly rounded reciprocal (any values
+    // shared by all triangles of the ray keep the test watertight)
     r.Sz = __fdiv_rn(1.0f, dz);
+    r.Sx = sel3(d, kx) * r.Sz;
+    r.Sy = sel3(d, ky) * r.Sz;
     return r;
 }
 

@@ -72,7 +72,9 @@ inline f3 cross(f3 a, f3 b) {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 inline float length(f3 a) { return std::sqrt(dot(a, a)); }
-inline f3 normalize(f3 a) { float l = length(a); return mk(a.x / l, a.y / l, a.z / l); }
+// GLSL's normalize restated: one correctly rounded reciprocal of the length,
+// then three products.
+inline f3 normalize(f3 a) { float r = 1.0f / length(a); return mk(a.x * r, a.y * r, a.z * r); }
 inline float fmin_(float a, float b) { return a < b ? a : b; }
 inline float fmax_(float a, float b) { return a > b ? a : b; }
 inline float comp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
@@ -171,9 +173,9 @@ inline TriRay tri_setup(f3 d) {
     int ky = kx + 1; if (ky == 3) ky = 0;
     if (comp(d, kz) < 0.0f) std::swap(kx, ky);
     r.kx = kx; r.ky = ky; r.kz = kz;
-    r.Sx = comp(d, kx) / comp(d, kz);
-    r.Sy = comp(d, ky) / comp(d, kz);
     r.Sz = 1.0f / comp(d, kz);
+    r.Sx = comp(d, kx) * r.Sz;
+    r.Sy = comp(d, ky) * r.Sz;
     return r;
 }
 
@@ -1322,7 +1324,7 @@ void shade_pixel(const orc_gbuffer* g, const orc_params* p, const uint8_t* shado
     auto add3 = [](Shade3<R> a, Shade3<R> b) { return Shade3<R>{a.x + b.x, a.y + b.y, a.z + b.z}; };
     auto dot3 = [](Shade3<R> a, Shade3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
     auto len3 = [&](Shade3<R> a) { return sqrt(dot3(a, a)); };
-    auto nrm3 = [&](Shade3<R> a) { R l = len3(a); return Shade3<R>{a.x / l, a.y / l, a.z / l}; };
+    auto nrm3 = [&](Shade3<R> a) { R r = (R)1 / len3(a); return Shade3<R>{a.x * r, a.y * r, a.z * r}; };
     auto mx = [](R a, R b) { return a > b ? a : b; };
     auto clamp01 = [](R a) { return a < (R)0 ? (R)0 : (a > (R)1 ? (R)1 : a); };
     const R PI = (R)3.14159265359;
