@@ -85,6 +85,7 @@ struct BatchTables {
 __global__ void k_batch_init(uint32_t* counters) {
     if (threadIdx.x < kCntCount && threadIdx.x != kCntBuildError && threadIdx.x != kCntError)
         counters[threadIdx.x] = 0u;
+    clear_claims(counters, threadIdx.x);
 }
 
 // Persistent closest-hit primary rays over the 8x8-tile pixel slots of every
@@ -529,12 +530,12 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
     B_ALLOC(b->dev_work, work_bytes);
     B_ALLOC(b->dev_tables, tables_bytes);
     B_ALLOC(b->dev_mats, nmats * sizeof(luzrt_material));
-    B_ALLOC(b->counters, kCntCount * 4);
+    B_ALLOC(b->counters, kCounterBytes);
     B_ALLOC(b->dev_tlas, count * sizeof(BatchTlas));
     B_ALLOC(b->dev_jobs, count * sizeof(TlasJob));
 #undef B_ALLOC
     // the error latch in the counters starts clear; frames never clear it
-    e = hipMemset(b->counters, 0, kCntCount * 4);
+    e = hipMemset(b->counters, 0, kCounterBytes);
     if (e != hipSuccess) { batch_free(b); return hip_fail(ctx, e, "hipMemset(batch counters)"); }
     e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
     if (e != hipSuccess) { batch_free(b); return hip_fail(ctx, e, "hipEventCreate"); }

@@ -1016,7 +1016,7 @@ inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b
 // Device flag set by build kernels on bad input or a malformed tree; read by
 // luzrt_get_timings / the build call's own check.
 uint32_t* build_error_flag(luzrt_ctx* ctx) {
-    if (ensure(ctx, ctx->counters, 64) != LUZRT_OK) return nullptr;
+    if (ensure(ctx, ctx->counters, kCounterBytes) != LUZRT_OK) return nullptr;
     return (uint32_t*)ctx->counters.ptr + kCntBuildError;
 }
 

@@ -68,7 +68,7 @@ extern "C" luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* 
     if (out->width == 0 || out->height == 0 || out->width > 16384 || out->height > 16384)
         return fail(ctx, LUZRT_E_LIMIT, "luzrt_primary_gbuffer: size %ux%u outside 1..16384",
                     out->width, out->height);
-    luzrt_status s = ensure(ctx, ctx->counters, 64);
+    luzrt_status s = ensure(ctx, ctx->counters, kCounterBytes);
     if (s != LUZRT_OK) return s;
     // the material table lives in device memory for the duration of the call
     size_t mbytes = (size_t)(nmats ? nmats : 1) * sizeof(luzrt_material);

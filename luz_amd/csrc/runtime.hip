@@ -294,8 +294,8 @@ luzrt_status luzrt_create(luzrt_ctx** out, int device) {
     }
     for (int s = 0; s < kStageCount; ++s)
         for (int k = 0; k < 2; ++k) hipEventCreate(&ctx->ev[s][k]);
-    if (ensure(ctx, ctx->counters, 64) != LUZRT_OK ||
-        hipMemset(ctx->counters.ptr, 0, 64) != hipSuccess) {
+    if (ensure(ctx, ctx->counters, kCounterBytes) != LUZRT_OK ||
+        hipMemset(ctx->counters.ptr, 0, kCounterBytes) != hipSuccess) {
         luzrt_destroy(ctx);
         return LUZRT_E_DEVICE;
     }

@@ -178,7 +178,7 @@ extern "C" luzrt_status luzrt_shadow_map_build(luzrt_ctx* ctx, const luzrt_tlas*
     if (m->light_type < 0 || m->light_type > 2) return fail(ctx, LUZRT_E_INVALID_ARG, "bad light type");
     if (m->light_type == LUZRT_LIGHT_POINT && !(m->z_far > 0.0f))
         return fail(ctx, LUZRT_E_INVALID_ARG, "point-light shadow map needs z_far > 0");
-    luzrt_status s = ensure(ctx, ctx->counters, 64);
+    luzrt_status s = ensure(ctx, ctx->counters, kCounterBytes);
     if (s != LUZRT_OK) return s;
     uint32_t groups = (uint32_t)ctx->num_cus * 4;
     if (groups == 0) groups = 1;

@@ -34,6 +34,7 @@ constexpr int kBlock = 256;
 __global__ void k_vis_init(uint32_t* counters, uint32_t ray_count) {
     if (threadIdx.x <= kCntRays && threadIdx.x != kCntError)
         counters[threadIdx.x] = threadIdx.x == kCntRays ? ray_count : 0u;
+    clear_claims(counters, threadIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_count(const luzrt_gbuffer gb, uint32_t tw,
@@ -84,6 +85,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, u
         counters[kCntRays] = total * ds->r_total;
         counters[kCntFetch] = 0;
     }
+    clear_claims(counters, threadIdx.x);
 }
 
 // Writes the active list, the per-pixel active index and the traversal's
@@ -188,18 +190,34 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     uint32_t ray = kNone;
     bool open = true;
     uint32_t chunk_next = 0, chunk_end = 0;  // wave-uniform
+    // Segment s of the rays is [s n / 8, (s + 1) n / 8).  Workgroups are
+    // dispatched to the 8 XCDs in turn, so the waves of one XCD start on one
+    // segment (a contiguous part of the image, whose BVH nodes then share that
+    // XCD's L2) and move on to the next segments when it is exhausted.
+    const uint32_t seg0 = blockIdx.x % kClaimSegments;
+    uint32_t seg = 0;   // segments this wave has found exhausted (wave-uniform)
     for (;;) {
         if (open) {
             uint64_t idle = __ballot(ray == kNone);
             uint32_t nidle = (uint32_t)__popcll(idle);
             if (nidle >= refill || nidle == 64u) {
-                if (chunk_next >= chunk_end) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&L.counters[kCntFetch], chunk);
-                    base = __shfl(base, 0);
-                    chunk_next = base;
-                    chunk_end = base < nrays ? (nrays - base > chunk ? base + chunk : nrays) : base;
-                    if (base >= nrays) open = false;
+                while (chunk_next >= chunk_end && open) {
+                    if (seg == kClaimSegments) {
+                        open = false;
+                        break;
+                    }
+                    const uint32_t s = (seg0 + seg) % kClaimSegments;
+                    const uint32_t lo = (uint32_t)(((uint64_t)nrays * s) / kClaimSegments);
+                    const uint32_t hi = (uint32_t)(((uint64_t)nrays * (s + 1)) / kClaimSegments);
+                    uint32_t got = 0;
+                    if (lane == 0) got = atomicAdd(&L.counters[kClaimBase + s * kClaimStride], chunk);
+                    const uint32_t base = lo + __shfl(got, 0);
+                    if (got < hi - lo) {
+                        chunk_next = base;
+                        chunk_end = hi - base > chunk ? base + chunk : hi;
+                    } else {
+                        ++seg;
+                    }
                 }
                 uint32_t avail = chunk_end > chunk_next ? chunk_end - chunk_next : 0u;
                 uint32_t take = nidle < avail ? nidle : avail;
@@ -339,7 +357,7 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     const size_t max_rays = npix * ds.r_total;
     if (max_rays > 0xFFFFFFFFull)
         return fail(ctx, LUZRT_E_LIMIT, "%zu rays exceed the 32-bit ray index", max_rays);
-    if ((s = ensure(ctx, ctx->counters, 64)) != LUZRT_OK) return s;
+    if ((s = ensure(ctx, ctx->counters, kCounterBytes)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->active, npix * 4)) != LUZRT_OK) return s;
     if ((s = ensure(ctx, ctx->act_rec, npix * 32)) != LUZRT_OK) return s;
     // act_index holds the per-pixel active index followed by the per-block counts
@@ -416,7 +434,7 @@ constexpr uint32_t kStaticGroups =
     (uint32_t)(kStaticSpillBytes / (kBlock * (kStaticDepth - kStaticShort) * 4));   // 768
 constexpr size_t kStaticBlockBytes = size_t(64) << 10;   // compaction block counts
 constexpr size_t kOffCounters = 0;
-constexpr size_t kOffScene = 256;
+constexpr size_t kOffScene = kCounterBytes;
 constexpr size_t kOffActive = 16384;
 constexpr size_t kOffIndex = kOffActive + kStaticPixels * 4;
 constexpr size_t kOffBlocks = kOffIndex + kStaticPixels * 4;
@@ -425,6 +443,7 @@ constexpr size_t kOffHits = kOffRec + kStaticPixels * 32;
 constexpr size_t kOffSpill = kOffHits + kStaticRays + 256;
 constexpr size_t kStaticBytes = kOffSpill + kStaticSpillBytes;   // 60.1 MiB
 static_assert(sizeof(DevScene) <= kOffActive - kOffScene, "scene block fits its slot");
+static_assert(kOffScene % 256 == 0, "aligned scene block");
 static_assert(kOffRec % 16 == 0 && kOffSpill % 256 == 0, "aligned regions");
 
 // Each GPU that loads the library holds its own copy until the process ends;
@@ -455,6 +474,7 @@ __global__ void k_put_scene(uint32_t* dst, SceneChunk c, uint32_t n) {
 // handed to the context) before the compaction sets them.
 __global__ void k_static_init(uint32_t* counters) {
     if (threadIdx.x < kCntCount) counters[threadIdx.x] = 0u;
+    clear_claims(counters, threadIdx.x);
 }
 
 // Hands the call's ray and pixel counts and its traversal error to the
@@ -481,7 +501,7 @@ extern "C" luzrt_status luzrt_trace_rays(luzrt_ctx* ctx, const luzrt_tlas* tlas,
     if (((uintptr_t)rays & 15u) != 0)
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_trace_rays: rays must be 16-byte aligned");
     if (count == 0) return LUZRT_OK;
-    luzrt_status s = ensure(ctx, ctx->counters, 64);
+    luzrt_status s = ensure(ctx, ctx->counters, kCounterBytes);
     if (s != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
     stage_begin(ctx, kStageTrace);
