@@ -211,8 +211,11 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                     const uint32_t hi = (uint32_t)(((uint64_t)nrays * (s + 1)) / kClaimSegments);
                     uint32_t got = 0;
                     if (lane == 0) got = atomicAdd(&L.counters[kClaimBase + s * kClaimStride], chunk);
-                    const uint32_t base = lo + __shfl(got, 0);
-                    if (got < hi - lo) {
+                    // only lane 0 holds the atomic's result: every decision
+                    // below uses the broadcast copy, so the claim stays uniform
+                    const uint32_t off = __shfl(got, 0);
+                    const uint32_t base = lo + off;
+                    if (off < hi - lo) {
                         chunk_next = base;
                         chunk_end = hi - base > chunk ? base + chunk : hi;
                     } else {
