@@ -208,15 +208,15 @@ struct DevScene {
 // Device counters of one visibility call (ctx->counters).
 enum Counter { kCntFetch = 0, kCntActive = 1, kCntError = 2, kCntRays = 3, kCntBuildError = 4,
                kCntPrimary = 5, kCntCount = 8 };
-// A traversal launch claims its rays from kClaimSegments counters, one per
-// contiguous eighth of its rays, each on a 128-byte line of its own after the
-// Counter array: one counter took every wave's device-scope atomic and
-// serialised the claims (about 90k per 1080p frame).  A counter array is
-// kCounterBytes long.
+// A traversal launch claims its rays in chunks from kClaimSegments counters,
+// counter c owning chunks c, c + 8, c + 16, ..., each on a 128-byte line of
+// its own after the Counter array: one counter took every wave's
+// device-scope atomic (about 90k per 1080p frame, 1.9M per 64-scene batched
+// frame).  A counter array is kCounterBytes long.
 constexpr uint32_t kClaimSegments = 8;
 constexpr uint32_t kClaimBase = 256;      // word index of segment 0's counter
 constexpr uint32_t kClaimStride = 32;     // words between segment counters
-constexpr size_t kCounterBytes = (kClaimBase + kClaimSegments * kClaimStride) * 4;   // 2 KiB
+constexpr size_t kCounterBytes = ((kClaimBase + kClaimSegments * kClaimStride) * 4 + 255) & ~size_t(255);
 
 // Zeroes the claim counters (kClaimSegments threads of the calling block).
 __device__ inline void clear_claims(uint32_t* counters, uint32_t tid) {

@@ -190,12 +190,13 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     uint32_t ray = kNone;
     bool open = true;
     uint32_t chunk_next = 0, chunk_end = 0;  // wave-uniform
-    // Segment s of the rays is [s n / 8, (s + 1) n / 8).  Workgroups are
-    // dispatched to the 8 XCDs in turn, so the waves of one XCD start on one
-    // segment (a contiguous part of the image, whose BVH nodes then share that
-    // XCD's L2) and move on to the next segments when it is exhausted.
+    // Claim counter c hands out chunks c, c + 8, c + 16, ... of the launch's
+    // rays, so the waves still advance one front through the rays (the BVH
+    // nodes in use stay few) while their atomics spread over 8 addresses.  A
+    // wave starts on the counter of its workgroup and moves to the next one
+    // when a counter's chunks run past the last ray.
     const uint32_t seg0 = blockIdx.x % kClaimSegments;
-    uint32_t seg = 0;   // segments this wave has found exhausted (wave-uniform)
+    uint32_t seg = 0;   // counters this wave has found exhausted (wave-uniform)
     for (;;) {
         if (open) {
             uint64_t idle = __ballot(ray == kNone);
@@ -206,18 +207,15 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                         open = false;
                         break;
                     }
-                    const uint32_t s = (seg0 + seg) % kClaimSegments;
-                    const uint32_t lo = (uint32_t)(((uint64_t)nrays * s) / kClaimSegments);
-                    const uint32_t hi = (uint32_t)(((uint64_t)nrays * (s + 1)) / kClaimSegments);
+                    const uint32_t c = (seg0 + seg) % kClaimSegments;
                     uint32_t got = 0;
-                    if (lane == 0) got = atomicAdd(&L.counters[kClaimBase + s * kClaimStride], chunk);
+                    if (lane == 0) got = atomicAdd(&L.counters[kClaimBase + c * kClaimStride], 1u);
                     // only lane 0 holds the atomic's result: every decision
                     // below uses the broadcast copy, so the claim stays uniform
-                    const uint32_t off = __shfl(got, 0);
-                    const uint32_t base = lo + off;
-                    if (off < hi - lo) {
-                        chunk_next = base;
-                        chunk_end = hi - base > chunk ? base + chunk : hi;
+                    const uint64_t base = ((uint64_t)__shfl(got, 0) * kClaimSegments + c) * chunk;
+                    if (base < nrays) {
+                        chunk_next = (uint32_t)base;
+                        chunk_end = nrays - chunk_next > chunk ? chunk_next + chunk : nrays;
                     } else {
                         ++seg;
                     }
