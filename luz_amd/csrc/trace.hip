@@ -112,15 +112,15 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const luzrt_gbuffer gb
     if (inside) act_index[p] = active ? a : kNone;
 }
 
-// Rays are claimed in chunks per wave (one device-scope atomic on a single
-// counter per chunk).  The chunk follows the launch's ray count, 1/kChunkShare
-// of an even share per wave, kept within kChunkMin .. kChunkMax rays: the rays
-// in flight on the whole GPU (waves x chunk) then cover few image rows of one
-// light at a time, so the BVH nodes they read stay in L2, while the claims stay
-// below the rate the one counter sustains.  Measured on 1080p sphere fields
+// Rays are claimed in chunks per wave (one device-scope atomic per chunk).
+// The chunk follows the launch's ray count, 1/kChunkShare of an even share per
+// wave, kept within kChunkMin .. kChunkMax rays: the rays in flight on the
+// whole GPU (waves x chunk) then cover few image rows of one light at a time,
+// so the BVH nodes they read stay in L2, while the claims stay below the rate
+// the counters sustain.  Measured on 1080p sphere fields with a single counter
 // (trace ms, 2048 / 512 / 256-ray chunks): 1 shadow ray per light + 1 AO ray
 // 4.40 / 2.11 / 1.82; 1 shadow ray per light + 16 AO rays 3.13 / 2.10 / 2.55;
-// chunks of 64 or 128 lose again to the counter (3.6 and 2.3 ms at 1 + 1).
+// chunks of 64 or 128 lost again to the counter (3.6 and 2.3 ms at 1 + 1).
 constexpr uint32_t kChunkMin = 256, kChunkMax = 512, kChunkShare = 12;
 
 __device__ inline uint32_t claim_chunk(uint32_t nrays, uint32_t nwaves) {
