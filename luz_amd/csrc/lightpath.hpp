@@ -569,7 +569,7 @@ __device__ inline void primary_pixel(const Node* tlas_nodes, const InstRec* recs
         ray_begin(r, st, v3(cam.eye[0], cam.eye[1], cam.eye[2]), d, cam.tmin, cam.tmax);
         for (;;) {
             int res = ray_step<true>(r, st, tlas_nodes, recs, aux, &best);
-            if (res == kStepDone) break;
+            if (res == kStepDone || (res == kStepContinue && r.next == kNone && st.sp == 0)) break;
             if (res == kStepOverflow) { atomicOr(error_flag, 1u); st.sp = st.lo = 0; break; }
         }
     }

@@ -85,7 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow_map(const Node* tlas_nodes, c
             ray_begin(rs, st, o, d, 0.0f, tmax);
             for (;;) {
                 int res_step = ray_step<true>(rs, st, tlas_nodes, recs, aux, &best);
-                if (res_step == kStepDone) break;
+                if (res_step == kStepDone || (res_step == kStepContinue && rs.next == kNone && st.sp == 0))
+                    break;
                 if (res_step == kStepOverflow) { atomicOr(error_flag, 1u); break; }
             }
         }

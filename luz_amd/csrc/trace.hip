@@ -274,6 +274,8 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
             for (;;) {
                 int res = ray_step<false, WIDE>(r, st, tnodes, trecs, nullptr, nullptr, tlas_lds,
                                                 L.max_steps);
+                // a ray with nothing left to visit ends in this step, not in one more
+                if (res == kStepContinue && r.next == kNone && st.sp == 0) res = kStepDone;
                 if (res != kStepContinue) {
                     L.hits[ray] = res == kStepHit ? 1 : 0;
                     if (res == kStepOverflow) atomicOr(&L.counters[kCntError], 1u);
