@@ -165,6 +165,8 @@ typedef struct luzrt_timings {
           post_ms;
     uint64_t rays;          /* rays cast by the last visibility/light call */
     uint64_t active_pixels; /* non-background pixels of that call */
+    uint64_t traced_rays;   /* of those, ray ids the traversal ran (64 per live row when the
+                               call culled ray groups, luzrt_set_group_culling; else = rays) */
 } luzrt_timings;
 
 /* ------------------------------------------------------------ context */
@@ -207,6 +209,17 @@ luzrt_status luzrt_set_step_limit(luzrt_ctx* ctx, uint32_t max_steps);
  * Batched frames use SLOT unless GROUPED is set before luzrt_batch_create. */
 enum { LUZRT_RAY_ORDER_AUTO = 0, LUZRT_RAY_ORDER_SLOT = 1, LUZRT_RAY_ORDER_GROUPED = 2 };
 luzrt_status luzrt_set_ray_order(luzrt_ctx* ctx, uint32_t order);
+
+/* Ray-group culling of visibility and light-pass calls (a launch parameter,
+ * like the above; no counterpart in the reference).  Enabled (default): when
+ * the rays are in GROUPED order, the TLAS holds at most 64 instances and a
+ * pixel casts at most 64 rays, the rays of each 64-pixel group and light (or
+ * the group's AO rays) are bounded by a region computed from the group's
+ * G-buffer texels; instances whose boxes miss that region are not visited,
+ * and a group and light with no such instance has its rays written as misses
+ * without tracing them.  The bounds are conservative, so the results are the
+ * same as without culling.  luzrt_get_timings reports the rays traced. */
+luzrt_status luzrt_set_group_culling(luzrt_ctx* ctx, int enable);
 
 /* Precision modes.  PLAIN: binary BVH with FP32 boxes, FP32 shading.
  * COMPACT: BLASes traversed as 8-wide nodes with 8-bit child boxes rounded

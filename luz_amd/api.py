@@ -130,7 +130,8 @@ class Timings(C.Structure):
                 ("trace_ms", C.c_float), ("resolve_ms", C.c_float), ("shade_ms", C.c_float),
                 ("primary_ms", C.c_float), ("denoise_ms", C.c_float), ("post_ms", C.c_float),
                 ("rays", C.c_uint64),
-                ("active_pixels", C.c_uint64)]
+                ("active_pixels", C.c_uint64),
+                ("traced_rays", C.c_uint64)]
 
 
 class BatchScene(C.Structure):
@@ -155,6 +156,7 @@ _sigs = {
     "luzrt_device_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "luzrt_set_ray_order": (C.c_int, [_P, C.c_uint32]),
+    "luzrt_set_group_culling": (C.c_int, [_P, C.c_int]),
     "luzrt_set_step_limit": (C.c_int, [_P, C.c_uint32]),
     "luzrt_trace_visibility_static": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P]),
     "luzrt_set_precision": (C.c_int, [_P, C.c_int]),
@@ -310,6 +312,10 @@ class Context:
     def set_ray_order(self, order: int) -> None:
         """RAY_ORDER_AUTO, RAY_ORDER_SLOT or RAY_ORDER_GROUPED (same results, different speed)."""
         self._check(lib.luzrt_set_ray_order(self.handle, order))
+
+    def set_group_culling(self, enable: bool) -> None:
+        """Ray-group culling of visibility calls on (default) or off (same results)."""
+        self._check(lib.luzrt_set_group_culling(self.handle, 1 if enable else 0))
 
     def set_step_limit(self, max_steps: int) -> None:
         """Traversal steps per ray before it is abandoned (0: the default 2^24)."""

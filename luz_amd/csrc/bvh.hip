@@ -867,6 +867,7 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         for (int k = 0; k < 12; ++k) w2o_out[s * 12 + k] = src.w2o[k];
         order_out[s] = src.input_index;
         for (int k = 0; k < 6; ++k) cb[s * 6 + k] = wbox[i * 6 + k];
+        for (int k = 0; k < 6; ++k) J.slot_box[s * 6 + k] = wbox[i * 6 + k];
         cr[s] = kLeaf | s;
     }
     __syncthreads();   // the sort keys are dead from here on
@@ -1201,8 +1202,9 @@ luzrt_status launch_refit_batched(luzrt_ctx* ctx, const RefitJob* dev_jobs, uint
     return LUZRT_OK;
 }
 
+// world boxes, two cluster-box planes, two reference planes, slot boxes
 size_t tlas_scratch_bytes(uint32_t capacity) {
-    return (size_t)capacity * (3 * 6 * sizeof(float) + 2 * sizeof(uint32_t));
+    return (size_t)capacity * (4 * 6 * sizeof(float) + 2 * sizeof(uint32_t));
 }
 
 size_t tlas_lds_bytes(uint32_t m) {
@@ -1230,7 +1232,15 @@ TlasJob tlas_job(luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
     J.aux = t->aux;
     J.w2o_out = t->w2o;
     J.order_out = t->order;
+    J.slot_box = reinterpret_cast<float*>(J.cref[1] + cap);
     return J;
+}
+
+const float* tlas_slot_boxes(const luzrt_tlas* t) {
+    const size_t cap = t->capacity;
+    return reinterpret_cast<const float*>(reinterpret_cast<const uint32_t*>(
+                                              reinterpret_cast<const float*>(t->scratch) + cap * 18) +
+                                          2 * cap);
 }
 
 luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {

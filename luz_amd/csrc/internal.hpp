@@ -31,6 +31,11 @@ struct luzrt_ctx {
     // workspace
     luz::DevBuf hits, active, act_index, act_rec, counters, spill, lut, scene, vis_shadow,
         vis_ao;
+    // ray-group culling of visibility calls: candidate masks, live-row masks
+    // and counts, the live-row list
+    luz::DevBuf cull_cand, cull_bits, cull_rows;
+    bool group_cull = true;        // luzrt_set_group_culling
+    bool last_culled = false;      // the last visibility call traced live rows only
     // timing
     hipEvent_t ev[luz::kStageCount][2] = {};
     bool ev_used[luz::kStageCount] = {};
@@ -159,8 +164,11 @@ struct TlasJob {
     InstAux* aux;
     float* w2o_out;
     uint32_t* order_out;
+    float* slot_box;        // padded world box of each slot (6 floats), for ray-group culling
 };
 TlasJob tlas_job(luzrt_tlas* t, const InstIn* dev_in, uint32_t n);
+// The padded world boxes of a built TLAS's slots (6 floats each, device).
+const float* tlas_slot_boxes(const luzrt_tlas* t);
 // Scratch bytes of a TLAS with room for `capacity` instances, and the dynamic
 // LDS bytes of its build kernel for m (a power of two >= the instance count).
 size_t tlas_scratch_bytes(uint32_t capacity);
@@ -207,7 +215,7 @@ struct DevScene {
 
 // Device counters of one visibility call (ctx->counters).
 enum Counter { kCntFetch = 0, kCntActive = 1, kCntError = 2, kCntRays = 3, kCntBuildError = 4,
-               kCntPrimary = 5, kCntCount = 8 };
+               kCntPrimary = 5, kCntLive = 6, kCntCount = 8 };
 // A traversal launch claims its rays in chunks from kClaimSegments counters,
 // counter c owning chunks c, c + 8, c + 16, ..., each on a 128-byte line of
 // its own after the Counter array: one counter took every wave's
@@ -251,6 +259,14 @@ struct TraceLaunch {
     bool gen;
     const struct DevScene* ds;
     const float4* act_rec;
+    // Ray-group culling (gen launches, grouped order): the launch traces only
+    // the live rows live_rows[0 .. counters[kCntLive]) (a row: slot j of the 64
+    // active pixels of group g, encoded g << 6 | j), and a ray of kind k (its
+    // light, or num_lights for AO) starts from the candidate slots
+    // cand[g * cand_kinds + k] instead of the TLAS root.  Null: no culling.
+    const uint32_t* live_rows;
+    const uint64_t* cand;
+    uint32_t cand_kinds;
 };
 
 // One scene of a multi-scene launch: its TLAS (nodes null: empty TLAS) and

@@ -247,26 +247,32 @@ __device__ inline V3 ao_dir(const PixelFrame& f, uint32_t frame, int s) {
               T.z * hx + B.z * hy + N.z * hz);
 }
 
-// The ray of slot j from the frame f of an active pixel.  Slot j < r_shadow
-// is shadow sample j - slot_base[l] of the last light l with
-// slot_base[l] <= j (lights without samples own no slots); slot j >= r_shadow
-// is AO sample j - r_shadow.  Shadow rays start at shadow_origin with
-// tmin 0.001 and end at the light (tmax = |light vector|); AO rays span
+// The light that owns shadow slot j < r_shadow: the last light l with
+// slot_base[l] <= j (lights without samples own no slots).
+__device__ inline int slot_light(const DevScene* ds, uint32_t j) {
+    int lo = 0, hi = ds->s.num_lights;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ds->slot_base[mid] <= j) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The ray of slot j from the frame f of an active pixel, slot j < r_shadow
+// belonging to light l = slot_light(j) (l is ignored for AO slots).  Slot
+// j < r_shadow is shadow sample j - slot_base[l] of light l; slot j >= r_shadow
+// is AO sample j - r_shadow.  Shadow rays start at shadow_origin with tmin
+// 0.001 and end at the light (tmax = |light vector|); AO rays span
 // [ao_min, ao_max] from ao_origin.
-__device__ inline void gen_ray(const DevScene* ds, const PixelFrame& f, uint32_t j, V3& o, V3& d,
-                               float& tmin, float& tmax) {
+__device__ inline void gen_ray_of(const DevScene* ds, const PixelFrame& f, uint32_t j, int l, V3& o,
+                                  V3& d, float& tmin, float& tmax) {
     const luzrt_scene& S = ds->s;
     const uint32_t frame = (uint32_t)S.frame;
     if (j < ds->r_shadow) {
-        int lo = 0, hi = S.num_lights;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (ds->slot_base[mid] <= j) lo = mid; else hi = mid;
-        }
-        const luzrt_light& L = S.lights[lo];
+        const luzrt_light& L = S.lights[l];
         const ShadowSetup ss = shadow_setup(L, f.P);
         o = shadow_origin(f);
-        d = shadow_dir(L, ss, f, frame, lo, (int)(j - ds->slot_base[lo]));
+        d = shadow_dir(L, ss, f, frame, l, (int)(j - ds->slot_base[l]));
         tmin = 0.001f;
         tmax = ss.tmax;
     } else {
@@ -275,6 +281,11 @@ __device__ inline void gen_ray(const DevScene* ds, const PixelFrame& f, uint32_t
         tmin = S.ao_min;
         tmax = S.ao_max;
     }
+}
+
+__device__ inline void gen_ray(const DevScene* ds, const PixelFrame& f, uint32_t j, V3& o, V3& d,
+                               float& tmin, float& tmax) {
+    gen_ray_of(ds, f, j, j < ds->r_shadow ? slot_light(ds, j) : 0, o, d, tmin, tmax);
 }
 
 // Ray order of a visibility launch over A active pixels with R slots each:

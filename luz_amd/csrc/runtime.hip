@@ -310,7 +310,7 @@ luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
     hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->hits, &ctx->active, &ctx->act_index, &ctx->act_rec,
                       &ctx->counters, &ctx->spill, &ctx->lut, &ctx->scene, &ctx->vis_shadow,
-                      &ctx->vis_ao};
+                      &ctx->vis_ao, &ctx->cull_cand, &ctx->cull_bits, &ctx->cull_rows};
     for (DevBuf* b : bufs)
         if (b->ptr) hipFree(b->ptr);
     for (int s = 0; s < kStageCount; ++s)
@@ -362,11 +362,13 @@ luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out) {
     }
     out->rays = 0;
     out->active_pixels = 0;
+    out->traced_rays = 0;
     if (ctx->counters.ptr) {
         uint32_t c[kCntCount];
         LUZ_HIP(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
         out->rays = c[kCntRays];
         out->active_pixels = c[kCntActive];
+        out->traced_rays = ctx->last_culled ? (uint64_t)c[kCntLive] * 64u : out->rays;
         if (c[kCntError]) return check_trace_latch(ctx);
         if (c[kCntBuildError])
             return fail(ctx, LUZRT_E_INVALID_ARG, "acceleration-structure build error flag %u",
@@ -388,6 +390,12 @@ luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint
     if (refill) ctx->refill_threshold = refill;
     if (stack_entries) ctx->stack_entries = stack_entries;
     if (tile_width) ctx->tile_width = tile_width;
+    return LUZRT_OK;
+}
+
+luzrt_status luzrt_set_group_culling(luzrt_ctx* ctx, int enable) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    ctx->group_cull = enable != 0;
     return LUZRT_OK;
 }
 

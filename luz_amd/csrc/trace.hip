@@ -112,6 +112,232 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const luzrt_gbuffer gb
     if (inside) act_index[p] = active ? a : kNone;
 }
 
+// ------------------------------------------------------ ray-group culling
+// The rays of a group of 64 consecutive active pixels (a ray group of the
+// grouped order) that share a kind (the samples of one light, or the AO
+// samples) lie in a region bounded cheaply from the group's pixel records:
+//   shadow rays of light l run from origins o in the box A of the group's
+//   shadow origins to end points within 2 x radius of o + Lvec (light.frag
+//   :111-146): Lvec = the light vector, so o + Lvec is the light position
+//   lifted by the origin's offset N x sb for point and spot lights, and is
+//   computed per pixel for directional lights (box C of the end points).
+//   Every point of such a ray is within max(|A|/2, |C|/2 + 2 radius) of the
+//   segment between the centres of A and C, at the same fraction of its
+//   length, so a TLAS slot whose padded world box, widened by that distance,
+//   misses the segment cannot hold a hit of any ray of the group and light;
+//   AO rays run from origins within 0.05 |N| of A for at most
+//   ao_max x max(|N|, |N|^2) (light.frag:86-97,229-231), so a slot box that
+//   misses A widened by that reach cannot hold one of their hits.
+// k_cull writes, per group, the candidate mask of each kind (bit s: TLAS slot
+// s) and the group's live rows (slots j whose kind has a candidate); rays of
+// other rows are misses without being traced.  The widening carries a margin
+// of 1e-3 + 2e-5 x the magnitude of the coordinates involved, far beyond the
+// FP32 rounding that separates these bounds from the traced rays.  A group
+// whose records hold a non-finite value gets every slot as candidate.
+constexpr uint32_t kCullGroup = 64;     // pixels per culling group (= kRayGroup)
+constexpr uint32_t kMaxCullInst = 64;   // TLAS slots a candidate mask covers
+constexpr uint32_t kMaxCullSlots = 64;  // ray slots per pixel a live-row mask covers
+constexpr uint32_t kMaxCand = 8;        // candidates entered directly; more start at the TLAS root
+
+__device__ inline float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline bool finite3(V3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+
+// Does the segment p0 + s (p1 - p0), s in [0, 1], meet the box [lo - w, hi + w]?
+__device__ inline bool segment_meets_box(const float p0[3], const float p1[3], const float lo[3],
+                                         const float hi[3], float w) {
+    float s0 = 0.0f, s1 = 1.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float a = lo[k] - w, b = hi[k] + w, d = p1[k] - p0[k];
+        if (d == 0.0f) {
+            if (p0[k] < a || p0[k] > b) return false;
+        } else {
+            float t0 = (a - p0[k]) / d, t1 = (b - p0[k]) / d;
+            if (t0 > t1) { const float t = t0; t0 = t1; t1 = t; }
+            s0 = fmax_(s0, t0);
+            s1 = fmin_(s1, t1);
+        }
+    }
+    return s0 <= s1;
+}
+
+// One wave per group (four per workgroup).
+__global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const uint32_t* counters,
+                                                 const DevScene* ds, const float* slot_box,
+                                                 uint32_t ninst, uint32_t ngroups, uint32_t kinds,
+                                                 uint64_t* cand, uint64_t* live_bits,
+                                                 uint32_t* live_count) {
+    __shared__ uint64_t s_mask[kBlock / 64][LUZRT_MAX_LIGHTS + 1];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
+    if (g >= ngroups) return;
+    const uint32_t A = counters[kCntActive];
+    if ((uint64_t)g * kCullGroup >= A) {
+        if (lane == 0) {
+            live_bits[g] = 0;
+            live_count[g] = 0;
+        }
+        return;
+    }
+    const luzrt_scene& S = ds->s;
+    const uint32_t a = g * kCullGroup + lane;
+    // lanes past the last active pixel repeat the group's first pixel
+    const uint32_t ar = a < A ? a : g * kCullGroup;
+    const PixelFrame f = pixel_from_record(act_rec[2 * (size_t)ar], act_rec[2 * (size_t)ar + 1]);
+    const V3 o = shadow_origin(f);
+    const float nlen = vlen(f.N);
+    const float lift = fmax_(f.lc * 0.01f, 0.05f) * nlen;
+    const bool bad = !(finite3(o) && finite3(f.P) && isfinite(lift));
+    const bool any_bad = __ballot(bad) != 0ull;
+    float alo[3] = {wave_min(o.x), wave_min(o.y), wave_min(o.z)};
+    float ahi[3] = {wave_max(o.x), wave_max(o.y), wave_max(o.z)};
+    const float liftmax = wave_max(lift), nmax = wave_max(nlen);
+    float a0[3], ra2 = 0.0f, mag = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        a0[k] = 0.5f * (alo[k] + ahi[k]);
+        const float h = 0.5f * (ahi[k] - alo[k]);
+        ra2 += h * h;
+        mag = fmax_(mag, fmax_(fabsf(alo[k]), fabsf(ahi[k])));
+    }
+    const float rA = sqrtf(ra2) * 1.0001f;
+    // this lane's TLAS slot box
+    const bool own = lane < ninst;
+    float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
+    if (own) {
+        for (int k = 0; k < 3; ++k) {
+            blo[k] = slot_box[lane * 6 + k];
+            bhi[k] = slot_box[lane * 6 + 3 + k];
+        }
+    }
+    const uint64_t all = ninst >= 64 ? ~0ull : ((1ull << ninst) - 1ull);
+    const uint32_t nl = (uint32_t)S.num_lights;
+    for (uint32_t l = 0; l < nl; ++l) {
+        const luzrt_light& Lt = S.lights[l];
+        uint64_t m = 0;
+        if (Lt.num_shadow_samples > 0) {
+            const V3 lpos = v3(Lt.position[0], Lt.position[1], Lt.position[2]);
+            const bool lbad = !(finite3(lpos) && isfinite(Lt.direction[0]) && isfinite(Lt.direction[1]) &&
+                                isfinite(Lt.direction[2]) && isfinite(Lt.radius));
+            float c0[3], rC, cmag;
+            if (Lt.type == LUZRT_LIGHT_DIRECTIONAL) {
+                // end points o + dir (dir . normalize(-dir)) |lpos - P| (shadow_setup)
+                const V3 dir = v3(Lt.direction[0], Lt.direction[1], Lt.direction[2]);
+                const float kd = vdot(dir, vnorm(v3(-dir.x, -dir.y, -dir.z)));
+                const V3 e = vadd(o, vmul(dir, kd * vlen(vsub(lpos, f.P))));
+                const float elo[3] = {wave_min(e.x), wave_min(e.y), wave_min(e.z)};
+                const float ehi[3] = {wave_max(e.x), wave_max(e.y), wave_max(e.z)};
+                float rc2 = 0.0f;
+                cmag = 0.0f;
+                for (int k = 0; k < 3; ++k) {
+                    c0[k] = 0.5f * (elo[k] + ehi[k]);
+                    const float h = 0.5f * (ehi[k] - elo[k]);
+                    rc2 += h * h;
+                    cmag = fmax_(cmag, fmax_(fabsf(elo[k]), fabsf(ehi[k])));
+                }
+                rC = sqrtf(rc2) * 1.0001f;
+            } else {
+                c0[0] = lpos.x; c0[1] = lpos.y; c0[2] = lpos.z;
+                rC = liftmax * 1.0001f;
+                cmag = fmax_(fabsf(lpos.x), fmax_(fabsf(lpos.y), fabsf(lpos.z)));
+            }
+            rC += 2.0f * fabsf(Lt.radius);
+            const float w = fmax_(rA, rC) + 1e-3f + 2e-5f * (mag + cmag + rA + rC);
+            const bool meet = own && segment_meets_box(a0, c0, blo, bhi, w);
+            m = __ballot(meet);
+            if (any_bad || lbad || !isfinite(w)) m = all;
+        }
+        if (lane == 0) s_mask[wave][l] = m;
+    }
+    {
+        const float amax = fmax_(S.ao_max, 0.0f);
+        const float reach = amax * fmax_(nmax, nmax * nmax) * 1.00001f + 0.05f * nmax;
+        const float w = reach + 1e-3f + 2e-5f * (mag + reach);
+        bool meet = own;
+        for (int k = 0; k < 3; ++k) meet = meet && alo[k] - w <= bhi[k] && ahi[k] + w >= blo[k];
+        uint64_t m = __ballot(meet);
+        if (any_bad || !isfinite(S.ao_max) || !isfinite(w)) m = all;
+        if (lane == 0) s_mask[wave][nl] = m;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane; k < kinds; k += 64) cand[(size_t)g * kinds + k] = s_mask[wave][k];
+    const uint32_t R = ds->r_total, j = lane;
+    bool live = false;
+    if (j < R) live = s_mask[wave][j < ds->r_shadow ? (uint32_t)slot_light(ds, j) : nl] != 0ull;
+    const uint64_t bits = __ballot(live);
+    if (lane == 0) {
+        live_bits[g] = bits;
+        live_count[g] = (uint32_t)__popcll(bits);
+    }
+}
+
+// Exclusive scan of the groups' live-row counts (one workgroup, as
+// k_compact_scan); the total goes to counters[kCntLive].
+__global__ __launch_bounds__(1024) void k_cull_scan(uint32_t* counts, uint32_t n, uint32_t* counters) {
+    __shared__ uint32_t wave_sum[1024 / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t per = (n + 1023u) / 1024u, begin = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (begin + k < n) sum += counts[begin + k];
+    uint32_t incl = sum;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t w = 0; w < 1024 / 64; ++w) {
+        if (w < wave) before += wave_sum[w];
+        total += wave_sum[w];
+    }
+    uint32_t run = before + incl - sum;
+    for (uint32_t k = 0; k < per; ++k) {
+        if (begin + k < n) {
+            const uint32_t v = counts[begin + k];
+            counts[begin + k] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0) counters[kCntLive] = total;
+}
+
+// One wave per group: lane j writes live row j of the group into the row list
+// (at the group's scanned base) or, for a culled row, its hit bytes (zero).
+__global__ __launch_bounds__(kBlock) void k_cull_rows(const uint32_t* counters, const DevScene* ds,
+                                                      uint32_t ngroups, const uint64_t* live_bits,
+                                                      const uint32_t* row_base, uint32_t* rows,
+                                                      uint8_t* hits) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
+    if (g >= ngroups) return;
+    const uint32_t A = counters[kCntActive];
+    if ((uint64_t)g * kCullGroup >= A) return;
+    const uint32_t R = ds->r_total, j = lane;
+    if (j >= R) return;
+    const uint64_t bits = live_bits[g];
+    if ((bits >> j) & 1ull) {
+        const uint64_t below = j ? (bits & (~0ull >> (64 - j))) : 0ull;
+        rows[row_base[g] + (uint32_t)__popcll(below)] = (g << 6) | j;
+    } else {
+        // grouped order: slot j of the group's w pixels is rays g*64*R + j*w ..
+        const uint32_t w = A - g * kCullGroup < kCullGroup ? A - g * kCullGroup : kCullGroup;
+        uint8_t* h = hits + (size_t)g * kCullGroup * R + (size_t)j * w;
+        if (w == kCullGroup) {
+            uint4* h4 = reinterpret_cast<uint4*>(h);
+            for (int k = 0; k < 4; ++k) h4[k] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            for (uint32_t k = 0; k < w; ++k) h[k] = 0;
+        }
+    }
+}
+
 // Rays are claimed in chunks per wave (one device-scope atomic per chunk).
 // The chunk follows the launch's ray count, 1/kChunkShare of an even share per
 // wave, kept within kChunkMin .. kChunkMax rays: the rays in flight on the
@@ -180,7 +406,9 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     const uint32_t total = gridDim.x * kBlock;
     StackT<SHORT, DEPTH> st;
     st.init(lds + tid, kBlock, L.spill + (size_t)blockIdx.x * kBlock + tid, total);
-    const uint32_t nrays = L.counters[kCntRays];
+    // culled gen launches trace the live rows only (64 ray ids per row)
+    const bool cull = SRC == kGen && L.live_rows != nullptr;
+    const uint32_t nrays = cull ? L.counters[kCntLive] * kCullGroup : L.counters[kCntRays];
     const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // active pixels
     const uint32_t R = SRC == kGen ? L.ds->r_total : 1u;             // ray slots per pixel
     const uint32_t G = SRC == kGen ? L.ds->ray_group : 0u;           // pixels per ray group
@@ -248,13 +476,46 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                             }
                         } else {
                             if (SRC == kGen) {
-                                uint32_t slot, a;
-                                ray_slot(rid, A, R, G, slot, a);
-                                const float4 r0 = L.act_rec[2 * (size_t)a], r1 = L.act_rec[2 * (size_t)a + 1];
-                                V3 o, d;
-                                float tmin, tmax;
-                                gen_ray(L.ds, pixel_from_record(r0, r1), slot, o, d, tmin, tmax);
-                                ray_begin(r, st, o, d, tmin, tmax);
+                                uint32_t slot = 0, a = 0;
+                                uint64_t cands = 0;
+                                const uint32_t rs = L.ds->r_shadow;
+                                if (cull) {
+                                    // live row g << 6 | j: slot j of the group's pixel i; its hit
+                                    // byte is the ray's grouped-order index
+                                    const uint32_t row = L.live_rows[rid >> 6], i = rid & 63u;
+                                    const uint32_t g = row >> 6, g0 = g * kCullGroup;
+                                    slot = row & 63u;
+                                    a = g0 + i;
+                                    if (a < A) {
+                                        const uint32_t w = A - g0 < kCullGroup ? A - g0 : kCullGroup;
+                                        ray = g0 * R + slot * w + i;
+                                        const uint32_t kind = slot < rs ? (uint32_t)slot_light(L.ds, slot)
+                                                                        : (uint32_t)L.ds->s.num_lights;
+                                        cands = L.cand[(size_t)g * L.cand_kinds + kind];
+                                    } else {
+                                        ray = kNone;   // past the last pixel of a partial group
+                                    }
+                                } else {
+                                    ray_slot(rid, A, R, G, slot, a);
+                                }
+                                if (ray != kNone) {
+                                    const float4 r0 = L.act_rec[2 * (size_t)a], r1 = L.act_rec[2 * (size_t)a + 1];
+                                    V3 o, d;
+                                    float tmin, tmax;
+                                    gen_ray_of(L.ds, pixel_from_record(r0, r1), slot,
+                                               slot < rs ? slot_light(L.ds, slot) : 0, o, d, tmin, tmax);
+                                    ray_begin(r, st, o, d, tmin, tmax);
+                                    if (cull && __popcll(cands) <= (int)kMaxCand) {
+                                        // enter the candidate slots directly: the first now, the
+                                        // others from the stack (no TLAS node is visited)
+                                        r.next = cands ? (kLeaf | (uint32_t)__builtin_ctzll(cands)) : kNone;
+                                        cands &= cands - 1;
+                                        while (cands) {
+                                            st.push(kLeaf | (uint32_t)__builtin_ctzll(cands));
+                                            cands &= cands - 1;
+                                        }
+                                    }
+                                }
                             } else {
                                 float4 o = L.ray_o[(size_t)rid * L.ray_stride];
                                 float4 d = L.ray_d[(size_t)rid * L.ray_stride];
@@ -385,14 +646,42 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
-    // rays are generated inside the traversal; the compaction scan set their count
-    stage_begin(ctx, kStageRaygen);
-    stage_end(ctx, kStageRaygen);
-    LUZ_HIP(ctx, hipGetLastError());
+    // rays are generated inside the traversal; the compaction scan set their
+    // count (the raygen stage times the culling kernels, when they run)
+
+    // ray-group culling (grouped order, <= 64 TLAS slots, <= 64 rays per pixel)
+    const uint32_t ngroups = (uint32_t)((npix + kCullGroup - 1) / kCullGroup);
+    const bool cull = ctx->group_cull && ds.r_total > 0 && ds.r_total <= kMaxCullSlots &&
+                      tlas->count > 0 && tlas->count <= kMaxCullInst && ds.ray_group == kCullGroup &&
+                      (uint64_t)ngroups * kCullGroup * ds.r_total <= 0xFFFFFFFFull;
+    ctx->last_culled = cull;
+    const uint32_t kinds = (uint32_t)scene->num_lights + 1;
+    if (cull) {
+        if ((s = ensure(ctx, ctx->cull_cand, (size_t)ngroups * kinds * 8)) != LUZRT_OK) return s;
+        if ((s = ensure(ctx, ctx->cull_bits, (size_t)ngroups * 12 + 16)) != LUZRT_OK) return s;
+        if ((s = ensure(ctx, ctx->cull_rows, (size_t)ngroups * ds.r_total * 4)) != LUZRT_OK) return s;
+        uint64_t* bits = (uint64_t*)ctx->cull_bits.ptr;
+        uint32_t* counts = (uint32_t*)(bits + ngroups);
+        const uint32_t gblocks = cdiv(ngroups, kBlock / 64);
+        stage_begin(ctx, kStageRaygen);
+        k_cull<<<gblocks, kBlock, 0, st>>>((const float4*)ctx->act_rec.ptr, counters, dsd,
+                                           tlas_slot_boxes(tlas), tlas->count, ngroups, kinds,
+                                           (uint64_t*)ctx->cull_cand.ptr, bits, counts);
+        k_cull_scan<<<1, 1024, 0, st>>>(counts, ngroups, counters);
+        k_cull_rows<<<gblocks, kBlock, 0, st>>>(counters, dsd, ngroups, bits, counts,
+                                                (uint32_t*)ctx->cull_rows.ptr, (uint8_t*)ctx->hits.ptr);
+        stage_end(ctx, kStageRaygen);
+        LUZ_HIP(ctx, hipGetLastError());
+    }
 
     stage_begin(ctx, kStageTrace);
     if (ds.r_total > 0 && tlas->count > 0) {
         TraceLaunch L{};
+        if (cull) {
+            L.live_rows = (const uint32_t*)ctx->cull_rows.ptr;
+            L.cand = (const uint64_t*)ctx->cull_cand.ptr;
+            L.cand_kinds = kinds;
+        }
         L.tlas_nodes = tlas->nodes;
         L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
         L.tlas_nodes_count = tlas->node_count;
@@ -598,9 +887,39 @@ extern "C" luzrt_status luzrt_trace_visibility_static(luzrt_ctx* ctx, const luzr
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
+    // ray-group culling (grouped order, <= 64 TLAS slots, <= 64 rays per pixel)
+    const uint32_t ngroups = (uint32_t)((npix + kCullGroup - 1) / kCullGroup);
+    const bool cull = ctx->group_cull && ds.r_total > 0 && ds.r_total <= kMaxCullSlots &&
+                      tlas->count > 0 && tlas->count <= kMaxCullInst && ds.ray_group == kCullGroup &&
+                      (uint64_t)ngroups * kCullGroup * ds.r_total <= 0xFFFFFFFFull;
+    ctx->last_culled = cull;
+    const uint32_t kinds = (uint32_t)scene->num_lights + 1;
+    if (cull) {
+        if ((s = ensure(ctx, ctx->cull_cand, (size_t)ngroups * kinds * 8)) != LUZRT_OK) return s;
+        if ((s = ensure(ctx, ctx->cull_bits, (size_t)ngroups * 12 + 16)) != LUZRT_OK) return s;
+        if ((s = ensure(ctx, ctx->cull_rows, (size_t)ngroups * ds.r_total * 4)) != LUZRT_OK) return s;
+        uint64_t* bits = (uint64_t*)ctx->cull_bits.ptr;
+        uint32_t* counts = (uint32_t*)(bits + ngroups);
+        const uint32_t gblocks = cdiv(ngroups, kBlock / 64);
+        stage_begin(ctx, kStageRaygen);
+        k_cull<<<gblocks, kBlock, 0, st>>>((const float4*)ctx->act_rec.ptr, counters, dsd,
+                                           tlas_slot_boxes(tlas), tlas->count, ngroups, kinds,
+                                           (uint64_t*)ctx->cull_cand.ptr, bits, counts);
+        k_cull_scan<<<1, 1024, 0, st>>>(counts, ngroups, counters);
+        k_cull_rows<<<gblocks, kBlock, 0, st>>>(counters, dsd, ngroups, bits, counts,
+                                                (uint32_t*)ctx->cull_rows.ptr, (uint8_t*)ctx->hits.ptr);
+        stage_end(ctx, kStageRaygen);
+        LUZ_HIP(ctx, hipGetLastError());
+    }
+
     stage_begin(ctx, kStageTrace);
     if (ds.r_total > 0 && tlas->count > 0) {
         TraceLaunch L{};
+        if (cull) {
+            L.live_rows = (const uint32_t*)ctx->cull_rows.ptr;
+            L.cand = (const uint64_t*)ctx->cull_cand.ptr;
+            L.cand_kinds = kinds;
+        }
         L.tlas_nodes = tlas->nodes;
         L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
         L.tlas_nodes_count = tlas->node_count;

@@ -177,6 +177,46 @@ def test_visibility_same_in_every_ray_order(ctx, order):
     g.close()
 
 
+@pytest.mark.parametrize("ao_max,radius", [(1.0, None), (16.0, None), (16.0, 3.0)])
+def test_group_culling_changes_no_count(ctx, ao_max, radius):
+    """Ray-group culling (on by default) against the oracle and against the
+    same call without it, on the sphere field with a partial last group of
+    pixels: AO radius 1 (beyond the reach of every group's AO rays) and 16
+    (occluders within reach), and point lights of radius 3 (wide shadow
+    shafts).  The culled call traces fewer rays than it casts."""
+    s, g = _gpu_scene(ctx, api.SCENE_SPHERES)
+    o = oracle_scene(s)
+    W, H, gd, got, _ = _gbuffers(ctx, s, g, o, 331, 187)
+    sb = s.pack()
+    sb.ao_max = ao_max
+    if radius is not None:
+        for l in range(sb.num_lights):
+            if sb.lights[l].type == api.LIGHT_POINT:
+                sb.lights[l].radius = radius
+    nl = sb.num_lights
+    out = {}
+    for cull in (True, False):
+        dsh, dao = ctx.buffer(nl * W * H), ctx.buffer(W * H)
+        ctx.set_group_culling(cull)
+        try:
+            print(f"ao_max {ao_max} radius {radius} culling {cull}", flush=True)
+            ctx.trace_visibility(g.tlas, gd.desc(), sb, dsh, dao)
+        finally:
+            ctx.set_group_culling(True)
+        t = ctx.timings()
+        out[cull] = (dsh.download(np.uint8, (nl, W * H)), dao.download(np.uint8, W * H), t.traced_rays,
+                     t.rays)
+    wsh, wao, _ = o.trace_visibility(got, W, H, sb)
+    print(f"rays {out[True][3]}, traced with culling {out[True][2]}, shadowed {int(wsh.sum())}, "
+          f"AO hits {int((sb.ao_num_samples - wao.astype(int)).sum())}", flush=True)
+    for cull in (True, False):
+        np.testing.assert_array_equal(out[cull][0], wsh, err_msg=f"shadow, culling {cull}")
+        np.testing.assert_array_equal(out[cull][1], wao, err_msg=f"AO, culling {cull}")
+    assert out[False][2] == out[False][3]
+    assert out[True][2] < out[True][3]
+    g.close()
+
+
 @pytest.mark.parametrize("kind,res", [(api.SCENE_CUBE, None), (api.SCENE_SPHERES, (480, 270))])
 def test_light_pass_radiance(ctx, kind, res):
     s, g = _gpu_scene(ctx, kind)
