@@ -183,7 +183,8 @@ def test_group_culling_changes_no_count(ctx, ao_max, radius):
     same call without it, on the sphere field with a partial last group of
     pixels: AO radius 1 (beyond the reach of every group's AO rays) and 16
     (occluders within reach), and point lights of radius 3 (wide shadow
-    shafts).  The culled call traces fewer rays than it casts."""
+    shafts).  At AO radius 1 the culled call traces fewer rays than it casts
+    (at 16 every group of this small image can reach an instance)."""
     s, g = _gpu_scene(ctx, api.SCENE_SPHERES)
     o = oracle_scene(s)
     W, H, gd, got, _ = _gbuffers(ctx, s, g, o, 331, 187)
@@ -213,7 +214,8 @@ def test_group_culling_changes_no_count(ctx, ao_max, radius):
         np.testing.assert_array_equal(out[cull][0], wsh, err_msg=f"shadow, culling {cull}")
         np.testing.assert_array_equal(out[cull][1], wao, err_msg=f"AO, culling {cull}")
     assert out[False][2] == out[False][3]
-    assert out[True][2] < out[True][3]
+    if ao_max == 1.0:
+        assert out[True][2] < out[True][3]
     g.close()
 
 
