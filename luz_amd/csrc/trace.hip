@@ -857,6 +857,7 @@ extern "C" luzrt_status luzrt_trace_visibility_static(luzrt_ctx* ctx, const luzr
                     ds.r_total, kStaticRays);
     if ((size_t)(nblocks + 2) * 4 > kStaticBlockBytes)
         return fail(ctx, LUZRT_E_LIMIT, "luzrt_trace_visibility_static: %u compaction blocks", nblocks);
+    ctx->last_culled = false;   // this path traces every ray
     uint8_t* base = (uint8_t*)ctx->static_scratch;
     uint32_t* counters = (uint32_t*)(base + kOffCounters);
     DevScene* dsd = (DevScene*)(base + kOffScene);
@@ -887,39 +888,9 @@ extern "C" luzrt_status luzrt_trace_visibility_static(luzrt_ctx* ctx, const luzr
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
-    // ray-group culling (grouped order, <= 64 TLAS slots, <= 64 rays per pixel)
-    const uint32_t ngroups = (uint32_t)((npix + kCullGroup - 1) / kCullGroup);
-    const bool cull = ctx->group_cull && ds.r_total > 0 && ds.r_total <= kMaxCullSlots &&
-                      tlas->count > 0 && tlas->count <= kMaxCullInst && ds.ray_group == kCullGroup &&
-                      (uint64_t)ngroups * kCullGroup * ds.r_total <= 0xFFFFFFFFull;
-    ctx->last_culled = cull;
-    const uint32_t kinds = (uint32_t)scene->num_lights + 1;
-    if (cull) {
-        if ((s = ensure(ctx, ctx->cull_cand, (size_t)ngroups * kinds * 8)) != LUZRT_OK) return s;
-        if ((s = ensure(ctx, ctx->cull_bits, (size_t)ngroups * 12 + 16)) != LUZRT_OK) return s;
-        if ((s = ensure(ctx, ctx->cull_rows, (size_t)ngroups * ds.r_total * 4)) != LUZRT_OK) return s;
-        uint64_t* bits = (uint64_t*)ctx->cull_bits.ptr;
-        uint32_t* counts = (uint32_t*)(bits + ngroups);
-        const uint32_t gblocks = cdiv(ngroups, kBlock / 64);
-        stage_begin(ctx, kStageRaygen);
-        k_cull<<<gblocks, kBlock, 0, st>>>((const float4*)ctx->act_rec.ptr, counters, dsd,
-                                           tlas_slot_boxes(tlas), tlas->count, ngroups, kinds,
-                                           (uint64_t*)ctx->cull_cand.ptr, bits, counts);
-        k_cull_scan<<<1, 1024, 0, st>>>(counts, ngroups, counters);
-        k_cull_rows<<<gblocks, kBlock, 0, st>>>(counters, dsd, ngroups, bits, counts,
-                                                (uint32_t*)ctx->cull_rows.ptr, (uint8_t*)ctx->hits.ptr);
-        stage_end(ctx, kStageRaygen);
-        LUZ_HIP(ctx, hipGetLastError());
-    }
-
     stage_begin(ctx, kStageTrace);
     if (ds.r_total > 0 && tlas->count > 0) {
         TraceLaunch L{};
-        if (cull) {
-            L.live_rows = (const uint32_t*)ctx->cull_rows.ptr;
-            L.cand = (const uint64_t*)ctx->cull_cand.ptr;
-            L.cand_kinds = kinds;
-        }
         L.tlas_nodes = tlas->nodes;
         L.recs = tlas->wide ? tlas->recs_w : tlas->recs;
         L.tlas_nodes_count = tlas->node_count;
