@@ -118,6 +118,7 @@ def main() -> int:
         vis_ms.append(t.compact_ms + t.raygen_ms + t.trace_ms + t.resolve_ms)
     rays = int(t.rays)
     active = int(t.active_pixels)
+    traced = int(t.traced_rays)
     ms_frame = elapsed / args.steps * 1e3
     value = rays * args.steps / elapsed / 1e6
     trace_avg = float(np.mean(trace_ms))
@@ -143,7 +144,7 @@ def main() -> int:
                          "64x64 ground grid, 1,032,192 instanced tris, 1 dir + 8 point lights, "
                          "1 shadow ray/light + 16 AO rays/px"),
             "width": W, "height": H, "instanced_tris": int(scene.instanced_triangles),
-            "rays_per_frame": rays, "active_pixels": active,
+            "rays_per_frame": rays, "active_pixels": active, "traced_rays_per_frame": traced,
             "frame_ms_breakdown": {"tlas_build": round(float(np.mean(build_ms)), 4),
                                    "visibility": round(float(np.mean(vis_ms)), 4),
                                    "trace_kernel": round(trace_avg, 4),
@@ -333,12 +334,13 @@ def ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb, out, W, H, threads, args,
         t = ctx.timings()
         trace_ms.append(t.trace_ms)
     rays = int(t.rays)
+    traced = int(t.traced_rays)
     par, st = visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, args.cpu_row_step * 4)
     nr = max(st.rays, 1)
     return {
         "ao_max": float(sb.ao_max), "steps": steps, "ms_per_frame": round(elapsed / steps * 1e3, 4),
         "mrays_per_s": round(rays * steps / elapsed / 1e6, 2), "trace_kernel_ms": round(float(np.median(trace_ms)), 4),
-        "rays_per_frame": rays,
+        "rays_per_frame": rays, "traced_rays_per_frame": traced,
         "per_ray_counts": {"node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
                            "inst": round(st.inst_visits / nr, 3)},
         "parity": par,

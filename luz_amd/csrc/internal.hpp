@@ -266,6 +266,7 @@ struct TraceLaunch {
     // cand[g * cand_kinds + k] instead of the TLAS root.  Null: no culling.
     const uint32_t* live_rows;
     const uint64_t* cand;
+    const float* slot_box;   // the TLAS slots' padded world boxes (6 floats each)
     uint32_t cand_kinds;
 };
 

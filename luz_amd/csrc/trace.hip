@@ -506,13 +506,20 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                                                slot < rs ? slot_light(L.ds, slot) : 0, o, d, tmin, tmax);
                                     ray_begin(r, st, o, d, tmin, tmax);
                                     if (cull && __popcll(cands) <= (int)kMaxCand) {
-                                        // enter the candidate slots directly: the first now, the
-                                        // others from the stack (no TLAS node is visited)
-                                        r.next = cands ? (kLeaf | (uint32_t)__builtin_ctzll(cands)) : kNone;
-                                        cands &= cands - 1;
+                                        // the candidate slots whose boxes the ray meets (the
+                                        // test the TLAS node above each slot would make), the
+                                        // first entered now, the others from the stack: no
+                                        // TLAS node is visited
+                                        r.next = kNone;
                                         while (cands) {
-                                            st.push(kLeaf | (uint32_t)__builtin_ctzll(cands));
+                                            const uint32_t sl = (uint32_t)__builtin_ctzll(cands);
                                             cands &= cands - 1;
+                                            const float* b = L.slot_box + sl * 6;
+                                            float tn;
+                                            if (box_hit(r.wbr, b[0], b[3], b[1], b[4], b[2], b[5], tn)) {
+                                                if (r.next == kNone) r.next = kLeaf | sl;
+                                                else st.push(kLeaf | sl);
+                                            }
                                         }
                                     }
                                 }
@@ -680,6 +687,7 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
         if (cull) {
             L.live_rows = (const uint32_t*)ctx->cull_rows.ptr;
             L.cand = (const uint64_t*)ctx->cull_cand.ptr;
+            L.slot_box = tlas_slot_boxes(tlas);
             L.cand_kinds = kinds;
         }
         L.tlas_nodes = tlas->nodes;
