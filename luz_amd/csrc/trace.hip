@@ -48,18 +48,28 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(const luzrt_gbuffer gb
     if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block counts (one workgroup: each thread scans a run
-// of consecutive counts, the runs' sums are scanned by wave shuffles and one
-// exchange of wave totals).  Writes the active-pixel and ray counts and zeroes
-// the claim and error counters of the traversal that follows.
-__global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, uint32_t nblocks,
-                                                       uint32_t* counters, const DevScene* ds) {
+// Exclusive scan in place of n counts (16-byte aligned) by one workgroup of
+// 1024 threads: thread t sums its run of `per` consecutive counts (a multiple
+// of 4, read as 16-byte vectors so that a run's loads are in flight
+// together), the runs' sums are scanned by wave shuffles and one exchange of
+// wave totals, and each run is rewritten with its prefixes.  Returns the total
+// (to every thread).
+__device__ inline uint32_t scan_runs_1024(uint32_t* counts, uint32_t n) {
     __shared__ uint32_t wave_sum[1024 / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t per = (nblocks + 1023u) / 1024u, begin = threadIdx.x * per;
+    const uint32_t per = (((n + 1023u) / 1024u) + 3u) & ~3u, begin = threadIdx.x * per;
+    uint4* c4 = reinterpret_cast<uint4*>(counts);
     uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (begin + k < nblocks) sum += block_counts[begin + k];
+#pragma unroll 8
+    for (uint32_t k = 0; k < per; k += 4) {
+        const uint32_t e = begin + k;
+        if (e + 3 < n) {
+            const uint4 v = c4[e >> 2];
+            sum += v.x + v.y + v.z + v.w;
+        } else {
+            for (uint32_t q = e; q < n && q < e + 4; ++q) sum += counts[q];
+        }
+    }
     uint32_t incl = sum;
     for (uint32_t off = 1; off < 64; off <<= 1) {
         const uint32_t y = __shfl_up(incl, off);
@@ -73,13 +83,33 @@ __global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, u
         total += wave_sum[w];
     }
     uint32_t run = before + incl - sum;
-    for (uint32_t k = 0; k < per; ++k) {
-        if (begin + k < nblocks) {
-            const uint32_t v = block_counts[begin + k];
-            block_counts[begin + k] = run;
-            run += v;
+#pragma unroll 8
+    for (uint32_t k = 0; k < per; k += 4) {
+        const uint32_t e = begin + k;
+        if (e + 3 < n) {
+            const uint4 v = c4[e >> 2];
+            uint4 o;
+            o.x = run; run += v.x;
+            o.y = run; run += v.y;
+            o.z = run; run += v.z;
+            o.w = run; run += v.w;
+            c4[e >> 2] = o;
+        } else {
+            for (uint32_t q = e; q < n && q < e + 4; ++q) {
+                const uint32_t v = counts[q];
+                counts[q] = run;
+                run += v;
+            }
         }
     }
+    return total;
+}
+
+// Exclusive scan of the block counts.  Writes the active-pixel and ray counts
+// and zeroes the claim and error counters of the traversal that follows.
+__global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* block_counts, uint32_t nblocks,
+                                                       uint32_t* counters, const DevScene* ds) {
+    const uint32_t total = scan_runs_1024(block_counts, nblocks);
     if (threadIdx.x == 0) {
         counters[kCntActive] = total;
         counters[kCntRays] = total * ds->r_total;
@@ -139,32 +169,53 @@ constexpr uint32_t kMaxCullInst = 64;   // TLAS slots a candidate mask covers
 constexpr uint32_t kMaxCullSlots = 64;  // ray slots per pixel a live-row mask covers
 constexpr uint32_t kMaxCand = 8;        // candidates entered directly; more start at the TLAS root
 
+// Wave-wide min / max (the result in every lane): DPP exchanges within each
+// row of 16 lanes (quad xor 1, quad xor 2, half-row mirror, row mirror), then
+// the four row values read by readlane.
+template <int CTRL>
+__device__ inline float dpp_of(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF,
+                                                      false));
+}
+__device__ inline float row_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 __device__ inline float wave_min(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
+    v = fminf(v, dpp_of<0xB1>(v));
+    v = fminf(v, dpp_of<0x4E>(v));
+    v = fminf(v, dpp_of<0x141>(v));
+    v = fminf(v, dpp_of<0x140>(v));
+    return fminf(fminf(row_value(v, 0), row_value(v, 16)), fminf(row_value(v, 32), row_value(v, 48)));
 }
 __device__ inline float wave_max(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
+    v = fmaxf(v, dpp_of<0xB1>(v));
+    v = fmaxf(v, dpp_of<0x4E>(v));
+    v = fmaxf(v, dpp_of<0x141>(v));
+    v = fmaxf(v, dpp_of<0x140>(v));
+    return fmaxf(fmaxf(row_value(v, 0), row_value(v, 16)), fmaxf(row_value(v, 32), row_value(v, 48)));
 }
 __device__ inline bool finite3(V3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
 
 // Does the segment p0 + s (p1 - p0), s in [0, 1], meet the box [lo - w, hi + w]?
-__device__ inline bool segment_meets_box(const float p0[3], const float p1[3], const float lo[3],
-                                         const float hi[3], float w) {
+// inv[k] = 1 / (p1[k] - p0[k]) by the hardware reciprocal (its one-ulp error
+// moves the crossing points by far less than the widening's margin); a zero
+// component tests p0 against the slab instead.
+__device__ inline bool segment_meets_box(const float p0[3], const float p1[3], const float inv[3],
+                                         const float lo[3], const float hi[3], float w) {
     float s0 = 0.0f, s1 = 1.0f;
+    bool in = true;
     for (int k = 0; k < 3; ++k) {
-        const float a = lo[k] - w, b = hi[k] + w, d = p1[k] - p0[k];
-        if (d == 0.0f) {
-            if (p0[k] < a || p0[k] > b) return false;
+        const float a = lo[k] - w, b = hi[k] + w;
+        if (p1[k] == p0[k]) {
+            in = in && p0[k] >= a && p0[k] <= b;
         } else {
-            float t0 = (a - p0[k]) / d, t1 = (b - p0[k]) / d;
+            float t0 = (a - p0[k]) * inv[k], t1 = (b - p0[k]) * inv[k];
             if (t0 > t1) { const float t = t0; t0 = t1; t1 = t; }
             s0 = fmax_(s0, t0);
             s1 = fmin_(s1, t1);
         }
     }
-    return s0 <= s1;
+    return in && s0 <= s1;
 }
 
 // One wave per group (four per workgroup).
@@ -248,7 +299,9 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
             }
             rC += 2.0f * fabsf(Lt.radius);
             const float w = fmax_(rA, rC) + 1e-3f + 2e-5f * (mag + cmag + rA + rC);
-            const bool meet = own && segment_meets_box(a0, c0, blo, bhi, w);
+            const float inv[3] = {__builtin_amdgcn_rcpf(c0[0] - a0[0]), __builtin_amdgcn_rcpf(c0[1] - a0[1]),
+                                  __builtin_amdgcn_rcpf(c0[2] - a0[2])};
+            const bool meet = own && segment_meets_box(a0, c0, inv, blo, bhi, w);
             m = __ballot(meet);
             if (any_bad || lbad || !isfinite(w)) m = all;
         }
@@ -276,35 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
     }
 }
 
-// Exclusive scan of the groups' live-row counts (one workgroup, as
-// k_compact_scan); the total goes to counters[kCntLive].
+// Exclusive scan of the groups' live-row counts; the total goes to
+// counters[kCntLive].
 __global__ __launch_bounds__(1024) void k_cull_scan(uint32_t* counts, uint32_t n, uint32_t* counters) {
-    __shared__ uint32_t wave_sum[1024 / 64];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t per = (n + 1023u) / 1024u, begin = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (begin + k < n) sum += counts[begin + k];
-    uint32_t incl = sum;
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off);
-        if (lane >= off) incl += y;
-    }
-    if (lane == 63) wave_sum[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (uint32_t w = 0; w < 1024 / 64; ++w) {
-        if (w < wave) before += wave_sum[w];
-        total += wave_sum[w];
-    }
-    uint32_t run = before + incl - sum;
-    for (uint32_t k = 0; k < per; ++k) {
-        if (begin + k < n) {
-            const uint32_t v = counts[begin + k];
-            counts[begin + k] = run;
-            run += v;
-        }
-    }
+    const uint32_t total = scan_runs_1024(counts, n);
     if (threadIdx.x == 0) counters[kCntLive] = total;
 }
 
@@ -634,7 +662,7 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     // act_index holds the per-pixel active index followed by the per-block counts
     const uint32_t tw = ctx->tile_width, th = 64u / tw;
     const size_t ntiles_px_sz = (size_t)((W + tw - 1) / tw) * ((H + th - 1) / th) * 64;
-    if ((s = ensure(ctx, ctx->act_index, npix * 4 + (ntiles_px_sz / kBlock + 2) * 4)) != LUZRT_OK)
+    if ((s = ensure(ctx, ctx->act_index, npix * 4 + 16 + (ntiles_px_sz / kBlock + 2) * 4)) != LUZRT_OK)
         return s;
     if ((s = ensure(ctx, ctx->hits, max_rays + 16)) != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
@@ -644,7 +672,7 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     stage_begin(ctx, kStageCompact);
     uint32_t ntiles_px = (uint32_t)ntiles_px_sz;
     uint32_t nblocks = cdiv(ntiles_px, kBlock);
-    uint32_t* block_counts = (uint32_t*)ctx->act_index.ptr + npix;
+    uint32_t* block_counts = (uint32_t*)ctx->act_index.ptr + ((npix + 3) & ~size_t(3));   // 16-byte aligned
     k_compact_count<<<nblocks, kBlock, 0, st>>>(gl, tw, ntiles_px, block_counts);
     k_compact_scan<<<1, 1024, 0, st>>>(block_counts, nblocks, counters, dsd);
     k_compact_write<<<nblocks, kBlock, 0, st>>>(gl, tw, ntiles_px, dsd,
@@ -665,10 +693,11 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     const uint32_t kinds = (uint32_t)scene->num_lights + 1;
     if (cull) {
         if ((s = ensure(ctx, ctx->cull_cand, (size_t)ngroups * kinds * 8)) != LUZRT_OK) return s;
-        if ((s = ensure(ctx, ctx->cull_bits, (size_t)ngroups * 12 + 16)) != LUZRT_OK) return s;
+        const size_t counts_off = ((size_t)ngroups * 8 + 255) & ~size_t(255);   // 16-byte aligned counts
+        if ((s = ensure(ctx, ctx->cull_bits, counts_off + (size_t)ngroups * 4 + 16)) != LUZRT_OK) return s;
         if ((s = ensure(ctx, ctx->cull_rows, (size_t)ngroups * ds.r_total * 4)) != LUZRT_OK) return s;
         uint64_t* bits = (uint64_t*)ctx->cull_bits.ptr;
-        uint32_t* counts = (uint32_t*)(bits + ngroups);
+        uint32_t* counts = (uint32_t*)((uint8_t*)ctx->cull_bits.ptr + counts_off);
         const uint32_t gblocks = cdiv(ngroups, kBlock / 64);
         stage_begin(ctx, kStageRaygen);
         k_cull<<<gblocks, kBlock, 0, st>>>((const float4*)ctx->act_rec.ptr, counters, dsd,
