@@ -58,18 +58,17 @@ __device__ inline uint32_t scan_runs_1024(uint32_t* counts, uint32_t n) {
     __shared__ uint32_t wave_sum[1024 / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t per = (((n + 1023u) / 1024u) + 3u) & ~3u, begin = threadIdx.x * per;
-    uint4* c4 = reinterpret_cast<uint4*>(counts);
+    // the run [begin, end): nv whole vectors, then single counts
+    const uint32_t end = begin + per < n ? begin + per : (begin < n ? n : begin);
+    const uint32_t nv = (end - begin) >> 2, tail = begin + nv * 4;
+    uint4* c4 = reinterpret_cast<uint4*>(counts) + (begin >> 2);
     uint32_t sum = 0;
 #pragma unroll 8
-    for (uint32_t k = 0; k < per; k += 4) {
-        const uint32_t e = begin + k;
-        if (e + 3 < n) {
-            const uint4 v = c4[e >> 2];
-            sum += v.x + v.y + v.z + v.w;
-        } else {
-            for (uint32_t q = e; q < n && q < e + 4; ++q) sum += counts[q];
-        }
+    for (uint32_t k = 0; k < nv; ++k) {
+        const uint4 v = c4[k];
+        sum += v.x + v.y + v.z + v.w;
     }
+    for (uint32_t q = tail; q < end; ++q) sum += counts[q];
     uint32_t incl = sum;
     for (uint32_t off = 1; off < 64; off <<= 1) {
         const uint32_t y = __shfl_up(incl, off);
@@ -84,23 +83,19 @@ __device__ inline uint32_t scan_runs_1024(uint32_t* counts, uint32_t n) {
     }
     uint32_t run = before + incl - sum;
 #pragma unroll 8
-    for (uint32_t k = 0; k < per; k += 4) {
-        const uint32_t e = begin + k;
-        if (e + 3 < n) {
-            const uint4 v = c4[e >> 2];
-            uint4 o;
-            o.x = run; run += v.x;
-            o.y = run; run += v.y;
-            o.z = run; run += v.z;
-            o.w = run; run += v.w;
-            c4[e >> 2] = o;
-        } else {
-            for (uint32_t q = e; q < n && q < e + 4; ++q) {
-                const uint32_t v = counts[q];
-                counts[q] = run;
-                run += v;
-            }
-        }
+    for (uint32_t k = 0; k < nv; ++k) {
+        const uint4 v = c4[k];
+        uint4 o;
+        o.x = run; run += v.x;
+        o.y = run; run += v.y;
+        o.z = run; run += v.z;
+        o.w = run; run += v.w;
+        c4[k] = o;
+    }
+    for (uint32_t q = tail; q < end; ++q) {
+        const uint32_t v = counts[q];
+        counts[q] = run;
+        run += v;
     }
     return total;
 }
@@ -242,13 +237,24 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
     const uint32_t ar = a < A ? a : g * kCullGroup;
     const PixelFrame f = pixel_from_record(act_rec[2 * (size_t)ar], act_rec[2 * (size_t)ar + 1]);
     const V3 o = shadow_origin(f);
-    const float nlen = vlen(f.N);
-    const float lift = fmax_(f.lc * 0.01f, 0.05f) * nlen;
-    const bool bad = !(finite3(o) && finite3(f.P) && isfinite(lift));
+    const float nlen = __builtin_amdgcn_sqrtf(vdot(f.N, f.N));
+    // the shadow origin's offset from the surface, N x sb: a ray towards a
+    // point or spot light ends near the light position plus this offset
+    const V3 lv = vmul(f.N, fmax_(f.lc * 0.01f, 0.05f));
+    const bool bad = !(finite3(o) && finite3(f.P) && finite3(lv) && isfinite(nlen));
     const bool any_bad = __ballot(bad) != 0ull;
     float alo[3] = {wave_min(o.x), wave_min(o.y), wave_min(o.z)};
     float ahi[3] = {wave_max(o.x), wave_max(o.y), wave_max(o.z)};
-    const float liftmax = wave_max(lift), nmax = wave_max(nlen);
+    const float llo[3] = {wave_min(lv.x), wave_min(lv.y), wave_min(lv.z)};
+    const float lhi[3] = {wave_max(lv.x), wave_max(lv.y), wave_max(lv.z)};
+    const float nmax = wave_max(nlen);
+    float l0[3], rl2 = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        l0[k] = 0.5f * (llo[k] + lhi[k]);
+        const float h = 0.5f * (lhi[k] - llo[k]);
+        rl2 += h * h;
+    }
+    const float rL = __builtin_amdgcn_sqrtf(rl2) * 1.0001f;
     float a0[3], ra2 = 0.0f, mag = 0.0f;
     for (int k = 0; k < 3; ++k) {
         a0[k] = 0.5f * (alo[k] + ahi[k]);
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
         ra2 += h * h;
         mag = fmax_(mag, fmax_(fabsf(alo[k]), fabsf(ahi[k])));
     }
-    const float rA = sqrtf(ra2) * 1.0001f;
+    const float rA = __builtin_amdgcn_sqrtf(ra2) * 1.0001f;
     // this lane's TLAS slot box
     const bool own = lane < ninst;
     float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
@@ -293,9 +299,11 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
                 }
                 rC = sqrtf(rc2) * 1.0001f;
             } else {
-                c0[0] = lpos.x; c0[1] = lpos.y; c0[2] = lpos.z;
-                rC = liftmax * 1.0001f;
-                cmag = fmax_(fabsf(lpos.x), fmax_(fabsf(lpos.y), fabsf(lpos.z)));
+                // end points o + Lvec = P + N sb + (lpos - P): the light position
+                // plus the group's offsets
+                c0[0] = lpos.x + l0[0]; c0[1] = lpos.y + l0[1]; c0[2] = lpos.z + l0[2];
+                rC = rL;
+                cmag = fmax_(fabsf(c0[0]), fmax_(fabsf(c0[1]), fabsf(c0[2]))) + rL;
             }
             rC += 2.0f * fabsf(Lt.radius);
             const float w = fmax_(rA, rC) + 1e-3f + 2e-5f * (mag + cmag + rA + rC);
