@@ -162,7 +162,6 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const luzrt_gbuffer gb
 constexpr uint32_t kCullGroup = 64;     // pixels per culling group (= kRayGroup)
 constexpr uint32_t kMaxCullInst = 64;   // TLAS slots a candidate mask covers
 constexpr uint32_t kMaxCullSlots = 64;  // ray slots per pixel a live-row mask covers
-constexpr uint32_t kMaxCand = 8;        // candidates entered directly; more start at the TLAS root
 
 // Wave-wide min / max (the result in every lane): DPP exchanges within each
 // row of 16 lanes (quad xor 1, quad xor 2, half-row mirror, row mirror), then
@@ -410,15 +409,19 @@ __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32
 // registers).
 enum RaySource { kRays = 0, kBatch = 1, kGen = 2 };
 
-template <bool WIDE, int SHORT, int SRC, int DEPTH = kSpill>
-__global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceLaunch L, uint32_t refill) {
+// CULLED (gen launches after k_cull): the live rows only, each ray starting
+// from its group's candidate slots; no TLAS node is visited, so only the
+// instance records are staged in LDS.
+template <bool WIDE, int SHORT, int SRC, int DEPTH = kSpill, bool CULLED = false>
+__global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceLaunch L,
+                                                                                      uint32_t refill) {
     constexpr bool BATCH = SRC == kBatch;
     __shared__ uint32_t lds[SHORT * kBlock];
     __shared__ uint32_t sbase[BATCH ? kMaxBatch + 1 : 1];
     // A small TLAS (nodes + instance records, 64 B each) is staged in LDS: every
     // ray starts there, so its reads stop competing with BLAS data in L1.  15 KB
     // keeps five 256-lane workgroups (16-entry stacks) resident per CU.
-    __shared__ float4 stlas[BATCH ? 4 : kTlasLdsEntries * 4];
+    __shared__ float4 stlas[BATCH ? 4 : (CULLED ? kMaxCullInst * 4 : kTlasLdsEntries * 4)];
     const uint32_t tid = threadIdx.x;
     if (BATCH) {
         for (uint32_t i = tid; i <= L.nscenes; i += kBlock) sbase[i] = L.scene_ray_base[i];
@@ -427,10 +430,11 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     const Node* tnodes = L.tlas_nodes;
     const InstRec* trecs = L.recs;
     bool tlas_lds = false;
-    if (kStageTlas && !BATCH && L.tlas_nodes_count + L.recs_count <= kTlasLdsEntries) {
+    if (kStageTlas && !BATCH &&
+        (CULLED ? L.recs_count <= kMaxCullInst : L.tlas_nodes_count + L.recs_count <= kTlasLdsEntries)) {
         const float4* gn = reinterpret_cast<const float4*>(L.tlas_nodes);
         const float4* gr = reinterpret_cast<const float4*>(L.recs);
-        const uint32_t nn = L.tlas_nodes_count * 4, nr = L.recs_count * 4;
+        const uint32_t nn = CULLED ? 0u : L.tlas_nodes_count * 4, nr = L.recs_count * 4;
         for (uint32_t i = tid; i < nn; i += kBlock) stlas[i] = gn[i];
         for (uint32_t i = tid; i < nr; i += kBlock) stlas[nn + i] = gr[i];
         __syncthreads();
@@ -443,7 +447,7 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     StackT<SHORT, DEPTH> st;
     st.init(lds + tid, kBlock, L.spill + (size_t)blockIdx.x * kBlock + tid, total);
     // culled gen launches trace the live rows only (64 ray ids per row)
-    const bool cull = SRC == kGen && L.live_rows != nullptr;
+    constexpr bool cull = CULLED && SRC == kGen;
     const uint32_t nrays = cull ? L.counters[kCntLive] * kCullGroup : L.counters[kCntRays];
     const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // active pixels
     const uint32_t R = SRC == kGen ? L.ds->r_total : 1u;             // ray slots per pixel
@@ -541,7 +545,7 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                                     gen_ray_of(L.ds, pixel_from_record(r0, r1), slot,
                                                slot < rs ? slot_light(L.ds, slot) : 0, o, d, tmin, tmax);
                                     ray_begin(r, st, o, d, tmin, tmax);
-                                    if (cull && __popcll(cands) <= (int)kMaxCand) {
+                                    if (cull) {
                                         // the candidate slots whose boxes the ray meets (the
                                         // test the TLAS node above each slot would make), the
                                         // first entered now, the others from the stack: no
@@ -576,8 +580,8 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
         }
         if (ray != kNone) {
             for (;;) {
-                int res = ray_step<false, WIDE>(r, st, tnodes, trecs, nullptr, nullptr, tlas_lds,
-                                                L.max_steps);
+                int res = ray_step<false, WIDE, StackT<SHORT, DEPTH>, CULLED>(r, st, tnodes, trecs, nullptr,
+                                                                          nullptr, tlas_lds, L.max_steps);
                 // a ray with nothing left to visit ends in this step, not in one more
                 if (res == kStepContinue && r.next == kNone && st.sp == 0) res = kStepDone;
                 if (res != kStepContinue) {
@@ -628,6 +632,16 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
         if (sd <= 8) launch(k_trace_any<false, 8, kBatch>);
         else if (sd <= 16) launch(k_trace_any<false, 16, kBatch>);
         else launch(k_trace_any<false, 32, kBatch>);
+    } else if (L.gen && L.live_rows) {
+        if (L.wide) {
+            if (sd <= 8) launch(k_trace_any<true, 8, kGen, kSpill, true>);
+            else if (sd <= 16) launch(k_trace_any<true, 16, kGen, kSpill, true>);
+            else launch(k_trace_any<true, 32, kGen, kSpill, true>);
+        } else {
+            if (sd <= 8) launch(k_trace_any<false, 8, kGen, kSpill, true>);
+            else if (sd <= 16) launch(k_trace_any<false, 16, kGen, kSpill, true>);
+            else launch(k_trace_any<false, 32, kGen, kSpill, true>);
+        }
     } else if (L.gen) {
         if (L.wide) {
             if (sd <= 8) launch(k_trace_any<true, 8, kGen>);

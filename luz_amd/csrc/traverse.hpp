@@ -199,7 +199,10 @@ __device__ inline int wide_visit(RayState& r, STACK& st) {
 }
 
 // tlas_lds: tlas_nodes and recs point into LDS (a staged TLAS).
-template <bool CLOSEST, bool WIDE = false, typename STACK = Stack>
+// TLAS_FREE: the ray visits no TLAS node (its stack holds instance slots and
+// BLAS nodes only, as after ray-group culling), so the world slab ray is not
+// kept and every node visited is a BLAS node.
+template <bool CLOSEST, bool WIDE = false, typename STACK = Stack, bool TLAS_FREE = false>
 __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, const InstRec* recs,
                                const InstAux* aux, Best* best, bool tlas_lds = false,
                                uint32_t max_steps = kMaxSteps) {
@@ -213,6 +216,8 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
             return kStepContinue;
         }
     }
+    // a TLAS-free ray holds only instance slots at the top level
+    if (TLAS_FREE && !r.in_blas && !(r.next & kLeaf)) return kStepOverflow;
     if (!r.in_blas && (r.next & kLeaf)) {
         uint32_t slot = r.next & ~kLeaf;
         float4 w0, w1, w2;
@@ -232,14 +237,17 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
         r.next = 0;
         return kStepContinue;
     }
-    if (WIDE && r.in_blas) return wide_visit(r, st);
+    if (WIDE && (TLAS_FREE || r.in_blas)) return wide_visit(r, st);
     float4 qx, qy, qz;
     uint4 qc;
-    load_rows(r.in_blas ? r.bnodes : tlas_nodes, r.next, tlas_lds && !r.in_blas, qx, qy, qz, qc);
+    if (TLAS_FREE)
+        load_rows(r.bnodes, r.next, false, qx, qy, qz, qc);
+    else
+        load_rows(r.in_blas ? r.bnodes : tlas_nodes, r.next, tlas_lds && !r.in_blas, qx, qy, qz, qc);
     float tn0, tn1;
     // the ray of the tree being walked, selected per lane (restoring the world
     // ray into a single slot on every return to the TLAS costs more)
-    const BoxRay cur = r.in_blas ? r.br : r.wbr;
+    const BoxRay cur = (TLAS_FREE || r.in_blas) ? r.br : r.wbr;
     const bool b0 = box_hit(cur, qx.x, qx.z, qy.x, qy.z, qz.x, qz.z, tn0);
     const bool b1 = box_hit(cur, qx.y, qx.w, qy.y, qy.w, qz.y, qz.w, tn1);
     const bool h0 = (qc.x != kEmpty) & b0;
@@ -252,7 +260,7 @@ __device__ inline int ray_step(RayState& r, STACK& st, const Node* tlas_nodes, c
         bool h = c == 0 ? h0 : h1;
         if (!h) continue;
         uint32_t ch = c == 0 ? qc.x : qc.y;
-        if (r.in_blas && (ch & kLeaf)) {
+        if ((TLAS_FREE || r.in_blas) && (ch & kLeaf)) {
             uint32_t slot = ch & ~kLeaf;
             V3 a, b, cc;
             load_tri(r.btris, slot, a, b, cc);
