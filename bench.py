@@ -189,7 +189,7 @@ def main() -> int:
         c0 = time.perf_counter()
         _, _, st1 = o.trace_visibility(host, W, H, sb, threads=1, row_step=args.cpu_row_step * 2)
         cpu1_s = time.perf_counter() - c0
-        record["roofline"] = roofline(ctx, st, rays, W * H, nl, trace_avg, traffic, prof)
+        record["roofline"] = roofline(ctx, st, rays, traced, W * H, nl, trace_avg, traffic, prof)
         record["cpu_baseline"] = {
             "value": round(st.rays / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
@@ -271,16 +271,17 @@ def visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, f64_step):
     return par, st
 
 
-def roofline(ctx, st, rays, pixels, nl, trace_ms, traffic, prof) -> dict:
+def roofline(ctx, st, rays, traced, pixels, nl, trace_ms, traffic, prof) -> dict:
     """The traversal kernel against its rooflines.  Algorithmic bytes per launch
-    follow the visibility formula sum over rays of (64 n_node + 36 n_tri +
-    64 n_inst) + 24 P + (L + 1) P, with per-ray counts from the oracle's
-    canonical traversal of the identical BVH; the VALU issue fraction is the
+    follow the visibility formula (64 n_node + 36 n_tri + 64 n_inst) per ray +
+    24 P + (L + 1) P, with per-ray counts from the oracle's canonical traversal
+    of the identical BVH, over the rays the launch traces (ray-group culling
+    resolves the others as misses before it); the VALU issue fraction is the
     profiled SQ_INSTS_VALU per launch over CUs x 4 SIMDs x clock / 4 (one
     wave64 VALU instruction per SIMD every 4 cycles) over the kernel time."""
     nr = max(st.rays, 1)
     per_ray = (64.0 * st.node_visits + 36.0 * st.tri_tests + 64.0 * st.inst_visits) / nr
-    b_vis = rays * per_ray + 24.0 * pixels + (nl + 1) * pixels
+    b_vis = traced * per_ray + 24.0 * pixels + (nl + 1) * pixels
     t = trace_ms * 1e-3
     achieved = b_vis / t / 1e9
     cus, clk_khz = ctx.device_info()
@@ -295,7 +296,7 @@ def roofline(ctx, st, rays, pixels, nl, trace_ms, traffic, prof) -> dict:
     return {
         "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_trace_any",
-        "algorithmic_bytes_per_launch": round(b_vis),
+        "algorithmic_bytes_per_launch": round(b_vis), "rays": rays, "traced_rays": traced,
         "per_ray_counts": {"node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
                            "inst": round(st.inst_visits / nr, 3)},
         # the BVH (about 11 MB) is served from L2 and the Infinity Cache: the

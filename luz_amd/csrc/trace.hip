@@ -219,6 +219,18 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
                                                  uint64_t* cand, uint64_t* live_bits,
                                                  uint32_t* live_count) {
     __shared__ uint64_t s_mask[kBlock / 64][LUZRT_MAX_LIGHTS + 1];
+    // the lights and slot bases, staged once per workgroup (the per-light loop
+    // below then reads LDS instead of waiting on scalar loads)
+    __shared__ luzrt_light s_light[LUZRT_MAX_LIGHTS];
+    __shared__ uint32_t s_slot[LUZRT_MAX_LIGHTS];
+    {
+        const uint32_t nlw = (uint32_t)ds->s.num_lights * (uint32_t)(sizeof(luzrt_light) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(ds->s.lights);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_light);
+        for (uint32_t i = threadIdx.x; i < nlw; i += kBlock) dst[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < (uint32_t)ds->s.num_lights; i += kBlock) s_slot[i] = ds->slot_base[i];
+        __syncthreads();
+    }
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
     if (g >= ngroups) return;
@@ -274,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
     const uint64_t all = ninst >= 64 ? ~0ull : ((1ull << ninst) - 1ull);
     const uint32_t nl = (uint32_t)S.num_lights;
     for (uint32_t l = 0; l < nl; ++l) {
-        const luzrt_light& Lt = S.lights[l];
+        const luzrt_light& Lt = s_light[l];
         uint64_t m = 0;
         if (Lt.num_shadow_samples > 0) {
             const V3 lpos = v3(Lt.position[0], Lt.position[1], Lt.position[2]);
@@ -328,7 +340,18 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
     for (uint32_t k = lane; k < kinds; k += 64) cand[(size_t)g * kinds + k] = s_mask[wave][k];
     const uint32_t R = ds->r_total, j = lane;
     bool live = false;
-    if (j < R) live = s_mask[wave][j < ds->r_shadow ? (uint32_t)slot_light(ds, j) : nl] != 0ull;
+    if (j < R) {
+        uint32_t kind = nl;
+        if (j < ds->r_shadow) {   // the last light whose slot base is <= j (slot_light)
+            int lo = 0, hi = (int)nl;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_slot[mid] <= j) lo = mid; else hi = mid;
+            }
+            kind = (uint32_t)lo;
+        }
+        live = s_mask[wave][kind] != 0ull;
+    }
     const uint64_t bits = __ballot(live);
     if (lane == 0) {
         live_bits[g] = bits;
