@@ -369,6 +369,15 @@ struct CountVisibility {
 // Visibility straight from the hit bytes of one active pixel (its slot-0 ray
 // `first` and the step between its slots, ray_row): the same fractions as
 // CountVisibility over the counts k_resolve would have written.
+// cnt / n for 0 <= cnt <= n, n > 0: exactly 0 and 1 at the ends, else the
+// product with the hardware reciprocal (within 2 ulps of the quotient; the
+// radiance it scales is compared to 1e-3).
+__device__ inline float count_fraction(uint32_t cnt, uint32_t n) {
+    if (cnt == 0u) return 0.0f;
+    if (cnt == n) return 1.0f;
+    return (float)cnt * __builtin_amdgcn_rcpf((float)n);
+}
+
 struct HitVisibility {
     const uint8_t* hits;
     const DevScene* ds;
@@ -379,7 +388,7 @@ struct HitVisibility {
         const uint32_t j0 = ds->slot_base[l];
         uint32_t cnt = 0;
         for (int s = 0; s < ns; ++s) cnt += hits[first + (j0 + s) * step];
-        return (float)cnt / (float)ns;
+        return count_fraction(cnt, (uint32_t)ns);
     }
     __device__ float ao_frac(const luzrt_scene& S, uint32_t) const {
         const int na = S.ao_num_samples;
@@ -387,7 +396,7 @@ struct HitVisibility {
         const uint32_t j0 = ds->r_shadow;
         uint32_t miss = 0;
         for (int s = 0; s < na; ++s) miss += 1u - hits[first + (j0 + s) * step];
-        return (float)miss / (float)na;
+        return count_fraction(miss, (uint32_t)na);
     }
 };
 

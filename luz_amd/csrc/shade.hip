@@ -31,10 +31,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(const luzrt_gbuffer g, const D
 // Light pass without count planes: thread q shades background pixel q and
 // active pixel q (active order), the latter's visibility summed from its hit
 // bytes (HitVisibility), as k_resolve + k_shade would.
-__global__ __launch_bounds__(kBlock) void k_shade_hits(const luzrt_gbuffer g, const DevScene* ds,
-                                                       const float* lut, const uint32_t* act_index,
-                                                       const uint32_t* act_list, const uint8_t* hits,
-                                                       const uint32_t* counters, float4* out) {
+// The read-only inputs are __restrict__: the scene block's lights then come
+// through scalar loads (uniform across the wave) instead of vector loads.
+__global__ __launch_bounds__(kBlock) void k_shade_hits(const luzrt_gbuffer g, const DevScene* __restrict__ ds,
+                                                       const float* __restrict__ lut,
+                                                       const uint32_t* __restrict__ act_index,
+                                                       const uint32_t* __restrict__ act_list,
+                                                       const uint8_t* __restrict__ hits,
+                                                       const uint32_t* __restrict__ counters,
+                                                       float4* __restrict__ out) {
     __shared__ float s_lut[256];
     s_lut[threadIdx.x] = lut[threadIdx.x];
     __syncthreads();
