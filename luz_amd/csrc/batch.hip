@@ -206,12 +206,19 @@ __global__ __launch_bounds__(kBlock) void k_bresolve(BatchTables T, const uint8_
     if (q < A) resolve_pixel(S.ds, q, A, h, S.npix, S.act_list[q], S.shadow, S.ao);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bshade(BatchTables T, const float* lut) {
+// The scene block is read through a __restrict__ pointer (nothing the kernel
+// writes aliases it), so its lights come through scalar loads; the sRGB table
+// is staged in LDS as in the single-scene light pass.
+__global__ __launch_bounds__(kBlock) void k_bshade(BatchTables T, const float* __restrict__ lut) {
+    __shared__ float s_lut[256];
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
     const uint32_t s = find_scene(T.pblk, T.n, blockIdx.x);
     const BatchSceneDev& S = T.sc[s];
     const uint32_t p = (blockIdx.x - T.pblk[s]) * kBlock + threadIdx.x;
     if (p >= S.npix) return;
-    S.out[p] = shade_pixel(S.g, S.ds, lut, CountVisibility{S.shadow, S.ao, S.npix}, p);
+    const DevScene* __restrict__ ds = S.ds;
+    S.out[p] = shade_pixel(S.g, ds, s_lut, CountVisibility{S.shadow, S.ao, S.npix}, p);
 }
 
 }  // namespace
