@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 
+#include "cull.hpp"
 #include "internal.hpp"
 #include "lightpath.hpp"
 
@@ -77,6 +78,7 @@ struct BatchTables {
     const uint32_t* ppx;    // [S+1] first primary tile-pixel of each scene (cameras only)
     uint32_t* active;       // [S] active pixels of the frame
     uint32_t* ray_base;     // [S+1] first ray of each scene; [S] = total
+    uint32_t* gbase;        // [S+1] first 64-pixel culling group of each scene; [S] = total
     uint32_t n;
 };
 
@@ -167,15 +169,69 @@ __global__ __launch_bounds__(1024) void k_bcompact_scan(BatchTables T) {
 
 // Ray ranges: scene s owns rays [ray_base[s], ray_base[s] + active[s] * r_total[s]).
 // The host has checked that the sum over all pixels fits 32 bits.
+// Culling groups: scene s owns groups [gbase[s], gbase[s + 1]), ceil(active[s] / 64).
 __global__ void k_bray_base(BatchTables T, uint32_t* counters) {
     if (threadIdx.x != 0) return;
-    uint32_t base = 0;
+    uint32_t base = 0, gb = 0;
     for (uint32_t s = 0; s < T.n; ++s) {
         T.ray_base[s] = base;
+        T.gbase[s] = gb;
         base += T.active[s] * T.sc[s].ds->r_total;
+        gb += (T.active[s] + kCullGroup - 1) / kCullGroup;
     }
     T.ray_base[T.n] = base;
+    T.gbase[T.n] = gb;
     counters[kCntRays] = base;
+}
+
+// Ray-group culling of a batched frame (cull.hpp): one wave per 64-pixel group
+// of the batch's group space (groups past the frame's total get no live row);
+// the group's pixel frames come from its scene's G-buffer planes as the
+// traversal generates them.
+__global__ __launch_bounds__(kBlock) void k_bcull(BatchTables T, const BatchTlas* tl, uint32_t groups_max,
+                                                  uint32_t kinds, uint64_t* cand, uint64_t* live_bits,
+                                                  uint32_t* live_count) {
+    __shared__ uint64_t s_mask[kBlock / 64][LUZRT_MAX_LIGHTS + 1];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
+    if (g >= groups_max) return;
+    if (g >= T.gbase[T.n]) {
+        if (lane == 0) {
+            live_bits[g] = 0;
+            live_count[g] = 0;
+        }
+        return;
+    }
+    const uint32_t s = find_scene(T.gbase, T.n, g);
+    const BatchTlas& bt = tl[s];
+    const DevScene* __restrict__ ds = bt.ds;
+    const uint32_t A = T.active[s], g0 = (g - T.gbase[s]) * kCullGroup;
+    const uint32_t a = g0 + lane;
+    // lanes past the last active pixel repeat the group's first pixel
+    const uint32_t ar = a < A ? a : g0;
+    const PixelFrame f = pixel_frame(bt.position, bt.normal, bt.width, ds->s, bt.act_list[ar]);
+    const uint64_t bits = cull_group(f, ds->s.lights, ds->slot_base, (uint32_t)ds->s.num_lights, ds->r_shadow,
+                                     ds->r_total, ds->s.ao_max, bt.slot_box, bt.ninst, s_mask[wave],
+                                     cand + (size_t)g * kinds);
+    if (lane == 0) {
+        live_bits[g] = bits;
+        live_count[g] = (uint32_t)__popcll(bits);
+    }
+}
+
+// Live rows of the batch's groups (codes g << 6 | j) and the culled rows' hit
+// bytes (misses), as k_cull_rows.
+__global__ __launch_bounds__(kBlock) void k_bcull_rows(BatchTables T, uint32_t groups_max,
+                                                       const uint64_t* live_bits, const uint32_t* row_base,
+                                                       uint32_t* rows, uint8_t* hits) {
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
+    if (g >= groups_max || g >= T.gbase[T.n]) return;
+    const uint32_t s = find_scene(T.gbase, T.n, g);
+    const uint32_t A = T.active[s], g0 = (g - T.gbase[s]) * kCullGroup;
+    const uint32_t R = T.sc[s].ds->r_total;
+    const uint32_t w = A - g0 < kCullGroup ? A - g0 : kCullGroup;
+    cull_rows_lane(live_bits[g], R, w, g << 6, rows + row_base[g], hits + T.ray_base[s] + (size_t)g0 * R);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bcompact_write(BatchTables T, uint32_t tw) {
@@ -259,6 +315,14 @@ struct luzrt_batch {
     uint64_t max_rays = 0;
     uint32_t total_cblk = 0, total_pblk = 0, total_ppx = 0;
     DevBuf hits;
+    // ray-group culling (decided at creation): candidate masks (kinds per
+    // group), live-row bits and counts, the live-row list
+    bool cull = false;
+    uint32_t cull_kinds = 0, groups_max = 0;
+    uint64_t* cull_cand = nullptr;
+    uint64_t* cull_bits = nullptr;
+    uint32_t* cull_counts = nullptr;
+    uint32_t* cull_rows = nullptr;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -305,7 +369,8 @@ luzrt_status upload_tables(luzrt_ctx* ctx, luzrt_batch* b) {
         bt[s].ds = b->sc[s].ds;
         bt[s].act_list = b->sc[s].act_list;
         bt[s].width = b->sc[s].g.width;
-        bt[s].pad = 0;
+        bt[s].ninst = b->inst_n[s];
+        bt[s].slot_box = tlas_slot_boxes(t);
     }
     LUZ_HIP(ctx, hipMemcpyAsync(b->dev_tlas, bt.data(), n * sizeof(BatchTlas), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -378,7 +443,22 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     k_bray_base<<<1, 64, 0, st>>>(b->tables, b->counters);
     k_bcompact_write<<<b->total_cblk, kBlock, 0, st>>>(b->tables, tw);
     LUZ_HIP(ctx, hipGetLastError());
+    if (b->cull) {
+        const uint32_t gblocks = cdiv(b->groups_max, kBlock / 64);
+        k_bcull<<<gblocks, kBlock, 0, st>>>(b->tables, b->dev_tlas, b->groups_max, b->cull_kinds, b->cull_cand,
+                                           b->cull_bits, b->cull_counts);
+        launch_cull_scan(st, b->cull_counts, b->groups_max, b->counters);
+        k_bcull_rows<<<gblocks, kBlock, 0, st>>>(b->tables, b->groups_max, b->cull_bits, b->cull_counts,
+                                                b->cull_rows, (uint8_t*)b->hits.ptr);
+        LUZ_HIP(ctx, hipGetLastError());
+    }
     TraceLaunch L{};
+    if (b->cull) {
+        L.live_rows = b->cull_rows;
+        L.cand = b->cull_cand;
+        L.cand_kinds = b->cull_kinds;
+        L.scene_group_base = b->tables.gbase;
+    }
     L.hits = (uint8_t*)b->hits.ptr;
     L.counters = b->counters;
     L.wide = false;
@@ -433,7 +513,8 @@ void batch_free(luzrt_batch* b) {
     if (b->host_stage) (void)hipHostFree(b->host_stage);
     for (void* p : {(void*)b->dev_stage, (void*)b->dev_work, (void*)b->dev_tables, (void*)b->dev_mats,
                     (void*)b->dev_tlas, (void*)b->dev_jobs, (void*)b->dev_refits,
-                    b->hits.ptr})
+                    b->hits.ptr, (void*)b->cull_cand, (void*)b->cull_bits, (void*)b->cull_counts,
+                    (void*)b->cull_rows})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -468,7 +549,6 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         if (!d.tlas || !d.rgba32f_out || !d.ao_counts || (d.scene.num_lights > 0 && !d.shadow_counts))
             return bad(LUZRT_E_INVALID_ARG, "null TLAS or output");
         if ((st = fill_scene(ctx, &d.scene, &ds[s], true)) != LUZRT_OK) { batch_free(b); return st; }
-        ds[s].ray_group = ctx->ray_order == LUZRT_RAY_ORDER_GROUPED ? kRayGroup : 0;
         luzrt_gbuffer g;
         if ((st = check_gbuffer(ctx, &d.gbuffer, &g)) != LUZRT_OK) { batch_free(b); return st; }
         // the batched kernels (and the primary rays that write the planes)
@@ -489,6 +569,24 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         return fail(ctx, LUZRT_E_LIMIT, "%llu rays per frame exceed the 32-bit ray index",
                     (unsigned long long)rays);
     }
+    // ray-group culling when every scene allows it (<= 64 TLAS slots, <= 64 rays
+    // per pixel): the scenes' rays then go in 64-pixel groups
+    uint64_t groups = 0;
+    uint32_t kinds = 1, rmax = 1;
+    bool cull = ctx->group_cull;
+    for (uint32_t s = 0; s < count; ++s) {
+        const uint64_t npix = (uint64_t)scenes[s].gbuffer.width * scenes[s].gbuffer.height;
+        groups += (npix + kCullGroup - 1) / kCullGroup;
+        kinds = std::max<uint32_t>(kinds, (uint32_t)scenes[s].scene.num_lights + 1);
+        rmax = std::max<uint32_t>(rmax, ds[s].r_total);
+        if (scenes[s].tlas->capacity > kMaxCullInst || ds[s].r_total > kMaxCullSlots) cull = false;
+    }
+    if (groups * kCullGroup > 0xFFFFFFFFull) cull = false;
+    b->cull = cull;
+    b->cull_kinds = kinds;
+    b->groups_max = (uint32_t)groups;
+    for (uint32_t s = 0; s < count; ++s)
+        ds[s].ray_group = (cull || ctx->ray_order == LUZRT_RAY_ORDER_GROUPED) ? kRayGroup : 0;
     // two scenes must not share a TLAS (one build job per TLAS and frame), an
     // output, or a G-buffer plane that either of them writes (camera set)
     for (uint32_t s = 0; s < count; ++s)
@@ -527,7 +625,7 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
     size_t nmats = 0;
     for (auto& m : b->mats) nmats += m.size();
     const size_t tables_bytes = count * sizeof(BatchSceneDev) + (count + 1) * 4 * 3 + count * 4 +
-                                (count + 1) * 4 + 256;
+                                (count + 1) * 4 * 2 + 256;
 #define B_ALLOC(ptr, bytes)                                                             \
     do {                                                                                \
         hipError_t ae = hipMalloc((void**)&(ptr), (bytes) ? (bytes) : 256);             \
@@ -540,6 +638,12 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
     B_ALLOC(b->counters, kCounterBytes);
     B_ALLOC(b->dev_tlas, count * sizeof(BatchTlas));
     B_ALLOC(b->dev_jobs, count * sizeof(TlasJob));
+    if (b->cull) {
+        B_ALLOC(b->cull_cand, groups * kinds * 8);
+        B_ALLOC(b->cull_bits, groups * 8);
+        B_ALLOC(b->cull_counts, ((groups + 3) & ~uint64_t(3)) * 4);
+        B_ALLOC(b->cull_rows, groups * rmax * 4);
+    }
 #undef B_ALLOC
     // the error latch in the counters starts clear; frames never clear it
     e = hipMemset(b->counters, 0, kCounterBytes);
@@ -556,7 +660,8 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
     b->tables.pblk = (const uint32_t*)tp; tp += (count + 1) * 4;
     b->tables.ppx = (const uint32_t*)tp; tp += (count + 1) * 4;
     b->tables.active = (uint32_t*)tp; tp += count * 4;
-    b->tables.ray_base = (uint32_t*)tp;
+    b->tables.ray_base = (uint32_t*)tp; tp += (count + 1) * 4;
+    b->tables.gbase = (uint32_t*)tp;
     b->tables.n = count;
     b->sc.resize(count);
     std::vector<BatchSceneDev>& sc = b->sc;

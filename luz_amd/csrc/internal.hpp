@@ -252,6 +252,7 @@ struct TraceLaunch {
     const uint32_t* scene_ray_base;   // device, nscenes + 1 entries
     const struct BatchTlas* scene_tlas;
     const uint32_t* scene_active;     // device, nscenes entries
+    const uint32_t* scene_group_base; // device, nscenes + 1: first culling group of each scene
     // Visibility launch with rays generated in place (gen): the ray of slot j
     // of active pixel a (ray_row order, A = counters[kCntActive]) is computed
     // from the pixel's record act_rec[2a], act_rec[2a + 1] (pixel_record) and
@@ -280,7 +281,8 @@ struct BatchTlas {
     const struct DevScene* ds;
     const uint32_t* act_list;
     uint32_t width;
-    uint32_t pad;
+    uint32_t ninst;          // TLAS slots of the frame (ray-group culling)
+    const float* slot_box;   // their padded world boxes (6 floats each)
 };
 constexpr uint32_t kMaxBatch = LUZRT_MAX_BATCH;
 constexpr uint32_t kRayGroup = 64;   // pixels per ray group of the grouped ray order (ray_row)
@@ -296,6 +298,9 @@ inline uint32_t primary_chunk(uint64_t ntiles, uint32_t groups) {
     return c < 1 ? 1u : (c > 64 ? 64u : (uint32_t)c);
 }
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L);
+// Exclusive scan in place of n live-row counts (16-byte aligned), the total to
+// counters[kCntLive] (one workgroup, trace.hip).
+void launch_cull_scan(hipStream_t st, uint32_t* counts, uint32_t n, uint32_t* counters);
 // Finds the device address of the no-allocation entry point's scratch array
 // on the context's device (current at the call); records a failure in the
 // context for that entry point to report.

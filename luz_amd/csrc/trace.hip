@@ -23,6 +23,7 @@
 #include "internal.hpp"
 #include "lightpath.hpp"
 #include "traverse.hpp"
+#include "cull.hpp"
 
 namespace luz {
 namespace {
@@ -46,58 +47,6 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(const luzrt_gbuffer gb
     uint32_t total;
     block_prefix(active, wave_tot, total);
     if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
-}
-
-// Exclusive scan in place of n counts (16-byte aligned) by one workgroup of
-// 1024 threads: thread t sums its run of `per` consecutive counts (a multiple
-// of 4, read as 16-byte vectors so that a run's loads are in flight
-// together), the runs' sums are scanned by wave shuffles and one exchange of
-// wave totals, and each run is rewritten with its prefixes.  Returns the total
-// (to every thread).
-__device__ inline uint32_t scan_runs_1024(uint32_t* counts, uint32_t n) {
-    __shared__ uint32_t wave_sum[1024 / 64];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t per = (((n + 1023u) / 1024u) + 3u) & ~3u, begin = threadIdx.x * per;
-    // the run [begin, end): nv whole vectors, then single counts
-    const uint32_t end = begin + per < n ? begin + per : (begin < n ? n : begin);
-    const uint32_t nv = (end - begin) >> 2, tail = begin + nv * 4;
-    uint4* c4 = reinterpret_cast<uint4*>(counts) + (begin >> 2);
-    uint32_t sum = 0;
-#pragma unroll 8
-    for (uint32_t k = 0; k < nv; ++k) {
-        const uint4 v = c4[k];
-        sum += v.x + v.y + v.z + v.w;
-    }
-    for (uint32_t q = tail; q < end; ++q) sum += counts[q];
-    uint32_t incl = sum;
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off);
-        if (lane >= off) incl += y;
-    }
-    if (lane == 63) wave_sum[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (uint32_t w = 0; w < 1024 / 64; ++w) {
-        if (w < wave) before += wave_sum[w];
-        total += wave_sum[w];
-    }
-    uint32_t run = before + incl - sum;
-#pragma unroll 8
-    for (uint32_t k = 0; k < nv; ++k) {
-        const uint4 v = c4[k];
-        uint4 o;
-        o.x = run; run += v.x;
-        o.y = run; run += v.y;
-        o.z = run; run += v.z;
-        o.w = run; run += v.w;
-        c4[k] = o;
-    }
-    for (uint32_t q = tail; q < end; ++q) {
-        const uint32_t v = counts[q];
-        counts[q] = run;
-        run += v;
-    }
-    return total;
 }
 
 // Exclusive scan of the block counts.  Writes the active-pixel and ray counts
@@ -137,90 +86,16 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const luzrt_gbuffer gb
     if (inside) act_index[p] = active ? a : kNone;
 }
 
-// ------------------------------------------------------ ray-group culling
-// The rays of a group of 64 consecutive active pixels (a ray group of the
-// grouped order) that share a kind (the samples of one light, or the AO
-// samples) lie in a region bounded cheaply from the group's pixel records:
-//   shadow rays of light l run from origins o in the box A of the group's
-//   shadow origins to end points within 2 x radius of o + Lvec (light.frag
-//   :111-146): Lvec = the light vector, so o + Lvec is the light position
-//   lifted by the origin's offset N x sb for point and spot lights, and is
-//   computed per pixel for directional lights (box C of the end points).
-//   Every point of such a ray is within max(|A|/2, |C|/2 + 2 radius) of the
-//   segment between the centres of A and C, at the same fraction of its
-//   length, so a TLAS slot whose padded world box, widened by that distance,
-//   misses the segment cannot hold a hit of any ray of the group and light;
-//   AO rays run from origins within 0.05 |N| of A for at most
-//   ao_max x max(|N|, |N|^2) (light.frag:86-97,229-231), so a slot box that
-//   misses A widened by that reach cannot hold one of their hits.
-// k_cull writes, per group, the candidate mask of each kind (bit s: TLAS slot
-// s) and the group's live rows (slots j whose kind has a candidate); rays of
-// other rows are misses without being traced.  The widening carries a margin
-// of 1e-3 + 2e-5 x the magnitude of the coordinates involved, far beyond the
-// FP32 rounding that separates these bounds from the traced rays.  A group
-// whose records hold a non-finite value gets every slot as candidate.
-constexpr uint32_t kCullGroup = 64;     // pixels per culling group (= kRayGroup)
-constexpr uint32_t kMaxCullInst = 64;   // TLAS slots a candidate mask covers
-constexpr uint32_t kMaxCullSlots = 64;  // ray slots per pixel a live-row mask covers
-
-// Wave-wide min / max (the result in every lane): DPP exchanges within each
-// row of 16 lanes (quad xor 1, quad xor 2, half-row mirror, row mirror), then
-// the four row values read by readlane.
-template <int CTRL>
-__device__ inline float dpp_of(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF,
-                                                      false));
-}
-__device__ inline float row_value(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-__device__ inline float wave_min(float v) {
-    v = fminf(v, dpp_of<0xB1>(v));
-    v = fminf(v, dpp_of<0x4E>(v));
-    v = fminf(v, dpp_of<0x141>(v));
-    v = fminf(v, dpp_of<0x140>(v));
-    return fminf(fminf(row_value(v, 0), row_value(v, 16)), fminf(row_value(v, 32), row_value(v, 48)));
-}
-__device__ inline float wave_max(float v) {
-    v = fmaxf(v, dpp_of<0xB1>(v));
-    v = fmaxf(v, dpp_of<0x4E>(v));
-    v = fmaxf(v, dpp_of<0x141>(v));
-    v = fmaxf(v, dpp_of<0x140>(v));
-    return fmaxf(fmaxf(row_value(v, 0), row_value(v, 16)), fmaxf(row_value(v, 32), row_value(v, 48)));
-}
-__device__ inline bool finite3(V3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
-
-// Does the segment p0 + s (p1 - p0), s in [0, 1], meet the box [lo - w, hi + w]?
-// inv[k] = 1 / (p1[k] - p0[k]) by the hardware reciprocal (its one-ulp error
-// moves the crossing points by far less than the widening's margin); a zero
-// component tests p0 against the slab instead.
-__device__ inline bool segment_meets_box(const float p0[3], const float p1[3], const float inv[3],
-                                         const float lo[3], const float hi[3], float w) {
-    float s0 = 0.0f, s1 = 1.0f;
-    bool in = true;
-    for (int k = 0; k < 3; ++k) {
-        const float a = lo[k] - w, b = hi[k] + w;
-        if (p1[k] == p0[k]) {
-            in = in && p0[k] >= a && p0[k] <= b;
-        } else {
-            float t0 = (a - p0[k]) * inv[k], t1 = (b - p0[k]) * inv[k];
-            if (t0 > t1) { const float t = t0; t0 = t1; t1 = t; }
-            s0 = fmax_(s0, t0);
-            s1 = fmin_(s1, t1);
-        }
-    }
-    return in && s0 <= s1;
-}
-
-// One wave per group (four per workgroup).
+// Ray-group culling (cull.hpp): one wave per 64-pixel group (four per
+// workgroup) runs cull_group on the group's pixel records, with the scene's
+// lights and slot bases staged in LDS once per workgroup (the per-light loop
+// then reads LDS instead of waiting on scalar loads).
 __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const uint32_t* counters,
                                                  const DevScene* ds, const float* slot_box,
                                                  uint32_t ninst, uint32_t ngroups, uint32_t kinds,
                                                  uint64_t* cand, uint64_t* live_bits,
                                                  uint32_t* live_count) {
     __shared__ uint64_t s_mask[kBlock / 64][LUZRT_MAX_LIGHTS + 1];
-    // the lights and slot bases, staged once per workgroup (the per-light loop
-    // below then reads LDS instead of waiting on scalar loads)
     __shared__ luzrt_light s_light[LUZRT_MAX_LIGHTS];
     __shared__ uint32_t s_slot[LUZRT_MAX_LIGHTS];
     {
@@ -242,117 +117,12 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
         }
         return;
     }
-    const luzrt_scene& S = ds->s;
     const uint32_t a = g * kCullGroup + lane;
     // lanes past the last active pixel repeat the group's first pixel
     const uint32_t ar = a < A ? a : g * kCullGroup;
     const PixelFrame f = pixel_from_record(act_rec[2 * (size_t)ar], act_rec[2 * (size_t)ar + 1]);
-    const V3 o = shadow_origin(f);
-    const float nlen = __builtin_amdgcn_sqrtf(vdot(f.N, f.N));
-    // the shadow origin's offset from the surface, N x sb: a ray towards a
-    // point or spot light ends near the light position plus this offset
-    const V3 lv = vmul(f.N, fmax_(f.lc * 0.01f, 0.05f));
-    const bool bad = !(finite3(o) && finite3(f.P) && finite3(lv) && isfinite(nlen));
-    const bool any_bad = __ballot(bad) != 0ull;
-    float alo[3] = {wave_min(o.x), wave_min(o.y), wave_min(o.z)};
-    float ahi[3] = {wave_max(o.x), wave_max(o.y), wave_max(o.z)};
-    const float llo[3] = {wave_min(lv.x), wave_min(lv.y), wave_min(lv.z)};
-    const float lhi[3] = {wave_max(lv.x), wave_max(lv.y), wave_max(lv.z)};
-    const float nmax = wave_max(nlen);
-    float l0[3], rl2 = 0.0f;
-    for (int k = 0; k < 3; ++k) {
-        l0[k] = 0.5f * (llo[k] + lhi[k]);
-        const float h = 0.5f * (lhi[k] - llo[k]);
-        rl2 += h * h;
-    }
-    const float rL = __builtin_amdgcn_sqrtf(rl2) * 1.0001f;
-    float a0[3], ra2 = 0.0f, mag = 0.0f;
-    for (int k = 0; k < 3; ++k) {
-        a0[k] = 0.5f * (alo[k] + ahi[k]);
-        const float h = 0.5f * (ahi[k] - alo[k]);
-        ra2 += h * h;
-        mag = fmax_(mag, fmax_(fabsf(alo[k]), fabsf(ahi[k])));
-    }
-    const float rA = __builtin_amdgcn_sqrtf(ra2) * 1.0001f;
-    // this lane's TLAS slot box
-    const bool own = lane < ninst;
-    float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
-    if (own) {
-        for (int k = 0; k < 3; ++k) {
-            blo[k] = slot_box[lane * 6 + k];
-            bhi[k] = slot_box[lane * 6 + 3 + k];
-        }
-    }
-    const uint64_t all = ninst >= 64 ? ~0ull : ((1ull << ninst) - 1ull);
-    const uint32_t nl = (uint32_t)S.num_lights;
-    for (uint32_t l = 0; l < nl; ++l) {
-        const luzrt_light& Lt = s_light[l];
-        uint64_t m = 0;
-        if (Lt.num_shadow_samples > 0) {
-            const V3 lpos = v3(Lt.position[0], Lt.position[1], Lt.position[2]);
-            const bool lbad = !(finite3(lpos) && isfinite(Lt.direction[0]) && isfinite(Lt.direction[1]) &&
-                                isfinite(Lt.direction[2]) && isfinite(Lt.radius));
-            float c0[3], rC, cmag;
-            if (Lt.type == LUZRT_LIGHT_DIRECTIONAL) {
-                // end points o + dir (dir . normalize(-dir)) |lpos - P| (shadow_setup)
-                const V3 dir = v3(Lt.direction[0], Lt.direction[1], Lt.direction[2]);
-                const float kd = vdot(dir, vnorm(v3(-dir.x, -dir.y, -dir.z)));
-                const V3 e = vadd(o, vmul(dir, kd * vlen(vsub(lpos, f.P))));
-                const float elo[3] = {wave_min(e.x), wave_min(e.y), wave_min(e.z)};
-                const float ehi[3] = {wave_max(e.x), wave_max(e.y), wave_max(e.z)};
-                float rc2 = 0.0f;
-                cmag = 0.0f;
-                for (int k = 0; k < 3; ++k) {
-                    c0[k] = 0.5f * (elo[k] + ehi[k]);
-                    const float h = 0.5f * (ehi[k] - elo[k]);
-                    rc2 += h * h;
-                    cmag = fmax_(cmag, fmax_(fabsf(elo[k]), fabsf(ehi[k])));
-                }
-                rC = sqrtf(rc2) * 1.0001f;
-            } else {
-                // end points o + Lvec = P + N sb + (lpos - P): the light position
-                // plus the group's offsets
-                c0[0] = lpos.x + l0[0]; c0[1] = lpos.y + l0[1]; c0[2] = lpos.z + l0[2];
-                rC = rL;
-                cmag = fmax_(fabsf(c0[0]), fmax_(fabsf(c0[1]), fabsf(c0[2]))) + rL;
-            }
-            rC += 2.0f * fabsf(Lt.radius);
-            const float w = fmax_(rA, rC) + 1e-3f + 2e-5f * (mag + cmag + rA + rC);
-            const float inv[3] = {__builtin_amdgcn_rcpf(c0[0] - a0[0]), __builtin_amdgcn_rcpf(c0[1] - a0[1]),
-                                  __builtin_amdgcn_rcpf(c0[2] - a0[2])};
-            const bool meet = own && segment_meets_box(a0, c0, inv, blo, bhi, w);
-            m = __ballot(meet);
-            if (any_bad || lbad || !isfinite(w)) m = all;
-        }
-        if (lane == 0) s_mask[wave][l] = m;
-    }
-    {
-        const float amax = fmax_(S.ao_max, 0.0f);
-        const float reach = amax * fmax_(nmax, nmax * nmax) * 1.00001f + 0.05f * nmax;
-        const float w = reach + 1e-3f + 2e-5f * (mag + reach);
-        bool meet = own;
-        for (int k = 0; k < 3; ++k) meet = meet && alo[k] - w <= bhi[k] && ahi[k] + w >= blo[k];
-        uint64_t m = __ballot(meet);
-        if (any_bad || !isfinite(S.ao_max) || !isfinite(w)) m = all;
-        if (lane == 0) s_mask[wave][nl] = m;
-    }
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t k = lane; k < kinds; k += 64) cand[(size_t)g * kinds + k] = s_mask[wave][k];
-    const uint32_t R = ds->r_total, j = lane;
-    bool live = false;
-    if (j < R) {
-        uint32_t kind = nl;
-        if (j < ds->r_shadow) {   // the last light whose slot base is <= j (slot_light)
-            int lo = 0, hi = (int)nl;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_slot[mid] <= j) lo = mid; else hi = mid;
-            }
-            kind = (uint32_t)lo;
-        }
-        live = s_mask[wave][kind] != 0ull;
-    }
-    const uint64_t bits = __ballot(live);
+    const uint64_t bits = cull_group(f, s_light, s_slot, (uint32_t)ds->s.num_lights, ds->r_shadow, ds->r_total,
+                                     ds->s.ao_max, slot_box, ninst, s_mask[wave], cand + (size_t)g * kinds);
     if (lane == 0) {
         live_bits[g] = bits;
         live_count[g] = (uint32_t)__popcll(bits);
@@ -366,34 +136,20 @@ __global__ __launch_bounds__(1024) void k_cull_scan(uint32_t* counts, uint32_t n
     if (threadIdx.x == 0) counters[kCntLive] = total;
 }
 
-// One wave per group: lane j writes live row j of the group into the row list
-// (at the group's scanned base) or, for a culled row, its hit bytes (zero).
+// One wave per group: its live rows into the row list, its culled rows'
+// hit bytes written as misses (cull_rows_lane).
 __global__ __launch_bounds__(kBlock) void k_cull_rows(const uint32_t* counters, const DevScene* ds,
                                                       uint32_t ngroups, const uint64_t* live_bits,
                                                       const uint32_t* row_base, uint32_t* rows,
                                                       uint8_t* hits) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t wave = threadIdx.x >> 6;
     const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
     if (g >= ngroups) return;
     const uint32_t A = counters[kCntActive];
     if ((uint64_t)g * kCullGroup >= A) return;
-    const uint32_t R = ds->r_total, j = lane;
-    if (j >= R) return;
-    const uint64_t bits = live_bits[g];
-    if ((bits >> j) & 1ull) {
-        const uint64_t below = j ? (bits & (~0ull >> (64 - j))) : 0ull;
-        rows[row_base[g] + (uint32_t)__popcll(below)] = (g << 6) | j;
-    } else {
-        // grouped order: slot j of the group's w pixels is rays g*64*R + j*w ..
-        const uint32_t w = A - g * kCullGroup < kCullGroup ? A - g * kCullGroup : kCullGroup;
-        uint8_t* h = hits + (size_t)g * kCullGroup * R + (size_t)j * w;
-        if (w == kCullGroup) {
-            uint4* h4 = reinterpret_cast<uint4*>(h);
-            for (int k = 0; k < 4; ++k) h4[k] = make_uint4(0u, 0u, 0u, 0u);
-        } else {
-            for (uint32_t k = 0; k < w; ++k) h[k] = 0;
-        }
-    }
+    const uint32_t R = ds->r_total, g0 = g * kCullGroup;
+    const uint32_t w = A - g0 < kCullGroup ? A - g0 : kCullGroup;
+    cull_rows_lane(live_bits[g], R, w, g << 6, rows + row_base[g], hits + (size_t)g0 * R);
 }
 
 // Rays are claimed in chunks per wave (one device-scope atomic per chunk).
@@ -441,13 +197,17 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     constexpr bool BATCH = SRC == kBatch;
     __shared__ uint32_t lds[SHORT * kBlock];
     __shared__ uint32_t sbase[BATCH ? kMaxBatch + 1 : 1];
+    __shared__ uint32_t sgbase[BATCH && CULLED ? kMaxBatch + 1 : 1];   // first culling group per scene
     // A small TLAS (nodes + instance records, 64 B each) is staged in LDS: every
     // ray starts there, so its reads stop competing with BLAS data in L1.  15 KB
     // keeps five 256-lane workgroups (16-entry stacks) resident per CU.
     __shared__ float4 stlas[BATCH ? 4 : (CULLED ? kMaxCullInst * 4 : kTlasLdsEntries * 4)];
     const uint32_t tid = threadIdx.x;
     if (BATCH) {
-        for (uint32_t i = tid; i <= L.nscenes; i += kBlock) sbase[i] = L.scene_ray_base[i];
+        for (uint32_t i = tid; i <= L.nscenes; i += kBlock) {
+            sbase[i] = L.scene_ray_base[i];
+            if (CULLED) sgbase[i] = L.scene_group_base[i];
+        }
         __syncthreads();
     }
     const Node* tnodes = L.tlas_nodes;
@@ -469,9 +229,10 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
     const uint32_t total = gridDim.x * kBlock;
     StackT<SHORT, DEPTH> st;
     st.init(lds + tid, kBlock, L.spill + (size_t)blockIdx.x * kBlock + tid, total);
-    // culled gen launches trace the live rows only (64 ray ids per row)
+    // culled launches trace the live rows only (64 ray ids per row)
     constexpr bool cull = CULLED && SRC == kGen;
-    const uint32_t nrays = cull ? L.counters[kCntLive] * kCullGroup : L.counters[kCntRays];
+    constexpr bool bcull = CULLED && BATCH;
+    const uint32_t nrays = CULLED ? L.counters[kCntLive] * kCullGroup : L.counters[kCntRays];
     const uint32_t A = SRC == kGen ? L.counters[kCntActive] : 1u;   // active pixels
     const uint32_t R = SRC == kGen ? L.ds->r_total : 1u;             // ray slots per pixel
     const uint32_t G = SRC == kGen ? L.ds->ray_group : 0u;           // pixels per ray group
@@ -518,7 +279,46 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                     if (rank < take) {
                         uint32_t rid = chunk_next + rank;
                         ray = rid;
-                        if (BATCH) {
+                        if (bcull) {
+                            // live row g << 6 | j of the batch's group space: the scene
+                            // from the group bases, the hit byte at the ray's index
+                            // in the scene's grouped order
+                            const uint32_t row = L.live_rows[rid >> 6], i = rid & 63u;
+                            const uint32_t g = row >> 6, slot = row & 63u;
+                            const uint32_t sc = scene_of_ray(sgbase, L.nscenes, g);
+                            const BatchTlas& bt = L.scene_tlas[sc];
+                            const uint32_t As = L.scene_active[sc], g0 = (g - sgbase[sc]) * kCullGroup;
+                            const uint32_t a = g0 + i;
+                            if (a >= As) {
+                                ray = kNone;   // past the last pixel of a partial group
+                            } else {
+                                const DevScene* dsc = bt.ds;
+                                const uint32_t Rs = dsc->r_total, rs = dsc->r_shadow;
+                                const uint32_t w = As - g0 < kCullGroup ? As - g0 : kCullGroup;
+                                ray = sbase[sc] + g0 * Rs + slot * w + i;
+                                trecs = bt.recs;
+                                const int l = slot < rs ? slot_light(dsc, slot) : 0;
+                                const uint32_t kind = slot < rs ? (uint32_t)l : (uint32_t)dsc->s.num_lights;
+                                uint64_t cands = L.cand[(size_t)g * L.cand_kinds + kind];
+                                V3 o, d;
+                                float tmin, tmax;
+                                gen_ray_of(dsc, pixel_frame(bt.position, bt.normal, bt.width, dsc->s,
+                                                            bt.act_list[a]),
+                                           slot, l, o, d, tmin, tmax);
+                                ray_begin(r, st, o, d, tmin, tmax);
+                                r.next = kNone;
+                                while (cands) {
+                                    const uint32_t sl = (uint32_t)__builtin_ctzll(cands);
+                                    cands &= cands - 1;
+                                    const float* b = bt.slot_box + sl * 6;
+                                    float tn;
+                                    if (box_hit(r.wbr, b[0], b[3], b[1], b[4], b[2], b[5], tn)) {
+                                        if (r.next == kNone) r.next = kLeaf | sl;
+                                        else st.push(kLeaf | sl);
+                                    }
+                                }
+                            }
+                        } else if (BATCH) {
                             const uint32_t sc = scene_of_ray(sbase, L.nscenes, rid);
                             const BatchTlas& bt = L.scene_tlas[sc];
                             tnodes = bt.nodes;
@@ -636,6 +436,10 @@ inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b
 
 }  // namespace
 
+void launch_cull_scan(hipStream_t st, uint32_t* counts, uint32_t n, uint32_t* counters) {
+    k_cull_scan<<<1, 1024, 0, st>>>(counts, n, counters);
+}
+
 luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
     uint32_t groups = (uint32_t)ctx->num_cus * ctx->groups_per_cu;
     if (groups == 0) groups = 1;
@@ -652,9 +456,15 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
         if (L.wide || L.nscenes > kMaxBatch)
             return fail(ctx, LUZRT_E_UNSUPPORTED, "multi-scene traversal: binary BVHs, <= %u scenes",
                         kMaxBatch);
-        if (sd <= 8) launch(k_trace_any<false, 8, kBatch>);
-        else if (sd <= 16) launch(k_trace_any<false, 16, kBatch>);
-        else launch(k_trace_any<false, 32, kBatch>);
+        if (L.live_rows) {
+            if (sd <= 8) launch(k_trace_any<false, 8, kBatch, kSpill, true>);
+            else if (sd <= 16) launch(k_trace_any<false, 16, kBatch, kSpill, true>);
+            else launch(k_trace_any<false, 32, kBatch, kSpill, true>);
+        } else {
+            if (sd <= 8) launch(k_trace_any<false, 8, kBatch>);
+            else if (sd <= 16) launch(k_trace_any<false, 16, kBatch>);
+            else launch(k_trace_any<false, 32, kBatch>);
+        }
     } else if (L.gen && L.live_rows) {
         if (L.wide) {
             if (sd <= 8) launch(k_trace_any<true, 8, kGen, kSpill, true>);

@@ -116,6 +116,32 @@ def test_batch_grouped_ray_order_matches_oracle(ctx):
         wl.close()
 
 
+def test_batch_without_culling_matches_oracle(ctx):
+    """A batch created with ray-group culling off (every ray traced, slot-major
+    order) gives the oracle's counts, as the default culled batch does."""
+    ctx.set_group_culling(False)
+    try:
+        wl = BatchWorkload(ctx, SCENES)
+    finally:
+        ctx.set_group_culling(True)
+    try:
+        frame = 6
+        wl.frame(frame, True)
+        for i in (1, 2, 3):
+            h = wl.hosts[i]
+            h.animate(frame)
+            gbuf, sh, ao, _ = wl.outputs(i)
+            sb = h.pack()
+            sb.frame = frame
+            W, H, _ = h.camera()
+            wsh, wao, _ = oracle_scene(h).trace_visibility(gbuf, W, H, sb)
+            print(f"unculled batch, scene {i}: {W}x{H}", flush=True)
+            np.testing.assert_array_equal(sh, wsh)
+            np.testing.assert_array_equal(ao, wao)
+    finally:
+        wl.close()
+
+
 def test_batch_mixes_cameras_and_caller_gbuffers(ctx):
     """Scenes without a camera keep the G-buffer their caller wrote (no primary
     tiles of their own in the batch's tile space, between scenes that have
