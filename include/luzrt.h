@@ -212,7 +212,7 @@ luzrt_status luzrt_set_ray_order(luzrt_ctx* ctx, uint32_t order);
 
 /* Ray-group culling of visibility and light-pass calls (a launch parameter,
  * like the above; no counterpart in the reference).  Enabled (default): when
- * the rays are in GROUPED order, the TLAS holds at most 64 instances and a
+ * the rays are in GROUPED order, the TLAS holds at most 128 instances and a
  * pixel casts at most 64 rays, the rays of each 64-pixel group and light (or
  * the group's AO rays) are bounded by a region computed from the group's
  * G-buffer texels; instances whose boxes miss that region are not visited,

@@ -191,7 +191,7 @@ __global__ void k_bray_base(BatchTables T, uint32_t* counters) {
 __global__ __launch_bounds__(kBlock) void k_bcull(BatchTables T, const BatchTlas* tl, uint32_t groups_max,
                                                   uint32_t kinds, uint64_t* cand, uint64_t* live_bits,
                                                   uint32_t* live_count) {
-    __shared__ uint64_t s_mask[kBlock / 64][LUZRT_MAX_LIGHTS + 1];
+    __shared__ uint64_t s_mask[kBlock / 64][(LUZRT_MAX_LIGHTS + 1) * kCandWords];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
     if (g >= groups_max) return;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_bcull(BatchTables T, const BatchTlas
     const PixelFrame f = pixel_frame(bt.position, bt.normal, bt.width, ds->s, bt.act_list[ar]);
     const uint64_t bits = cull_group(f, ds->s.lights, ds->slot_base, (uint32_t)ds->s.num_lights, ds->r_shadow,
                                      ds->r_total, ds->s.ao_max, bt.slot_box, bt.ninst, s_mask[wave],
-                                     cand + (size_t)g * kinds);
+                                     cand + (size_t)g * kinds * kCandWords);
     if (lane == 0) {
         live_bits[g] = bits;
         live_count[g] = (uint32_t)__popcll(bits);
@@ -569,7 +569,7 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         return fail(ctx, LUZRT_E_LIMIT, "%llu rays per frame exceed the 32-bit ray index",
                     (unsigned long long)rays);
     }
-    // ray-group culling when every scene allows it (<= 64 TLAS slots, <= 64 rays
+    // ray-group culling when every scene allows it (<= 128 TLAS slots, <= 64 rays
     // per pixel): the scenes' rays then go in 64-pixel groups
     uint64_t groups = 0;
     uint32_t kinds = 1, rmax = 1;
@@ -639,7 +639,7 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
     B_ALLOC(b->dev_tlas, count * sizeof(BatchTlas));
     B_ALLOC(b->dev_jobs, count * sizeof(TlasJob));
     if (b->cull) {
-        B_ALLOC(b->cull_cand, groups * kinds * 8);
+        B_ALLOC(b->cull_cand, groups * kinds * kCandWords * 8);
         B_ALLOC(b->cull_bits, groups * 8);
         B_ALLOC(b->cull_counts, ((groups + 3) & ~uint64_t(3)) * 4);
         B_ALLOC(b->cull_rows, groups * rmax * 4);

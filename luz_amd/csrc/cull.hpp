@@ -28,7 +28,8 @@
 namespace luz {
 
 constexpr uint32_t kCullGroup = 64;     // pixels per culling group (= kRayGroup)
-constexpr uint32_t kMaxCullInst = 64;   // TLAS slots a candidate mask covers
+constexpr uint32_t kMaxCullInst = 128;  // TLAS slots a candidate mask covers (kCandWords x 64)
+constexpr uint32_t kCandWords = 2;      // u64 words of a candidate mask
 constexpr uint32_t kMaxCullSlots = 64;  // ray slots per pixel a live-row mask covers
 
 // Wave-wide min / max (the result in every lane): DPP exchanges within each
@@ -83,10 +84,11 @@ __device__ inline bool segment_meets_box(const float p0[3], const float p1[3], c
 // The culling of one group, by one wave: lane i holds the frame f of the
 // group's pixel i (lanes past its last pixel repeat the first one).  lights
 // and slot_base are the scene's (LDS or global), slot_box and ninst its TLAS
-// slots' padded world boxes (ninst <= kMaxCullInst); wave_masks is this wave's
-// scratch of nl + 1 masks (LDS).  Writes the nl + 1 candidate masks of the
-// group to cand and returns its live-row bits (bit j: slot j < R has a
-// candidate).
+// slots' padded world boxes (ninst <= kMaxCullInst: lane i tests slots i and
+// i + 64); wave_masks is this wave's scratch of nl + 1 masks of kCandWords
+// words (LDS).  Writes the nl + 1 candidate masks of the group to cand (word w
+// of kind k at cand[k * kCandWords + w]) and returns its live-row bits (bit j:
+// slot j < R has a candidate).
 template <typename LIGHT, typename SLOT>
 __device__ inline uint64_t cull_group(const PixelFrame& f, const LIGHT* lights, const SLOT* slot_base,
                                       uint32_t nl, uint32_t r_shadow, uint32_t R, float ao_max,
@@ -120,19 +122,27 @@ __device__ inline uint64_t cull_group(const PixelFrame& f, const LIGHT* lights, 
         mag = fmax_(mag, fmax_(fabsf(alo[k]), fabsf(ahi[k])));
     }
     const float rA = __builtin_amdgcn_sqrtf(ra2) * 1.0001f;
-    // this lane's TLAS slot box
-    const bool own = lane < ninst;
+    // this lane's TLAS slot boxes (slots lane and lane + 64)
+    const bool own = lane < ninst, own1 = lane + 64 < ninst, two = ninst > 64;
     float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
+    float blo1[3] = {0.0f, 0.0f, 0.0f}, bhi1[3] = {0.0f, 0.0f, 0.0f};
     if (own) {
         for (int k = 0; k < 3; ++k) {
             blo[k] = slot_box[lane * 6 + k];
             bhi[k] = slot_box[lane * 6 + 3 + k];
         }
     }
+    if (own1) {
+        for (int k = 0; k < 3; ++k) {
+            blo1[k] = slot_box[(lane + 64) * 6 + k];
+            bhi1[k] = slot_box[(lane + 64) * 6 + 3 + k];
+        }
+    }
     const uint64_t all = ninst >= 64 ? ~0ull : ((1ull << ninst) - 1ull);
+    const uint64_t all1 = ninst >= 128 ? ~0ull : (ninst > 64 ? ((1ull << (ninst - 64)) - 1ull) : 0ull);
     for (uint32_t l = 0; l < nl; ++l) {
         const LIGHT& Lt = lights[l];
-        uint64_t m = 0;
+        uint64_t m = 0, m1 = 0;
         if (Lt.num_shadow_samples > 0) {
             const V3 lpos = v3(Lt.position[0], Lt.position[1], Lt.position[2]);
             const bool lbad = !(finite3(lpos) && isfinite(Lt.direction[0]) && isfinite(Lt.direction[1]) &&
@@ -167,22 +177,38 @@ __device__ inline uint64_t cull_group(const PixelFrame& f, const LIGHT* lights, 
                                   __builtin_amdgcn_rcpf(c0[2] - a0[2])};
             const bool meet = own && segment_meets_box(a0, c0, inv, blo, bhi, w);
             m = __ballot(meet);
-            if (any_bad || lbad || !isfinite(w)) m = all;
+            if (two) m1 = __ballot(own1 && segment_meets_box(a0, c0, inv, blo1, bhi1, w));
+            if (any_bad || lbad || !isfinite(w)) {
+                m = all;
+                m1 = all1;
+            }
         }
-        if (lane == 0) wave_masks[l] = m;
+        if (lane == 0) {
+            wave_masks[l * kCandWords] = m;
+            wave_masks[l * kCandWords + 1] = m1;
+        }
     }
     {
         const float amax = fmax_(ao_max, 0.0f);
         const float reach = amax * fmax_(nmax, nmax * nmax) * 1.00001f + 0.05f * nmax;
         const float w = reach + 1e-3f + 2e-5f * (mag + reach);
-        bool meet = own;
-        for (int k = 0; k < 3; ++k) meet = meet && alo[k] - w <= bhi[k] && ahi[k] + w >= blo[k];
-        uint64_t m = __ballot(meet);
-        if (any_bad || !isfinite(ao_max) || !isfinite(w)) m = all;
-        if (lane == 0) wave_masks[nl] = m;
+        bool meet = own, meet1 = own1;
+        for (int k = 0; k < 3; ++k) {
+            meet = meet && alo[k] - w <= bhi[k] && ahi[k] + w >= blo[k];
+            meet1 = meet1 && alo[k] - w <= bhi1[k] && ahi[k] + w >= blo1[k];
+        }
+        uint64_t m = __ballot(meet), m1 = __ballot(meet1);
+        if (any_bad || !isfinite(ao_max) || !isfinite(w)) {
+            m = all;
+            m1 = all1;
+        }
+        if (lane == 0) {
+            wave_masks[nl * kCandWords] = m;
+            wave_masks[nl * kCandWords + 1] = m1;
+        }
     }
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t k = lane; k <= nl; k += 64) cand[k] = wave_masks[k];
+    for (uint32_t k = lane; k < (nl + 1) * kCandWords; k += 64) cand[k] = wave_masks[k];
     const uint32_t j = lane;
     bool live = false;
     if (j < R) {
@@ -195,7 +221,7 @@ __device__ inline uint64_t cull_group(const PixelFrame& f, const LIGHT* lights, 
             }
             kind = (uint32_t)lo;
         }
-        live = wave_masks[kind] != 0ull;
+        live = (wave_masks[kind * kCandWords] | wave_masks[kind * kCandWords + 1]) != 0ull;
     }
     return __ballot(live);
 }

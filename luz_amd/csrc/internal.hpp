@@ -263,8 +263,9 @@ struct TraceLaunch {
     // Ray-group culling (gen launches, grouped order): the launch traces only
     // the live rows live_rows[0 .. counters[kCntLive]) (a row: slot j of the 64
     // active pixels of group g, encoded g << 6 | j), and a ray of kind k (its
-    // light, or num_lights for AO) starts from the candidate slots
-    // cand[g * cand_kinds + k] instead of the TLAS root.  Null: no culling.
+    // light, or num_lights for AO) starts from the candidate slots of mask
+    // words cand[(g * cand_kinds + k) * kCandWords ..] instead of the TLAS root.
+    // Null: no culling.
     const uint32_t* live_rows;
     const uint64_t* cand;
     const float* slot_box;   // the TLAS slots' padded world boxes (6 floats each)

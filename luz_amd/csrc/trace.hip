@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
                                                  uint32_t ninst, uint32_t ngroups, uint32_t kinds,
                                                  uint64_t* cand, uint64_t* live_bits,
                                                  uint32_t* live_count) {
-    __shared__ uint64_t s_mask[kBlock / 64][LUZRT_MAX_LIGHTS + 1];
+    __shared__ uint64_t s_mask[kBlock / 64][(LUZRT_MAX_LIGHTS + 1) * kCandWords];
     __shared__ luzrt_light s_light[LUZRT_MAX_LIGHTS];
     __shared__ uint32_t s_slot[LUZRT_MAX_LIGHTS];
     {
@@ -122,7 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_cull(const float4* act_rec, const ui
     const uint32_t ar = a < A ? a : g * kCullGroup;
     const PixelFrame f = pixel_from_record(act_rec[2 * (size_t)ar], act_rec[2 * (size_t)ar + 1]);
     const uint64_t bits = cull_group(f, s_light, s_slot, (uint32_t)ds->s.num_lights, ds->r_shadow, ds->r_total,
-                                     ds->s.ao_max, slot_box, ninst, s_mask[wave], cand + (size_t)g * kinds);
+                                     ds->s.ao_max, slot_box, ninst, s_mask[wave],
+                                     cand + (size_t)g * kinds * kCandWords);
     if (lane == 0) {
         live_bits[g] = bits;
         live_count[g] = (uint32_t)__popcll(bits);
@@ -187,6 +188,27 @@ __device__ inline uint32_t scene_of_ray(const uint32_t* base, uint32_t n, uint32
 // ray index at claim time; the per-lane TLAS pointers then live in vector
 // registers).
 enum RaySource { kRays = 0, kBatch = 1, kGen = 2 };
+
+// A culled ray's start: the candidate slots (mask words cw) whose padded
+// boxes it meets, the test the TLAS node above each slot would make; the
+// first is entered now, the others from the stack, and no TLAS node is visited.
+template <typename STACK>
+__device__ inline void enter_candidates(RayState& r, STACK& st, const uint64_t* cw, const float* slot_box) {
+    r.next = kNone;
+    for (uint32_t w = 0; w < kCandWords; ++w) {
+        uint64_t m = cw[w];
+        while (m) {
+            const uint32_t sl = 64u * w + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const float* b = slot_box + sl * 6;
+            float tn;
+            if (box_hit(r.wbr, b[0], b[3], b[1], b[4], b[2], b[5], tn)) {
+                if (r.next == kNone) r.next = kLeaf | sl;
+                else st.push(kLeaf | sl);
+            }
+        }
+    }
+}
 
 // CULLED (gen launches after k_cull): the live rows only, each ray starting
 // from its group's candidate slots; no TLAS node is visited, so only the
@@ -299,24 +321,14 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                                 trecs = bt.recs;
                                 const int l = slot < rs ? slot_light(dsc, slot) : 0;
                                 const uint32_t kind = slot < rs ? (uint32_t)l : (uint32_t)dsc->s.num_lights;
-                                uint64_t cands = L.cand[(size_t)g * L.cand_kinds + kind];
+                                const uint64_t* cw = L.cand + ((size_t)g * L.cand_kinds + kind) * kCandWords;
                                 V3 o, d;
                                 float tmin, tmax;
                                 gen_ray_of(dsc, pixel_frame(bt.position, bt.normal, bt.width, dsc->s,
                                                             bt.act_list[a]),
                                            slot, l, o, d, tmin, tmax);
                                 ray_begin(r, st, o, d, tmin, tmax);
-                                r.next = kNone;
-                                while (cands) {
-                                    const uint32_t sl = (uint32_t)__builtin_ctzll(cands);
-                                    cands &= cands - 1;
-                                    const float* b = bt.slot_box + sl * 6;
-                                    float tn;
-                                    if (box_hit(r.wbr, b[0], b[3], b[1], b[4], b[2], b[5], tn)) {
-                                        if (r.next == kNone) r.next = kLeaf | sl;
-                                        else st.push(kLeaf | sl);
-                                    }
-                                }
+                                enter_candidates(r, st, cw, bt.slot_box);
                             }
                         } else if (BATCH) {
                             const uint32_t sc = scene_of_ray(sbase, L.nscenes, rid);
@@ -340,7 +352,7 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                         } else {
                             if (SRC == kGen) {
                                 uint32_t slot = 0, a = 0;
-                                uint64_t cands = 0;
+                                const uint64_t* cw = nullptr;
                                 const uint32_t rs = L.ds->r_shadow;
                                 if (cull) {
                                     // live row g << 6 | j: slot j of the group's pixel i; its hit
@@ -354,7 +366,7 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                                         ray = g0 * R + slot * w + i;
                                         const uint32_t kind = slot < rs ? (uint32_t)slot_light(L.ds, slot)
                                                                         : (uint32_t)L.ds->s.num_lights;
-                                        cands = L.cand[(size_t)g * L.cand_kinds + kind];
+                                        cw = L.cand + ((size_t)g * L.cand_kinds + kind) * kCandWords;
                                     } else {
                                         ray = kNone;   // past the last pixel of a partial group
                                     }
@@ -368,23 +380,7 @@ __global__ __launch_bounds__(kBlock, SHORT > 16 ? 1 : 5) void k_trace_any(TraceL
                                     gen_ray_of(L.ds, pixel_from_record(r0, r1), slot,
                                                slot < rs ? slot_light(L.ds, slot) : 0, o, d, tmin, tmax);
                                     ray_begin(r, st, o, d, tmin, tmax);
-                                    if (cull) {
-                                        // the candidate slots whose boxes the ray meets (the
-                                        // test the TLAS node above each slot would make), the
-                                        // first entered now, the others from the stack: no
-                                        // TLAS node is visited
-                                        r.next = kNone;
-                                        while (cands) {
-                                            const uint32_t sl = (uint32_t)__builtin_ctzll(cands);
-                                            cands &= cands - 1;
-                                            const float* b = L.slot_box + sl * 6;
-                                            float tn;
-                                            if (box_hit(r.wbr, b[0], b[3], b[1], b[4], b[2], b[5], tn)) {
-                                                if (r.next == kNone) r.next = kLeaf | sl;
-                                                else st.push(kLeaf | sl);
-                                            }
-                                        }
-                                    }
+                                    if (cull) enter_candidates(r, st, cw, L.slot_box);
                                 }
                             } else {
                                 float4 o = L.ray_o[(size_t)rid * L.ray_stride];
@@ -539,7 +535,7 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     // rays are generated inside the traversal; the compaction scan set their
     // count (the raygen stage times the culling kernels, when they run)
 
-    // ray-group culling (grouped order, <= 64 TLAS slots, <= 64 rays per pixel)
+    // ray-group culling (grouped order, <= 128 TLAS slots, <= 64 rays per pixel)
     const uint32_t ngroups = (uint32_t)((npix + kCullGroup - 1) / kCullGroup);
     const bool cull = ctx->group_cull && ds.r_total > 0 && ds.r_total <= kMaxCullSlots &&
                       tlas->count > 0 && tlas->count <= kMaxCullInst && ds.ray_group == kCullGroup &&
@@ -547,7 +543,7 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     ctx->last_culled = cull;
     const uint32_t kinds = (uint32_t)scene->num_lights + 1;
     if (cull) {
-        if ((s = ensure(ctx, ctx->cull_cand, (size_t)ngroups * kinds * 8)) != LUZRT_OK) return s;
+        if ((s = ensure(ctx, ctx->cull_cand, (size_t)ngroups * kinds * kCandWords * 8)) != LUZRT_OK) return s;
         const size_t counts_off = ((size_t)ngroups * 8 + 255) & ~size_t(255);   // 16-byte aligned counts
         if ((s = ensure(ctx, ctx->cull_bits, counts_off + (size_t)ngroups * 4 + 16)) != LUZRT_OK) return s;
         if ((s = ensure(ctx, ctx->cull_rows, (size_t)ngroups * ds.r_total * 4)) != LUZRT_OK) return s;
