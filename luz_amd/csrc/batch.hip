@@ -170,18 +170,23 @@ __global__ __launch_bounds__(1024) void k_bcompact_scan(BatchTables T) {
 // Ray ranges: scene s owns rays [ray_base[s], ray_base[s] + active[s] * r_total[s]).
 // The host has checked that the sum over all pixels fits 32 bits.
 // Culling groups: scene s owns groups [gbase[s], gbase[s + 1]), ceil(active[s] / 64).
-__global__ void k_bray_base(BatchTables T, uint32_t* counters) {
+// A culled frame starts every scene's rays at a multiple of `align` (16: the
+// culled rows' hit bytes are then written as 16-byte vectors); kCntRays is
+// the number of rays either way.
+__global__ void k_bray_base(BatchTables T, uint32_t* counters, uint32_t align) {
     if (threadIdx.x != 0) return;
-    uint32_t base = 0, gb = 0;
+    uint32_t base = 0, gb = 0, rays = 0;
     for (uint32_t s = 0; s < T.n; ++s) {
         T.ray_base[s] = base;
         T.gbase[s] = gb;
-        base += T.active[s] * T.sc[s].ds->r_total;
+        const uint32_t r = T.active[s] * T.sc[s].ds->r_total;
+        rays += r;
+        base = (base + r + align - 1) / align * align;
         gb += (T.active[s] + kCullGroup - 1) / kCullGroup;
     }
     T.ray_base[T.n] = base;
     T.gbase[T.n] = gb;
-    counters[kCntRays] = base;
+    counters[kCntRays] = rays;
 }
 
 // Ray-group culling of a batched frame (cull.hpp): one wave per 64-pixel group
@@ -192,10 +197,27 @@ __global__ __launch_bounds__(kBlock) void k_bcull(BatchTables T, const BatchTlas
                                                   uint32_t kinds, uint64_t* cand, uint64_t* live_bits,
                                                   uint32_t* live_count) {
     __shared__ uint64_t s_mask[kBlock / 64][(LUZRT_MAX_LIGHTS + 1) * kCandWords];
+    __shared__ luzrt_light s_light[LUZRT_MAX_LIGHTS];
+    __shared__ uint32_t s_slot[LUZRT_MAX_LIGHTS];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t g = blockIdx.x * (kBlock / 64) + wave;
+    // when the workgroup's four groups are of one scene (all but a few), its
+    // lights and slot bases are staged in LDS as in k_cull
+    const uint32_t total = T.gbase[T.n];
+    const uint32_t gf = blockIdx.x * (kBlock / 64);
+    const uint32_t gl = min(gf + kBlock / 64, total) - 1;
+    const bool one_scene = gf < total && find_scene(T.gbase, T.n, gf) == find_scene(T.gbase, T.n, gl);
+    if (one_scene) {
+        const DevScene* ds0 = tl[find_scene(T.gbase, T.n, gf)].ds;
+        const uint32_t nlw = (uint32_t)ds0->s.num_lights * (uint32_t)(sizeof(luzrt_light) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(ds0->s.lights);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_light);
+        for (uint32_t i = threadIdx.x; i < nlw; i += kBlock) dst[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < (uint32_t)ds0->s.num_lights; i += kBlock) s_slot[i] = ds0->slot_base[i];
+        __syncthreads();
+    }
     if (g >= groups_max) return;
-    if (g >= T.gbase[T.n]) {
+    if (g >= total) {
         if (lane == 0) {
             live_bits[g] = 0;
             live_count[g] = 0;
@@ -210,9 +232,13 @@ __global__ __launch_bounds__(kBlock) void k_bcull(BatchTables T, const BatchTlas
     // lanes past the last active pixel repeat the group's first pixel
     const uint32_t ar = a < A ? a : g0;
     const PixelFrame f = pixel_frame(bt.position, bt.normal, bt.width, ds->s, bt.act_list[ar]);
-    const uint64_t bits = cull_group(f, ds->s.lights, ds->slot_base, (uint32_t)ds->s.num_lights, ds->r_shadow,
-                                     ds->r_total, ds->s.ao_max, bt.slot_box, bt.ninst, s_mask[wave],
-                                     cand + (size_t)g * kinds * kCandWords);
+    uint64_t bits;
+    if (one_scene)
+        bits = cull_group(f, s_light, s_slot, (uint32_t)ds->s.num_lights, ds->r_shadow, ds->r_total,
+                          ds->s.ao_max, bt.slot_box, bt.ninst, s_mask[wave], cand + (size_t)g * kinds * kCandWords);
+    else
+        bits = cull_group(f, ds->s.lights, ds->slot_base, (uint32_t)ds->s.num_lights, ds->r_shadow, ds->r_total,
+                          ds->s.ao_max, bt.slot_box, bt.ninst, s_mask[wave], cand + (size_t)g * kinds * kCandWords);
     if (lane == 0) {
         live_bits[g] = bits;
         live_count[g] = (uint32_t)__popcll(bits);
@@ -440,7 +466,7 @@ luzrt_status issue_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     const uint32_t tw = ctx->tile_width;
     k_bcompact_count<<<b->total_cblk, kBlock, 0, st>>>(b->tables, tw);
     k_bcompact_scan<<<b->n, 1024, 0, st>>>(b->tables);
-    k_bray_base<<<1, 64, 0, st>>>(b->tables, b->counters);
+    k_bray_base<<<1, 64, 0, st>>>(b->tables, b->counters, b->cull ? 16u : 1u);
     k_bcompact_write<<<b->total_cblk, kBlock, 0, st>>>(b->tables, tw);
     LUZ_HIP(ctx, hipGetLastError());
     if (b->cull) {
@@ -481,7 +507,8 @@ luzrt_status prepare_frame(luzrt_ctx* ctx, luzrt_batch* b) {
     uint32_t groups = (uint32_t)ctx->num_cus * std::max<uint32_t>(ctx->groups_per_cu, kPrimaryGroupsPerCu);
     if (groups == 0) groups = 1;
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * kBlock * kSpill * 4)) != LUZRT_OK) return s;
-    if ((s = ensure(ctx, b->hits, b->max_rays + 16)) != LUZRT_OK) return s;
+    // a culled frame's scenes start at multiples of 16 rays (k_bray_base)
+    if ((s = ensure(ctx, b->hits, b->max_rays + 16 * ((size_t)b->n + 1))) != LUZRT_OK) return s;
     return LUZRT_OK;
 }
 
@@ -564,7 +591,7 @@ luzrt_status luzrt_batch_create(luzrt_ctx* ctx, const luzrt_batch_scene* scenes,
         b->has_cam.push_back(d.camera ? 1u : 0u);
         b->mats.emplace_back(d.materials, d.materials + d.material_count);
     }
-    if (rays > 0xFFFFFFFFull) {
+    if (rays + 16ull * (count + 1) > 0xFFFFFFFFull) {
         batch_free(b);
         return fail(ctx, LUZRT_E_LIMIT, "%llu rays per frame exceed the 32-bit ray index",
                     (unsigned long long)rays);
