@@ -63,10 +63,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kIn = LUZRT_DN_FEATURES, kHid = LUZRT_DN_HIDDEN, kOut = 2;
 constexpr int kBlock = 256;
 constexpr uint32_t kChunkRows = 1024;
-// inference: 16-pixel tiles per wave, and 8 waves per group sharing one copy
-// of the weights in LDS (measured on the 1080p frame: 1 tile x 512 threads
-// 0.93 ms, 1 x 256 1.01 ms, 2 x 512 1.22 ms, 2 x 256 1.49 ms, 4 x 256 2.15 ms)
-constexpr int kInferTiles = 1;
+// inference: one 16-pixel tile per wave at a time, 8 waves per group sharing
+// one copy of the weights in LDS
 constexpr int kInferBlock = 512;
 
 constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn;
@@ -534,13 +532,139 @@ __device__ inline uint32_t dn_nb_index(const DnFrameDev& F, uint32_t x, uint32_t
     return (uint32_t)qy * F.width + (uint32_t)qx;
 }
 
-// Inference, one 16-pixel tile per lane column j, Q tiles per wave: the
-// light-independent part of layer 1 once per pixel (K-steps FG, bias folded
-// into the accumulators), then for every light the visibility K-step FV on
-// top of it and layers 2..5.  A pixel's AO is the mean of its lights' AO
-// outputs, summed in light order; background pixels give 0.
-template <int Q>
-__global__ __launch_bounds__(kInferBlock, 4) void k_dn_infer(InferArgs A) {
+// The per-pixel state a tile's light groups share: lane (j, g) holds pixel j
+// of the tile and neighbours 8g + e (e < 8) of its 5x5 window.
+struct InferTile {
+    uint32_t vq[8];   // neighbour indices (0 off the window and for invalid pixels)
+    uint32_t vfg;     // foreground bits of those neighbours
+    float cnt, bao;   // foreground pixels of the window, box base of AO
+    uint32_t pp;      // pixel index (0 beyond the frame)
+    bool valid, inb;
+};
+
+// Lights l0 .. l0+P-1 of one 16-pixel tile: the visibility K-step FV on top of
+// the light-independent accumulators `ag`, layers 2..4 on MFMA and layer 5 in
+// FP32.  Every weight fragment, bias and layer-5 weight read from LDS feeds the
+// products of all P lights, and the group's 8 P shadow bytes are loaded
+// together before the first is used.  AO outputs are added to `aosum` in light
+// order; the visibilities are written.
+template <int P>
+__device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, const float* sbias,
+                                      const float* sw5, const uint16_t* svis, const float* svisf, uint32_t l0,
+                                      int lane, int g, const f32x4 (&ag)[4], const InferTile& T, float b50,
+                                      float b51, float& aosum) {
+    const DnFrameDev& F = A.F;
+    const uint32_t npix = F.width * F.height;
+    const uint32_t L = F.num_lights;
+    // the weight fragments are read from LDS at every use: an opaque lane
+    // offset keeps the compiler from holding all 28 of them in registers
+    // across the groups (which would leave one wave per SIMD)
+    uint32_t ln = (uint32_t)lane;
+    asm volatile("" : "+v"(ln));
+    uint32_t s8[P][8];
+    if (L) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const uint8_t* sh = F.shadow + (size_t)(l0 + p) * npix;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s8[p][e] = sh[T.vq[e]];
+        }
+    }
+    f32x4 acc[P][4];
+    uint4 hb[P][2];
+    float bvis[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        float sv = 0.0f;
+        if (L) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool on = (T.vfg >> e) & 1u;
+                w[e >> 1] |= (on ? (uint32_t)svis[s8[p][e]] : 0u) << (16 * (e & 1));
+                if (on) sv += svisf[s8[p][e]];
+            }
+        }
+        sv += __shfl_xor(sv, 16);
+        sv += __shfl_xor(sv, 32);
+        bvis[p] = sv / T.cnt;
+        hb[p][0] = make_uint4(w[0], w[1], w[2], w[3]);   // the visibility B operand
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const bf16x8 a = as_bf16x8(sfrag[(kIFV + t) * 64 + ln]);
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p][t] = mfma(a, as_bf16x8(hb[p][0]), ag[t]);
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) dn_relu_pack(acc[p], hb[p]);
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        const int fb = k == 1 ? kIF2 : (k == 2 ? kIF3 : kIF4);
+        uint4 fr[8];   // fragment (t, s) at t * 2 + s, all 8 in flight together
+#pragma unroll
+        for (int i = 0; i < 8; ++i) fr[i] = sfrag[(fb + i) * 64 + ln];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const f32x4 bk = *reinterpret_cast<const f32x4*>(&sbias[k * kHid + 16 * t + 4 * g]);
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[p][t] = bk;
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bf16x8 a = as_bf16x8(fr[t * 2 + s]);
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p][t] = mfma(a, as_bf16x8(hb[p][s]), acc[p][t]);
+            }
+        if (k < 3) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) dn_relu_pack(acc[p], hb[p]);
+        }
+    }
+    // layer 5 in FP32 on layer 4's activations: per lane in (t, i) order, then
+    // over the lane groups as ((g0 + g1) + (g2 + g3)), the order of dn_out_fp32
+    float o0[P], o1[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        o0[p] = 0.0f;
+        o1[p] = 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const f32x4 wa = *reinterpret_cast<const f32x4*>(&sw5[16 * t + 4 * g]);
+        const f32x4 wb = *reinterpret_cast<const f32x4*>(&sw5[kHid + 16 * t + 4 * g]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float h = relu_bits(acc[p][t][i]);
+                o0[p] = fmaf(wa[i], h, o0[p]);
+                o1[p] = fmaf(wb[i], h, o1[p]);
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        o0[p] += __shfl_xor(o0[p], 16);
+        o1[p] += __shfl_xor(o1[p], 16);
+        o0[p] += __shfl_xor(o0[p], 32);
+        o1[p] += __shfl_xor(o1[p], 32);
+        if (g == 0 && T.inb) {
+            aosum += T.valid ? (o0[p] + b50) + T.bao : 0.0f;
+            if (L)
+                A.vis_out[(size_t)(l0 + p) * npix + T.pp] = T.valid ? clamp01f((o1[p] + b51) + bvis[p]) : 0.0f;
+        }
+    }
+}
+
+// Inference, one 16-pixel tile per wave at a time (lane column j: pixel j of
+// the tile): the light-independent part of layer 1 once per pixel (K-steps FG,
+// bias folded into the accumulators), then the lights in groups of 3 (the
+// last group 1 or 2), each light with the visibility K-step FV on top of it
+// and layers 2..5.  A pixel's AO is the mean of its lights' AO outputs, summed
+// in light order; background pixels give 0.
+__global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     __shared__ uint4 sfrag[kFragInfer * 64];
     __shared__ __align__(16) float sbias[4 * kHid + 4];
     __shared__ uint16_t svis[256];   // BF16 of 1 - s / shadow_n
@@ -566,204 +690,109 @@ __global__ __launch_bounds__(kInferBlock, 4) void k_dn_infer(InferArgs A) {
     const uint32_t wave = lb * (kInferBlock / 64) + (threadIdx.x >> 6);
     const uint32_t nwaves = nb * (kInferBlock / 64);
     const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
-    for (uint32_t tile0 = wave * Q; tile0 < A.ntiles; tile0 += nwaves * Q) {
-        uint32_t pp[Q], px[Q], py[Q];
-        bool valid[Q], inb[Q];
-        float dc[Q];
-        bool any = false;
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const uint32_t p = (tile0 + q) * 16u + (uint32_t)j;
-            inb[q] = p < npix;
-            const uint32_t pc = inb[q] ? p : 0u;
-            const float d = F.pix[pc].x;
-            valid[q] = inb[q] && d >= 0.0f;
-            dc[q] = valid[q] ? d : 0.0f;
-            pp[q] = pc; px[q] = pc % F.width; py[q] = pc / F.width;
-            any |= valid[q];
-        }
-        if (__ballot(any) == 0ull) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (g == 0 && inb[q]) {
-                    A.ao_out[pp[q]] = 0.0f;
-                    for (uint32_t l = 0; l < L; ++l) A.vis_out[(size_t)l * npix + pp[q]] = 0.0f;
-                }
+    for (uint32_t tile = wave; tile < A.ntiles; tile += nwaves) {
+        InferTile T;
+        const uint32_t p = tile * 16u + (uint32_t)j;
+        T.inb = p < npix;
+        T.pp = T.inb ? p : 0u;
+        const float d = F.pix[T.pp].x;
+        T.valid = T.inb && d >= 0.0f;
+        const float dc = T.valid ? d : 0.0f;
+        const uint32_t px = T.pp % F.width, py = T.pp / F.width;
+        if (__ballot(T.valid) == 0ull) {
+            if (g == 0 && T.inb) {
+                A.ao_out[T.pp] = 0.0f;
+                for (uint32_t l = 0; l < L; ++l) A.vis_out[(size_t)l * npix + T.pp] = 0.0f;
+            }
             continue;
         }
         // ---- layer 1, light-independent features: K-steps 0..2 (AO, normal)
-        f32x4 ag[Q][4];
+        // of neighbours 8s + 2g + h, then K-step 3: relative depth of
+        // neighbours 8g + e, and the neighbour indices and foreground bits that
+        // the visibility step reads per light.  Every neighbour index is
+        // clamped to the image, so all loads are issued before the first is
+        // used (invalid pixels select zeros afterwards).
+        uint32_t na[6];
+        float4 va[6];
+        float aa[6];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const f32x4 b1 = *reinterpret_cast<const f32x4*>(&sbias[16 * t + 4 * g]);
+        for (int i = 0; i < 6; ++i) na[i] = dn_nb_index(F, px, py, 2 * (4 * (i >> 1) + g) + (i & 1));
+        uint32_t nv[8];
+        float dv[8], av[8];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) ag[q][t] = b1;
+        for (int e = 0; e < 8; ++e) nv[e] = dn_nb_index(F, px, py, 8 * g + e < 25 ? 8 * g + e : 0);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            va[i] = F.pix[na[i]];
+            aa[i] = F.ao_frac[na[i]];
         }
+        const float4 v24 = F.pix[nv[0]];   // neighbour 24 on lane group 3
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            dv[e] = e == 0 ? v24.x : F.pix[nv[e]].x;
+            av[e] = F.ao_frac[nv[e]];
+        }
+        f32x4 ag[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ag[t] = *reinterpret_cast<const f32x4*>(&sbias[16 * t + 4 * g]);
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            uint4 fb[Q];
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            if (T.valid) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                uint32_t w[4] = {0u, 0u, 0u, 0u};
-                if (valid[q]) {
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const uint32_t nq = dn_nb_index(F, px[q], py[q], 2 * (4 * s + g) + h);
-                        const float4 v = F.pix[nq];
-                        const bool fg = v.x >= 0.0f;
-                        const float ao = fg ? F.ao_frac[nq] : 0.0f;
-                        w[2 * h] = bf16_pack2(ao, v.y);     // background normals are zero
-                        w[2 * h + 1] = bf16_pack2(v.z, v.w);
-                    }
+                for (int h = 0; h < 2; ++h) {
+                    const float4 v = va[2 * s + h];
+                    const float ao = v.x >= 0.0f ? aa[2 * s + h] : 0.0f;
+                    w[2 * h] = bf16_pack2(ao, v.y);     // background normals are zero
+                    w[2 * h + 1] = bf16_pack2(v.z, v.w);
                 }
-                fb[q] = make_uint4(w[0], w[1], w[2], w[3]);
             }
+            const bf16x8 fb = as_bf16x8(make_uint4(w[0], w[1], w[2], w[3]));
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const bf16x8 a = as_bf16x8(sfrag[(kIFG + t * 4 + s) * 64 + lane]);
-#pragma unroll
-                for (int q = 0; q < Q; ++q) ag[q][t] = mfma(a, as_bf16x8(fb[q]), ag[q][t]);
-            }
+            for (int t = 0; t < 4; ++t) ag[t] = mfma(as_bf16x8(sfrag[(kIFG + t * 4 + s) * 64 + lane]), fb, ag[t]);
         }
-        // ---- K-step 3: relative depth of neighbours 8g + e, and the neighbour
-        // indices and foreground bits that the visibility step reads per light
-        uint32_t vq[Q][8];
-        uint32_t vfg[Q];
-        float bao[Q], cnt[Q];   // box base of AO; foreground pixels of the window
         {
-            uint4 fb[Q];
+            float f[8];
+            float sa = 0.0f;
+            T.vfg = 0u;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                float f[8];
-                float sa = 0.0f;
-                vfg[q] = 0u;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int n = 8 * g + e;
-                    f[e] = 0.0f;
-                    vq[q][e] = 0u;
-                    if (valid[q] && n < 25) {
-                        const uint32_t nq = dn_nb_index(F, px[q], py[q], n);
-                        const float d = F.pix[nq].x;
-                        vq[q][e] = nq;
-                        if (d >= 0.0f) {
-                            vfg[q] |= 1u << e;
-                            sa += F.ao_frac[nq];
-                            const float rel = dc[q] > 0.0f ? (d - dc[q]) / dc[q] * 16.0f : 0.0f;
-                            f[e] = rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel);
-                        } else {
-                            f[e] = 1.0f;
-                        }
-                    }
-                }
-                if (g == 3 && valid[q]) {   // (AO, normal) of neighbour 24 in elements 1..4
-                    const uint32_t nq = vq[q][0];
-                    const float4 v = F.pix[nq];
-                    const bool fg = (vfg[q] & 1u) != 0u;
-                    f[1] = fg ? F.ao_frac[nq] : 0.0f;
-                    f[2] = v.y; f[3] = v.z; f[4] = v.w;
-                }
-                fb[q] = make_uint4(bf16_pack2(f[0], f[1]), bf16_pack2(f[2], f[3]), bf16_pack2(f[4], f[5]),
-                                   bf16_pack2(f[6], f[7]));
-                // sums over the four lane groups of a pixel column: ((g0 + g1) + (g2 + g3))
-                sa += __shfl_xor(sa, 16);
-                sa += __shfl_xor(sa, 32);
-                int c = __popc(vfg[q]);
-                c += __shfl_xor(c, 16);
-                c += __shfl_xor(c, 32);
-                cnt[q] = (float)(c ? c : 1);
-                bao[q] = sa / cnt[q];
+            for (int e = 0; e < 8; ++e) {
+                const bool in = T.valid && 8 * g + e < 25;
+                const bool fg = in && dv[e] >= 0.0f;
+                T.vq[e] = in ? nv[e] : 0u;
+                T.vfg |= (fg ? 1u : 0u) << e;
+                sa += fg ? av[e] : 0.0f;
+                const float rel = dc > 0.0f ? (dv[e] - dc) / dc * 16.0f : 0.0f;
+                f[e] = !in ? 0.0f : (!fg ? 1.0f : (rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel)));
             }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const bf16x8 a = as_bf16x8(sfrag[(kIFG + t * 4 + 3) * 64 + lane]);
-#pragma unroll
-                for (int q = 0; q < Q; ++q) ag[q][t] = mfma(a, as_bf16x8(fb[q]), ag[q][t]);
+            if (g == 3 && T.valid) {   // (AO, normal) of neighbour 24 in elements 1..4
+                const bool fg = (T.vfg & 1u) != 0u;
+                f[1] = fg ? av[0] : 0.0f;
+                f[2] = v24.y; f[3] = v24.z; f[4] = v24.w;
             }
+            const bf16x8 fb = as_bf16x8(make_uint4(bf16_pack2(f[0], f[1]), bf16_pack2(f[2], f[3]),
+                                                   bf16_pack2(f[4], f[5]), bf16_pack2(f[6], f[7])));
+            // sums over the four lane groups of a pixel column: ((g0 + g1) + (g2 + g3))
+            sa += __shfl_xor(sa, 16);
+            sa += __shfl_xor(sa, 32);
+            int c = __popc(T.vfg);
+            c += __shfl_xor(c, 16);
+            c += __shfl_xor(c, 32);
+            T.cnt = (float)(c ? c : 1);
+            T.bao = sa / T.cnt;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) ag[t] = mfma(as_bf16x8(sfrag[(kIFG + t * 4 + 3) * 64 + lane]), fb, ag[t]);
         }
-        // ---- per light: visibility K-step, layers 2..5
-        float aosum[Q];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) aosum[q] = 0.0f;
+        // ---- per light group: visibility K-step, layers 2..5
+        float aosum = 0.0f;
         const uint32_t nl = A.lights;
-        for (uint32_t l = 0; l < nl; ++l) {
-            f32x4 acc[Q][4];
-            uint4 hb[Q][2];
-            // the weight fragments are read from LDS at every use: an opaque
-            // lane offset keeps the compiler from holding all 30 of them in
-            // registers across the light loop (which would allow one wave per SIMD)
-            uint32_t ln = (uint32_t)lane;
-            asm volatile("" : "+v"(ln));
-            float bvis[Q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                uint32_t w[4] = {0u, 0u, 0u, 0u};
-                float sv = 0.0f;
-                if (L) {
-                    const uint8_t* sh = F.shadow + (size_t)l * npix;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const bool on = (vfg[q] >> e) & 1u;
-                        const uint32_t s8 = on ? (uint32_t)sh[vq[q][e]] : 0u;
-                        w[e >> 1] |= (on ? (uint32_t)svis[s8] : 0u) << (16 * (e & 1));
-                        if (on) sv += svisf[s8];
-                    }
-                }
-                sv += __shfl_xor(sv, 16);
-                sv += __shfl_xor(sv, 32);
-                bvis[q] = sv / cnt[q];
-                const bf16x8 bv = as_bf16x8(make_uint4(w[0], w[1], w[2], w[3]));
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    acc[q][t] = mfma(as_bf16x8(sfrag[(kIFV + t) * 64 + ln]), bv, ag[q][t]);
-            }
-#pragma unroll
-            for (int q = 0; q < Q; ++q) dn_relu_pack(acc[q], hb[q]);
-#pragma unroll
-            for (int k = 1; k < 4; ++k) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const f32x4 bk = *reinterpret_cast<const f32x4*>(&sbias[k * kHid + 16 * t + 4 * g]);
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) acc[q][t] = bk;
-                }
-                const int fb = k == 1 ? kIF2 : (k == 2 ? kIF3 : kIF4);
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const bf16x8 a = as_bf16x8(sfrag[(fb + t * 2 + s) * 64 + ln]);
-#pragma unroll
-                        for (int q = 0; q < Q; ++q) acc[q][t] = mfma(a, as_bf16x8(hb[q][s]), acc[q][t]);
-                    }
-                if (k < 3) {
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) dn_relu_pack(acc[q], hb[q]);
-                }
-            }
-            // layer 5 in FP32 on layer 4's activations
-            float out[Q][2];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                f32x4 h[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) h[t][i] = relu_bits(acc[q][t][i]);
-                // the opaque offset (zero) keeps the weights in LDS rather than registers
-                dn_out_fp32(h, sw5 + (ln - (uint32_t)lane), g, out[q][0], out[q][1]);
-            }
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (g == 0 && inb[q]) {
-                    aosum[q] += valid[q] ? (out[q][0] + b50) + bao[q] : 0.0f;
-                    if (L)
-                        A.vis_out[(size_t)l * npix + pp[q]] = valid[q] ? clamp01f((out[q][1] + b51) + bvis[q]) : 0.0f;
-                }
-        }
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (g == 0 && inb[q]) A.ao_out[pp[q]] = clamp01f(aosum[q] / (float)nl);
+        uint32_t l = 0;
+        for (; l + 3 <= nl; l += 3) dn_light_group<3>(A, sfrag, sbias, sw5, svis, svisf, l, lane, g, ag, T, b50, b51, aosum);
+        if (nl - l == 2)
+            dn_light_group<2>(A, sfrag, sbias, sw5, svis, svisf, l, lane, g, ag, T, b50, b51, aosum);
+        else if (nl - l == 1)
+            dn_light_group<1>(A, sfrag, sbias, sw5, svis, svisf, l, lane, g, ag, T, b50, b51, aosum);
+        if (g == 0 && T.inb) A.ao_out[T.pp] = clamp01f(aosum / (float)nl);
     }
 }
 
@@ -1074,13 +1103,13 @@ luzrt_status luzrt_denoise(luzrt_ctx* ctx, luzrt_denoiser* d, const luzrt_dn_fra
     // as many groups as are resident at once (48 KB of LDS and the registers
     // of 8 waves each), so that no group waits for another to finish
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dn_infer<kInferTiles>, kInferBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dn_infer, kInferBlock, 0) !=
             hipSuccess || per_cu < 1)
         per_cu = 1;
-    uint32_t groups = cdiv(A.ntiles, (kInferBlock / 64) * kInferTiles);
+    uint32_t groups = cdiv(A.ntiles, kInferBlock / 64);
     const uint32_t maxg = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1) * (uint32_t)per_cu;
     if (groups > maxg) groups = maxg;
-    k_dn_infer<kInferTiles><<<groups, kInferBlock, 0, ctx->stream>>>(A);
+    k_dn_infer<<<groups, kInferBlock, 0, ctx->stream>>>(A);
     stage_end(ctx, kStageDenoise);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
