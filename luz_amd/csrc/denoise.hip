@@ -542,17 +542,48 @@ struct InferTile {
     bool valid, inb;
 };
 
+// Sum of a value over the four lane groups (rows of 16 lanes) of a pixel
+// column, as ((g0 + g1) + (g2 + g3)), in every lane: v_permlane16_swap pairs
+// rows 0/1 and 2/3, v_permlane32_swap the two halves (VALU exchanges, no LDS
+// round trip).
+__device__ inline float col_sum(float v) {
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    const float s = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+__device__ inline uint32_t col_sum(uint32_t v) {
+    const auto a = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    const uint32_t s = a[0] + a[1];
+    const auto b = __builtin_amdgcn_permlane32_swap(s, s, false, false);
+    return b[0] + b[1];
+}
+
+// The shadow bytes of lights l0 .. l0+2 (clamped to the last light) at the
+// tile's neighbour indices.
+__device__ inline void dn_shadow_bytes(const DnFrameDev& F, uint32_t l0, const uint32_t (&vq)[8],
+                                       uint32_t (&s8)[3][8]) {
+    const uint32_t npix = F.width * F.height;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const uint32_t l = l0 + p < F.num_lights ? l0 + p : F.num_lights - 1;
+        const uint8_t* sh = F.shadow + (size_t)l * npix;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s8[p][e] = sh[vq[e]];
+    }
+}
+
 // Lights l0 .. l0+P-1 of one 16-pixel tile: the visibility K-step FV on top of
 // the light-independent accumulators `ag`, layers 2..4 on MFMA and layer 5 in
 // FP32.  Every weight fragment, bias and layer-5 weight read from LDS feeds the
-// products of all P lights, and the group's 8 P shadow bytes are loaded
-// together before the first is used.  AO outputs are added to `aosum` in light
-// order; the visibilities are written.
+// products of all P lights; the group's shadow bytes `s8` were loaded by the
+// caller one group ahead.  AO outputs are added to `aosum` in light order; the
+// visibilities are written.
 template <int P>
 __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, const float* sbias,
-                                      const float* sw5, const uint16_t* svis, const float* svisf, uint32_t l0,
-                                      int lane, int g, const f32x4 (&ag)[4], const InferTile& T, float b50,
-                                      float b51, float& aosum) {
+                                      const float* sw5, const uint2* svt, uint32_t l0, int lane, int g,
+                                      const f32x4 (&ag)[4], const InferTile& T, const uint32_t (&s8)[3][8],
+                                      float b50, float b51, float& aosum) {
     const DnFrameDev& F = A.F;
     const uint32_t npix = F.width * F.height;
     const uint32_t L = F.num_lights;
@@ -561,15 +592,6 @@ __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, co
     // across the groups (which would leave one wave per SIMD)
     uint32_t ln = (uint32_t)lane;
     asm volatile("" : "+v"(ln));
-    uint32_t s8[P][8];
-    if (L) {
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const uint8_t* sh = F.shadow + (size_t)(l0 + p) * npix;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) s8[p][e] = sh[T.vq[e]];
-        }
-    }
     f32x4 acc[P][4];
     uint4 hb[P][2];
     float bvis[P];
@@ -578,16 +600,18 @@ __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, co
         uint32_t w[4] = {0u, 0u, 0u, 0u};
         float sv = 0.0f;
         if (L) {
+            uint2 tv[8];   // (BF16 bits, FP32 bits) of 1 - s / shadow_n
+#pragma unroll
+            for (int e = 0; e < 8; ++e) tv[e] = svt[s8[p][e]];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const bool on = (T.vfg >> e) & 1u;
-                w[e >> 1] |= (on ? (uint32_t)svis[s8[p][e]] : 0u) << (16 * (e & 1));
-                if (on) sv += svisf[s8[p][e]];
+                // all-ones for a foreground neighbour, else zero (+0.0f, which the sum keeps)
+                const uint32_t on = 0u - ((T.vfg >> e) & 1u);
+                w[e >> 1] |= (tv[e].x & on) << (16 * (e & 1));
+                sv += __uint_as_float(tv[e].y & on);
             }
         }
-        sv += __shfl_xor(sv, 16);
-        sv += __shfl_xor(sv, 32);
-        bvis[p] = sv / T.cnt;
+        bvis[p] = col_sum(sv) / T.cnt;
         hb[p][0] = make_uint4(w[0], w[1], w[2], w[3]);   // the visibility B operand
     }
 #pragma unroll
@@ -646,10 +670,8 @@ __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, co
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        o0[p] += __shfl_xor(o0[p], 16);
-        o1[p] += __shfl_xor(o1[p], 16);
-        o0[p] += __shfl_xor(o0[p], 32);
-        o1[p] += __shfl_xor(o1[p], 32);
+        o0[p] = col_sum(o0[p]);
+        o1[p] = col_sum(o1[p]);
         if (g == 0 && T.inb) {
             aosum += T.valid ? (o0[p] + b50) + T.bao : 0.0f;
             if (L)
@@ -667,8 +689,7 @@ __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, co
 __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     __shared__ uint4 sfrag[kFragInfer * 64];
     __shared__ __align__(16) float sbias[4 * kHid + 4];
-    __shared__ uint16_t svis[256];   // BF16 of 1 - s / shadow_n
-    __shared__ float svisf[256];     // 1 - s / shadow_n
+    __shared__ uint2 svt[256];   // (BF16 bits, FP32 bits) of 1 - s / shadow_n
     __shared__ __align__(16) float sw5[kOut * kHid];
     for (int i = threadIdx.x; i < 24 * 64; i += kInferBlock) sfrag[kIF2 * 64 + i] = A.frags[kF2 * 64 + i];
     for (int i = threadIdx.x; i < kOut * kHid; i += kInferBlock) sw5[i] = A.params[oW5 + i];
@@ -677,8 +698,8 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
         sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
     const DnFrameDev& F = A.F;
     for (int i = threadIdx.x; i < 256; i += kInferBlock) {
-        svisf[i] = 1.0f - (float)i / F.shadow_n;
-        svis[i] = (uint16_t)bf16_bits(svisf[i]);
+        const float v = 1.0f - (float)i / F.shadow_n;
+        svt[i] = make_uint2(bf16_bits(v), __float_as_uint(v));
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
@@ -773,12 +794,9 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
             const bf16x8 fb = as_bf16x8(make_uint4(bf16_pack2(f[0], f[1]), bf16_pack2(f[2], f[3]),
                                                    bf16_pack2(f[4], f[5]), bf16_pack2(f[6], f[7])));
             // sums over the four lane groups of a pixel column: ((g0 + g1) + (g2 + g3))
-            sa += __shfl_xor(sa, 16);
-            sa += __shfl_xor(sa, 32);
-            int c = __popc(T.vfg);
-            c += __shfl_xor(c, 16);
-            c += __shfl_xor(c, 32);
-            T.cnt = (float)(c ? c : 1);
+            sa = col_sum(sa);
+            const uint32_t c = col_sum((uint32_t)__popc(T.vfg));
+            T.cnt = (float)(c ? c : 1u);
             T.bao = sa / T.cnt;
 #pragma unroll
             for (int t = 0; t < 4; ++t) ag[t] = mfma(as_bf16x8(sfrag[(kIFG + t * 4 + 3) * 64 + lane]), fb, ag[t]);
@@ -786,12 +804,18 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
         // ---- per light group: visibility K-step, layers 2..5
         float aosum = 0.0f;
         const uint32_t nl = A.lights;
+        uint32_t sb[3][8];
+        if (L) dn_shadow_bytes(F, 0, T.vq, sb);
         uint32_t l = 0;
-        for (; l + 3 <= nl; l += 3) dn_light_group<3>(A, sfrag, sbias, sw5, svis, svisf, l, lane, g, ag, T, b50, b51, aosum);
+        for (; l + 3 <= nl; l += 3) {
+            if (L && l > 0) dn_shadow_bytes(F, l, T.vq, sb);
+            dn_light_group<3>(A, sfrag, sbias, sw5, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
+        }
+        if (L && l > 0 && l < nl) dn_shadow_bytes(F, l, T.vq, sb);
         if (nl - l == 2)
-            dn_light_group<2>(A, sfrag, sbias, sw5, svis, svisf, l, lane, g, ag, T, b50, b51, aosum);
+            dn_light_group<2>(A, sfrag, sbias, sw5, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
         else if (nl - l == 1)
-            dn_light_group<1>(A, sfrag, sbias, sw5, svis, svisf, l, lane, g, ag, T, b50, b51, aosum);
+            dn_light_group<1>(A, sfrag, sbias, sw5, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
         if (g == 0 && T.inb) A.ao_out[T.pp] = clamp01f(aosum / (float)nl);
     }
 }
