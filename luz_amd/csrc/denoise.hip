@@ -284,15 +284,23 @@ __device__ inline void dn_hidden_epilogue(const f32x4 (&acc)[4], const float* bi
     hb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
 }
 
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
 // ReLU of a hidden layer's 4x4 tiles (bias already in the accumulators) into
-// the next layer's B fragments (inference).
+// the next layer's B fragments (inference).  The ReLU is taken after rounding,
+// on each BF16 half as a signed 16-bit integer (v_pk_max_i16 with 0): a value
+// with its sign bit set, -0 and negative NaN included, becomes +0, as
+// relu_bits before the rounding gives, and every other value is kept.
 __device__ inline void dn_relu_pack(const f32x4 (&acc)[4], uint4 (&hb)[2]) {
     uint32_t w[2][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; i += 2)
-            w[t >> 1][((t & 1) * 4 + i) >> 1] = bf16_pack2(relu_bits(acc[t][i]), relu_bits(acc[t][i + 1]));
+        for (int i = 0; i < 4; i += 2) {
+            const s16x2 h = __builtin_bit_cast(s16x2, bf16_pack2(acc[t][i], acc[t][i + 1]));
+            w[t >> 1][((t & 1) * 4 + i) >> 1] =
+                __builtin_bit_cast(uint32_t, __builtin_elementwise_max(h, (s16x2){0, 0}));
+        }
     hb[0] = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
     hb[1] = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
 }
@@ -536,7 +544,8 @@ __device__ inline uint32_t dn_nb_index(const DnFrameDev& F, uint32_t x, uint32_t
 // of the tile and neighbours 8g + e (e < 8) of its 5x5 window.
 struct InferTile {
     uint32_t vq[8];   // neighbour indices (0 off the window and for invalid pixels)
-    uint32_t vfg;     // foreground bits of those neighbours
+    uint32_t bo[8];   // visibility table offset: 0 for a foreground neighbour, else
+                      // 2048 bytes (the table's zero half)
     float cnt, bao;   // foreground pixels of the window, box base of AO
     uint32_t pp;      // pixel index (0 beyond the frame)
     bool valid, inb;
@@ -559,6 +568,18 @@ __device__ inline uint32_t col_sum(uint32_t v) {
     return b[0] + b[1];
 }
 
+// Buffer resource over `bytes` bytes at p for raw loads (32-bit offsets in a
+// VGPR, no 64-bit address arithmetic per load).
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+
+// The prepass record (camera distance or -1, normal) of pixel q.
+__device__ inline float4 load_pix(__amdgpu_buffer_rsrc_t r, uint32_t q) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, q * 16u, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
 // The shadow bytes of lights l0 .. l0+2 (clamped to the last light) at the
 // tile's neighbour indices.
 __device__ inline void dn_shadow_bytes(const DnFrameDev& F, uint32_t l0, const uint32_t (&vq)[8],
@@ -567,9 +588,9 @@ __device__ inline void dn_shadow_bytes(const DnFrameDev& F, uint32_t l0, const u
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const uint32_t l = l0 + p < F.num_lights ? l0 + p : F.num_lights - 1;
-        const uint8_t* sh = F.shadow + (size_t)l * npix;
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(F.shadow + (size_t)l * npix, npix);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) s8[p][e] = sh[vq[e]];
+        for (int e = 0; e < 8; ++e) s8[p][e] = __builtin_amdgcn_raw_buffer_load_b8(r, vq[e], 0, 0);
     }
 }
 
@@ -600,15 +621,17 @@ __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, co
         uint32_t w[4] = {0u, 0u, 0u, 0u};
         float sv = 0.0f;
         if (L) {
-            uint2 tv[8];   // (BF16 bits, FP32 bits) of 1 - s / shadow_n
+            // (BF16 bits, FP32 bits) of 1 - s / shadow_n, or zeros (+0.0f, which
+            // the sum keeps) for a neighbour off the foreground
+            uint2 tv[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) tv[e] = svt[s8[p][e]];
+            for (int e = 0; e < 8; ++e)
+                tv[e] = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(svt) +
+                                                        ((s8[p][e] << 3) | T.bo[e]));
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                // all-ones for a foreground neighbour, else zero (+0.0f, which the sum keeps)
-                const uint32_t on = 0u - ((T.vfg >> e) & 1u);
-                w[e >> 1] |= (tv[e].x & on) << (16 * (e & 1));
-                sv += __uint_as_float(tv[e].y & on);
+                w[e >> 1] |= tv[e].x << (16 * (e & 1));
+                sv += __uint_as_float(tv[e].y);
             }
         }
         bvis[p] = col_sum(sv) / T.cnt;
@@ -689,7 +712,7 @@ __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, co
 __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     __shared__ uint4 sfrag[kFragInfer * 64];
     __shared__ __align__(16) float sbias[4 * kHid + 4];
-    __shared__ uint2 svt[256];   // (BF16 bits, FP32 bits) of 1 - s / shadow_n
+    __shared__ uint2 svt[512];   // (BF16 bits, FP32 bits) of 1 - s / shadow_n; zeros
     __shared__ __align__(16) float sw5[kOut * kHid];
     for (int i = threadIdx.x; i < 24 * 64; i += kInferBlock) sfrag[kIF2 * 64 + i] = A.frags[kF2 * 64 + i];
     for (int i = threadIdx.x; i < kOut * kHid; i += kInferBlock) sw5[i] = A.params[oW5 + i];
@@ -700,6 +723,7 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     for (int i = threadIdx.x; i < 256; i += kInferBlock) {
         const float v = 1.0f - (float)i / F.shadow_n;
         svt[i] = make_uint2(bf16_bits(v), __float_as_uint(v));
+        svt[256 + i] = make_uint2(0u, 0u);
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
@@ -711,12 +735,13 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     const uint32_t wave = lb * (kInferBlock / 64) + (threadIdx.x >> 6);
     const uint32_t nwaves = nb * (kInferBlock / 64);
     const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
+    const __amdgpu_buffer_rsrc_t rpix = buf_rsrc(F.pix, npix * 16u), rao = buf_rsrc(F.ao_frac, npix * 4u);
     for (uint32_t tile = wave; tile < A.ntiles; tile += nwaves) {
         InferTile T;
         const uint32_t p = tile * 16u + (uint32_t)j;
         T.inb = p < npix;
         T.pp = T.inb ? p : 0u;
-        const float d = F.pix[T.pp].x;
+        const float d = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rpix, T.pp * 16u, 0, 0));
         T.valid = T.inb && d >= 0.0f;
         const float dc = T.valid ? d : 0.0f;
         const uint32_t px = T.pp % F.width, py = T.pp / F.width;
@@ -744,14 +769,14 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
         for (int e = 0; e < 8; ++e) nv[e] = dn_nb_index(F, px, py, 8 * g + e < 25 ? 8 * g + e : 0);
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-            va[i] = F.pix[na[i]];
-            aa[i] = F.ao_frac[na[i]];
+            va[i] = load_pix(rpix, na[i]);
+            aa[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rao, na[i] * 4u, 0, 0));
         }
-        const float4 v24 = F.pix[nv[0]];   // neighbour 24 on lane group 3
+        const float4 v24 = load_pix(rpix, nv[0]);   // neighbour 24 on lane group 3
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            dv[e] = e == 0 ? v24.x : F.pix[nv[e]].x;
-            av[e] = F.ao_frac[nv[e]];
+            dv[e] = e == 0 ? v24.x : __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rpix, nv[e] * 16u, 0, 0));
+            av[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rao, nv[e] * 4u, 0, 0));
         }
         f32x4 ag[4];
 #pragma unroll
@@ -775,19 +800,20 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
         {
             float f[8];
             float sa = 0.0f;
-            T.vfg = 0u;
+            uint32_t vfg = 0u;   // foreground bits of the neighbours 8g + e
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const bool in = T.valid && 8 * g + e < 25;
                 const bool fg = in && dv[e] >= 0.0f;
                 T.vq[e] = in ? nv[e] : 0u;
-                T.vfg |= (fg ? 1u : 0u) << e;
+                T.bo[e] = fg ? 0u : 2048u;
+                vfg |= (fg ? 1u : 0u) << e;
                 sa += fg ? av[e] : 0.0f;
                 const float rel = dc > 0.0f ? (dv[e] - dc) / dc * 16.0f : 0.0f;
                 f[e] = !in ? 0.0f : (!fg ? 1.0f : (rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel)));
             }
             if (g == 3 && T.valid) {   // (AO, normal) of neighbour 24 in elements 1..4
-                const bool fg = (T.vfg & 1u) != 0u;
+                const bool fg = (vfg & 1u) != 0u;
                 f[1] = fg ? av[0] : 0.0f;
                 f[2] = v24.y; f[3] = v24.z; f[4] = v24.w;
             }
@@ -795,7 +821,7 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
                                                    bf16_pack2(f[4], f[5]), bf16_pack2(f[6], f[7])));
             // sums over the four lane groups of a pixel column: ((g0 + g1) + (g2 + g3))
             sa = col_sum(sa);
-            const uint32_t c = col_sum((uint32_t)__popc(T.vfg));
+            const uint32_t c = col_sum((uint32_t)__popc(vfg));
             T.cnt = (float)(c ? c : 1u);
             T.bao = sa / T.cnt;
 #pragma unroll
