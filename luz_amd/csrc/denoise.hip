@@ -89,10 +89,13 @@ constexpr int kBOff[5] = {oB1, oB2, oB3, oB4, oB5};
 //   FG, K-step 3: element e is the relative depth of neighbour 8g + e (< 25);
 //       lane group 3 also holds (AO, n.x, n.y, n.z) of neighbour 24 in e = 1..4;
 //   FV: element e is the visibility of neighbour 8g + e (< 25).
+// F5(s) s < 2 holds layer 5 for inference, where it runs on MFMA: output row 0
+// and 1 are the BF16 rounding of W5's rows, rows 2 and 3 the BF16 rounding of
+// what that leaves (W5 - hi), rows 4..15 zero, K in the hidden-layer order.
 constexpr int kF1 = 0, kF2 = 20, kF3 = 28, kF4 = 36, kG2 = 44, kG3 = 52, kG4 = 60,
-              kG5 = 68, kFG = 72, kFV = 88, kFragTrain = 72, kFragAll = 92;
-// inference fragments in LDS: F2..F4 (24), then FG and FV (20)
-constexpr int kIF2 = 0, kIF3 = 8, kIF4 = 16, kIFG = 24, kIFV = 40, kFragInfer = 44;
+              kG5 = 68, kFG = 72, kFV = 88, kF5 = 92, kFragTrain = 72, kFragAll = 94;
+// inference fragments in LDS: F2..F4 (24), then FG and FV (20), then F5 (2)
+constexpr int kIF2 = 0, kIF3 = 8, kIF4 = 16, kIFG = 24, kIFV = 40, kIF5 = 44, kFragInfer = 46;
 
 // Original feature index (6 n + channel) of element e of lane group g in
 // inference K-step s (s == 4: the visibility step), or -1 for padding.
@@ -595,14 +598,18 @@ __device__ inline void dn_shadow_bytes(const DnFrameDev& F, uint32_t l0, const u
 }
 
 // Lights l0 .. l0+P-1 of one 16-pixel tile: the visibility K-step FV on top of
-// the light-independent accumulators `ag`, layers 2..4 on MFMA and layer 5 in
-// FP32.  Every weight fragment, bias and layer-5 weight read from LDS feeds the
-// products of all P lights; the group's shadow bytes `s8` were loaded by the
-// caller one group ahead.  AO outputs are added to `aosum` in light order; the
-// visibilities are written.
+// the light-independent accumulators `ag`, then layers 2..5 on MFMA.  Every
+// weight fragment and bias read from LDS feeds the products of all P lights;
+// the caller loaded the group's shadow bytes `s8`.  AO outputs are added to
+// `aosum` in light order; the visibilities are written.
+//
+// Layer 5 takes layer 4's activations in BF16, as every other layer takes its
+// inputs, and W5 as the sum of two BF16 parts (F5), so only the activation
+// rounding (relative 2^-9 per term) separates it from the FP32 layer 5 of
+// training.
 template <int P>
 __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, const float* sbias,
-                                      const float* sw5, const uint2* svt, uint32_t l0, int lane, int g,
+                                      const uint2* svt, uint32_t l0, int lane, int g,
                                       const f32x4 (&ag)[4], const InferTile& T, const uint32_t (&s8)[3][8],
                                       float b50, float b51, float& aosum) {
     const DnFrameDev& F = A.F;
@@ -665,40 +672,26 @@ __device__ inline void dn_light_group(const InferArgs& A, const uint4* sfrag, co
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p][t] = mfma(a, as_bf16x8(hb[p][s]), acc[p][t]);
             }
-        if (k < 3) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) dn_relu_pack(acc[p], hb[p]);
-        }
+        for (int p = 0; p < P; ++p) dn_relu_pack(acc[p], hb[p]);
     }
-    // layer 5 in FP32 on layer 4's activations: per lane in (t, i) order, then
-    // over the lane groups as ((g0 + g1) + (g2 + g3)), the order of dn_out_fp32
-    float o0[P], o1[P];
+    // layer 5: lane group 0 receives rows 0..3 of its pixel column, the high
+    // and low parts of the two outputs
+    f32x4 y[P];
+    {
+        const bf16x8 a0 = as_bf16x8(sfrag[(kIF5 + 0) * 64 + ln]), a1 = as_bf16x8(sfrag[(kIF5 + 1) * 64 + ln]);
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-        o0[p] = 0.0f;
-        o1[p] = 0.0f;
-    }
+        for (int p = 0; p < P; ++p) y[p] = mfma(a0, as_bf16x8(hb[p][0]), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const f32x4 wa = *reinterpret_cast<const f32x4*>(&sw5[16 * t + 4 * g]);
-        const f32x4 wb = *reinterpret_cast<const f32x4*>(&sw5[kHid + 16 * t + 4 * g]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const float h = relu_bits(acc[p][t][i]);
-                o0[p] = fmaf(wa[i], h, o0[p]);
-                o1[p] = fmaf(wb[i], h, o1[p]);
-            }
+        for (int p = 0; p < P; ++p) y[p] = mfma(a1, as_bf16x8(hb[p][1]), y[p]);
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        o0[p] = col_sum(o0[p]);
-        o1[p] = col_sum(o1[p]);
         if (g == 0 && T.inb) {
-            aosum += T.valid ? (o0[p] + b50) + T.bao : 0.0f;
+            const float o0 = y[p][0] + y[p][2], o1 = y[p][1] + y[p][3];
+            aosum += T.valid ? (o0 + b50) + T.bao : 0.0f;
             if (L)
-                A.vis_out[(size_t)(l0 + p) * npix + T.pp] = T.valid ? clamp01f((o1[p] + b51) + bvis[p]) : 0.0f;
+                A.vis_out[(size_t)(l0 + p) * npix + T.pp] = T.valid ? clamp01f((o1 + b51) + bvis[p]) : 0.0f;
         }
     }
 }
@@ -713,10 +706,9 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     __shared__ uint4 sfrag[kFragInfer * 64];
     __shared__ __align__(16) float sbias[4 * kHid + 4];
     __shared__ uint2 svt[512];   // (BF16 bits, FP32 bits) of 1 - s / shadow_n; zeros
-    __shared__ __align__(16) float sw5[kOut * kHid];
     for (int i = threadIdx.x; i < 24 * 64; i += kInferBlock) sfrag[kIF2 * 64 + i] = A.frags[kF2 * 64 + i];
-    for (int i = threadIdx.x; i < kOut * kHid; i += kInferBlock) sw5[i] = A.params[oW5 + i];
     for (int i = threadIdx.x; i < 20 * 64; i += kInferBlock) sfrag[kIFG * 64 + i] = A.frags[kFG * 64 + i];
+    for (int i = threadIdx.x; i < 2 * 64; i += kInferBlock) sfrag[kIF5 * 64 + i] = A.frags[kF5 * 64 + i];
     for (int i = threadIdx.x; i < 4 * kHid + kOut; i += kInferBlock)
         sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
     const DnFrameDev& F = A.F;
@@ -835,13 +827,13 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
         uint32_t l = 0;
         for (; l + 3 <= nl; l += 3) {
             if (L && l > 0) dn_shadow_bytes(F, l, T.vq, sb);
-            dn_light_group<3>(A, sfrag, sbias, sw5, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
+            dn_light_group<3>(A, sfrag, sbias, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
         }
         if (L && l > 0 && l < nl) dn_shadow_bytes(F, l, T.vq, sb);
         if (nl - l == 2)
-            dn_light_group<2>(A, sfrag, sbias, sw5, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
+            dn_light_group<2>(A, sfrag, sbias, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
         else if (nl - l == 1)
-            dn_light_group<1>(A, sfrag, sbias, sw5, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
+            dn_light_group<1>(A, sfrag, sbias, svt, l, lane, g, ag, T, sb, b50, b51, aosum);
         if (g == 0 && T.inb) A.ao_out[T.pp] = clamp01f(aosum / (float)nl);
     }
 }
@@ -941,10 +933,14 @@ __global__ void k_dn_pack(const float* P, uint16_t* frags) {
     } else if (f < kFG) {
         const int t = f - kG5, o = khid(0, g, e);
         if (o < kOut) val = P[oW5 + o * kHid + (16 * t + j)];
-    } else {
+    } else if (f < kF5) {
         const int t = f < kFV ? (f - kFG) / 4 : f - kFV, s = f < kFV ? (f - kFG) % 4 : 4;
         const int k = infer_feature(s, g, e);
         if (k >= 0) val = P[oW1 + (16 * t + j) * kIn + k];
+    } else if (j < 4) {
+        const float w = P[oW5 + (j & 1) * kHid + khid(f - kF5, g, e)];
+        const float hi = bf16_val(bf16_bits(w));
+        val = j < 2 ? hi : w - hi;
     }
     frags[id] = (uint16_t)bf16_bits(val);
 }
