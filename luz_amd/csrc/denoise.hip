@@ -83,32 +83,30 @@ constexpr int kBOff[5] = {oB1, oB2, oB3, oB4, oB5};
 // Layer 5 (64 -> 2) runs in FP32 (dn_out_fp32); G5 holds W5^T for the backward pass.
 // FG and FV hold layer 1 again in the inference order, which splits the
 // features of a (pixel, light) sample into the 125 that do not depend on the
-// light (one product per pixel) and the 25 visibilities (one K-step per light):
-//   FG, K-steps s < 3, slot k = 4s + g: element e < 4 is channel (AO, n.x, n.y,
-//       n.z)[e] of neighbour 2k, e >= 4 the same of neighbour 2k + 1;
-//   FG, K-step 3: element e is the relative depth of neighbour 8g + e (< 25);
-//       lane group 3 also holds (AO, n.x, n.y, n.z) of neighbour 24 in e = 1..4;
-//   FV: element e is the visibility of neighbour 8g + e (< 25).
+// light (one product per pixel) and the 25 visibilities (one K-step per light).
+// Lane group g only ever reads neighbours 8g .. 8g + 7 (those below 25):
+//   FG, K-steps s < 4: element e is channel (AO, n.x, n.y, n.z)[e % 4] of
+//       neighbour 8g + 2s + e / 4;
+//   FG, K-step 4: element e is the relative depth of neighbour 8g + e;
+//   FV: element e is the visibility of neighbour 8g + e.
 // F5(s) s < 2 holds layer 5 for inference, where it runs on MFMA: output row 0
 // and 1 are the BF16 rounding of W5's rows, rows 2 and 3 the BF16 rounding of
 // what that leaves (W5 - hi), rows 4..15 zero, K in the hidden-layer order.
 constexpr int kF1 = 0, kF2 = 20, kF3 = 28, kF4 = 36, kG2 = 44, kG3 = 52, kG4 = 60,
-              kG5 = 68, kFG = 72, kFV = 88, kF5 = 92, kFragTrain = 72, kFragAll = 94;
-// inference fragments in LDS: F2..F4 (24), then FG and FV (20), then F5 (2)
-constexpr int kIF2 = 0, kIF3 = 8, kIF4 = 16, kIFG = 24, kIFV = 40, kIF5 = 44, kFragInfer = 46;
+              kG5 = 68, kFG = 72, kFV = 92, kF5 = 96, kFragTrain = 72, kFragAll = 98;
+// inference fragments in LDS: F2..F4 (24), then FG and FV (24), then F5 (2)
+constexpr int kIF2 = 0, kIF3 = 8, kIF4 = 16, kIFG = 24, kIFV = 44, kIF5 = 48, kFragInfer = 50;
 
 // Original feature index (6 n + channel) of element e of lane group g in
-// inference K-step s (s == 4: the visibility step), or -1 for padding.
+// inference K-step s (s == 5: the visibility step), or -1 for padding.
 __host__ __device__ inline int infer_feature(int s, int g, int e) {
-    if (s < 3) {
-        const int n = 2 * (4 * s + g) + (e >> 2), ch = e & 3;
-        return 6 * n + (ch == 0 ? 0 : ch + 2);
+    if (s < 4) {
+        const int n = 8 * g + 2 * s + (e >> 2), ch = e & 3;
+        return n < 25 ? 6 * n + (ch == 0 ? 0 : ch + 2) : -1;
     }
     const int n = 8 * g + e;
-    if (s == 4) return n < 25 ? 6 * n + 1 : -1;
-    if (n < 25) return 6 * n + 2;
-    if (g == 3 && e >= 1 && e <= 4) return 6 * 24 + (e == 1 ? 0 : e + 1);
-    return -1;
+    if (n >= 25) return -1;
+    return s == 4 ? 6 * n + 2 : 6 * n + 1;
 }
 
 __host__ __device__ inline int khid(int s, int g, int e) { return 16 * (2 * s + (e >> 2)) + 4 * g + (e & 3); }
@@ -707,7 +705,7 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     __shared__ __align__(16) float sbias[4 * kHid + 4];
     __shared__ uint2 svt[512];   // (BF16 bits, FP32 bits) of 1 - s / shadow_n; zeros
     for (int i = threadIdx.x; i < 24 * 64; i += kInferBlock) sfrag[kIF2 * 64 + i] = A.frags[kF2 * 64 + i];
-    for (int i = threadIdx.x; i < 20 * 64; i += kInferBlock) sfrag[kIFG * 64 + i] = A.frags[kFG * 64 + i];
+    for (int i = threadIdx.x; i < 24 * 64; i += kInferBlock) sfrag[kIFG * 64 + i] = A.frags[kFG * 64 + i];
     for (int i = threadIdx.x; i < 2 * 64; i += kInferBlock) sfrag[kIF5 * 64 + i] = A.frags[kF5 * 64 + i];
     for (int i = threadIdx.x; i < 4 * kHid + kOut; i += kInferBlock)
         sbias[i] = A.params[i < 4 * kHid ? kBOff[i / kHid] + i % kHid : oB5 + (i - 4 * kHid)];
@@ -744,50 +742,43 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
             }
             continue;
         }
-        // ---- layer 1, light-independent features: K-steps 0..2 (AO, normal)
-        // of neighbours 8s + 2g + h, then K-step 3: relative depth of
+        // ---- layer 1, light-independent features: K-steps 0..3 (AO, normal)
+        // of neighbours 8g + 2s + e / 4, then K-step 4: relative depth of
         // neighbours 8g + e, and the neighbour indices and foreground bits that
         // the visibility step reads per light.  Every neighbour index is
         // clamped to the image, so all loads are issued before the first is
-        // used (invalid pixels select zeros afterwards).
-        uint32_t na[6];
-        float4 va[6];
-        float aa[6];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) na[i] = dn_nb_index(F, px, py, 2 * (4 * (i >> 1) + g) + (i & 1));
+        // used (invalid pixels and neighbours beyond the 25th select zeros).
         uint32_t nv[8];
-        float dv[8], av[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) nv[e] = dn_nb_index(F, px, py, 8 * g + e < 25 ? 8 * g + e : 0);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            va[i] = load_pix(rpix, na[i]);
-            aa[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rao, na[i] * 4u, 0, 0));
-        }
-        const float4 v24 = load_pix(rpix, nv[0]);   // neighbour 24 on lane group 3
+        float4 vv[8];
+        float av[8];
+        bool in[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            dv[e] = e == 0 ? v24.x : __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rpix, nv[e] * 16u, 0, 0));
+            nv[e] = dn_nb_index(F, px, py, 8 * g + e < 25 ? 8 * g + e : 0);
+            in[e] = T.valid && 8 * g + e < 25;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            vv[e] = load_pix(rpix, nv[e]);
             av[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rao, nv[e] * 4u, 0, 0));
         }
         f32x4 ag[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) ag[t] = *reinterpret_cast<const f32x4*>(&sbias[16 * t + 4 * g]);
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            uint32_t w[4] = {0u, 0u, 0u, 0u};
-            if (T.valid) {
+        for (int s = 0; s < 4; ++s) {
+            uint32_t w[4];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const float4 v = va[2 * s + h];
-                    const float ao = v.x >= 0.0f ? aa[2 * s + h] : 0.0f;
-                    w[2 * h] = bf16_pack2(ao, v.y);     // background normals are zero
-                    w[2 * h + 1] = bf16_pack2(v.z, v.w);
-                }
+            for (int h = 0; h < 2; ++h) {
+                const int e = 2 * s + h;
+                const float4 v = in[e] ? vv[e] : make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+                const float ao = v.x >= 0.0f ? av[e] : 0.0f;
+                w[2 * h] = bf16_pack2(ao, v.y);     // background normals are zero
+                w[2 * h + 1] = bf16_pack2(v.z, v.w);
             }
             const bf16x8 fb = as_bf16x8(make_uint4(w[0], w[1], w[2], w[3]));
 #pragma unroll
-            for (int t = 0; t < 4; ++t) ag[t] = mfma(as_bf16x8(sfrag[(kIFG + t * 4 + s) * 64 + lane]), fb, ag[t]);
+            for (int t = 0; t < 4; ++t) ag[t] = mfma(as_bf16x8(sfrag[(kIFG + t * 5 + s) * 64 + lane]), fb, ag[t]);
         }
         {
             float f[8];
@@ -795,19 +786,14 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
             uint32_t vfg = 0u;   // foreground bits of the neighbours 8g + e
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const bool in = T.valid && 8 * g + e < 25;
-                const bool fg = in && dv[e] >= 0.0f;
-                T.vq[e] = in ? nv[e] : 0u;
+                const float dn = vv[e].x;
+                const bool fg = in[e] && dn >= 0.0f;
+                T.vq[e] = in[e] ? nv[e] : 0u;
                 T.bo[e] = fg ? 0u : 2048u;
                 vfg |= (fg ? 1u : 0u) << e;
                 sa += fg ? av[e] : 0.0f;
-                const float rel = dc > 0.0f ? (dv[e] - dc) / dc * 16.0f : 0.0f;
-                f[e] = !in ? 0.0f : (!fg ? 1.0f : (rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel)));
-            }
-            if (g == 3 && T.valid) {   // (AO, normal) of neighbour 24 in elements 1..4
-                const bool fg = (vfg & 1u) != 0u;
-                f[1] = fg ? av[0] : 0.0f;
-                f[2] = v24.y; f[3] = v24.z; f[4] = v24.w;
+                const float rel = dc > 0.0f ? (dn - dc) / dc * 16.0f : 0.0f;
+                f[e] = !in[e] ? 0.0f : (!fg ? 1.0f : (rel < -1.0f ? -1.0f : (rel > 1.0f ? 1.0f : rel)));
             }
             const bf16x8 fb = as_bf16x8(make_uint4(bf16_pack2(f[0], f[1]), bf16_pack2(f[2], f[3]),
                                                    bf16_pack2(f[4], f[5]), bf16_pack2(f[6], f[7])));
@@ -817,7 +803,7 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
             T.cnt = (float)(c ? c : 1u);
             T.bao = sa / T.cnt;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) ag[t] = mfma(as_bf16x8(sfrag[(kIFG + t * 4 + 3) * 64 + lane]), fb, ag[t]);
+            for (int t = 0; t < 4; ++t) ag[t] = mfma(as_bf16x8(sfrag[(kIFG + t * 5 + 4) * 64 + lane]), fb, ag[t]);
         }
         // ---- per light group: visibility K-step, layers 2..5
         float aosum = 0.0f;
@@ -934,7 +920,7 @@ __global__ void k_dn_pack(const float* P, uint16_t* frags) {
         const int t = f - kG5, o = khid(0, g, e);
         if (o < kOut) val = P[oW5 + o * kHid + (16 * t + j)];
     } else if (f < kF5) {
-        const int t = f < kFV ? (f - kFG) / 4 : f - kFV, s = f < kFV ? (f - kFG) % 4 : 4;
+        const int t = f < kFV ? (f - kFG) / 5 : f - kFV, s = f < kFV ? (f - kFG) % 5 : 5;
         const int k = infer_feature(s, g, e);
         if (k >= 0) val = P[oW1 + (16 * t + j) * kIn + k];
     } else if (j < 4) {
