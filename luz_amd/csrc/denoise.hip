@@ -66,6 +66,7 @@ constexpr uint32_t kChunkRows = 1024;
 // inference: one 16-pixel tile per wave at a time, 8 waves per group sharing
 // one copy of the weights in LDS
 constexpr int kInferBlock = 512;
+constexpr uint32_t kInferRounds = 4;   // bands of tiles per chunk an XCD sweeps in order
 
 constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn;
 constexpr int oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
@@ -719,14 +720,26 @@ __global__ __launch_bounds__(kInferBlock, 2) void k_dn_infer(InferArgs A) {
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
     const uint32_t npix = F.width * F.height;
     const uint32_t L = F.num_lights;
-    // blocks of one XCD (dispatched round-robin over 8) take neighbouring tiles
-    const uint32_t nb = gridDim.x;
-    const uint32_t lb = nb % 8 == 0 ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
-    const uint32_t wave = lb * (kInferBlock / 64) + (threadIdx.x >> 6);
-    const uint32_t nwaves = nb * (kInferBlock / 64);
+    // The tiles go out in chunks of kInferRounds bands (a band: one tile per
+    // wave of an XCD), chunk c to XCD c % 8 (blocks are dispatched round-robin
+    // over the 8 XCDs): an XCD's waves sweep each of its chunks band after
+    // band, so the window rows that neighbouring bands share are still in its
+    // L2 when the next band reads them, and the chunks interleave the image
+    // over the XCDs, which keeps their work even where whole regions are
+    // background.  With a grid not a multiple of 8 every wave strides alone.
+    const uint32_t nb = gridDim.x, wpb = kInferBlock / 64;
+    const bool xcd = nb % 8 == 0;
+    const uint32_t W = xcd ? (nb / 8) * wpb : nb * wpb;   // waves per band
+    const uint32_t wx = (xcd ? blockIdx.x / 8 : blockIdx.x) * wpb + (threadIdx.x >> 6);
+    const uint32_t c0 = xcd ? blockIdx.x % 8 : 0u, cstep = xcd ? 8u : 1u;
+    const uint32_t C = kInferRounds * W;                   // tiles per chunk
     const float b50 = sbias[4 * kHid], b51 = sbias[4 * kHid + 1];
     const __amdgpu_buffer_rsrc_t rpix = buf_rsrc(F.pix, npix * 16u), rao = buf_rsrc(F.ao_frac, npix * 4u);
-    for (uint32_t tile = wave; tile < A.ntiles; tile += nwaves) {
+    for (uint32_t k = 0;; ++k) {
+        const uint32_t base = (c0 + cstep * (k / kInferRounds)) * C;   // first tile of the chunk
+        if (base >= A.ntiles) break;
+        const uint32_t tile = base + (k % kInferRounds) * W + wx;
+        if (tile >= A.ntiles) continue;   // the frame's last chunk ends early
         InferTile T;
         const uint32_t p = tile * 16u + (uint32_t)j;
         T.inb = p < npix;
