@@ -638,44 +638,95 @@ constexpr int kPlocRadius = 16;                              // clustering searc
 constexpr int kPlocTile = kTlasThreads + 2 * kPlocRadius;    // staged boxes per neighbour pass
 
 // Threads of a TLAS build launch for m (a power of two >= the instance count):
-// one wave up to 64 instances (every step of the build then fits one wave and
-// its barriers cost little), else kTlasThreads.
-inline uint32_t tlas_threads(uint32_t m) { return m <= 64 ? 64u : (uint32_t)kTlasThreads; }
+// four waves up to 64 instances (the neighbour search then gives each cluster
+// four threads, one per SIMD, and the barriers stay cheap), else kTlasThreads.
+inline uint32_t tlas_threads(uint32_t m) { return m <= 64 ? 256u : (uint32_t)kTlasThreads; }
 
-// Merge cost of a cluster box (lo xyz, hi xyz) with box t of a staged tile
-// (six planes of `plane` floats): half the surface area of the union, FP32 as
-// written, NaN as +inf.
-__device__ inline float merge_cost(const float* a, const float* tile, int plane, int t) {
-    const float lx = fmin_(a[0], tile[t]), ly = fmin_(a[1], tile[plane + t]),
-                lz = fmin_(a[2], tile[2 * plane + t]);
-    const float hx = fmax_(a[3], tile[3 * plane + t]), hy = fmax_(a[4], tile[4 * plane + t]),
-                hz = fmax_(a[5], tile[5 * plane + t]);
+// Merge cost of two cluster boxes as the tile holds them ((lo.xyz, hi.x) and
+// (hi.y, hi.z)): half the surface area of the union, FP32 as written, NaN as
+// +inf.
+__device__ inline float merge_cost(float4 a, float2 a2, float4 b, float2 b2) {
+    const float lx = fmin_(a.x, b.x), ly = fmin_(a.y, b.y), lz = fmin_(a.z, b.z);
+    const float hx = fmax_(a.w, b.w), hy = fmax_(a2.x, b2.x), hz = fmax_(a2.y, b2.y);
     const float ex = hx - lx, ey = hy - ly, ez = hz - lz;
     const float c = ex * ey + ey * ez + ez * ex;
     return c == c ? c : INFINITY;
 }
 
-// Nearest neighbour of cluster i among clusters i - kPlocRadius .. i + kPlocRadius
-// (within 0 .. count - 1), from a tile that holds cluster c at c - base +
-// kPlocRadius: least merge cost, the smaller index on equal cost.
-__device__ inline uint32_t ploc_nearest(const float* tile, uint32_t i, uint32_t base, uint32_t count) {
+// The LDS tile of cluster boxes: slot t holds (lo.x, lo.y, lo.z, hi.x) in
+// a[t] and (hi.y, hi.z) in b[t], 24 bytes read by two loads.
+struct BoxTile {
+    float4* a;
+    float2* b;
+};
+__device__ inline void tile_put(BoxTile T, int t, const float* box) {
+    T.a[t] = make_float4(box[0], box[1], box[2], box[3]);
+    T.b[t] = make_float2(box[4], box[5]);
+}
+__device__ inline void tile_get(BoxTile T, int t, float* box) {
+    const float4 a = T.a[t];
+    const float2 b = T.b[t];
+    box[0] = a.x; box[1] = a.y; box[2] = a.z;
+    box[3] = a.w; box[4] = b.x; box[5] = b.y;
+}
+
+// Part q (< 4) of the neighbour search of cluster i among clusters
+// i - kPlocRadius .. i + kPlocRadius (within 0 .. count - 1), from a tile that
+// holds cluster c at slot c - base + kPlocRadius: candidates 8q .. 8q + 7 of
+// the 32 in index order, their boxes read from LDS together (slots outside
+// 0 .. count - 1 are read, as the tile spans them, and skipped).  Keeps in
+// (best, bj) the least merge cost, the smaller index on equal cost.
+__device__ inline void ploc_part(BoxTile tile, uint32_t i, uint32_t base, uint32_t count, int q, float& best,
+                                 uint32_t& bj) {
     const int ti = (int)(i - base) + kPlocRadius;
-    float box[6];
-    for (int k = 0; k < 6; ++k) box[k] = tile[k * kPlocTile + ti];
-    const int j0 = (int)i - kPlocRadius < 0 ? 0 : (int)i - kPlocRadius;
-    const int j1 = (int)i + kPlocRadius > (int)count - 1 ? (int)count - 1 : (int)i + kPlocRadius;
+    const float4 a = tile.a[ti];
+    const float2 a2 = tile.b[ti];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int k = 8 * q + u, d = k < kPlocRadius ? k - kPlocRadius : k - kPlocRadius + 1;
+        const int j = (int)i + d;
+        const float c = merge_cost(a, a2, tile.a[ti + d], tile.b[ti + d]);
+        if (j >= 0 && j < (int)count && (bj == kNone || c < best)) { best = c; bj = (uint32_t)j; }
+    }
+}
+
+// Nearest neighbour of cluster i: least merge cost, the smaller index on
+// equal cost.
+__device__ inline uint32_t ploc_nearest(BoxTile tile, uint32_t i, uint32_t base, uint32_t count) {
     float best = INFINITY;
     uint32_t bj = kNone;
-    for (int j = j0; j <= j1; ++j) {
-        if (j == (int)i) continue;
-        const float c = merge_cost(box, tile, kPlocTile, j - (int)base + kPlocRadius);
-        if (bj == kNone || c < best) { best = c; bj = (uint32_t)j; }
+    for (int q = 0; q < 4; ++q) ploc_part(tile, i, base, count, q, best, bj);
+    return bj;
+}
+
+// The same search by four consecutive threads (a quad), thread q taking part
+// q; the quad's parts combine as (cost, index) pairs, least first, which is
+// the order of the one-thread search.  All threads of the wave call it; the
+// result is valid in every thread of quad i (i < count).
+__device__ inline uint32_t ploc_nearest_quad(BoxTile tile, uint32_t i, uint32_t count) {
+    float best = INFINITY;
+    uint32_t bj = kNone;
+    if (i < count) ploc_part(tile, i, 0, count, (int)(threadIdx.x & 3u), best, bj);
+    for (int o = 1; o <= 2; o <<= 1) {
+        const float ob = __shfl_xor(best, o);
+        const uint32_t oj = __shfl_xor(bj, o);
+        if (ob < best || (ob == best && oj < bj)) { best = ob; bj = oj; }
     }
     return bj;
 }
 
+// Workgroup barrier ordering LDS accesses only: global stores still in
+// flight are not waited for.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Exclusive ranks of this thread's head and keep flags among the workgroup's
 // threads (thread order), and their totals.  All threads must call it.
+// LDS_ONLY: its barrier orders LDS accesses only.
+template <bool LDS_ONLY = false>
 __device__ inline void block_ranks(bool head, bool keep, uint32_t (*s_wave)[kTlasThreads / 64],
                                    uint32_t& hr, uint32_t& kr, uint32_t& th, uint32_t& tk) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -685,7 +736,8 @@ __device__ inline void block_ranks(bool head, bool keep, uint32_t (*s_wave)[kTla
         s_wave[0][wave] = (uint32_t)__popcll(bh);
         s_wave[1][wave] = (uint32_t)__popcll(bk);
     }
-    __syncthreads();
+    if (LDS_ONLY) lds_barrier();
+    else __syncthreads();
     uint32_t oh = 0, ok = 0;
     th = 0;
     tk = 0;
@@ -715,6 +767,24 @@ __device__ inline void ploc_node(Node* nodes, uint32_t idx, const float* a, cons
     if (!(rb & kLeaf)) nodes[rb].parent = idx;
 }
 
+// The same node written without its parent field, which the merge that takes
+// the node in sets (kNone for the root, node 0): every byte of a node is then
+// stored once, so rounds that create and link nodes need not wait for each
+// other's stores.
+__device__ inline void ploc_node_unlinked(Node* nodes, uint32_t idx, const float* a, const float* b,
+                                          uint32_t ra, uint32_t rb) {
+    Node* nd = nodes + idx;
+    float4* q = reinterpret_cast<float4*>(nd);
+    q[0] = make_float4(a[0], b[0], a[3], b[3]);   // lox[2], hix[2]
+    q[1] = make_float4(a[1], b[1], a[4], b[4]);   // loy[2], hiy[2]
+    q[2] = make_float4(a[2], b[2], a[5], b[5]);   // loz[2], hiz[2]
+    *reinterpret_cast<uint2*>(nd->child) = make_uint2(ra, rb);
+    nd->pad = 0;
+    if (idx == 0) nd->parent = kNone;
+    if (!(ra & kLeaf)) nodes[ra].parent = idx;
+    if (!(rb & kLeaf)) nodes[rb].parent = idx;
+}
+
 __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     const InstIn* in = J.in;
     const uint32_t n = J.n, m = J.m;
@@ -728,7 +798,7 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
     // tlas_lds_bytes(m) bytes, m a power of two >= n: the sort keys, then the
     // clustering's box tile, neighbour and reference arrays in the same memory
     // (within 64 KB for 8192 instances).
-    extern __shared__ uint64_t skeys[];
+    extern __shared__ __align__(16) uint64_t skeys[];
     const int tid = threadIdx.x;
     // launched with 64 threads (one wave, cheap barriers) up to 64 instances,
     // else with kTlasThreads
@@ -877,8 +947,9 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
         // least merge cost, mutual pairs merge into a new node at the lower
         // position, and the survivors are compacted in order.  New nodes are
         // numbered downwards from n - 2 in merge order, so the root is node 0.
-        float* tile = reinterpret_cast<float*>(skeys);          // 6 planes of kPlocTile boxes
-        uint32_t* snn = reinterpret_cast<uint32_t*>(tile + 6 * kPlocTile);   // m entries
+        BoxTile tile{reinterpret_cast<float4*>(skeys), nullptr};   // kPlocTile boxes
+        tile.b = reinterpret_cast<float2*>(tile.a + kPlocTile);
+        uint32_t* snn = reinterpret_cast<uint32_t*>(tile.b + kPlocTile);   // m entries
         uint32_t* sref = snn + m;                                              // kTlasThreads entries
         __shared__ uint32_t s_wave[2][kTlasThreads / 64];
         uint32_t count = n, next = n - 1;
@@ -888,8 +959,7 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
             for (uint32_t base = 0; base < count; base += nt) {
                 for (int t = tid; t < (int)nt + 2 * kPlocRadius; t += (int)nt) {
                     const int c = (int)base - kPlocRadius + t;
-                    if (c >= 0 && c < (int)count)
-                        for (int k = 0; k < 6; ++k) tile[k * kPlocTile + t] = cb[c * 6 + k];
+                    if (c >= 0 && c < (int)count) tile_put(tile, t, cb + (size_t)c * 6);
                 }
                 __syncthreads();
                 const uint32_t i = base + tid;
@@ -934,21 +1004,26 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
             uint32_t* tr = cr; cr = cr2; cr2 = tr;
         }
         // rounds over at most one cluster per thread: cluster i lives in thread
-        // i's registers and in the LDS tile (box at i + kPlocRadius) and sref
+        // i's registers and in the LDS tile (box at i + kPlocRadius) and sref;
+        // the nodes these rounds create are only stored, so their barriers
+        // order LDS accesses alone
         float box[6];
         uint32_t ref = 0;
         if (tid < count) {
-            for (int k = 0; k < 6; ++k) {
-                box[k] = cb[tid * 6 + k];
-                tile[k * kPlocTile + tid + kPlocRadius] = box[k];
-            }
+            for (int k = 0; k < 6; ++k) box[k] = cb[tid * 6 + k];
+            tile_put(tile, tid + kPlocRadius, box);
             ref = cr[tid];
             sref[tid] = ref;
         }
         __syncthreads();
         while (count > 1 && !failed) {
-            if (tid < count) snn[tid] = ploc_nearest(tile, tid, 0, count);
-            __syncthreads();
+            if (4 * count <= nt) {   // four threads per cluster
+                const uint32_t i = (uint32_t)tid >> 2, b = ploc_nearest_quad(tile, i, count);
+                if (i < count && (tid & 3) == 0) snn[i] = b;
+            } else if (tid < count) {
+                snn[tid] = ploc_nearest(tile, tid, 0, count);
+            }
+            lds_barrier();
             bool head = false, keep = false;
             uint32_t j = 0;
             if (tid < count) {
@@ -958,29 +1033,29 @@ __device__ inline void tlas_build_body(const TlasJob& J, uint32_t* error) {
                 keep = !(mutual && tid > j);
             }
             uint32_t hr, kr, heads, kept;
-            block_ranks(head, keep, s_wave, hr, kr, heads, kept);
+            block_ranks<true>(head, keep, s_wave, hr, kr, heads, kept);
             if (head) {
                 const uint32_t idx = next - 1 - hr;
                 float other[6];
-                for (int k = 0; k < 6; ++k) other[k] = tile[k * kPlocTile + j + kPlocRadius];
-                ploc_node(nodes, idx, box, other, ref, sref[j]);
+                tile_get(tile, j + kPlocRadius, other);
+                ploc_node_unlinked(nodes, idx, box, other, ref, sref[j]);
                 for (int k = 0; k < 3; ++k) {
                     box[k] = fmin_(box[k], other[k]);
                     box[3 + k] = fmax_(box[3 + k], other[3 + k]);
                 }
                 ref = idx;
             }
-            __syncthreads();   // every read of this round's tile and sref is done
+            lds_barrier();   // every read of this round's tile and sref is done
             if (keep) {
-                for (int k = 0; k < 6; ++k) tile[k * kPlocTile + kr + kPlocRadius] = box[k];
+                tile_put(tile, kr + kPlocRadius, box);
                 sref[kr] = ref;
             }
-            __syncthreads();
+            lds_barrier();
             failed = heads == 0;
             count = kept;
             next -= heads;
             if (tid < count) {
-                for (int k = 0; k < 6; ++k) box[k] = tile[k * kPlocTile + tid + kPlocRadius];
+                tile_get(tile, tid + kPlocRadius, box);
                 ref = sref[tid];
             }
         }
@@ -1209,7 +1284,7 @@ size_t tlas_scratch_bytes(uint32_t capacity) {
 
 size_t tlas_lds_bytes(uint32_t m) {
     const size_t sort = (size_t)(m > 1024 ? m : 1024) * sizeof(uint64_t);
-    const size_t ploc = (size_t)6 * kPlocTile * sizeof(float) + ((size_t)m + kTlasThreads) * sizeof(uint32_t);
+    const size_t ploc = (size_t)kPlocTile * (sizeof(float4) + sizeof(float2)) + ((size_t)m + kTlasThreads) * sizeof(uint32_t);
     return sort > ploc ? sort : ploc;
 }
 
