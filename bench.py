@@ -54,8 +54,9 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC counter calibration)")
-    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "k_trace_any_traffic.json"),
-                    help="HBM bytes per traversal launch measured with rocprofv3 --pmc (JSON)")
+    ap.add_argument("--traffic", default=None,
+                    help="HBM bytes and VALU instructions per traversal launch measured with rocprofv3 "
+                         "--pmc (JSON; default: the workload's file under profiles/)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
@@ -157,7 +158,8 @@ def main() -> int:
                                         ("groups_per_cu", "refill", "stack_entries", "tile_width", "table")}
 
     prof = {}
-    tpath = Path(args.traffic)
+    tpath = Path(args.traffic) if args.traffic else (
+        ROOT / "profiles" / ("k_trace_any_terrain4k_traffic.json" if compact else "k_trace_any_traffic.json"))
     if tpath.exists():
         try:
             prof = json.loads(tpath.read_text())
@@ -166,7 +168,7 @@ def main() -> int:
     traffic = prof.get("hbm_bytes_per_launch")
 
     if not args.no_cpu and compact:
-        record.update(compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays))
+        record.update(compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays, prof))
     elif not args.no_cpu:
         from oracle import oracle as orc
         host = gbuf.download()
@@ -625,9 +627,12 @@ def bench_denoise(ctx, args) -> int:
     return 0
 
 
-def compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays) -> dict:
+def compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays, prof) -> dict:
     """Precision form: hit/miss mismatch and radiance RMSE against the FP64 oracle,
-    exactness against the FP32 oracle, on row samples of the 4K frame."""
+    exactness against the FP32 oracle, on row samples of the 4K frame; the
+    8-wide traversal kernel against its rooflines as roofline() rates the
+    binary one (algorithmic bytes with 96 bytes per 8-wide node, the bound from
+    the profiled HBM traffic and VALU instructions per launch)."""
     import numpy as np
 
     from oracle import oracle as orc
@@ -661,11 +666,23 @@ def compact_parity(ctx, gs, scene, gbuf, sb, out, W, H, trace_avg, rays) -> dict
     na = max(sb.ao_num_samples, 1)
     per_ray = (96.0 * st.wide_node_visits + 64.0 * st.node_visits + 36.0 * st.tri_tests +
                64.0 * st.inst_visits) / max(st.rays, 1)
-    achieved = rays * (per_ray + 33.0) / (trace_avg * 1e-3) / 1e9   # + per-pixel record (32 B) + hit byte
+    pixels = int(act.sum())
+    t = trace_avg * 1e-3
+    achieved = (rays * per_ray + 24.0 * pixels + (nl + 1) * pixels) / t / 1e9
+    cus, clk_khz = ctx.device_info()
+    clock_hz = max(clk_khz, 1) * 1e3
+    traffic, valu = prof.get("hbm_bytes_per_launch"), prof.get("valu_insts_per_launch")
+    valu_frac = valu / (cus * 4 * clock_hz / 4) / t if valu else None
+    hbm_frac = traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None
+    bound = "valu" if valu_frac is not None and valu_frac > (hbm_frac or 0.0) else "hbm"
     return {
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace_any<wide>",
-                     "algorithmic_bytes_per_ray": round(per_ray + 33.0, 2)},
+        "roofline": {"bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_trace_any<wide>",
+                     "algorithmic_bytes_per_ray": round(per_ray, 2),
+                     "traffic_frac": round(hbm_frac, 4) if hbm_frac else None,
+                     "valu_issue_frac": round(valu_frac, 4) if valu_frac else None,
+                     "valu_insts_per_launch": valu, "counters_from": prof.get("source"),
+                     "bytes_per_ray_with_pixel_records": round(per_ray + 33.0, 2)},
         "cpu_baseline": {"value": round(st.rays / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads,
                          "kind": "port",
                          "sample": f"oracle FP32 8-wide visibility on every {step32}th row ({st.rays} rays)"},
