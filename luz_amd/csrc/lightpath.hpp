@@ -325,11 +325,43 @@ __device__ inline void ray_slot(uint32_t rid, uint32_t A, uint32_t R, uint32_t G
     a = g * G + (within - j * w);
 }
 
+// Samples n of a pixel's hit mask from slot j0 (0 < n, j0 + n <= 64).
+__device__ inline uint64_t slot_bits(uint64_t m, uint32_t j0, uint32_t n) {
+    return (m >> j0) & (n >= 64 ? ~0ull : (1ull << n) - 1ull);
+}
+
 // Per-pixel counts from hit bytes: occluded shadow samples per light
 // (plane l of `shadow`), unoccluded AO samples; a = kNone for background.
+// Up to 64 rays per pixel the hit bytes (0 or 1) are gathered into a bit mask
+// first, the loads of 32 slots in flight together, and each count is a
+// population count of its slots.
 __device__ inline void resolve_pixel(const DevScene* ds, uint32_t a, uint32_t A, const uint8_t* hits,
                                      uint32_t npix, uint32_t p, uint8_t* shadow, uint8_t* ao) {
     const luzrt_scene& S = ds->s;
+    const uint32_t R = ds->r_total;
+    if (a != kNone && R > 0 && R <= 64) {
+        uint32_t first, step;
+        ray_row(a, A, R, ds->ray_group, first, step);
+        uint64_t m = 0;
+        for (uint32_t c = 0; c < R; c += 32) {
+            uint32_t w = 0;
+#pragma unroll
+            for (uint32_t u = 0; u < 32; ++u) {
+                const uint32_t j = c + u < R ? c + u : R - 1;   // slots past R re-read the last
+                w |= (uint32_t)(hits[first + j * step] != 0) << u;
+            }
+            if (R - c < 32) w &= (1u << (R - c)) - 1u;
+            m |= (uint64_t)w << c;
+        }
+        for (int l = 0; l < S.num_lights; ++l) {
+            const int n = S.lights[l].num_shadow_samples;
+            const uint32_t cnt = n > 0 ? (uint32_t)__popcll(slot_bits(m, ds->slot_base[l], (uint32_t)n)) : 0u;
+            shadow[(size_t)l * npix + p] = (uint8_t)cnt;
+        }
+        const int na = S.ao_num_samples;
+        ao[p] = (uint8_t)(na > 0 ? (uint32_t)na - (uint32_t)__popcll(slot_bits(m, ds->r_shadow, (uint32_t)na)) : 0u);
+        return;
+    }
     uint32_t first = 0, step = 0;
     if (a != kNone) ray_row(a, A, ds->r_total, ds->ray_group, first, step);
     for (int l = 0; l < S.num_lights; ++l) {
