@@ -167,14 +167,16 @@ __device__ inline int wide_visit(RayState& r, STACK& st) {
     const float sx = __int_as_float((ex + 127) << 23), sy = __int_as_float((ey + 127) << 23),
                 sz = __int_as_float((ez + 127) << 23);
     uint32_t mask = 0;
+    // q x 2^e (q < 256, 2^e a normal power of two) is exact, so the fused
+    // multiply-add rounds once where o + q x 2^e would: the same decoded bounds
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        float lx = ox + (float)byte_of(qa.x, qa.y, k) * sx;
-        float ly = oy + (float)byte_of(qa.z, qa.w, k) * sy;
-        float lz = oz + (float)byte_of(qb.x, qb.y, k) * sz;
-        float hx = ox + (float)byte_of(qb.z, qb.w, k) * sx;
-        float hy = oy + (float)byte_of(qc.x, qc.y, k) * sy;
-        float hz = oz + (float)byte_of(qc.z, qc.w, k) * sz;
+        float lx = __builtin_fmaf((float)byte_of(qa.x, qa.y, k), sx, ox);
+        float ly = __builtin_fmaf((float)byte_of(qa.z, qa.w, k), sy, oy);
+        float lz = __builtin_fmaf((float)byte_of(qb.x, qb.y, k), sz, oz);
+        float hx = __builtin_fmaf((float)byte_of(qb.z, qb.w, k), sx, ox);
+        float hy = __builtin_fmaf((float)byte_of(qc.x, qc.y, k), sy, oy);
+        float hz = __builtin_fmaf((float)byte_of(qc.z, qc.w, k), sz, oz);
         float tn;
         if (k < nchild && box_hit(r.br, lx, hx, ly, hy, lz, hz, tn)) mask |= 1u << k;
     }
