@@ -255,12 +255,10 @@ __device__ inline BoxRay box_setup(V3 o, V3 d, float tmin, float tmax) {
 }
 
 // Slab test of child c of a node given its per-axis (lo0, lo1, hi0, hi1).
-__device__ inline bool box_hit(const BoxRay& r, float lox, float hix, float loy, float hiy,
-                               float loz, float hiz, float& tnear) {
-    bool sx = r.inv.x < 0.0f, sy = r.inv.y < 0.0f, sz = r.inv.z < 0.0f;
-    float nx = sx ? hix : lox, fx = sx ? lox : hix;
-    float ny = sy ? hiy : loy, fy = sy ? loy : hiy;
-    float nz = sz ? hiz : loz, fz = sz ? loz : hiz;
+// The slab test on the box's near and far planes per axis (near: the lower
+// plane where the ray's reciprocal direction is >= 0, else the upper one).
+__device__ inline bool box_hit_nf(const BoxRay& r, float nx, float fx, float ny, float fy, float nz, float fz,
+                                  float& tnear) {
     float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
     float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
     float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
@@ -271,6 +269,13 @@ __device__ inline bool box_hit(const BoxRay& r, float lox, float hix, float loy,
     tf = tf + fabsf(tf) * 0x1p-20f;
     tnear = tn;
     return tn <= tf;
+}
+
+__device__ inline bool box_hit(const BoxRay& r, float lox, float hix, float loy, float hiy,
+                               float loz, float hiz, float& tnear) {
+    bool sx = r.inv.x < 0.0f, sy = r.inv.y < 0.0f, sz = r.inv.z < 0.0f;
+    return box_hit_nf(r, sx ? hix : lox, sx ? lox : hix, sy ? hiy : loy, sy ? loy : hiy, sz ? hiz : loz,
+                      sz ? loz : hiz, tnear);
 }
 
 __device__ inline V3 xform_point(const float* w, V3 p) {

@@ -167,18 +167,24 @@ __device__ inline int wide_visit(RayState& r, STACK& st) {
     const float sx = __int_as_float((ex + 127) << 23), sy = __int_as_float((ey + 127) << 23),
                 sz = __int_as_float((ez + 127) << 23);
     uint32_t mask = 0;
+    // the near and far planes' byte words per axis, chosen once per node by
+    // the ray's direction signs (box_hit chooses them per child)
+    const bool gx = r.br.inv.x < 0.0f, gy = r.br.inv.y < 0.0f, gz = r.br.inv.z < 0.0f;
+    const uint32_t nxa = gx ? qb.z : qa.x, nxb = gx ? qb.w : qa.y, fxa = gx ? qa.x : qb.z, fxb = gx ? qa.y : qb.w;
+    const uint32_t nya = gy ? qc.x : qa.z, nyb = gy ? qc.y : qa.w, fya = gy ? qa.z : qc.x, fyb = gy ? qa.w : qc.y;
+    const uint32_t nza = gz ? qc.z : qb.x, nzb = gz ? qc.w : qb.y, fza = gz ? qb.x : qc.z, fzb = gz ? qb.y : qc.w;
     // q x 2^e (q < 256, 2^e a normal power of two) is exact, so the fused
     // multiply-add rounds once where o + q x 2^e would: the same decoded bounds
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        float lx = __builtin_fmaf((float)byte_of(qa.x, qa.y, k), sx, ox);
-        float ly = __builtin_fmaf((float)byte_of(qa.z, qa.w, k), sy, oy);
-        float lz = __builtin_fmaf((float)byte_of(qb.x, qb.y, k), sz, oz);
-        float hx = __builtin_fmaf((float)byte_of(qb.z, qb.w, k), sx, ox);
-        float hy = __builtin_fmaf((float)byte_of(qc.x, qc.y, k), sy, oy);
-        float hz = __builtin_fmaf((float)byte_of(qc.z, qc.w, k), sz, oz);
+        const float nx = __builtin_fmaf((float)byte_of(nxa, nxb, k), sx, ox);
+        const float fx = __builtin_fmaf((float)byte_of(fxa, fxb, k), sx, ox);
+        const float ny = __builtin_fmaf((float)byte_of(nya, nyb, k), sy, oy);
+        const float fy = __builtin_fmaf((float)byte_of(fya, fyb, k), sy, oy);
+        const float nz = __builtin_fmaf((float)byte_of(nza, nzb, k), sz, oz);
+        const float fz = __builtin_fmaf((float)byte_of(fza, fzb, k), sz, oz);
         float tn;
-        if (k < nchild && box_hit(r.br, lx, hx, ly, hy, lz, hz, tn)) mask |= 1u << k;
+        if (k < nchild && box_hit_nf(r.br, nx, fx, ny, fy, nz, fz, tn)) mask |= 1u << k;
     }
     const uint64_t meta = ((uint64_t)h1.w << 32) | h1.z;
     r.next = kNone;
