@@ -219,6 +219,32 @@ def test_group_culling_changes_no_count(ctx, ao_max, radius):
     g.close()
 
 
+def test_visibility_more_than_64_rays_per_pixel(ctx):
+    """8 shadow samples per light and 24 AO samples (96 rays per pixel): past
+    the 64 ray slots that ray-group culling and the resolve's hit mask cover,
+    so the call traces every ray and resolves each pixel's counts slot by
+    slot.  Counts against the oracle, and the light pass against shading from
+    them."""
+    s, g = _gpu_scene(ctx, api.SCENE_SPHERES)
+    o = oracle_scene(s)
+    W, H, gd, got, _ = _gbuffers(ctx, s, g, o, 160, 96)
+    sb = s.pack()
+    sb.ao_max = 16.0
+    for l in range(sb.num_lights):
+        sb.lights[l].num_shadow_samples = 8
+    sb.ao_num_samples = 24
+    nl = sb.num_lights
+    dsh, dao = ctx.buffer(nl * W * H), ctx.buffer(W * H)
+    ctx.trace_visibility(g.tlas, gd.desc(), sb, dsh, dao)
+    t = ctx.timings()
+    print(f"{t.rays} rays, {t.traced_rays} traced", flush=True)
+    assert t.traced_rays == t.rays
+    wsh, wao, _ = o.trace_visibility(got, W, H, sb)
+    np.testing.assert_array_equal(dsh.download(np.uint8, (nl, W * H)), wsh)
+    np.testing.assert_array_equal(dao.download(np.uint8, W * H), wao)
+    g.close()
+
+
 @pytest.mark.parametrize("kind,res", [(api.SCENE_CUBE, None), (api.SCENE_SPHERES, (480, 270))])
 def test_light_pass_radiance(ctx, kind, res):
     s, g = _gpu_scene(ctx, kind)
