@@ -223,8 +223,7 @@ def test_visibility_more_than_64_rays_per_pixel(ctx):
     """8 shadow samples per light and 24 AO samples (96 rays per pixel): past
     the 64 ray slots that ray-group culling and the resolve's hit mask cover,
     so the call traces every ray and resolves each pixel's counts slot by
-    slot.  Counts against the oracle, and the light pass against shading from
-    them."""
+    slot.  Counts against the oracle."""
     s, g = _gpu_scene(ctx, api.SCENE_SPHERES)
     o = oracle_scene(s)
     W, H, gd, got, _ = _gbuffers(ctx, s, g, o, 160, 96)
