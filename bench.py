@@ -47,6 +47,9 @@ def main() -> int:
                          "terrain4k: 3840x2160 terrain + spheres, ~5.2M tris, precision form; "
                          "batch64: 64 animated scenes of uneven size and resolution per frame; "
                          "denoise1080p: 1-spp + learned denoiser against the plain multi-spp tracer")
+    ap.add_argument("--bvh", default="wide", choices=["wide", "binary"],
+                    help="spheres1080p: trace the 8-wide BVH with FP32 shading (PRECISION_WIDE, the "
+                         "default) or the binary BVH (PRECISION_PLAIN); counts and radiance are the same")
     ap.add_argument("--scenes", type=int, default=64, help="batch64: scenes per frame")
     ap.add_argument("--autotune", action="store_true",
                     help="search the traversal launch shape on this workload before timing")
@@ -77,8 +80,11 @@ def main() -> int:
     if args.workload == "denoise1080p":
         return bench_denoise(ctx, args)
     compact = args.workload == "terrain4k"
+    wide = compact or args.bvh == "wide"
     if compact:
         ctx.set_precision(api.PRECISION_COMPACT)
+    elif wide:
+        ctx.set_precision(api.PRECISION_WIDE)
     scene = api.HostScene(api.SCENE_TERRAIN if compact else api.SCENE_SPHERES, 0)
     gs = api.GPUScene(ctx, scene)
     W, H, cam = scene.camera()
@@ -97,17 +103,20 @@ def main() -> int:
         gs.update_resources_gpu(scene)            # TLAS rebuild (GPUScene.cpp:354-365)
         ctx.light_pass(gs.tlas, gbuf.desc(), sb, out)
 
+    # Frames are submitted back to back, as a renderer does: the host packs
+    # frame k+1's instances while the GPU runs frame k (the library does not
+    # wait for the GPU inside a frame).  Stage events are off meanwhile: each
+    # one holds the next kernel back by about 8 us.
+    ctx.set_stage_timing(False)
     for _ in range(args.warmup):
         frame()
     ctx.sync()
-    # Frames are submitted back to back, as a renderer does: the host packs
-    # frame k+1's instances while the GPU runs frame k (the library does not
-    # wait for the GPU inside a frame).
     t0 = time.perf_counter()
     for _ in range(args.steps):
         frame()
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    ctx.set_stage_timing(True)
     # per-stage GPU times (HIP events) from frames that are each waited for
     trace_ms, build_ms, shade_ms, vis_ms = [], [], [], []
     for _ in range(min(args.steps, 10)):
@@ -143,7 +152,10 @@ def main() -> int:
                          "+ 8 AO rays/px") if compact else
                         ("1920x1080 sphere field: 50 instances of 5 fBm icospheres (20,480 tris) + "
                          "64x64 ground grid, 1,032,192 instanced tris, 1 dir + 8 point lights, "
-                         "1 shadow ray/light + 16 AO rays/px"),
+                         "1 shadow ray/light + 16 AO rays/px, " +
+                         ("8-wide BVH with 8-bit child boxes" if wide else "binary BVH") +
+                         ", FP32 shading"),
+            "bvh": "8-wide" if wide else "binary",
             "width": W, "height": H, "instanced_tris": int(scene.instanced_triangles),
             "rays_per_frame": rays, "active_pixels": active, "traced_rays_per_frame": traced,
             "frame_ms_breakdown": {"tlas_build": round(float(np.mean(build_ms)), 4),
@@ -159,7 +171,8 @@ def main() -> int:
 
     prof = {}
     tpath = Path(args.traffic) if args.traffic else (
-        ROOT / "profiles" / ("k_trace_any_terrain4k_traffic.json" if compact else "k_trace_any_traffic.json"))
+        ROOT / "profiles" / ("k_trace_any_terrain4k_traffic.json" if compact else
+                             "k_trace_any_traffic.json" if wide else "k_trace_any_binary_traffic.json"))
     if tpath.exists():
         try:
             prof = json.loads(tpath.read_text())
@@ -178,8 +191,10 @@ def main() -> int:
             o.add_mesh(v, idx)
         o2w, mesh_ids, _ = scene.nodes()
         o.set_instances(o2w, mesh_ids)
+        if wide:
+            o.build_wide()
         threads, cores_avail = cpu_threads()
-        par, st = visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, args.cpu_row_step * 4)
+        par, st = visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, args.cpu_row_step * 4, wide)
         # CPU baseline: the oracle's FP32 visibility of the whole frame, median of 5
         o.trace_visibility(host, W, H, sb, threads=threads)      # warm-up frame
         times = []
@@ -191,7 +206,7 @@ def main() -> int:
         c0 = time.perf_counter()
         _, _, st1 = o.trace_visibility(host, W, H, sb, threads=1, row_step=args.cpu_row_step * 2)
         cpu1_s = time.perf_counter() - c0
-        record["roofline"] = roofline(ctx, st, rays, traced, W * H, nl, trace_avg, traffic, prof)
+        record["roofline"] = roofline(ctx, st, rays, traced, W * H, nl, trace_avg, traffic, prof, wide)
         record["cpu_baseline"] = {
             "value": round(st.rays / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
@@ -214,7 +229,7 @@ def main() -> int:
         C.memmove(C.byref(sb16), C.byref(sb), C.sizeof(api.SceneBlock))
         sb16.ao_max = 16.0
         record["ao_radius16"] = ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb16, out, W, H, threads,
-                                                args, traffic)
+                                                args, wide)
     print(json.dumps(record), flush=True)
     gs.close()
     ctx.close()
@@ -241,9 +256,10 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, f64_step):
-    """GPU counts against the FP32 oracle on every pixel and the FP64 oracle
-    on every f64_step-th row; the oracle's stats (canonical traversal counts)."""
+def visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, f64_step, wide=False):
+    """GPU counts against the FP32 oracle (binary BVH) on every pixel and the
+    FP64 oracle on every f64_step-th row; the oracle's stats (canonical
+    traversal counts) of the BVH the GPU traced: the 8-wide one when wide."""
     import numpy as np
 
     nl = sb.num_lights
@@ -258,6 +274,8 @@ def visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, f64_step):
     act = np.abs(host["normal"]).sum(1) != 0
     m64 = rows64.reshape(-1) & act
     dsh, dao, _ = o.trace_visibility(host, W, H, sb, threads=threads, row_step=f64_step, f64=True)
+    if wide:
+        _, _, st = o.trace_visibility(host, W, H, sb, threads=threads, wide=True)
     na = max(sb.ao_num_samples, 1)
     ao_rmse = float(np.sqrt(np.mean((gao[m64].astype(np.float64) / na - dao[m64] / na) ** 2))) if m64.any() else 0.0
     par = {
@@ -273,16 +291,18 @@ def visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, f64_step):
     return par, st
 
 
-def roofline(ctx, st, rays, traced, pixels, nl, trace_ms, traffic, prof) -> dict:
+def roofline(ctx, st, rays, traced, pixels, nl, trace_ms, traffic, prof, wide=False) -> dict:
     """The traversal kernel against its rooflines.  Algorithmic bytes per launch
-    follow the visibility formula (64 n_node + 36 n_tri + 64 n_inst) per ray +
-    24 P + (L + 1) P, with per-ray counts from the oracle's canonical traversal
-    of the identical BVH, over the rays the launch traces (ray-group culling
+    follow the visibility formula (96 n_wide + 64 n_node + 36 n_tri + 64 n_inst)
+    per ray + 24 P + (L + 1) P, with per-ray counts from the oracle's canonical
+    traversal of the identical BVH (n_wide: 8-wide BLAS nodes, n_node: binary
+    nodes), over the rays the launch traces (ray-group culling
     resolves the others as misses before it); the VALU issue fraction is the
     profiled SQ_INSTS_VALU per launch over CUs x 4 SIMDs x clock / 4 (one
     wave64 VALU instruction per SIMD every 4 cycles) over the kernel time."""
     nr = max(st.rays, 1)
-    per_ray = (64.0 * st.node_visits + 36.0 * st.tri_tests + 64.0 * st.inst_visits) / nr
+    per_ray = (96.0 * st.wide_node_visits + 64.0 * st.node_visits + 36.0 * st.tri_tests +
+               64.0 * st.inst_visits) / nr
     b_vis = traced * per_ray + 24.0 * pixels + (nl + 1) * pixels
     t = trace_ms * 1e-3
     achieved = b_vis / t / 1e9
@@ -297,9 +317,11 @@ def roofline(ctx, st, rays, traced, pixels, nl, trace_ms, traffic, prof) -> dict
     bound = "valu" if valu_frac is not None and valu_frac > (hbm_frac or 0.0) else "hbm"
     return {
         "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_trace_any",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "kernel": "k_trace_any<wide>" if wide else "k_trace_any",
         "algorithmic_bytes_per_launch": round(b_vis), "rays": rays, "traced_rays": traced,
-        "per_ray_counts": {"node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
+        "per_ray_counts": {"wide_node": round(st.wide_node_visits / nr, 3),
+                           "node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
                            "inst": round(st.inst_visits / nr, 3)},
         # the BVH (about 11 MB) is served from L2 and the Infinity Cache: the
         # measured HBM bytes per launch over the same time give the HBM fraction drawn
@@ -314,7 +336,7 @@ def roofline(ctx, st, rays, traced, pixels, nl, trace_ms, traffic, prof) -> dict
     }
 
 
-def ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb, out, W, H, threads, args, traffic) -> dict:
+def ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb, out, W, H, threads, args, wide) -> dict:
     """The headline frame with AO radius 16: timing, parity and traversal counts."""
     import numpy as np
 
@@ -322,6 +344,7 @@ def ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb, out, W, H, threads, args,
         gs.update_resources_gpu(scene)
         ctx.light_pass(gs.tlas, gbuf.desc(), sb, out)
 
+    ctx.set_stage_timing(False)
     for _ in range(max(args.warmup, 1)):
         frame()
     ctx.sync()
@@ -331,6 +354,7 @@ def ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb, out, W, H, threads, args,
         frame()
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    ctx.set_stage_timing(True)
     trace_ms = []
     for _ in range(5):
         frame()
@@ -338,13 +362,14 @@ def ao_radius_frame(ctx, gs, scene, gbuf, o, host, sb, out, W, H, threads, args,
         trace_ms.append(t.trace_ms)
     rays = int(t.rays)
     traced = int(t.traced_rays)
-    par, st = visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, args.cpu_row_step * 4)
+    par, st = visibility_parity(ctx, gs, gbuf, o, host, sb, W, H, threads, args.cpu_row_step * 4, wide)
     nr = max(st.rays, 1)
     return {
         "ao_max": float(sb.ao_max), "steps": steps, "ms_per_frame": round(elapsed / steps * 1e3, 4),
         "mrays_per_s": round(rays * steps / elapsed / 1e6, 2), "trace_kernel_ms": round(float(np.median(trace_ms)), 4),
         "rays_per_frame": rays, "traced_rays_per_frame": traced,
-        "per_ray_counts": {"node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
+        "per_ray_counts": {"wide_node": round(st.wide_node_visits / nr, 3),
+                           "node": round(st.node_visits / nr, 3), "tri": round(st.tri_tests / nr, 3),
                            "inst": round(st.inst_visits / nr, 3)},
         "parity": par,
     }

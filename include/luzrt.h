@@ -181,8 +181,16 @@ const char*  luzrt_status_string(luzrt_status s);
  * device latches the condition; no later call clears it).  The reference has
  * no such limit (the driver's traversal) and no error path (Base.hpp:51-59). */
 luzrt_status luzrt_sync(luzrt_ctx* ctx);
-/* Per-stage GPU times of the last build/trace/light/primary call (synchronises). */
+/* Per-stage GPU times of the last build/trace/light/primary call (synchronises);
+ * 0 for stages run while stage timing is off. */
 luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out);
+/* Stage timing on (the default) or off.  On, every stage records a HIP event
+ * at each end, which holds the next kernel back by about 8 us; frames
+ * submitted back to back run without them when it is off.  The reference
+ * always writes its timestamps (CmdBeginTimeStamp / CmdEndTimeStamp,
+ * VulkanWrapper.h:250-267, VulkanWrapper.cpp:1276-1296), which cost its
+ * queue nothing comparable. */
+luzrt_status luzrt_set_stage_timing(luzrt_ctx* ctx, int enable);
 /* The context's device: compute units and peak engine clock (kHz), as the
  * HIP runtime reports them (for throughput and issue-rate figures). */
 luzrt_status luzrt_device_info(const luzrt_ctx* ctx, uint32_t* compute_units, uint32_t* clock_khz);
@@ -225,10 +233,16 @@ luzrt_status luzrt_set_group_culling(luzrt_ctx* ctx, int enable);
  * COMPACT: BLASes traversed as 8-wide nodes with 8-bit child boxes rounded
  * outward (96-byte nodes), FP32 watertight triangle tests, GGX shading with
  * the Fresnel and diffuse terms in packed FP16 (two lights per half2), the
- * specular scalar D*G/denominator in FP32, FP32 accumulation.  Takes effect
- * at the next TLAS build. */
+ * specular scalar D*G/denominator in FP32, FP32 accumulation.
+ * WIDE: COMPACT's 8-wide BVH with PLAIN's FP32 shading.  Every box test is
+ * conservative (the far slab bound is widened past its rounding error) and
+ * the triangle tests are the same FP32 watertight tests, so a ray meets the
+ * same triangles through either tree: counts and radiance are PLAIN's, bit
+ * for bit, from fewer dependent node reads.  Batched frames are PLAIN only.
+ * A mode takes effect at the next TLAS build. */
 #define LUZRT_PRECISION_PLAIN 0
 #define LUZRT_PRECISION_COMPACT 1
+#define LUZRT_PRECISION_WIDE 2
 luzrt_status luzrt_set_precision(luzrt_ctx* ctx, int mode);
 
 /* ------------------------------------------------------------ memory */

@@ -28,7 +28,7 @@ MAX_INSTANCES = 8192
 LIGHT_POINT, LIGHT_SPOT, LIGHT_DIRECTIONAL = 0, 1, 2
 SCENE_CUBE, SCENE_SPHERES, SCENE_TERRAIN, SCENE_RANDOM = 1, 2, 4, 5
 RAY_ORDER_AUTO, RAY_ORDER_SLOT, RAY_ORDER_GROUPED = 0, 1, 2
-PRECISION_PLAIN, PRECISION_COMPACT = 0, 1
+PRECISION_PLAIN, PRECISION_COMPACT, PRECISION_WIDE = 0, 1, 2
 SHADOW_DISABLED, SHADOW_RAYTRACED, SHADOW_MAP = 0, 1, 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "device error", -4: "in use",
@@ -157,6 +157,7 @@ _sigs = {
     "luzrt_set_trace_config": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "luzrt_set_ray_order": (C.c_int, [_P, C.c_uint32]),
     "luzrt_set_group_culling": (C.c_int, [_P, C.c_int]),
+    "luzrt_set_stage_timing": (C.c_int, [_P, C.c_int]),
     "luzrt_set_step_limit": (C.c_int, [_P, C.c_uint32]),
     "luzrt_trace_visibility_static": (C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(SceneBlock), _P, _P]),
     "luzrt_set_precision": (C.c_int, [_P, C.c_int]),
@@ -317,12 +318,17 @@ class Context:
         """Ray-group culling of visibility calls on (default) or off (same results)."""
         self._check(lib.luzrt_set_group_culling(self.handle, 1 if enable else 0))
 
+    def set_stage_timing(self, enable: bool) -> None:
+        """Per-stage HIP events on (default) or off; off, timings() reports 0 per stage."""
+        self._check(lib.luzrt_set_stage_timing(self.handle, 1 if enable else 0))
+
     def set_step_limit(self, max_steps: int) -> None:
         """Traversal steps per ray before it is abandoned (0: the default 2^24)."""
         self._check(lib.luzrt_set_step_limit(self.handle, max_steps))
 
     def set_precision(self, mode: int) -> None:
-        """PRECISION_PLAIN or PRECISION_COMPACT (8-wide 8-bit BVH, packed-FP16 shading)."""
+        """PRECISION_PLAIN, PRECISION_COMPACT (8-wide 8-bit BVH, packed-FP16 shading) or
+        PRECISION_WIDE (the 8-wide BVH with FP32 shading: PLAIN's results)."""
         self._check(lib.luzrt_set_precision(self.handle, mode))
 
     def buffer(self, nbytes: int) -> DeviceBuffer:

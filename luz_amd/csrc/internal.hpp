@@ -39,6 +39,7 @@ struct luzrt_ctx {
     // timing
     hipEvent_t ev[luz::kStageCount][2] = {};
     bool ev_used[luz::kStageCount] = {};
+    bool timing = true;            // luzrt_set_stage_timing
     uint64_t last_rays = 0, last_active = 0;
     // traversal launch configuration
     uint32_t groups_per_cu = 6;      // launch shape chosen by the autotuner on MI355X

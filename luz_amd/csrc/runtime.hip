@@ -74,10 +74,13 @@ luzrt_status zero_fill(luzrt_ctx* ctx, void* p, size_t bytes) {
 }
 
 void stage_begin(luzrt_ctx* ctx, Stage s) {
+    if (!ctx->timing) return;
     hipEventRecord(ctx->ev[s][0], ctx->stream);
     ctx->ev_used[s] = true;
 }
-void stage_end(luzrt_ctx* ctx, Stage s) { hipEventRecord(ctx->ev[s][1], ctx->stream); }
+void stage_end(luzrt_ctx* ctx, Stage s) {
+    if (ctx->timing) hipEventRecord(ctx->ev[s][1], ctx->stream);
+}
 
 luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g, luzrt_gbuffer* out) {
     if (g->width == 0 || g->height == 0 || g->width > 16384 || g->height > 16384)
@@ -292,8 +295,11 @@ luzrt_status luzrt_create(luzrt_ctx** out, int device) {
         delete ctx;
         return LUZRT_E_DEVICE;
     }
+    // Stage events only time: luzrt_get_timings reads them after a full stream
+    // synchronisation, so they skip the system-scope fence (an L2 write-back and
+    // invalidate between the stages' kernels, which cost about 10 us per record).
     for (int s = 0; s < kStageCount; ++s)
-        for (int k = 0; k < 2; ++k) hipEventCreate(&ctx->ev[s][k]);
+        for (int k = 0; k < 2; ++k) hipEventCreateWithFlags(&ctx->ev[s][k], hipEventDisableSystemFence);
     if (ensure(ctx, ctx->counters, kCounterBytes) != LUZRT_OK ||
         hipMemset(ctx->counters.ptr, 0, kCounterBytes) != hipSuccess) {
         luzrt_destroy(ctx);
@@ -393,6 +399,12 @@ luzrt_status luzrt_set_trace_config(luzrt_ctx* ctx, uint32_t groups_per_cu, uint
     return LUZRT_OK;
 }
 
+luzrt_status luzrt_set_stage_timing(luzrt_ctx* ctx, int enable) {
+    if (!ctx) return LUZRT_E_INVALID_ARG;
+    ctx->timing = enable != 0;
+    return LUZRT_OK;
+}
+
 luzrt_status luzrt_set_group_culling(luzrt_ctx* ctx, int enable) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     ctx->group_cull = enable != 0;
@@ -424,7 +436,7 @@ luzrt_status luzrt_set_ray_order(luzrt_ctx* ctx, uint32_t order) {
 
 luzrt_status luzrt_set_precision(luzrt_ctx* ctx, int mode) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
-    if (mode != LUZRT_PRECISION_PLAIN && mode != LUZRT_PRECISION_COMPACT)
+    if (mode != LUZRT_PRECISION_PLAIN && mode != LUZRT_PRECISION_COMPACT && mode != LUZRT_PRECISION_WIDE)
         return fail(ctx, LUZRT_E_INVALID_ARG, "unknown precision mode %d", mode);
     ctx->precision = mode;
     return LUZRT_OK;
@@ -628,7 +640,7 @@ luzrt_status pack_instances(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instance*
         r.tris = in.blas->tris;
         r.nrms = in.blas->nrms;
         r.prims = in.blas->prims;
-        if (ctx->precision == LUZRT_PRECISION_COMPACT && in.blas->tri_count > 0) {
+        if (ctx->precision != LUZRT_PRECISION_PLAIN && in.blas->tri_count > 0) {
             luzrt_blas* b = const_cast<luzrt_blas*>(in.blas);
             if (!b->wide_valid) {
                 luzrt_status ws = blas_wide_device(ctx, b);
@@ -661,7 +673,7 @@ luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instanc
     luzrt_status s = pack_instances(ctx, t, inst, count, host);
     if (s != LUZRT_OK) return s;
     LUZ_HIP(ctx, hipSetDevice(ctx->device));
-    t->wide = ctx->precision == LUZRT_PRECISION_COMPACT;
+    t->wide = ctx->precision != LUZRT_PRECISION_PLAIN;   // COMPACT and WIDE trace the 8-wide form
     if (t->wide && !t->recs_w)
         LUZ_HIP(ctx, hipMalloc((void**)&t->recs_w, (size_t)t->capacity * sizeof(InstRec)));
     if ((s = stage_upload(ctx, t->input, host.data(), host.size() * sizeof(InstIn))) != LUZRT_OK)

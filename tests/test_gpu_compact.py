@@ -80,3 +80,45 @@ def test_compact_visibility_bit_exact_and_fp16_radiance(compact):
     assert np.isfinite(rad).all()
     assert rmse / scale < 5e-3
     g.close()
+
+
+@pytest.mark.parametrize("ao_max", [None, 16.0])
+def test_wide_mode_equals_plain(ctx, ao_max):
+    """PRECISION_WIDE traces the 8-wide BVH and shades in FP32.  On the sphere
+    field, with the scene's AO radius and with radius 16 (where AO rays meet
+    occluders), its counts equal the oracle's binary FP32 traversal and its
+    light-pass radiance equals PLAIN's bit for bit, from the same G-buffer."""
+    s = api.HostScene(api.SCENE_SPHERES, 0)
+    g = api.GPUScene(ctx, s)
+    o = oracle_scene(s)
+    W, H = 480, 270
+    _, _, cam = s.camera()
+    _, _, mats = s.nodes()
+    gd = api.GBufferDevice(ctx, W, H)
+    ctx.primary_gbuffer(g.tlas, cam, mats, gd.desc())
+    host = gd.download()
+    sb = s.pack()
+    if ao_max is not None:
+        sb.ao_max = ao_max
+    nl = sb.num_lights
+    out = ctx.buffer(W * H * 16)
+    ctx.light_pass(g.tlas, gd.desc(), sb, out)
+    want = out.download(np.float32, (W * H, 4))
+    try:
+        ctx.set_precision(api.PRECISION_WIDE)
+        g.update_resources_gpu(s)                 # the TLAS over the 8-wide BLASes
+        dsh, dao = ctx.buffer(nl * W * H), ctx.buffer(W * H)
+        ctx.trace_visibility(g.tlas, gd.desc(), sb, dsh, dao)
+        gsh = dsh.download(np.uint8, (nl, W * H))
+        gao = dao.download(np.uint8, W * H)
+        out2 = ctx.buffer(W * H * 16)
+        ctx.light_pass(g.tlas, gd.desc(), sb, out2)
+        got = out2.download(np.float32, (W * H, 4))
+    finally:
+        ctx.set_precision(api.PRECISION_PLAIN)
+    wsh, wao, _ = o.trace_visibility(host, W, H, sb)
+    print(f"ao_max {ao_max}: shadowed {int(wsh.sum())}, AO misses {int(wao.sum())}", flush=True)
+    np.testing.assert_array_equal(gsh, wsh)
+    np.testing.assert_array_equal(gao, wao)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    g.close()

@@ -262,7 +262,37 @@ def test_light_pass_radiance(ctx, kind, res):
     g.close()
 
 
-@pytest.mark.parametrize("shadow_type", [api.SHADOW_RAYTRACED, api.SHADOW_DISABLED, api.SHADOW_MAP])
+def test_light_pass_without_stage_timing(ctx):
+    """With stage timing off the light pass writes the same radiance, bit for
+    bit, the timings report 0 ms for every stage and still count the rays;
+    turned back on, the stages are timed again."""
+    s, g = _gpu_scene(ctx, api.SCENE_SPHERES)
+    o = oracle_scene(s)
+    W, H, gd, _, _ = _gbuffers(ctx, s, g, o, 480, 270)
+    sb = s.pack()
+    out = ctx.buffer(W * H * 16)
+    ctx.light_pass(g.tlas, gd.desc(), sb, out)
+    on = ctx.timings()
+    want = out.download(np.float32, (W * H, 4))
+    try:
+        ctx.set_stage_timing(False)
+        out2 = ctx.buffer(W * H * 16)
+        ctx.light_pass(g.tlas, gd.desc(), sb, out2)
+        off = ctx.timings()
+        got = out2.download(np.float32, (W * H, 4))
+    finally:
+        ctx.set_stage_timing(True)
+    print(f"timed: trace {on.trace_ms:.4f} ms, {on.rays} rays; untimed: {off.rays} rays", flush=True)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert on.trace_ms > 0 and on.shade_ms > 0
+    assert (off.build_ms, off.compact_ms, off.raygen_ms, off.trace_ms, off.resolve_ms, off.shade_ms) == (0,) * 6
+    assert off.rays == on.rays > 0 and off.traced_rays == on.traced_rays
+    ctx.light_pass(g.tlas, gd.desc(), sb, out2)
+    assert ctx.timings().trace_ms > 0
+    g.close()
+
+
+@pytest.mark.parametrize("shadow_type",[api.SHADOW_RAYTRACED, api.SHADOW_DISABLED, api.SHADOW_MAP])
 def test_spot_light_and_shadow_types(ctx, shadow_type):
     """The cube scene with a spot light and a point light beside its
     directional light, two soft-shadow samples per light and 8 AO samples,
