@@ -810,6 +810,7 @@ luzrt_status luzrt_batch_frame(luzrt_ctx* ctx, luzrt_batch* b, uint32_t frame, i
         b->graph_key = key();
         b->graph_n = b->inst_n;
     }
+    ctx->release_clean = false;   // the frame refits BLASes and reads TLASes
     if (use_graph) {
         if (!b->exec) {
             LUZ_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

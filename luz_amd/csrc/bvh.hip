@@ -1098,6 +1098,7 @@ uint32_t* build_error_flag(luzrt_ctx* ctx) {
 
 luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices,
                                const uint32_t* indices) {
+    ctx->release_clean = false;   // BLAS data written on the context's stream
     hipStream_t st = ctx->stream;
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
@@ -1155,6 +1156,7 @@ luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
 }
 
 luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices) {
+    ctx->release_clean = false;   // BLAS data written on the context's stream
     hipStream_t st = ctx->stream;
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
@@ -1175,6 +1177,7 @@ luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
 }
 
 luzrt_status blas_wide_device(luzrt_ctx* ctx, luzrt_blas* b) {
+    ctx->release_clean = false;   // BLAS data written on the context's stream
     hipStream_t st = ctx->stream;
     const uint32_t n = b->tri_count;
     const size_t cap = n > 1 ? n - 1 : 1;   // wide nodes <= binary internal nodes
@@ -1318,12 +1321,13 @@ const float* tlas_slot_boxes(const luzrt_tlas* t) {
                                           2 * cap);
 }
 
-luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n) {
+luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n,
+                               hipStream_t st) {
     uint32_t* error = build_error_flag(ctx);
     if (!error) return fail(ctx, LUZRT_E_OUT_OF_MEMORY, "no device memory for the error flag");
     TlasJob J = tlas_job(t, dev_in, n);
     const size_t lds = tlas_lds_bytes(J.m);
-    k_tlas_build<<<1, tlas_threads(J.m), lds, ctx->stream>>>(J, error);
+    k_tlas_build<<<1, tlas_threads(J.m), lds, st>>>(J, error);
     LUZ_HIP(ctx, hipGetLastError());
     return LUZRT_OK;
 }

@@ -26,6 +26,19 @@ enum Stage { kStageBuild, kStageCompact, kStageRaygen, kStageTrace, kStageResolv
 struct luzrt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // TLAS builds run on a stream of their own (default priority), after all
+    // earlier work of `stream` (build_mark); the first later call that reads
+    // the TLAS waits for its build (tlas_acquire), so the build overlaps the
+    // work between the two on `stream`.
+    hipStream_t build_stream = nullptr;
+    hipEvent_t build_mark = nullptr;
+    int pending_builds = 0;        // TLASes whose build no call has waited for
+    // Recorded on `stream` after a visibility call's traversal; while
+    // release_clean holds, nothing enqueued on `stream` since reads a TLAS or
+    // writes a BLAS, so the next TLAS build waits for this event instead of
+    // all of `stream` (it then overlaps the call's resolve or shading).
+    hipEvent_t tlas_release = nullptr;
+    bool release_clean = false;
     std::string error;
     int num_cus = 0;
     // workspace
@@ -104,6 +117,8 @@ struct luzrt_tlas {
     void* scratch = nullptr;
     std::vector<const luzrt_blas*> referenced;
     bool wide = false;             // records point to 8-wide BLASes (compact form)
+    hipEvent_t built = nullptr;    // recorded on the build stream after each build
+    bool build_pending = false;    // no call on the context's stream has waited for it yet
 };
 
 namespace luz {
@@ -114,8 +129,8 @@ luzrt_status ensure(luzrt_ctx* ctx, DevBuf& b, size_t bytes);
 // Zeroes `bytes` bytes at `p` on the context's stream with a kernel rather
 // than a memset node (calls on this path may be captured into a graph).
 luzrt_status zero_fill(luzrt_ctx* ctx, void* p, size_t bytes);
-void stage_begin(luzrt_ctx* ctx, Stage s);
-void stage_end(luzrt_ctx* ctx, Stage s);
+void stage_begin(luzrt_ctx* ctx, Stage s, hipStream_t st = nullptr);   // null: the context's stream
+void stage_end(luzrt_ctx* ctx, Stage s, hipStream_t st = nullptr);
 
 #define LUZ_HIP(ctx, call)                                          \
     do {                                                            \
@@ -144,7 +159,16 @@ luzrt_status blas_build_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertic
                                const uint32_t* indices);
 luzrt_status blas_refit_device(luzrt_ctx* ctx, luzrt_blas* b, const void* vertices);
 luzrt_status blas_wide_device(luzrt_ctx* ctx, luzrt_blas* b);
-luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n);
+luzrt_status tlas_build_device(luzrt_ctx* ctx, luzrt_tlas* t, const InstIn* dev_in, uint32_t n,
+                               hipStream_t st);
+// Before the first kernel of a call that reads `tlas` on the context's stream:
+// waits there for the TLAS's pending build (on the host while the stream is
+// being captured).
+luzrt_status tlas_acquire(luzrt_ctx* ctx, const luzrt_tlas* tlas);
+// Waits on the host for the build stream.
+luzrt_status build_stream_sync(luzrt_ctx* ctx);
+// After the last kernel of a call that reads a TLAS: records tlas_release.
+luzrt_status tlas_release(luzrt_ctx* ctx);
 uint32_t* build_error_flag(luzrt_ctx* ctx);
 
 // Host packing of TLAS instances (FP64 inverses, mask/empty filtering); updates
@@ -324,7 +348,8 @@ luzrt_status fill_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds, 
 // context's stream through a pinned staging buffer: `src` may be reused as soon
 // as the call returns, and the call does not wait for the GPU (a staging buffer
 // is reused once its previous copy has finished).
-luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes);
+luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes,
+                          hipStream_t st = nullptr);   // null: the context's stream
 // fill_scene, then uploads the block to ctx->scene with the given ray group.
 luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy,
                           uint32_t ray_group = 0, bool trace = false);

@@ -70,6 +70,7 @@ extern "C" luzrt_status luzrt_primary_gbuffer(luzrt_ctx* ctx, const luzrt_tlas* 
                     out->width, out->height);
     luzrt_status s = ensure(ctx, ctx->counters, kCounterBytes);
     if (s != LUZRT_OK) return s;
+    if ((s = tlas_acquire(ctx, tlas)) != LUZRT_OK) return s;
     // the material table lives in device memory for the duration of the call
     size_t mbytes = (size_t)(nmats ? nmats : 1) * sizeof(luzrt_material);
     static_assert(sizeof(luzrt_material) == 16, "material layout");

@@ -73,13 +73,42 @@ luzrt_status zero_fill(luzrt_ctx* ctx, void* p, size_t bytes) {
     return LUZRT_OK;
 }
 
-void stage_begin(luzrt_ctx* ctx, Stage s) {
+void stage_begin(luzrt_ctx* ctx, Stage s, hipStream_t st) {
     if (!ctx->timing) return;
-    hipEventRecord(ctx->ev[s][0], ctx->stream);
+    hipEventRecord(ctx->ev[s][0], st ? st : ctx->stream);
     ctx->ev_used[s] = true;
 }
-void stage_end(luzrt_ctx* ctx, Stage s) {
-    if (ctx->timing) hipEventRecord(ctx->ev[s][1], ctx->stream);
+void stage_end(luzrt_ctx* ctx, Stage s, hipStream_t st) {
+    if (ctx->timing) hipEventRecord(ctx->ev[s][1], st ? st : ctx->stream);
+}
+
+luzrt_status tlas_acquire(luzrt_ctx* ctx, const luzrt_tlas* tlas) {
+    luzrt_tlas* t = const_cast<luzrt_tlas*>(tlas);
+    ctx->release_clean = false;   // this call reads a TLAS after any earlier release
+    if (!t->build_pending) return LUZRT_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LUZ_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        LUZ_HIP(ctx, hipEventSynchronize(t->built));
+    else
+        LUZ_HIP(ctx, hipStreamWaitEvent(ctx->stream, t->built, 0));
+    t->build_pending = false;
+    ctx->pending_builds--;
+    return LUZRT_OK;
+}
+
+luzrt_status tlas_release(luzrt_ctx* ctx) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LUZ_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return LUZRT_OK;
+    LUZ_HIP(ctx, hipEventRecord(ctx->tlas_release, ctx->stream));
+    ctx->release_clean = true;
+    return LUZRT_OK;
+}
+
+luzrt_status build_stream_sync(luzrt_ctx* ctx) {
+    if (ctx->build_stream) LUZ_HIP(ctx, hipStreamSynchronize(ctx->build_stream));
+    return LUZRT_OK;
 }
 
 luzrt_status check_gbuffer(luzrt_ctx* ctx, const luzrt_gbuffer* g, luzrt_gbuffer* out) {
@@ -168,8 +197,9 @@ luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* ds
     return stage_upload(ctx, ctx->scene.ptr, ds, sizeof(DevScene));
 }
 
-luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st) {
     if (bytes == 0) return LUZRT_OK;
+    if (!st) st = ctx->stream;
     luzrt_ctx::Staging& sl = ctx->staging[ctx->staging_next];
     ctx->staging_next = (ctx->staging_next + 1) % luzrt_ctx::kStagingSlots;
     if (sl.pending) {
@@ -186,8 +216,8 @@ luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t byt
         sl.bytes = sz;
     }
     std::memcpy(sl.host, src, bytes);
-    LUZ_HIP(ctx, hipMemcpyAsync(dst, sl.host, bytes, hipMemcpyHostToDevice, ctx->stream));
-    LUZ_HIP(ctx, hipEventRecord(sl.done, ctx->stream));
+    LUZ_HIP(ctx, hipMemcpyAsync(dst, sl.host, bytes, hipMemcpyHostToDevice, st));
+    LUZ_HIP(ctx, hipEventRecord(sl.done, st));
     sl.pending = true;
     return LUZRT_OK;
 }
@@ -295,6 +325,12 @@ luzrt_status luzrt_create(luzrt_ctx** out, int device) {
         delete ctx;
         return LUZRT_E_DEVICE;
     }
+    if (hipStreamCreateWithFlags(&ctx->build_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->build_mark, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->tlas_release, hipEventDisableTiming) != hipSuccess) {
+        luzrt_destroy(ctx);
+        return LUZRT_E_DEVICE;
+    }
     // Stage events only time: luzrt_get_timings reads them after a full stream
     // synchronisation, so they skip the system-scope fence (an L2 write-back and
     // invalidate between the stages' kernels, which cost about 10 us per record).
@@ -314,6 +350,7 @@ luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    if (ctx->build_stream) hipStreamSynchronize(ctx->build_stream);
     DevBuf* bufs[] = {&ctx->hits, &ctx->active, &ctx->act_index, &ctx->act_rec,
                       &ctx->counters, &ctx->spill, &ctx->lut, &ctx->scene, &ctx->vis_shadow,
                       &ctx->vis_ao, &ctx->cull_cand, &ctx->cull_bits, &ctx->cull_rows};
@@ -325,7 +362,10 @@ luzrt_status luzrt_destroy(luzrt_ctx* ctx) {
         if (sl.host) hipHostFree(sl.host);
         if (sl.done) hipEventDestroy(sl.done);
     }
-    hipStreamDestroy(ctx->stream);
+    if (ctx->build_mark) hipEventDestroy(ctx->build_mark);
+    if (ctx->tlas_release) hipEventDestroy(ctx->tlas_release);
+    if (ctx->build_stream) hipStreamDestroy(ctx->build_stream);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return LUZRT_OK;
 }
@@ -349,12 +389,14 @@ static luzrt_status check_trace_latch(luzrt_ctx* ctx) {
 luzrt_status luzrt_sync(luzrt_ctx* ctx) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (build_stream_sync(ctx) != LUZRT_OK) return LUZRT_E_DEVICE;
     return check_trace_latch(ctx);
 }
 
 luzrt_status luzrt_get_timings(luzrt_ctx* ctx, luzrt_timings* out) {
     if (!ctx || !out) return LUZRT_E_INVALID_ARG;
     LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (build_stream_sync(ctx) != LUZRT_OK) return LUZRT_E_DEVICE;
     float* f[kStageCount] = {&out->build_ms, &out->compact_ms, &out->raygen_ms, &out->trace_ms,
                              &out->resolve_ms, &out->shade_ms, &out->primary_ms, &out->denoise_ms,
                              &out->post_ms};
@@ -484,6 +526,7 @@ luzrt_status luzrt_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t byt
 luzrt_status luzrt_download(luzrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!ctx || (bytes && (!dst || !src))) return LUZRT_E_INVALID_ARG;
     if (!bytes) return LUZRT_OK;
+    if (build_stream_sync(ctx) != LUZRT_OK) return LUZRT_E_DEVICE;   // a TLAS being inspected
     LUZ_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     LUZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return LUZRT_OK;
@@ -550,6 +593,8 @@ luzrt_status luzrt_blas_refit(luzrt_ctx* ctx, luzrt_blas* b, const void* vertice
     if (!ctx) return LUZRT_E_INVALID_ARG;
     if (!b || (b->vertex_count && !vertices))
         return fail(ctx, LUZRT_E_INVALID_ARG, "luzrt_blas_refit: null argument");
+    // a TLAS build still pending may be reading this BLAS's boxes
+    if (ctx->pending_builds > 0 && build_stream_sync(ctx) != LUZRT_OK) return LUZRT_E_DEVICE;
     stage_begin(ctx, kStageBuild);
     luzrt_status s = blas_refit_device(ctx, b, vertices);
     stage_end(ctx, kStageBuild);
@@ -676,18 +721,40 @@ luzrt_status luzrt_tlas_build(luzrt_ctx* ctx, luzrt_tlas* t, const luzrt_instanc
     t->wide = ctx->precision != LUZRT_PRECISION_PLAIN;   // COMPACT and WIDE trace the 8-wide form
     if (t->wide && !t->recs_w)
         LUZ_HIP(ctx, hipMalloc((void**)&t->recs_w, (size_t)t->capacity * sizeof(InstRec)));
-    if ((s = stage_upload(ctx, t->input, host.data(), host.size() * sizeof(InstIn))) != LUZRT_OK)
+    if (!t->built) LUZ_HIP(ctx, hipEventCreateWithFlags(&t->built, hipEventDisableTiming));
+    // On the build stream after the work enqueued so far on the context's
+    // stream that reads a TLAS or writes a BLAS (all of it, unless the last
+    // such work was a visibility call's traversal: tlas_release); on the
+    // context's stream itself while a caller captures it into a graph.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LUZ_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    hipStream_t bs = ctx->stream;
+    if (cs == hipStreamCaptureStatusNone) {
+        bs = ctx->build_stream;
+        if (!ctx->release_clean) LUZ_HIP(ctx, hipEventRecord(ctx->build_mark, ctx->stream));
+        LUZ_HIP(ctx, hipStreamWaitEvent(bs, ctx->release_clean ? ctx->tlas_release : ctx->build_mark, 0));
+    }
+    if ((s = stage_upload(ctx, t->input, host.data(), host.size() * sizeof(InstIn), bs)) != LUZRT_OK)
         return s;
-    stage_begin(ctx, kStageBuild);
-    s = tlas_build_device(ctx, t, (const InstIn*)t->input, (uint32_t)host.size());
-    stage_end(ctx, kStageBuild);
-    return s;
+    stage_begin(ctx, kStageBuild, bs);
+    s = tlas_build_device(ctx, t, (const InstIn*)t->input, (uint32_t)host.size(), bs);
+    stage_end(ctx, kStageBuild, bs);
+    if (s != LUZRT_OK) return s;
+    if (bs != ctx->stream) {
+        LUZ_HIP(ctx, hipEventRecord(t->built, bs));
+        if (!t->build_pending) ctx->pending_builds++;
+        t->build_pending = true;
+    }
+    return LUZRT_OK;
 }
 
 luzrt_status luzrt_tlas_destroy(luzrt_ctx* ctx, luzrt_tlas* t) {
     if (!ctx) return LUZRT_E_INVALID_ARG;
     if (!t) return LUZRT_OK;
     hipStreamSynchronize(ctx->stream);
+    if (ctx->build_stream) hipStreamSynchronize(ctx->build_stream);
+    if (t->built) hipEventDestroy(t->built);
+    if (t->build_pending) ctx->pending_builds--;
     for (const luzrt_blas* b : t->referenced) const_cast<luzrt_blas*>(b)->refs--;
     void* ptrs[] = {t->nodes, t->recs, t->recs_w, t->aux, t->w2o, t->order, t->input, t->scratch};
     for (void* p : ptrs)

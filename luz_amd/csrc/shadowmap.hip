@@ -181,6 +181,7 @@ extern "C" luzrt_status luzrt_shadow_map_build(luzrt_ctx* ctx, const luzrt_tlas*
         return fail(ctx, LUZRT_E_INVALID_ARG, "point-light shadow map needs z_far > 0");
     luzrt_status s = ensure(ctx, ctx->counters, kCounterBytes);
     if (s != LUZRT_OK) return s;
+    if ((s = tlas_acquire(ctx, tlas)) != LUZRT_OK) return s;
     uint32_t groups = (uint32_t)ctx->num_cus * 4;
     if (groups == 0) groups = 1;
     if ((s = ensure(ctx, ctx->spill, (size_t)groups * kBlock * kSpill * 4)) != LUZRT_OK) return s;

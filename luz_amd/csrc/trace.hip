@@ -533,7 +533,9 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
     LUZ_HIP(ctx, hipGetLastError());
 
     // rays are generated inside the traversal; the compaction scan set their
-    // count (the raygen stage times the culling kernels, when they run)
+    // count (the raygen stage times the culling kernels, when they run).  The
+    // compaction reads no TLAS data, so it overlaps the TLAS's pending build.
+    if ((s = tlas_acquire(ctx, tlas)) != LUZRT_OK) return s;
 
     // ray-group culling (grouped order, <= 128 TLAS slots, <= 64 rays per pixel)
     const uint32_t ngroups = (uint32_t)((npix + kCullGroup - 1) / kCullGroup);
@@ -586,6 +588,7 @@ luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt
         if ((s = zero_fill(ctx, ctx->hits.ptr, max_rays)) != LUZRT_OK) return s;
     }
     stage_end(ctx, kStageTrace);
+    if ((s = tlas_release(ctx)) != LUZRT_OK) return s;   // the rest reads no TLAS data
 
     if (!resolve) return LUZRT_OK;
     stage_begin(ctx, kStageResolve);
@@ -683,6 +686,7 @@ extern "C" luzrt_status luzrt_trace_rays(luzrt_ctx* ctx, const luzrt_tlas* tlas,
     if (count == 0) return LUZRT_OK;
     luzrt_status s = ensure(ctx, ctx->counters, kCounterBytes);
     if (s != LUZRT_OK) return s;
+    if ((s = tlas_acquire(ctx, tlas)) != LUZRT_OK) return s;
     uint32_t* counters = (uint32_t*)ctx->counters.ptr;
     stage_begin(ctx, kStageTrace);
     k_vis_init<<<1, 64, 0, ctx->stream>>>(counters, count);
@@ -776,6 +780,7 @@ extern "C" luzrt_status luzrt_trace_visibility_static(luzrt_ctx* ctx, const luzr
     stage_end(ctx, kStageCompact);
     LUZ_HIP(ctx, hipGetLastError());
 
+    if ((s = tlas_acquire(ctx, tlas)) != LUZRT_OK) return s;
     stage_begin(ctx, kStageTrace);
     if (ds.r_total > 0 && tlas->count > 0) {
         TraceLaunch L{};

@@ -75,6 +75,39 @@ def test_tlas_clustering_bytes_match_oracle(ctx, count, ties):
     blas.destroy()
 
 
+def test_tlas_rebuilt_between_calls_without_waiting(ctx):
+    """TLAS builds run on a stream of their own.  Four rebuilds of one TLAS
+    with different instance placements, each followed at once by a ray call
+    into its own buffer, nothing waited for in between: every call sees the
+    instances of the build before it and none of the build after it."""
+    rng = np.random.default_rng(5)
+    v, idx = cube_mesh()
+    blas = ctx.create_blas(ctx.to_device(v), 48, len(v), ctx.to_device(idx), len(idx))
+    count = 40
+    tlas = ctx.create_tlas(count)
+    rays = random_rays(rng, 30000, (0, 0, 0), 30.0, 60.0)
+    dr = ctx.to_device(rays)
+    placements, outs = [], []
+    for k in range(4):
+        xf = scaled_instances(rng.uniform(-25, 25, (count, 3)), rng.uniform(0.5, 3.0, count))
+        tlas.build([api.make_instance(blas, xf[i], i) for i in range(count)])
+        dh = ctx.buffer(len(rays))
+        ctx.trace_rays(tlas, dr, len(rays), dh)
+        placements.append(xf)
+        outs.append(dh)
+    got = [dh.download(np.uint8, len(rays)) for dh in outs]
+    for k, xf in enumerate(placements):
+        o = orc.OracleScene()
+        m = o.add_mesh(v, idx)
+        o.set_instances(xf, [m] * count)
+        want, _ = o.any_hit(rays)
+        print(f"build {k}: {int(want.sum())} of {len(rays)} rays hit", flush=True)
+        assert 0 < want.sum() < len(rays)
+        np.testing.assert_array_equal(got[k], want)
+    tlas.destroy()
+    blas.destroy()
+
+
 @pytest.mark.parametrize("kind", [api.SCENE_CUBE, api.SCENE_SPHERES])
 def test_trace_rays_match_oracle(ctx, kind):
     s, g = _gpu_scene(ctx, kind)
