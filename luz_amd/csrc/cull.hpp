@@ -32,30 +32,47 @@ constexpr uint32_t kMaxCullInst = 128;  // TLAS slots a candidate mask covers (k
 constexpr uint32_t kCandWords = 2;      // u64 words of a candidate mask
 constexpr uint32_t kMaxCullSlots = 64;  // ray slots per pixel a live-row mask covers
 
-// Wave-wide min / max (the result in every lane): DPP exchanges within each
-// row of 16 lanes (quad xor 1, quad xor 2, half-row mirror, row mirror), then
-// the four row values read by readlane.
+// Wave-wide min / max of finite floats (the result, uniform, in every lane):
+// each float mapped to an integer of the same order (sign and magnitude to
+// two's complement), v_min_i32 / v_max_i32 with DPP exchanges within each row
+// of 16 lanes (quad xor 1, quad xor 2, half-row mirror, row mirror), the four
+// row values combined on the scalar unit after readlane, and mapped back.
+// Exact, as min and max do not round; a non-finite input may give any value
+// (cull_group takes every slot as a candidate for such a group).
+__device__ inline int order_key(float v) {
+    const int b = __float_as_int(v);
+    return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ inline float key_value(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
 template <int CTRL>
-__device__ inline float dpp_of(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF,
-                                                      false));
+__device__ inline int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
-__device__ inline float row_value(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-__device__ inline float wave_min(float v) {
-    v = fminf(v, dpp_of<0xB1>(v));
-    v = fminf(v, dpp_of<0x4E>(v));
-    v = fminf(v, dpp_of<0x141>(v));
-    v = fminf(v, dpp_of<0x140>(v));
-    return fminf(fminf(row_value(v, 0), row_value(v, 16)), fminf(row_value(v, 32), row_value(v, 48)));
-}
-__device__ inline float wave_max(float v) {
-    v = fmaxf(v, dpp_of<0xB1>(v));
-    v = fmaxf(v, dpp_of<0x4E>(v));
-    v = fmaxf(v, dpp_of<0x141>(v));
-    v = fmaxf(v, dpp_of<0x140>(v));
-    return fmaxf(fmaxf(row_value(v, 0), row_value(v, 16)), fmaxf(row_value(v, 32), row_value(v, 48)));
+// The bounds lo[i] = wave min, hi[i] = wave max of v[i], i < N, with the
+// exchanges of all 2N values interleaved step by step (so each DPP read finds
+// its operand written several instructions earlier).
+template <int N>
+__device__ inline void wave_bounds(const float (&v)[N], float (&lo)[N], float (&hi)[N]) {
+    int a[N], b[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[i] = b[i] = order_key(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) { a[i] = min(a[i], dpp_i<0xB1>(a[i])); b[i] = max(b[i], dpp_i<0xB1>(b[i])); }
+#pragma unroll
+    for (int i = 0; i < N; ++i) { a[i] = min(a[i], dpp_i<0x4E>(a[i])); b[i] = max(b[i], dpp_i<0x4E>(b[i])); }
+#pragma unroll
+    for (int i = 0; i < N; ++i) { a[i] = min(a[i], dpp_i<0x141>(a[i])); b[i] = max(b[i], dpp_i<0x141>(b[i])); }
+#pragma unroll
+    for (int i = 0; i < N; ++i) { a[i] = min(a[i], dpp_i<0x140>(a[i])); b[i] = max(b[i], dpp_i<0x140>(b[i])); }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int a0 = __builtin_amdgcn_readlane(a[i], 0), a1 = __builtin_amdgcn_readlane(a[i], 16),
+                  a2 = __builtin_amdgcn_readlane(a[i], 32), a3 = __builtin_amdgcn_readlane(a[i], 48);
+        const int b0 = __builtin_amdgcn_readlane(b[i], 0), b1 = __builtin_amdgcn_readlane(b[i], 16),
+                  b2 = __builtin_amdgcn_readlane(b[i], 32), b3 = __builtin_amdgcn_readlane(b[i], 48);
+        lo[i] = key_value(min(min(a0, a1), min(a2, a3)));
+        hi[i] = key_value(max(max(b0, b1), max(b2, b3)));
+    }
 }
 __device__ inline bool finite3(V3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
 
@@ -102,11 +119,14 @@ __device__ inline uint64_t cull_group(const PixelFrame& f, const LIGHT* lights, 
     const V3 lv = vmul(f.N, fmax_(f.lc * 0.01f, 0.05f));
     const bool bad = !(finite3(o) && finite3(f.P) && finite3(lv) && isfinite(nlen));
     const bool any_bad = __ballot(bad) != 0ull;
-    const float alo[3] = {wave_min(o.x), wave_min(o.y), wave_min(o.z)};
-    const float ahi[3] = {wave_max(o.x), wave_max(o.y), wave_max(o.z)};
-    const float llo[3] = {wave_min(lv.x), wave_min(lv.y), wave_min(lv.z)};
-    const float lhi[3] = {wave_max(lv.x), wave_max(lv.y), wave_max(lv.z)};
-    const float nmax = wave_max(nlen);
+    float alo[3], ahi[3], llo[3], lhi[3], nlo[1], nhi[1];
+    {
+        const float ov[3] = {o.x, o.y, o.z}, lvv[3] = {lv.x, lv.y, lv.z}, nv[1] = {nlen};
+        wave_bounds(ov, alo, ahi);
+        wave_bounds(lvv, llo, lhi);
+        wave_bounds(nv, nlo, nhi);
+    }
+    const float nmax = nhi[0];
     float l0[3], rl2 = 0.0f;
     for (int k = 0; k < 3; ++k) {
         l0[k] = 0.5f * (llo[k] + lhi[k]);
@@ -153,8 +173,9 @@ __device__ inline uint64_t cull_group(const PixelFrame& f, const LIGHT* lights, 
                 const V3 dir = v3(Lt.direction[0], Lt.direction[1], Lt.direction[2]);
                 const float kd = vdot(dir, vnorm(v3(-dir.x, -dir.y, -dir.z)));
                 const V3 e = vadd(o, vmul(dir, kd * vlen(vsub(lpos, f.P))));
-                const float elo[3] = {wave_min(e.x), wave_min(e.y), wave_min(e.z)};
-                const float ehi[3] = {wave_max(e.x), wave_max(e.y), wave_max(e.z)};
+                const float ev[3] = {e.x, e.y, e.z};
+                float elo[3], ehi[3];
+                wave_bounds(ev, elo, ehi);
                 float rc2 = 0.0f;
                 cmag = 0.0f;
                 for (int k = 0; k < 3; ++k) {
