@@ -271,6 +271,14 @@ __device__ inline bool box_hit_nf(const BoxRay& r, float nx, float fx, float ny,
     return tn <= tf;
 }
 
+// The same test from the slab distances themselves (near and far per axis).
+__device__ inline bool box_hit_t(const BoxRay& r, float tnx, float tfx, float tny, float tfy, float tnz, float tfz) {
+    const float tn = fmaxf(fmaxf(fmaxf(r.tlo, tnx), tny), tnz);
+    float tf = fminf(fminf(fminf(r.thi, tfx), tfy), tfz);
+    tf = tf + fabsf(tf) * 0x1p-20f;
+    return tn <= tf;
+}
+
 __device__ inline bool box_hit(const BoxRay& r, float lox, float hix, float loy, float hiy,
                                float loz, float hiz, float& tnear) {
     bool sx = r.inv.x < 0.0f, sy = r.inv.y < 0.0f, sz = r.inv.z < 0.0f;

@@ -173,18 +173,26 @@ __device__ inline int wide_visit(RayState& r, STACK& st) {
     const uint32_t nxa = gx ? qb.z : qa.x, nxb = gx ? qb.w : qa.y, fxa = gx ? qa.x : qb.z, fxb = gx ? qa.y : qb.w;
     const uint32_t nya = gy ? qc.x : qa.z, nyb = gy ? qc.y : qa.w, fya = gy ? qa.z : qc.x, fyb = gy ? qa.w : qc.y;
     const uint32_t nza = gz ? qc.z : qb.x, nzb = gz ? qc.w : qb.y, fza = gz ? qb.x : qc.z, fzb = gz ? qb.y : qc.w;
-    // q x 2^e (q < 256, 2^e a normal power of two) is exact, so the fused
-    // multiply-add rounds once where o + q x 2^e would: the same decoded bounds
+    // A child plane o + q x 2^e is met at t = (o + q x 2^e - ro) x inv =
+    // q x S + O, with S = 2^e x inv (exact: a power-of-two scaling) and
+    // O = (o - ro) x inv per node and axis: one fused multiply-add per plane.
+    // Its rounding error, like that of decoding the plane first, is a few
+    // ulps of the terms, covered as before by the far bound's widening and
+    // the child boxes' outward rounding; an infinite reciprocal (a direction
+    // component of zero) gives infinite or NaN distances that the max/min
+    // ignore or that keep the box, as before.
+    const float Sx = sx * r.br.inv.x, Sy = sy * r.br.inv.y, Sz = sz * r.br.inv.z;
+    const float Ox = (ox - r.br.o.x) * r.br.inv.x, Oy = (oy - r.br.o.y) * r.br.inv.y,
+                Oz = (oz - r.br.o.z) * r.br.inv.z;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const float nx = __builtin_fmaf((float)byte_of(nxa, nxb, k), sx, ox);
-        const float fx = __builtin_fmaf((float)byte_of(fxa, fxb, k), sx, ox);
-        const float ny = __builtin_fmaf((float)byte_of(nya, nyb, k), sy, oy);
-        const float fy = __builtin_fmaf((float)byte_of(fya, fyb, k), sy, oy);
-        const float nz = __builtin_fmaf((float)byte_of(nza, nzb, k), sz, oz);
-        const float fz = __builtin_fmaf((float)byte_of(fza, fzb, k), sz, oz);
-        float tn;
-        if (k < nchild && box_hit_nf(r.br, nx, fx, ny, fy, nz, fz, tn)) mask |= 1u << k;
+        const float tnx = __builtin_fmaf((float)byte_of(nxa, nxb, k), Sx, Ox);
+        const float tfx = __builtin_fmaf((float)byte_of(fxa, fxb, k), Sx, Ox);
+        const float tny = __builtin_fmaf((float)byte_of(nya, nyb, k), Sy, Oy);
+        const float tfy = __builtin_fmaf((float)byte_of(fya, fyb, k), Sy, Oy);
+        const float tnz = __builtin_fmaf((float)byte_of(nza, nzb, k), Sz, Oz);
+        const float tfz = __builtin_fmaf((float)byte_of(fza, fzb, k), Sz, Oz);
+        if (k < nchild && box_hit_t(r.br, tnx, tfx, tny, tfy, tnz, tfz)) mask |= 1u << k;
     }
     const uint64_t meta = ((uint64_t)h1.w << 32) | h1.z;
     r.next = kNone;
