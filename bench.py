@@ -55,6 +55,7 @@ def main() -> int:
                     help="search the traversal launch shape on this workload before timing")
     ap.add_argument("--groups-per-cu", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--stack-entries", type=int, default=0, help="LDS stack entries per lane (8, 16 or 32)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC counter calibration)")
     ap.add_argument("--traffic", default=None,
@@ -73,8 +74,8 @@ def main() -> int:
     from luz_amd import api
 
     ctx = api.Context(0)
-    if args.groups_per_cu or args.refill:
-        ctx.set_trace_config(args.groups_per_cu, args.refill)
+    if args.groups_per_cu or args.refill or args.stack_entries:
+        ctx.set_trace_config(args.groups_per_cu, args.refill, args.stack_entries)
     if args.workload == "batch64":
         return bench_batch(ctx, args)
     if args.workload == "denoise1080p":

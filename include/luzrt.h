@@ -212,8 +212,10 @@ luzrt_status luzrt_set_step_limit(luzrt_ctx* ctx, uint32_t max_steps);
  * above; no counterpart in the reference, whose ray queries run per
  * fragment).  SLOT: all pixels' first sample, then all second samples, ...;
  * GROUPED: the samples of 64 neighbouring pixels together, one sample after
- * the other; AUTO (default): GROUPED while the distinct BLASes of the TLAS
- * total at most 32 MB, SLOT above.  Results are the same in every order.
+ * the other; AUTO (default): GROUPED when the call can cull its ray groups
+ * (luzrt_set_group_culling on, at most 128 TLAS instances and 64 rays per
+ * pixel) or while the distinct BLASes of the TLAS total at most 32 MB, SLOT
+ * otherwise.  Results are the same in every order.
  * Batched frames use SLOT unless GROUPED is set before luzrt_batch_create. */
 enum { LUZRT_RAY_ORDER_AUTO = 0, LUZRT_RAY_ORDER_SLOT = 1, LUZRT_RAY_ORDER_GROUPED = 2 };
 luzrt_status luzrt_set_ray_order(luzrt_ctx* ctx, uint32_t order);

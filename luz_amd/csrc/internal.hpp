@@ -354,7 +354,7 @@ luzrt_status stage_upload(luzrt_ctx* ctx, void* dst, const void* src, size_t byt
 luzrt_status upload_scene(luzrt_ctx* ctx, const luzrt_scene* scene, DevScene* host_copy,
                           uint32_t ray_group = 0, bool trace = false);
 // Ray group of a visibility call on `tlas` under the context's ray order.
-uint32_t ray_group_for(const luzrt_ctx* ctx, const luzrt_tlas* tlas);
+uint32_t ray_group_for(const luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_scene* scene = nullptr);
 // Validates a G-buffer descriptor and writes it with its layout filled in
 // (zero strides and pitches replaced by the packed defaults; the depth
 // pointer cleared when a position plane is given): the form kernels take.

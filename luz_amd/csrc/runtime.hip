@@ -14,6 +14,7 @@
 
 #include "internal.hpp"
 #include "traverse.hpp"
+#include "cull.hpp"
 
 namespace luz {
 
@@ -308,9 +309,20 @@ static bool invert_3x4(const float m[12], float out[12], float nmat[9]) {
 // 4.2 M triangles, 36 lights + 8 AO rays: 32.8 ms grouped, 29.3 ms by slot.
 constexpr size_t kGroupedBvhBytes = 32u << 20;   // the L2 of all XCDs together
 
-uint32_t ray_group_for(const luzrt_ctx* ctx, const luzrt_tlas* tlas) {
+uint32_t ray_group_for(const luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_scene* scene) {
     if (ctx->ray_order == LUZRT_RAY_ORDER_SLOT) return 0;
     if (ctx->ray_order == LUZRT_RAY_ORDER_GROUPED) return kRayGroup;
+    // a call that can cull its ray groups (which needs the grouped order)
+    // gains more from the culling than the slot order's coherent BVH reads
+    // give: on the 4K terrain (a 270 MB BVH) 20.7 against 21.9 ms
+    if (scene && ctx->group_cull && tlas->count > 0 && tlas->count <= kMaxCullInst &&
+        scene->num_lights >= 0 && scene->num_lights <= LUZRT_MAX_LIGHTS) {
+        uint32_t rays = scene->ao_num_samples > 0 ? (uint32_t)scene->ao_num_samples : 0u;
+        if (scene->shadow_type == LUZRT_SHADOW_RAYTRACED)
+            for (int l = 0; l < scene->num_lights; ++l)
+                rays += scene->lights[l].num_shadow_samples > 0 ? (uint32_t)scene->lights[l].num_shadow_samples : 0u;
+        if (rays > 0 && rays <= kMaxCullSlots) return kRayGroup;
+    }
     std::vector<const luzrt_blas*> seen(tlas->referenced);
     std::sort(seen.begin(), seen.end());
     seen.erase(std::unique(seen.begin(), seen.end()), seen.end());

@@ -498,7 +498,7 @@ luzrt_status launch_trace(luzrt_ctx* ctx, const TraceLaunch& L) {
 luzrt_status visibility_core(luzrt_ctx* ctx, const luzrt_tlas* tlas, const luzrt_gbuffer* g,
                              const luzrt_scene* scene, uint8_t* shadow, uint8_t* ao, bool resolve) {
     DevScene ds;
-    luzrt_status s = upload_scene(ctx, scene, &ds, ray_group_for(ctx, tlas), true);
+    luzrt_status s = upload_scene(ctx, scene, &ds, ray_group_for(ctx, tlas, scene), true);
     if (s != LUZRT_OK) return s;
     luzrt_gbuffer gl;
     if ((s = check_gbuffer(ctx, g, &gl)) != LUZRT_OK) return s;
